@@ -1,0 +1,216 @@
+/* mfc.h — C ABI of the MI355X-native 2D SPH orthogonal-cutting solver.
+ *
+ * The reference (iwf-inspire/mfree_iwf-ul_cut_gpu) has no FFI: main() calls C++ free
+ * functions and classes directly. The entry points below are the C-ABI form of that
+ * call sequence, one per reference interface:
+ *
+ *   setup_ref_cut(ny, &grid, rake, clear, chamfer, speed, feed)  benchmarks.h:48,
+ *       benchmarks.cpp:255-537                         -> mfc_config_ref_cut + mfc_create
+ *   new leap_frog(N, num_cell)                         leap_frog.h:49, leap_frog.cu:157-166
+ *                                                      -> part of mfc_create
+ *   leap_frog::step(particles, grid)                   leap_frog.h:48, leap_frog.cu:94-155
+ *       + global_step++ (mfree_gpu.cpp:308-309)        -> mfc_step
+ *   float2_t report_force(particle_gpu*)               output.h:26, output.cu:31-39
+ *                                                      -> mfc_tool_force
+ *   cudaMemcpy of particle_gpu members (debug.cpp:33-41, vtk_writer.cpp:59-71)
+ *                                                      -> mfc_get_field
+ *   grid_base::get_cells(particles, start, end)        grid.h:280, grid_gpu_green.cu:207-210
+ *                                                      -> mfc_get_cells
+ *   tool::get_segments / get_fillet (tool.h:75-76)     -> mfc_get_tool
+ *
+ * Conventions: every function returns 0 on success and a negative MFC_E_* code on
+ * failure (mfc_last_error() gives the message of the calling thread's last failure);
+ * nothing exits or asserts. The handle owns all device memory; the caller owns host
+ * buffers. All constants live in the handle, so several handles may coexist in one
+ * process. One handle is driven by one host thread at a time.
+ *
+ * Units are the reference's "bomb units" (benchmarks.h:47): cm, microseconds, grams,
+ * Mbar; temperatures in K.
+ */
+#ifndef MFC_H_
+#define MFC_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MFC_OK            0
+#define MFC_E_INVALID    -1   /* bad argument or configuration */
+#define MFC_E_HIP        -2   /* HIP runtime failure */
+#define MFC_E_NOMEM      -3   /* allocation failed */
+#define MFC_E_RANGE      -4   /* size or index out of range */
+#define MFC_E_STATE      -5   /* call not valid in the handle's current state */
+
+/* thermal solver, compile-time switch in the reference (types.h:94-95) */
+#define MFC_THERMAL_BROOKSHAW 0
+#define MFC_THERMAL_PSE       1
+
+/* arithmetic of the step */
+#define MFC_PREC_FP64   0   /* plain FP64, bit-exact with the oracle */
+#define MFC_PREC_FP32C  1   /* FP32 neighbour sums with Kahan compensation, FP64 islands */
+
+/* lattice rule */
+#define MFC_LATTICE_REF      0  /* setup_ref_cut: lo_y = hi_y - 3 feed, dx = (hi_y-lo_y)/(ny-1),
+                                   nx = (hi_x-lo_x)/dx (benchmarks.cpp:266-273) */
+#define MFC_LATTICE_EXPLICIT 1  /* nx x ny particles at spacing dx, top row at hi_y */
+
+typedef struct mfc_config {
+  /* workpiece lattice */
+  int32_t lattice;        /* MFC_LATTICE_* */
+  int32_t nx, ny;         /* particle rows/columns (nx ignored for MFC_LATTICE_REF) */
+  double dx;              /* spacing (MFC_LATTICE_EXPLICIT) */
+  double lo_x, hi_x, hi_y;/* workpiece extent; hi_x only used by MFC_LATTICE_REF */
+  double hdx;             /* h = hdx dx */
+  double jitter;          /* initial positions += U(-jitter, jitter) dx per coordinate */
+  uint64_t seed;          /* SplitMix64 seed of the jitter */
+  int32_t fixed_bottom_rows; /* rows fixed at the bottom (reference: 1, benchmarks.cpp:374) */
+  int32_t fix_left;       /* fix left edge below half height (benchmarks.cpp:376) */
+  int32_t fix_right;      /* fix right edge (benchmarks.cpp:378) */
+
+  /* material (phys_constants, constants_structs.h:27-35) */
+  double E, nu, rho0;
+  /* Johnson-Cook (joco_constants, constants_structs.h:61-75) */
+  double jc_A, jc_B, jc_C, jc_n, jc_m, jc_Tmelt, jc_Tref, jc_eps_dot_ref;
+  /* workpiece thermal constants (trml_constants, constants_structs.h:39-46) */
+  double cp, tq, k_wp, eta, T_init;
+  /* artificial viscosity / XSPH / artificial stress (corr_constants, constants_structs.h:50-57) */
+  double av_alpha, av_beta, av_eta, xsph_eps, stress_eps;
+
+  /* tool (tool.cpp:342-375, benchmarks.cpp:389-421) */
+  double tool_tl_x, tool_tl_y, tool_length, tool_height;
+  double rake_deg, clear_deg, edge_radius;  /* edge_radius == 0: sharp tool */
+  double feed;            /* depth of cut */
+  double speed;           /* cm/us */
+  double mu;              /* Coulomb friction coefficient */
+  double contact_alpha;   /* penalty scaling (benchmarks.cpp:419) */
+  int32_t sample_tool;    /* tool particles for thermal contact (benchmarks.cpp:263,424-459) */
+  double rho_tool, cp_tool, k_tool;
+
+  /* solver */
+  int32_t thermal_solver; /* MFC_THERMAL_* */
+  int32_t precision;      /* MFC_PREC_* */
+  double cfl;             /* dt = cfl h / (c0 + sqrt(speed)) (benchmarks.cpp:530 uses 0.5) */
+  int32_t pse_flag_quirk; /* 1: PSE tool flag reads 0 for index >= N/2 (interactions_gpu.cu:405) */
+  int32_t reserved[7];
+} mfc_config;
+
+typedef struct mfc_info {
+  int32_t n_particles;    /* workpiece + tool particles */
+  int32_t n_tool;         /* sampled tool particles (stored after the workpiece initially) */
+  int32_t grid_nx, grid_ny, num_cell;
+  int32_t precision;
+  double dt, t_final;     /* global_dt, global_t_final (benchmarks.cpp:530-531) */
+  double mass;            /* particle mass dx^2 rho0 */
+  double bbmin_x, bbmin_y, cell_size;
+  int64_t step;           /* number of completed steps (global_step) */
+} mfc_info;
+
+/* particle fields, returned in the handle's current (cell-sorted) order */
+typedef enum mfc_field {
+  MFC_FIELD_POS = 0,      /* 2 doubles */
+  MFC_FIELD_VEL = 1,      /* 2 */
+  MFC_FIELD_S = 2,        /* 4: Sxx, Sxy, Syy, Szz */
+  MFC_FIELD_RHO = 3,      /* 1 */
+  MFC_FIELD_P = 4,        /* 1 */
+  MFC_FIELD_T = 5,        /* 1 */
+  MFC_FIELD_EPS_PL = 6,   /* 1 */
+  MFC_FIELD_EPS_PL_DOT = 7, /* 1 */
+  MFC_FIELD_FC = 8,       /* 2 contact force */
+  MFC_FIELD_FT = 9,       /* 2 friction force */
+  MFC_FIELD_N = 10,       /* 2 contact normal (positional, as particle_gpu::n) */
+  MFC_FIELD_H = 11,       /* 1 */
+  MFC_FIELD_FLAGS = 12,   /* 1 int32: bit0 fixed, bit1 blanked, bit2 tool particle */
+  MFC_FIELD_ID = 13,      /* 1 int32: particle index at creation */
+  MFC_FIELD_HASH = 14,    /* 1 int32: cell id the current order was sorted by */
+  MFC_FIELD_POS_T = 15,   /* 2 */
+  MFC_FIELD_VEL_T = 16,   /* 2 */
+  MFC_FIELD_RHO_T = 17,   /* 1 */
+  MFC_FIELD_S_T = 18,     /* 4 */
+  MFC_FIELD_T_T = 19,     /* 1 */
+  MFC_FIELD_R = 20,       /* 3 artificial stress Rxx, Rxy, Ryy */
+  MFC_FIELD_COUNT = 21
+} mfc_field;
+
+#define MFC_FLAG_FIXED   1
+#define MFC_FLAG_BLANKED 2
+#define MFC_FLAG_TOOL    4
+
+/* tool state: up to 5 segments + 1 fillet (tool_gpu.cuh:32-52) */
+typedef struct mfc_tool_state {
+  double left[5][2], right[5][2], n[5][2];
+  double line_a[5], line_b[5];
+  int32_t vertical[5];
+  int32_t num_seg, has_fillet;
+  double fillet_p[2], fillet_r, fillet_t1, fillet_t2;
+  double vel[2];
+} mfc_tool_state;
+
+typedef struct mfc_sim mfc_sim;
+
+/* number of components of a field (1..4), or 0 for an unknown field */
+int mfc_field_components(int field);
+
+/* the defaults of setup_ref_cut(ny, grid, rake, clear, chamfer, speed, feed)
+ * (benchmarks.cpp:255-537, benchmarks.h:48): Ti6Al4V after Johnson 1985, thermal
+ * constants, tool sampling, Brookshaw conduction, FP64, CFL 0.5. speed == 0 selects
+ * the reference's 500 m/min (benchmarks.cpp:384). */
+int mfc_config_ref_cut(mfc_config *cfg, int ny, double rake, double clear, double chamfer,
+                       double speed, double feed);
+
+/* setup_ref_cut + particle_gpu + grid_gpu_green + leap_frog construction */
+int mfc_create(const mfc_config *cfg, mfc_sim **out);
+void mfc_destroy(mfc_sim *sim);
+
+/* leap_frog::step, nsteps times (hipGraph replay when nsteps > 1) */
+int mfc_step(mfc_sim *sim, int nsteps);
+/* wait for queued steps to finish */
+int mfc_sync(mfc_sim *sim);
+
+/* run nsteps steps without graphs, timing each phase with HIP events on the handle's
+ * stream; ms[phase] receives the mean milliseconds per step of: 0 binning (hash + stable
+ * counting sort + cell offsets), 1 gather/predict/EOS/artificial stress, 2 neighbour
+ * interactions + per-particle update + contact, 3 tool-force reduction */
+#define MFC_PROFILE_PHASES 4
+int mfc_profile(mfc_sim *sim, int nsteps, double *ms);
+
+/* report_force: F = sum_i fc_i + sum_i ft_i after the last step, summed in a fixed
+ * order (tiles of 256 particles in storage order reduced by halving strides, tiles
+ * summed in ascending order; sum of fc and sum of ft added last) */
+int mfc_tool_force(mfc_sim *sim, double F[2]);
+/* forces of every step since creation or the last reset, as F[step][2];
+ * *n receives the number of steps copied (at most cap); with F == NULL, *n receives the
+ * number of recorded steps (at most 2^20 are kept) */
+int mfc_force_history(mfc_sim *sim, double *F, int64_t cap, int64_t *n);
+int mfc_force_history_reset(mfc_sim *sim);
+
+int mfc_get_info(const mfc_sim *sim, mfc_info *info);
+/* copy a field to host memory: bytes must be N * components * sizeof(double)
+ * (sizeof(int32_t) for FLAGS, ID, HASH) */
+int mfc_get_field(mfc_sim *sim, int field, void *host, size_t bytes);
+/* overwrite a field from host memory (same layout); used to inject states in tests */
+int mfc_set_field(mfc_sim *sim, int field, const void *host, size_t bytes);
+/* cell ranges of the last binning: offsets[c]..offsets[c+1] is cell c in storage order,
+ * an array of num_cell + 1 ints */
+int mfc_get_cells(mfc_sim *sim, int32_t *offsets, size_t count);
+int mfc_get_tool(mfc_sim *sim, mfc_tool_state *tool);
+
+/* host-side setup only (no device work): the initial particles and tool of a
+ * configuration, in creation order. Arrays may be NULL; non-NULL arrays hold
+ * info->n_particles entries (pos: 2 per particle). */
+int mfc_scenario(const mfc_config *cfg, mfc_info *info, double *pos, double *rho, double *h, double *T,
+                 int32_t *flags, mfc_tool_state *tool);
+
+/* run the shared transcendental library on the device (diagnostics):
+ * fn 0 exp, 1 log, 2 pow(x,y), 3 atan2(x,y), 4 sin, 5 cos, 6 atan2f, 7 cosf, 8 sinf,
+ * 9 sqrt, 10 sqrtf, 11 x/y, 12 tanh */
+int mfc_device_math(int fn, const double *x, const double *y, double *out, int64_t n);
+
+const char *mfc_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MFC_H_ */

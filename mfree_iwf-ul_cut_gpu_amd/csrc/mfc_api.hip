@@ -1,0 +1,482 @@
+// mfc_api.hip — C ABI of the solver (declared in include/mfc.h).
+//
+// A handle owns every device buffer of one simulation, its own HIP stream (default priority)
+// and the hipGraph of two consecutive steps. Steps are queued asynchronously; calls that
+// return data synchronise the handle's stream.
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/mfc.h"
+#include "mfc_kernels.h"
+#include "mfc_setup.h"
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string &msg) {
+  g_err = msg;
+  return code;
+}
+
+#define HIP_TRY(expr)                                                                         \
+  do {                                                                                        \
+    hipError_t e_ = (expr);                                                                   \
+    if (e_ != hipSuccess) return fail(MFC_E_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+int field_components(int f) {
+  switch (f) {
+    case MFC_FIELD_POS: case MFC_FIELD_VEL: case MFC_FIELD_FC: case MFC_FIELD_FT: case MFC_FIELD_N:
+    case MFC_FIELD_POS_T: case MFC_FIELD_VEL_T: return 2;
+    case MFC_FIELD_S: case MFC_FIELD_S_T: return 4;
+    case MFC_FIELD_R: return 3;
+    case MFC_FIELD_RHO: case MFC_FIELD_P: case MFC_FIELD_T: case MFC_FIELD_EPS_PL: case MFC_FIELD_EPS_PL_DOT:
+    case MFC_FIELD_H: case MFC_FIELD_FLAGS: case MFC_FIELD_ID: case MFC_FIELD_HASH: case MFC_FIELD_RHO_T:
+    case MFC_FIELD_T_T: return 1;
+    default: return 0;
+  }
+}
+bool field_is_int(int f) { return f == MFC_FIELD_FLAGS || f == MFC_FIELD_ID || f == MFC_FIELD_HASH; }
+
+constexpr int64_t HIST_CAP = 1 << 20;
+
+}  // namespace
+
+struct mfc_sim {
+  mfc::scenario sc;
+  mfc::dev_consts k;
+  int ntiles = 0;
+  hipStream_t stream = nullptr;
+  std::vector<void *> allocs;
+  mfc::dstate set[2];
+  mfc::dbins bins;
+  mfc::dpositional pos;
+  double *rec = nullptr;
+  mfc_tool_state *tool = nullptr;  // two states, alternating each step
+  double *partials = nullptr, *force_last = nullptr, *hist = nullptr;
+  int64_t *hist_ctr = nullptr;
+  int64_t hist_base = 0;           // steps recorded before the last history reset
+  int cur = 0;                     // set holding the current state, and index of the current tool state
+  int64_t step = 0;
+  hipGraphExec_t graph[2] = {nullptr, nullptr};  // two steps starting from parity 0 / 1
+  bool use_graphs = true;
+
+  template <class T>
+  int alloc(T **p, size_t n) {
+    void *q = nullptr;
+    if (n == 0) n = 1;
+    hipError_t e = hipMalloc(&q, n * sizeof(T));
+    if (e != hipSuccess) return fail(MFC_E_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+    allocs.push_back(q);
+    *p = (T *)q;
+    return MFC_OK;
+  }
+};
+
+namespace {
+
+int alloc_state(mfc_sim *s, mfc::dstate &d, int N) {
+  double **dp[mfc::DSTATE_DOUBLES] = {&d.px, &d.py, &d.vx, &d.vy, &d.sxx, &d.sxy, &d.syy, &d.szz, &d.rho,
+                                      &d.T, &d.eps, &d.epsd, &d.h, &d.fcx, &d.fcy, &d.ftx, &d.fty, &d.ptx,
+                                      &d.pty, &d.vtx, &d.vty, &d.rhot, &d.stxx, &d.stxy, &d.styy, &d.stzz, &d.Tt};
+  for (int q = 0; q < mfc::DSTATE_DOUBLES; q++) {
+    int rc = s->alloc(dp[q], N);
+    if (rc) return rc;
+  }
+  int rc = s->alloc(&d.flags, N);
+  if (rc) return rc;
+  return s->alloc(&d.id, N);
+}
+
+void enqueue_step(mfc_sim *s, int c) {
+  const int n = 1 - c;
+  mfc::launch_binning(s->k, s->set[c], s->bins, s->stream);
+  mfc::launch_predict(s->k, s->set[c], s->set[n], s->bins, s->pos, s->rec, s->tool + c, s->tool + n, s->stream);
+  mfc::launch_interact(s->k, s->set[n], s->bins, s->pos, s->rec, s->tool + n, s->partials, s->stream);
+  mfc::launch_force_final(s->ntiles, s->partials, s->force_last, s->hist, HIST_CAP, s->hist_ctr, s->stream);
+}
+
+int build_graph(mfc_sim *s, int parity) {
+  hipGraph_t g = nullptr;
+  HIP_TRY(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
+  enqueue_step(s, parity);
+  enqueue_step(s, 1 - parity);
+  HIP_TRY(hipStreamEndCapture(s->stream, &g));
+  hipError_t e = hipGraphInstantiate(&s->graph[parity], g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  if (e != hipSuccess) return fail(MFC_E_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+  return MFC_OK;
+}
+
+int copy_out(mfc_sim *s, void *dst, const void *src, size_t bytes) {
+  HIP_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, s->stream));
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  return MFC_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *mfc_last_error(void) { return g_err.c_str(); }
+
+int mfc_field_components(int field) { return field_components(field); }
+
+int mfc_config_ref_cut(mfc_config *cfg, int ny, double rake, double clear, double chamfer, double speed, double feed) {
+  if (!cfg) return fail(MFC_E_INVALID, "null config");
+  mfc::config_ref_cut(*cfg, ny, rake, clear, chamfer, speed, feed);
+  return MFC_OK;
+}
+
+void mfc_destroy(mfc_sim *s) {
+  if (!s) return;
+  if (s->stream) (void)hipStreamSynchronize(s->stream);
+  for (int p = 0; p < 2; p++)
+    if (s->graph[p]) (void)hipGraphExecDestroy(s->graph[p]);
+  for (void *q : s->allocs) (void)hipFree(q);
+  if (s->stream) (void)hipStreamDestroy(s->stream);
+  delete s;
+}
+
+int mfc_create(const mfc_config *cfg, mfc_sim **out) {
+  if (!cfg || !out) return fail(MFC_E_INVALID, "null argument");
+  *out = nullptr;
+  if (cfg->precision != MFC_PREC_FP64) return fail(MFC_E_INVALID, "only MFC_PREC_FP64 is available in this build");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(MFC_E_HIP, "no HIP device");
+  mfc_sim *s = new mfc_sim();
+  std::string err;
+  int rc = mfc::build_scenario(*cfg, s->sc, err);
+  if (rc) { delete s; return fail(rc, err); }
+  const mfc::sim_constants &c = s->sc.k;
+  mfc::dev_consts &k = s->k;
+  std::memset(&k, 0, sizeof(k));
+  k.N = c.N; k.gnx = c.gnx; k.gny = c.gny; k.num_cell = c.num_cell;
+  k.bbmin_x = c.bbmin_x; k.bbmin_y = c.bbmin_y; k.cell = c.cell;
+  k.dt = c.dt; k.mass = c.mass; k.K = c.K; k.G = c.G; k.rho0 = c.rho0; k.c0 = c.c0;
+  k.wdeltap = c.wdeltap; k.av_alpha = c.av_alpha; k.av_beta = c.av_beta; k.av_eta = c.av_eta;
+  k.xsph_eps = c.xsph_eps; k.stress_eps = c.stress_eps;
+  k.alpha_wp = c.alpha_wp; k.alpha_tool = c.alpha_tool; k.cp = c.cp; k.tq = c.tq; k.eta_heat = c.eta_heat;
+  k.T_init = c.T_init;
+  k.jc_A = c.jc_A; k.jc_B = c.jc_B; k.jc_C = c.jc_C; k.jc_n = c.jc_n; k.jc_m = c.jc_m;
+  k.jc_Tmelt = c.jc_Tmelt; k.jc_Tref = c.jc_Tref; k.jc_eps_dot_ref = c.jc_eps_dot_ref;
+  k.contact_alpha = c.contact_alpha; k.mu = c.mu; k.tool_vx = c.tool_vx; k.tool_vy = c.tool_vy;
+  k.thermal_bc = c.thermal_bc; k.fric_heat = c.fric_heat; k.thermal_wp = c.thermal_wp; k.pse = c.pse;
+  k.pse_quirk = c.pse_quirk;
+  const int N = c.N;
+  s->ntiles = (N + mfc::TILE - 1) / mfc::TILE;
+
+  auto bail = [&](int code) { mfc_destroy(s); return code; };
+  if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess) return bail(fail(MFC_E_HIP, "stream"));
+  for (int q = 0; q < 2; q++)
+    if ((rc = alloc_state(s, s->set[q], N))) return bail(rc);
+  mfc::dbins &b = s->bins;
+  if ((rc = s->alloc(&b.key, N)) || (rc = s->alloc(&b.key_sorted, N)) || (rc = s->alloc(&b.slot, N)) ||
+      (rc = s->alloc(&b.count, c.num_cell)) || (rc = s->alloc(&b.offsets, (size_t)c.num_cell + 1)) ||
+      (rc = s->alloc(&b.bucket, N)) || (rc = s->alloc(&b.perm, N)) ||
+      (rc = s->alloc(&b.bsum, mfc::scan_blocks(c.num_cell))) || (rc = s->alloc(&b.clamped, 1)))
+    return bail(rc);
+  mfc::dpositional &p = s->pos;
+  if ((rc = s->alloc(&p.nx, N)) || (rc = s->alloc(&p.ny, N)) || (rc = s->alloc(&p.p, N)) ||
+      (rc = s->alloc(&p.rxx, N)) || (rc = s->alloc(&p.rxy, N)) || (rc = s->alloc(&p.ryy, N)))
+    return bail(rc);
+  if ((rc = s->alloc(&s->rec, (size_t)mfc::REC * N)) || (rc = s->alloc(&s->tool, 2)) ||
+      (rc = s->alloc(&s->partials, (size_t)4 * s->ntiles)) || (rc = s->alloc(&s->force_last, 2)) ||
+      (rc = s->alloc(&s->hist, (size_t)2 * HIST_CAP)) || (rc = s->alloc(&s->hist_ctr, 1)))
+    return bail(rc);
+  // zero everything, then upload the initial state (particle_gpu.cu:20-125)
+  {
+    std::vector<double> zero(N, 0.);
+    std::vector<int> ids(N);
+    for (int i = 0; i < N; i++) ids[i] = i;
+    auto up = [&](void *dst, const void *src, size_t bytes) {
+      return hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s->stream);
+    };
+    for (int q = 0; q < 2; q++) {
+      mfc::dstate &d = s->set[q];
+      double *all[mfc::DSTATE_DOUBLES] = {d.px, d.py, d.vx, d.vy, d.sxx, d.sxy, d.syy, d.szz, d.rho,
+                                          d.T, d.eps, d.epsd, d.h, d.fcx, d.fcy, d.ftx, d.fty, d.ptx,
+                                          d.pty, d.vtx, d.vty, d.rhot, d.stxx, d.stxy, d.styy, d.stzz, d.Tt};
+      for (double *a : all)
+        if (up(a, zero.data(), sizeof(double) * N) != hipSuccess) return bail(fail(MFC_E_HIP, "upload"));
+      if (up(d.flags, zero.data(), sizeof(int) * N) != hipSuccess) return bail(fail(MFC_E_HIP, "upload"));
+      if (up(d.id, ids.data(), sizeof(int) * N) != hipSuccess) return bail(fail(MFC_E_HIP, "upload"));
+    }
+    mfc::dstate &d = s->set[0];
+    const mfc::scenario &S = s->sc;
+    if (up(d.px, S.px.data(), sizeof(double) * N) || up(d.py, S.py.data(), sizeof(double) * N) ||
+        up(d.rho, S.rho.data(), sizeof(double) * N) || up(d.h, S.h.data(), sizeof(double) * N) ||
+        up(d.T, S.T.data(), sizeof(double) * N) || up(d.flags, S.flags.data(), sizeof(int) * N))
+      return bail(fail(MFC_E_HIP, "upload"));
+    double *pz[6] = {p.nx, p.ny, p.p, p.rxx, p.rxy, p.ryy};
+    for (double *a : pz)
+      if (up(a, zero.data(), sizeof(double) * N) != hipSuccess) return bail(fail(MFC_E_HIP, "upload"));
+    if (hipMemsetAsync(b.key_sorted, 0, sizeof(int) * N, s->stream) != hipSuccess ||
+        hipMemsetAsync(b.count, 0, sizeof(int) * c.num_cell, s->stream) != hipSuccess ||
+        hipMemsetAsync(b.offsets, 0, sizeof(int) * ((size_t)c.num_cell + 1), s->stream) != hipSuccess ||
+        hipMemsetAsync(b.clamped, 0, sizeof(unsigned long long), s->stream) != hipSuccess ||
+        hipMemsetAsync(s->hist_ctr, 0, sizeof(int64_t), s->stream) != hipSuccess ||
+        hipMemsetAsync(s->force_last, 0, 2 * sizeof(double), s->stream) != hipSuccess ||
+        hipMemsetAsync(s->rec, 0, sizeof(double) * mfc::REC * (size_t)N, s->stream) != hipSuccess)
+      return bail(fail(MFC_E_HIP, "memset"));
+    mfc_tool_state t2[2] = {S.tool, S.tool};
+    if (up(s->tool, t2, sizeof(t2)) != hipSuccess) return bail(fail(MFC_E_HIP, "upload tool"));
+    if (hipStreamSynchronize(s->stream) != hipSuccess) return bail(fail(MFC_E_HIP, "sync"));
+  }
+  *out = s;
+  return MFC_OK;
+}
+
+int mfc_step(mfc_sim *s, int nsteps) {
+  if (!s || nsteps < 0) return fail(MFC_E_INVALID, "bad argument");
+  int done = 0;
+  if (s->use_graphs && nsteps >= 4) {
+    while (nsteps - done >= 2) {
+      int par = s->cur;
+      if (!s->graph[par]) {
+        int rc = build_graph(s, par);
+        if (rc) return rc;
+      }
+      HIP_TRY(hipGraphLaunch(s->graph[par], s->stream));
+      done += 2;
+      s->step += 2;
+    }
+  }
+  for (; done < nsteps; done++) {
+    enqueue_step(s, s->cur);
+    s->cur = 1 - s->cur;
+    s->step++;
+  }
+  HIP_TRY(hipGetLastError());
+  return MFC_OK;
+}
+
+int mfc_profile(mfc_sim *s, int nsteps, double *ms) {
+  if (!s || !ms || nsteps < 1) return fail(MFC_E_INVALID, "bad argument");
+  hipEvent_t ev[5];
+  for (int q = 0; q < 5; q++) HIP_TRY(hipEventCreate(&ev[q]));
+  double acc[MFC_PROFILE_PHASES] = {0., 0., 0., 0.};
+  int rc = MFC_OK;
+  for (int it = 0; it < nsteps && rc == MFC_OK; it++) {
+    const int c = s->cur, n = 1 - c;
+    (void)hipEventRecord(ev[0], s->stream);
+    mfc::launch_binning(s->k, s->set[c], s->bins, s->stream);
+    (void)hipEventRecord(ev[1], s->stream);
+    mfc::launch_predict(s->k, s->set[c], s->set[n], s->bins, s->pos, s->rec, s->tool + c, s->tool + n, s->stream);
+    (void)hipEventRecord(ev[2], s->stream);
+    mfc::launch_interact(s->k, s->set[n], s->bins, s->pos, s->rec, s->tool + n, s->partials, s->stream);
+    (void)hipEventRecord(ev[3], s->stream);
+    mfc::launch_force_final(s->ntiles, s->partials, s->force_last, s->hist, HIST_CAP, s->hist_ctr, s->stream);
+    (void)hipEventRecord(ev[4], s->stream);
+    s->cur = n;
+    s->step++;
+    if (hipEventSynchronize(ev[4]) != hipSuccess) { rc = fail(MFC_E_HIP, "event sync"); break; }
+    for (int q = 0; q < MFC_PROFILE_PHASES; q++) {
+      float t = 0.f;
+      if (hipEventElapsedTime(&t, ev[q], ev[q + 1]) != hipSuccess) { rc = fail(MFC_E_HIP, "event time"); break; }
+      acc[q] += t;
+    }
+  }
+  for (int q = 0; q < 5; q++) (void)hipEventDestroy(ev[q]);
+  if (rc) return rc;
+  for (int q = 0; q < MFC_PROFILE_PHASES; q++) ms[q] = acc[q] / nsteps;
+  return MFC_OK;
+}
+
+int mfc_sync(mfc_sim *s) {
+  if (!s) return fail(MFC_E_INVALID, "null handle");
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  return MFC_OK;
+}
+
+int mfc_tool_force(mfc_sim *s, double F[2]) {
+  if (!s || !F) return fail(MFC_E_INVALID, "null argument");
+  if (s->step == 0) { F[0] = 0.; F[1] = 0.; return MFC_OK; }
+  return copy_out(s, F, s->force_last, 2 * sizeof(double));
+}
+
+int mfc_force_history(mfc_sim *s, double *F, int64_t cap, int64_t *n) {
+  if (!s || !n || cap < 0) return fail(MFC_E_INVALID, "bad argument");
+  int64_t ctr = 0;
+  int rc = copy_out(s, &ctr, s->hist_ctr, sizeof(int64_t));
+  if (rc) return rc;
+  int64_t m = ctr < HIST_CAP ? ctr : HIST_CAP;
+  if (!F) { *n = m; return MFC_OK; }
+  if (m > cap) m = cap;
+  if (m > 0) {
+    rc = copy_out(s, F, s->hist, sizeof(double) * 2 * (size_t)m);
+    if (rc) return rc;
+  }
+  *n = m;
+  return MFC_OK;
+}
+
+int mfc_force_history_reset(mfc_sim *s) {
+  if (!s) return fail(MFC_E_INVALID, "null handle");
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  HIP_TRY(hipMemsetAsync(s->hist_ctr, 0, sizeof(int64_t), s->stream));
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  return MFC_OK;
+}
+
+int mfc_get_info(const mfc_sim *s, mfc_info *info) {
+  if (!s || !info) return fail(MFC_E_INVALID, "null argument");
+  const mfc::sim_constants &c = s->sc.k;
+  std::memset(info, 0, sizeof(*info));
+  info->n_particles = c.N; info->n_tool = c.n_tool;
+  info->grid_nx = c.gnx; info->grid_ny = c.gny; info->num_cell = c.num_cell;
+  info->precision = MFC_PREC_FP64;
+  info->dt = c.dt; info->t_final = c.t_final; info->mass = c.mass;
+  info->bbmin_x = c.bbmin_x; info->bbmin_y = c.bbmin_y; info->cell_size = c.cell;
+  info->step = s->step;
+  return MFC_OK;
+}
+
+static int field_ptrs(mfc_sim *s, int field, void **ptr, int &nc) {
+  const mfc::dstate &d = s->set[s->cur];
+  nc = field_components(field);
+  for (int q = 0; q < 4; q++) ptr[q] = nullptr;
+  switch (field) {
+    case MFC_FIELD_POS: ptr[0] = d.px; ptr[1] = d.py; break;
+    case MFC_FIELD_VEL: ptr[0] = d.vx; ptr[1] = d.vy; break;
+    case MFC_FIELD_S: ptr[0] = d.sxx; ptr[1] = d.sxy; ptr[2] = d.syy; ptr[3] = d.szz; break;
+    case MFC_FIELD_RHO: ptr[0] = d.rho; break;
+    case MFC_FIELD_P: ptr[0] = s->pos.p; break;
+    case MFC_FIELD_T: ptr[0] = d.T; break;
+    case MFC_FIELD_EPS_PL: ptr[0] = d.eps; break;
+    case MFC_FIELD_EPS_PL_DOT: ptr[0] = d.epsd; break;
+    case MFC_FIELD_FC: ptr[0] = d.fcx; ptr[1] = d.fcy; break;
+    case MFC_FIELD_FT: ptr[0] = d.ftx; ptr[1] = d.fty; break;
+    case MFC_FIELD_N: ptr[0] = s->pos.nx; ptr[1] = s->pos.ny; break;
+    case MFC_FIELD_H: ptr[0] = d.h; break;
+    case MFC_FIELD_FLAGS: ptr[0] = d.flags; break;
+    case MFC_FIELD_ID: ptr[0] = d.id; break;
+    case MFC_FIELD_HASH: ptr[0] = s->bins.key_sorted; break;
+    case MFC_FIELD_POS_T: ptr[0] = d.ptx; ptr[1] = d.pty; break;
+    case MFC_FIELD_VEL_T: ptr[0] = d.vtx; ptr[1] = d.vty; break;
+    case MFC_FIELD_RHO_T: ptr[0] = d.rhot; break;
+    case MFC_FIELD_S_T: ptr[0] = d.stxx; ptr[1] = d.stxy; ptr[2] = d.styy; ptr[3] = d.stzz; break;
+    case MFC_FIELD_T_T: ptr[0] = d.Tt; break;
+    case MFC_FIELD_R: ptr[0] = s->pos.rxx; ptr[1] = s->pos.rxy; ptr[2] = s->pos.ryy; break;
+    default: return fail(MFC_E_INVALID, "unknown field");
+  }
+  return MFC_OK;
+}
+
+int mfc_get_field(mfc_sim *s, int field, void *host, size_t bytes) {
+  if (!s || !host) return fail(MFC_E_INVALID, "null argument");
+  void *ptr[4];
+  int nc;
+  int rc = field_ptrs(s, field, ptr, nc);
+  if (rc) return rc;
+  const int N = s->k.N;
+  const size_t es = field_is_int(field) ? sizeof(int) : sizeof(double);
+  if (bytes != (size_t)N * nc * es) return fail(MFC_E_RANGE, "buffer size does not match the field");
+  if (nc == 1) {
+    rc = copy_out(s, host, ptr[0], es * N);
+    if (rc) return rc;
+    if (field == MFC_FIELD_FLAGS) {
+      int *f = (int *)host;
+      for (int i = 0; i < N; i++) f[i] &= (MFC_FLAG_FIXED | MFC_FLAG_BLANKED | MFC_FLAG_TOOL);
+    }
+    return MFC_OK;
+  }
+  std::vector<double> tmp((size_t)N);
+  double *out = (double *)host;
+  for (int c = 0; c < nc; c++) {
+    rc = copy_out(s, tmp.data(), ptr[c], sizeof(double) * N);
+    if (rc) return rc;
+    for (int i = 0; i < N; i++) out[(size_t)nc * i + c] = tmp[i];
+  }
+  return MFC_OK;
+}
+
+int mfc_set_field(mfc_sim *s, int field, const void *host, size_t bytes) {
+  if (!s || !host) return fail(MFC_E_INVALID, "null argument");
+  if (field == MFC_FIELD_ID || field == MFC_FIELD_HASH) return fail(MFC_E_INVALID, "field is read-only");
+  void *ptr[4];
+  int nc;
+  int rc = field_ptrs(s, field, ptr, nc);
+  if (rc) return rc;
+  const int N = s->k.N;
+  const size_t es = field_is_int(field) ? sizeof(int) : sizeof(double);
+  if (bytes != (size_t)N * nc * es) return fail(MFC_E_RANGE, "buffer size does not match the field");
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  if (nc == 1) {
+    HIP_TRY(hipMemcpy(ptr[0], host, es * N, hipMemcpyHostToDevice));
+    return MFC_OK;
+  }
+  std::vector<double> tmp((size_t)N);
+  const double *in = (const double *)host;
+  for (int c = 0; c < nc; c++) {
+    for (int i = 0; i < N; i++) tmp[i] = in[(size_t)nc * i + c];
+    HIP_TRY(hipMemcpy(ptr[c], tmp.data(), sizeof(double) * N, hipMemcpyHostToDevice));
+  }
+  return MFC_OK;
+}
+
+int mfc_get_cells(mfc_sim *s, int32_t *offsets, size_t count) {
+  if (!s || !offsets) return fail(MFC_E_INVALID, "null argument");
+  if (count != (size_t)s->k.num_cell + 1) return fail(MFC_E_RANGE, "count must be num_cell + 1");
+  return copy_out(s, offsets, s->bins.offsets, sizeof(int) * count);
+}
+
+int mfc_get_tool(mfc_sim *s, mfc_tool_state *tool) {
+  if (!s || !tool) return fail(MFC_E_INVALID, "null argument");
+  return copy_out(s, tool, s->tool + s->cur, sizeof(mfc_tool_state));
+}
+
+int mfc_scenario(const mfc_config *cfg, mfc_info *info, double *pos, double *rho, double *h, double *T,
+                 int32_t *flags, mfc_tool_state *tool) {
+  if (!cfg || !info) return fail(MFC_E_INVALID, "null argument");
+  mfc::scenario S;
+  std::string err;
+  int rc = mfc::build_scenario(*cfg, S, err);
+  if (rc) return fail(rc, err);
+  const mfc::sim_constants &c = S.k;
+  std::memset(info, 0, sizeof(*info));
+  info->n_particles = c.N; info->n_tool = c.n_tool;
+  info->grid_nx = c.gnx; info->grid_ny = c.gny; info->num_cell = c.num_cell;
+  info->precision = cfg->precision;
+  info->dt = c.dt; info->t_final = c.t_final; info->mass = c.mass;
+  info->bbmin_x = c.bbmin_x; info->bbmin_y = c.bbmin_y; info->cell_size = c.cell;
+  for (int i = 0; i < c.N; i++) {
+    if (pos) { pos[2 * i] = S.px[i]; pos[2 * i + 1] = S.py[i]; }
+    if (rho) rho[i] = S.rho[i];
+    if (h) h[i] = S.h[i];
+    if (T) T[i] = S.T[i];
+    if (flags) flags[i] = S.flags[i];
+  }
+  if (tool) *tool = S.tool;
+  return MFC_OK;
+}
+
+int mfc_device_math(int fn, const double *x, const double *y, double *out, int64_t n) {
+  if (!x || !out || n < 0 || fn < 0 || fn > 12) return fail(MFC_E_INVALID, "bad argument");
+  if (n == 0) return MFC_OK;
+  double *dx = nullptr, *dy = nullptr, *dout = nullptr;
+  size_t bytes = sizeof(double) * (size_t)n;
+  int rc = MFC_OK;
+  if (hipMalloc(&dx, bytes) != hipSuccess || hipMalloc(&dout, bytes) != hipSuccess ||
+      (y && hipMalloc(&dy, bytes) != hipSuccess)) {
+    rc = fail(MFC_E_NOMEM, "hipMalloc");
+  } else if (hipMemcpy(dx, x, bytes, hipMemcpyHostToDevice) != hipSuccess ||
+             (y && hipMemcpy(dy, y, bytes, hipMemcpyHostToDevice) != hipSuccess)) {
+    rc = fail(MFC_E_HIP, "upload");
+  } else {
+    mfc::launch_device_math(fn, dx, dy, dout, n, nullptr);
+    if (hipDeviceSynchronize() != hipSuccess || hipMemcpy(out, dout, bytes, hipMemcpyDeviceToHost) != hipSuccess)
+      rc = fail(MFC_E_HIP, "device math");
+  }
+  if (dx) (void)hipFree(dx);
+  if (dy) (void)hipFree(dy);
+  if (dout) (void)hipFree(dout);
+  return rc;
+}
+
+}  // extern "C"

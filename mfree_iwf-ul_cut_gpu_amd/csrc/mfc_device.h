@@ -1,0 +1,293 @@
+// mfc_device.h — per-particle physics of the SPH cutting step as gfx950 device functions.
+//
+// Every function keeps the operand order of the reference expression it implements (cited
+// per function) so that, compiled with -ffp-contract=off, the FP64 kernels reproduce the
+// host oracle bit for bit. Single-precision truncations of the reference's FP64 build
+// (sqrtf, fmaxf, atan2f/cosf/sinf with double arguments) are kept as such.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/mfc.h"
+#include "mfc_math.h"
+
+namespace mfc {
+
+constexpr int TILE = 256;           // particles per work-group of the particle kernels
+constexpr double PI_FAC = 0.454728408833987;   // 10/(7 pi), kernels.cuh:28
+constexpr double FLT_MAX_D = 3.4028234663852886e38;
+
+// constants of one simulation, passed by value to every kernel
+struct dev_consts {
+  int N, gnx, gny, num_cell;
+  double bbmin_x, bbmin_y, cell;
+  double dt, mass, K, G, rho0, c0;
+  double wdeltap, av_alpha, av_beta, av_eta, xsph_eps, stress_eps;
+  double alpha_wp, alpha_tool, cp, tq, eta_heat, T_init;
+  double jc_A, jc_B, jc_C, jc_n, jc_m, jc_Tmelt, jc_Tref, jc_eps_dot_ref;
+  double contact_alpha, mu, tool_vx, tool_vy;
+  int thermal_bc, fric_heat, thermal_wp, pse, pse_quirk;
+};
+
+__device__ inline double f32sqrt(double x) { return (double)sqrtf((float)x); }
+
+// Work-groups are dispatched round-robin over the 8 XCDs of MI355X; map block b so that
+// each XCD owns one contiguous range of tiles (its neighbour data then stays in that XCD's
+// L2). Returns -1 for padding blocks.
+__device__ inline int xcd_tile(int b, int ntiles) {
+  int per = (ntiles + 7) >> 3;
+  int t = (b & 7) * per + (b >> 3);
+  return t < ntiles ? t : -1;
+}
+
+// cell coordinate: C++ truncation of a double, saturated outside the int range
+__device__ inline int64_t cell_coord(double f) {
+  if (!(f == f)) return -1;
+  if (f >= 1073741824.0) return 1073741824;
+  if (f <= -1073741824.0) return -1073741824;
+  return (int64_t)(int)f;
+}
+
+// cell id of a position (grid.cu:34-51); out-of-grid ids are clamped and reported
+__device__ inline int cell_of(const dev_consts &k, double x, double y, bool &clamped) {
+  int64_t ix = cell_coord((x - k.bbmin_x) / k.cell);
+  int64_t iy = cell_coord((y - k.bbmin_y) / k.cell);
+  int64_t h = ix * k.gny + iy;
+  clamped = false;
+  if (h < 0) { h = 0; clamped = true; }
+  if (h >= k.num_cell) { h = k.num_cell - 1; clamped = true; }
+  return (int)h;
+}
+
+// artificial stress R of Gray & Monaghan in the principal frame (actions_gpu.cu:35-97)
+__device__ inline void artificial_stress(double rhoi, double pi, double Sxx, double Sxy, double Syy, double eps,
+                                         double &Rxx, double &Rxy, double &Ryy) {
+  double sxx = Sxx - pi, sxy = Sxy, syy = Syy - pi;
+  double rhoi21 = 1. / (rhoi * rhoi);
+  double theta = 0.5 * (double)mfcm::atan2f(2. * sxy, sxx - syy + 0.);
+  double ct = (double)mfcm::cosf(theta), st = (double)mfcm::sinf(theta);
+  double ct2 = ct * ct, st2 = st * st;
+  double rsxx = ct2 * sxx + 2.0 * ct * st * sxy + st2 * syy;
+  double rsyy = st2 * sxx - 2.0 * ct * st * sxy + ct2 * syy;
+  double rrxx = 0., rryy = 0.;
+  if (rsxx > 0) rrxx = -eps * rsxx * rhoi21;
+  if (rsyy > 0) rryy = -eps * rsyy * rhoi21;
+  Rxx = ct2 * rrxx + st2 * rryy;
+  Rxy = ct * st * (rrxx - rryy);
+  Ryy = st2 * rrxx + ct2 * rryy;
+}
+
+// Jaumann stress rate (actions_gpu.cu:99-133). The glm::dmat3 expression
+// 2G(epsdot - tr/3 I) + Omega S + S Omega^T, expanded: the products with zero entries of
+// Omega, S and I only add or subtract exact zeros, which leaves every non-zero sum unchanged.
+__device__ inline void jaumann(double G, double vx_x, double vx_y, double vy_x, double vy_y, double Sxx, double Sxy,
+                               double Syy, double Szz, double &txx, double &txy, double &tyy, double &tzz) {
+  double exy = 0.5 * (vx_y + vy_x);
+  double w = 0.5 * (vy_x - vx_y);   // Omega[0][1]
+  double wt = 0.5 * (vx_y - vy_x);  // Omega[1][0]
+  double tr = vx_x + vy_y + 0.;
+  double s13 = double(1. / 3.) * tr;
+  double twoG = double(2.) * G;
+  // (Omega S)[c][r] = sum_k Omega[k][r] S[c][k];  (S Omega^T)[c][r] = sum_k S[k][r] Omega[r... ]
+  double a00 = 0. * Sxx + wt * Sxy + 0. * 0.;
+  double b00 = Sxx * 0. + Sxy * wt + 0. * 0.;
+  double a01 = w * Sxx + 0. * Sxy + 0. * 0.;
+  double b01 = Sxy * 0. + Syy * wt + 0. * 0.;
+  double a11 = w * Sxy + 0. * Syy + 0. * 0.;
+  double b11 = Sxy * w + Syy * 0. + 0. * 0.;
+  double a22 = 0. * 0. + 0. * 0. + 0. * Szz;
+  double b22 = 0. * 0. + 0. * 0. + Szz * 0.;
+  txx = twoG * (vx_x - s13 * 1.) + a00 + b00;
+  txy = twoG * (exy - s13 * 0.) + a01 + b01;
+  tyy = twoG * (vy_y - s13 * 1.) + a11 + b11;
+  tzz = twoG * (0. - s13 * 1.) + a22 + b22;
+}
+
+// Johnson-Cook flow stress (plasticity.cuh:30-46)
+__device__ inline double sigma_yield(const dev_consts &k, double eps_pl, double eps_pl_dot, double t) {
+  double theta = (t - k.jc_Tref) / (k.jc_Tmelt - k.jc_Tref);
+  double TA = k.jc_A + k.jc_B * mfcm::pow(eps_pl, k.jc_n);
+  double TB = 1.0;
+  double ed = eps_pl_dot / k.jc_eps_dot_ref;
+  if (ed > 1.0) TB = 1.0 + k.jc_C * mfcm::log(ed);
+  else TB = mfcm::pow((1.0 + ed), k.jc_C);
+  double TC = 1.0 - mfcm::pow(theta, k.jc_m);
+  return TA * TB * TC;
+}
+
+// residual of the radial return (plasticity.cuh:74-83)
+__device__ inline double rr_residual(const dev_consts &k, double dl, double norm, double eps0, double t) {
+  const double s23 = 0.816496580927726;  // sqrt(2/3), correctly rounded
+  double e = eps0 + s23 * fmax(dl, 0.);
+  double ed = s23 * fmax(dl, 0.) / k.dt;
+  double sY = sigma_yield(k, e, ed, t);
+  return norm - 2 * k.G * dl - s23 * sY;
+}
+
+// secant iteration of the radial return (plasticity.cuh:85-119)
+__device__ inline double rr_secant(const dev_consts &k, double init, double norm, double eps0, double t) {
+  double dl = init, dl_old;
+  int iter = 0;
+  while (true) {
+    double g = rr_residual(k, dl, norm, eps0, t);
+    double dl1 = dl * 1.01;
+    double g1 = rr_residual(k, dl1, norm, eps0, t);
+    double slope = (g - g1) / (dl - dl1);
+    dl_old = dl;
+    dl = dl - g / slope;
+    if (dl < 0.) dl = -0.1 * dl;
+    if (fabs(dl - dl_old) < 1e-6) return dl;
+    if (iter > 100) return dl;
+    iter++;
+  }
+}
+
+// Johnson-Cook plasticity by radial return on one particle (actions_gpu.cu:226-285)
+__device__ inline void plasticity(const dev_consts &k, double p, double rho, double &Sxx, double &Sxy, double &Syy,
+                                  double &Szz, double &eps_pl, double &eps_pl_dot, double &T) {
+  double norm = f32sqrt(Sxx * Sxx + Syy * Syy + Szz * Szz + Sxy * Sxy);
+  double cxx = Sxx - p, cyy = Syy - p, czz = Szz - p, cxy = Sxy;
+  double svm = f32sqrt((cxx * cxx + cyy * cyy + czz * czz) - cxx * cyy - cxx * czz - cyy * czz + 3.0 * cxy * cxy);
+  double sY = sigma_yield(k, eps_pl, eps_pl_dot, T);
+  if (svm < sY) { eps_pl_dot = 0.; return; }
+  const double s23 = 0.816496580927726;
+  double dl = rr_secant(k, fmax(eps_pl_dot * k.dt * s23, 1e-8), norm, eps_pl, T);
+  float inc = sqrtf((float)(2.0 / 3.0)) * fmaxf((float)dl, 0.f);
+  double e_new = eps_pl + (double)inc;
+  double ed_new = (double)inc / k.dt;
+  double mu = k.G;
+  double nxx = Sxx - Sxx / norm * dl * 2. * mu;
+  double nyy = Syy - Syy / norm * dl * 2. * mu;
+  double nzz = Szz - Szz / norm * dl * 2. * mu;
+  double nxy = Sxy - Sxy / norm * dl * 2. * mu;
+  Sxx = nxx; Syy = nyy; Szz = nzz; Sxy = nxy;
+  double e_old = eps_pl;
+  eps_pl = e_new; eps_pl_dot = ed_new;
+  if (k.tq != 0.) {
+    double deps = e_new - e_old;
+    double sY2 = sigma_yield(k, e_new, ed_new, T);
+    T += k.tq / (k.cp * rho) * deps * sY2;
+  }
+}
+
+// ---------------------------------------------------------------- tool contact (tool_gpu.cu)
+__device__ inline double fillet_angle(double y, double x) {  // myatan2, tool_gpu.cu:29-36
+  double t = mfcm::atan2(y, x);
+  return (t > 0.) ? t : t + 2 * M_PI;
+}
+
+// distance to the fillet arc, FLT_MAX outside its angular range (tool_gpu.cu:38-63)
+__device__ inline double fillet_distance(const mfc_tool_state &t, double qx, double qy) {
+  double Ax = t.fillet_p[0], Ay = t.fillet_p[1], r = t.fillet_r;
+  double d = sqrt((qx - Ax) * (qx - Ax) + (qy - Ay) * (qy - Ay));
+  double Cx = Ax + r * (qx - Ax) / d;
+  double Cy = Ay + r * (qy - Ay) / d;
+  double a = fillet_angle(Ay - Cy, Ax - Cx);
+  bool valid = a > fmin(t.fillet_t1, t.fillet_t2) && a < fmax(t.fillet_t1, t.fillet_t2);
+  if (!valid) return FLT_MAX_D;
+  double ex = Cx - qx, ey = Cy - qy;
+  return sqrt(ex * ex + ey * ey);
+}
+
+// penetration test and closest tool point / normal (tool_gpu.cu:65-139)
+__device__ inline bool tool_contact(const mfc_tool_state &t, double qx, double qy, double &cx, double &cy, double &nx,
+                                    double &ny) {
+  const bool fil = t.num_seg == 5;
+  bool in = true;
+  for (int i = 0; i < t.num_seg; i++) {
+    double dx = t.left[i][0] - qx, dy = t.left[i][1] - qy;
+    in = in && (dx * t.n[i][0] + dy * t.n[i][1]) < 0.;
+  }
+  if (fil) {
+    double dx = t.fillet_p[0] - qx, dy = t.fillet_p[1] - qy;
+    in = in || sqrt(dx * dx + dy * dy) < t.fillet_r;
+  }
+  if (!in) return false;
+  double depth = FLT_MAX_D;
+  for (int i = 0; i < t.num_seg; i++) {
+    double d;
+    if (t.vertical[i]) {
+      double ddx = t.left[i][0] - qx;
+      d = fabs(ddx);
+      if (d < depth) {
+        depth = d;
+        double is = 1.0 / sqrt(ddx * ddx + 0. * 0.);
+        nx = ddx * is; ny = 0. * is;
+        cx = t.left[i][0]; cy = qy;
+      }
+    } else if (fil && i == 2) {
+      d = fillet_distance(t, qx, qy);
+      if (d < depth) {
+        depth = d;
+        double vx = qx - t.fillet_p[0], vy = qy - t.fillet_p[1];
+        double is = 1.0 / sqrt(vx * vx + vy * vy);
+        nx = vx * is; ny = vy * is;
+        cx = t.fillet_p[0] + nx * t.fillet_r;
+        cy = t.fillet_p[1] + ny * t.fillet_r;
+      }
+    } else {
+      double a = t.line_a[i], b = t.line_b[i];
+      double den = a * a + (-1.) * (-1.);
+      double px = ((-1.) * ((-1.) * qx - a * qy) - a * b) / den;
+      double py = (a * (-(-1.) * qx + a * qy) - (-1.) * b) / den;
+      double ex = px - qx, ey = py - qy;
+      d = sqrt(ex * ex + ey * ey);
+      if (d < depth) { depth = d; nx = -t.n[i][0]; ny = -t.n[i][1]; cx = px; cy = py; }
+    }
+  }
+  return true;
+}
+
+// penalty normal force and LS-DYNA style Coulomb friction with history (tool_gpu.cu:151-207);
+// returns false (and zero forces) outside the tool
+__device__ inline bool contact_force(const dev_consts &k, const mfc_tool_state &t, double x, double y, double vx,
+                                     double vy, double &fcx, double &fcy, double &ftx, double &fty, double &nx,
+                                     double &ny) {
+  double cx = 0., cy = 0.;
+  nx = 0.; ny = 0.;
+  if (!tool_contact(t, x, y, cx, cy, nx, ny)) {
+    fcx = fcy = ftx = fty = 0.; nx = ny = 0.;
+    return false;
+  }
+  double dt2 = k.dt * k.dt;
+  double gN = (x - cx) * nx + (y - cy) * ny;
+  double sc = -k.mass * gN;
+  fcx = ((sc * nx) / dt2) * k.contact_alpha;
+  fcy = ((sc * ny) / dt2) * k.contact_alpha;
+  double tx = 0., ty = 0.;
+  if (k.mu != 0.) {
+    double rvx = vx - k.tool_vx, rvy = vy - k.tool_vy;
+    double vrx = rvx - rvx * nx, vry = rvy - rvy * ny;
+    double km = k.contact_alpha * k.mass;
+    double kx = (km * vrx) / k.dt, ky = (km * vry) / k.dt;
+    double fy = k.mu * sqrt(fcx * fcx + fcy * fcy);
+    double sx = ftx - kx, sy = fty - ky;
+    double sl = sqrt(sx * sx + sy * sy);
+    if (sl > fy) { tx = (fy * sx) / sl; ty = (fy * sy) / sl; }
+    else { tx = sx; ty = sy; }
+  }
+  ftx = tx; fty = ty;
+  return true;
+}
+
+// one incremental tool move by dt: end points, line coefficients, fillet centre
+// (tool.cpp:211-221, geometric_primitives.cpp:70-85)
+__device__ inline void tool_move(mfc_tool_state &t, double dt) {
+  double sx = dt * t.vel[0], sy = dt * t.vel[1];
+  for (int i = 0; i < t.num_seg; i++) {
+    double lx = t.left[i][0] + sx, ly = t.left[i][1] + sy;
+    double rx = t.right[i][0] + sx, ry = t.right[i][1] + sy;
+    t.left[i][0] = lx; t.left[i][1] = ly; t.right[i][0] = rx; t.right[i][1] = ry;
+    double det = lx * 1. - 1. * rx;
+    if (fabs(det) < 1e-12) { t.vertical[i] = 1; t.line_a[i] = 1.7976931348623157e308; t.line_b[i] = lx; }
+    else {
+      t.vertical[i] = 0;
+      t.line_a[i] = (ly * 1. - 1. * ry) / det;
+      t.line_b[i] = (ry * lx - rx * ly) / det;
+    }
+  }
+  if (t.has_fillet) { t.fillet_p[0] = t.fillet_p[0] + sx; t.fillet_p[1] = t.fillet_p[1] + sy; }
+}
+
+}  // namespace mfc

@@ -1,0 +1,57 @@
+// mfc_kernels.h — launch interface of the gfx950 kernels of one SPH cutting step.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "mfc_device.h"
+
+namespace mfc {
+
+// one set of the permuted persistent particle fields, structure of arrays in storage order
+struct dstate {
+  double *px, *py, *vx, *vy;
+  double *sxx, *sxy, *syy, *szz;
+  double *rho, *T, *eps, *epsd, *h;
+  double *fcx, *fcy, *ftx, *fty;
+  double *ptx, *pty, *vtx, *vty, *rhot, *stxx, *stxy, *styy, *stzz, *Tt;
+  int *flags, *id;
+};
+constexpr int DSTATE_DOUBLES = 27;
+
+// binning scratch and cell table
+struct dbins {
+  int *key;          // cell id per particle in the current (unsorted) order
+  int *key_sorted;   // cell id per particle in sorted order (= reference's sorted hash)
+  int *slot;         // arrival slot inside the cell (order-free, fixed by the rank kernel)
+  int *count;        // particles per cell, zero between steps
+  int *offsets;      // num_cell + 1 exclusive prefix sums: cell c = [offsets[c], offsets[c+1])
+  int *bucket;       // particle indices grouped by cell, unordered inside a cell
+  int *perm;         // sorted position -> current position
+  int *bsum;         // scan block sums
+  unsigned long long *clamped;  // particles whose cell id was clamped into the grid
+};
+
+// positional (not permuted) arrays, as particle_gpu::{n,p,R} (particle_gpu.h:39-47)
+struct dpositional {
+  double *nx, *ny, *p, *rxx, *rxy, *ryy;
+};
+
+constexpr int REC = 16;  // doubles per neighbour record
+
+// internal flag: some value of the particle's neighbour record is NaN or Inf
+constexpr int FLAG_NONFINITE = 8;
+
+// step kernels
+void launch_binning(const dev_consts &k, const dstate &cur, dbins &b, hipStream_t s);
+void launch_predict(const dev_consts &k, const dstate &cur, const dstate &nxt, const dbins &b, const dpositional &pos,
+                    double *rec, const mfc_tool_state *tool_in, mfc_tool_state *tool_out, hipStream_t s);
+void launch_interact(const dev_consts &k, const dstate &st, const dbins &b, const dpositional &pos, const double *rec,
+                     const mfc_tool_state *tool_now, double *partials, hipStream_t s);
+void launch_force_final(int ntiles, const double *partials, double *force_last, double *hist, int64_t cap,
+                        int64_t *ctr, hipStream_t s);
+
+int scan_blocks(int num_cell);
+void launch_device_math(int fn, const double *x, const double *y, double *out, int64_t n, hipStream_t s);
+
+}  // namespace mfc

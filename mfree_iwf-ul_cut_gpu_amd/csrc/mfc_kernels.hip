@@ -1,0 +1,577 @@
+// mfc_kernels.hip — gfx950 kernels of one FP64 SPH cutting step.
+//
+// Per step (the reference's ~20 launches of leap_frog::step, leap_frog.cu:94-155, become 8):
+//   binning      k_hash_count, k_scan_reduce, k_scan_top, k_scan_down, k_place, k_rank
+//                cell ids (grid.cu:34-51) and a stable counting sort by cell
+//                (thrust::sort_by_key + reorder_data_and_find_cell_start, grid_gpu_green.cu:22-116,177-201)
+//   k_predict    gather into sorted order + leap-frog init/predict (leap_frog.cu:30-66) + EOS
+//                (actions_gpu.cu:41-51) + artificial stress (:53-97) + neighbour record + tool move
+//   k_interact   neighbour sums (interactions_gpu.cu:88-345) fused with every per-particle action
+//                that follows them in the step: Jaumann, continuity, momentum, advection, correct,
+//                friction heat, Johnson-Cook return, tool-particle move, contact, boundary
+//                conditions, NaN blanking, and the per-tile partial sums of the tool force
+//   k_force_final  tool force of the step from the tile partials (output.cu:31-39)
+//
+// Memory layout: the persistent particle state is a structure of arrays in cell-sorted order,
+// double-buffered across the gather. Neighbour data are packed per particle into one 128-byte
+// record (two 64-byte lines) so that the divergent per-lane neighbour reads of k_interact are
+// 16-byte vector loads from one cache line. Work-group to tile mapping is XCD-aware.
+#include "mfc_kernels.h"
+
+namespace mfc {
+
+namespace {
+
+constexpr int SCAN_THREADS = 256;
+constexpr int SCAN_ITEMS = 8;
+constexpr int SCAN_CHUNK = SCAN_THREADS * SCAN_ITEMS;
+
+__device__ inline double bits_to_double(int f) { return __longlong_as_double((long long)f); }
+__device__ inline int double_to_bits(double d) { return (int)__double_as_longlong(d); }
+__device__ inline bool finite_d(double v) { return !mfcm::is_nan(v) && !mfcm::is_inf(v); }
+
+// exclusive scan of one int per thread over a 256-thread block; total receives the block sum
+__device__ inline int block_exclusive_scan(int v, int &total) {
+  __shared__ int wsum[SCAN_THREADS / 64];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  int x = v;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    int y = __shfl_up(x, o, 64);
+    if (lane >= o) x += y;
+  }
+  if (lane == 63) wsum[w] = x;
+  __syncthreads();
+  int pre = 0, tot = 0;
+#pragma unroll
+  for (int q = 0; q < SCAN_THREADS / 64; q++) {
+    if (q < w) pre += wsum[q];
+    tot += wsum[q];
+  }
+  total = tot;
+  __syncthreads();
+  return pre + x - v;
+}
+
+// ------------------------------------------------------------------------------- binning
+// cell id of every particle (grid.cu:34-51; blanked particles keep their last sorted id,
+// grid.cu:39-41) and its arrival slot inside the cell
+__global__ __launch_bounds__(256) void k_hash_count(dev_consts k, const double *__restrict__ px,
+                                                    const double *__restrict__ py, const int *__restrict__ flags,
+                                                    const int *__restrict__ key_sorted, int *__restrict__ key,
+                                                    int *__restrict__ slot, int *__restrict__ count,
+                                                    unsigned long long *__restrict__ clamped) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= k.N) return;
+  int h;
+  if (flags[i] & MFC_FLAG_BLANKED) {
+    h = key_sorted[i];
+  } else {
+    bool cl;
+    h = cell_of(k, px[i], py[i], cl);
+    if (cl) atomicAdd(clamped, 1ull);
+  }
+  key[i] = h;
+  slot[i] = atomicAdd(&count[h], 1);
+}
+
+__global__ __launch_bounds__(SCAN_THREADS) void k_scan_reduce(const int *__restrict__ count, int n,
+                                                               int *__restrict__ bsum) {
+  int base = blockIdx.x * SCAN_CHUNK + threadIdx.x * SCAN_ITEMS;
+  int s = 0;
+#pragma unroll
+  for (int q = 0; q < SCAN_ITEMS; q++)
+    if (base + q < n) s += count[base + q];
+  int total;
+  block_exclusive_scan(s, total);
+  if (threadIdx.x == 0) bsum[blockIdx.x] = total;
+}
+
+__global__ __launch_bounds__(SCAN_THREADS) void k_scan_top(int *__restrict__ bsum, int nb) {
+  int carry = 0;
+  for (int c0 = 0; c0 < nb; c0 += SCAN_CHUNK) {
+    int base = c0 + threadIdx.x * SCAN_ITEMS;
+    int v[SCAN_ITEMS];
+    int s = 0;
+#pragma unroll
+    for (int q = 0; q < SCAN_ITEMS; q++) {
+      v[q] = (base + q < nb) ? bsum[base + q] : 0;
+      s += v[q];
+    }
+    int total;
+    int pre = block_exclusive_scan(s, total) + carry;
+#pragma unroll
+    for (int q = 0; q < SCAN_ITEMS; q++) {
+      if (base + q < nb) bsum[base + q] = pre;
+      pre += v[q];
+    }
+    carry += total;
+    __syncthreads();
+  }
+}
+
+// cell offsets from the counts; the counts are cleared for the next step
+__global__ __launch_bounds__(SCAN_THREADS) void k_scan_down(int *__restrict__ count, int n,
+                                                             const int *__restrict__ bsum, int *__restrict__ offsets) {
+  int base = blockIdx.x * SCAN_CHUNK + threadIdx.x * SCAN_ITEMS;
+  int v[SCAN_ITEMS];
+  int s = 0;
+#pragma unroll
+  for (int q = 0; q < SCAN_ITEMS; q++) {
+    v[q] = (base + q < n) ? count[base + q] : 0;
+    s += v[q];
+  }
+  int total;
+  int pre = block_exclusive_scan(s, total) + bsum[blockIdx.x];
+#pragma unroll
+  for (int q = 0; q < SCAN_ITEMS; q++) {
+    int c = base + q;
+    if (c < n) {
+      offsets[c] = pre;
+      count[c] = 0;
+      if (c == n - 1) offsets[n] = pre + v[q];
+    }
+    pre += v[q];
+  }
+}
+
+__global__ __launch_bounds__(256) void k_place(int N, const int *__restrict__ key, const int *__restrict__ slot,
+                                               const int *__restrict__ offsets, int *__restrict__ bucket) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  bucket[offsets[key[i]] + slot[i]] = i;
+}
+
+// stable position: the cell's offset plus the number of particles of the same cell that
+// precede this one in the current order (thrust::sort_by_key on (hash, 0..N-1))
+__global__ __launch_bounds__(256) void k_rank(int N, const int *__restrict__ key, const int *__restrict__ bucket,
+                                              const int *__restrict__ offsets, int *__restrict__ perm,
+                                              int *__restrict__ key_sorted) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  int h = key[i];
+  int b = offsets[h], e = offsets[h + 1];
+  int r = 0;
+  for (int q = b; q < e; q++) r += bucket[q] < i;
+  perm[b + r] = i;
+  key_sorted[b + r] = h;
+}
+
+// ------------------------------------------------------------------------------- predict
+// gather into sorted order, leap-frog init + predict, EOS, artificial stress, neighbour record
+__global__ __launch_bounds__(TILE) void k_predict(dev_consts k, int ntiles, const int *__restrict__ perm, dstate cur,
+                                                  dstate nxt, dpositional pos, double *__restrict__ rec,
+                                                  const mfc_tool_state *__restrict__ tool_in,
+                                                  mfc_tool_state *__restrict__ tool_out) {
+  const int t = xcd_tile(blockIdx.x, ntiles);
+  if (t < 0) return;
+  if (t == 0 && threadIdx.x == 0) {
+    mfc_tool_state ts = *tool_in;
+    tool_move(ts, k.dt);
+    *tool_out = ts;
+  }
+  const int i = t * TILE + threadIdx.x;
+  if (i >= k.N) return;
+  const int s = perm[i];
+  const double x0 = cur.px[s], y0 = cur.py[s], u0 = cur.vx[s], v0 = cur.vy[s];
+  const double sxx0 = cur.sxx[s], sxy0 = cur.sxy[s], syy0 = cur.syy[s], szz0 = cur.szz[s];
+  const double rho0 = cur.rho[s], T0 = cur.T[s];
+  const double ptx = cur.ptx[s], pty = cur.pty[s], vtx = cur.vtx[s], vty = cur.vty[s];
+  const double rhot = cur.rhot[s], stxx = cur.stxx[s], stxy = cur.stxy[s], styy = cur.styy[s], stzz = cur.stzz[s];
+  const double Tt = cur.Tt[s];
+  const int fl = cur.flags[s] & (MFC_FLAG_FIXED | MFC_FLAG_BLANKED | MFC_FLAG_TOOL);
+  const double h = cur.h[s];
+  nxt.px[i] = x0; nxt.py[i] = y0; nxt.vx[i] = u0; nxt.vy[i] = v0;
+  nxt.sxx[i] = sxx0; nxt.sxy[i] = sxy0; nxt.syy[i] = syy0; nxt.szz[i] = szz0;
+  nxt.rho[i] = rho0; nxt.T[i] = T0; nxt.h[i] = h;
+  nxt.eps[i] = cur.eps[s]; nxt.epsd[i] = cur.epsd[s];
+  nxt.fcx[i] = cur.fcx[s]; nxt.fcy[i] = cur.fcy[s]; nxt.ftx[i] = cur.ftx[s]; nxt.fty[i] = cur.fty[s];
+  nxt.ptx[i] = ptx; nxt.pty[i] = pty; nxt.vtx[i] = vtx; nxt.vty[i] = vty;
+  nxt.rhot[i] = rhot; nxt.stxx[i] = stxx; nxt.stxy[i] = stxy; nxt.styy[i] = styy; nxt.stzz[i] = stzz;
+  nxt.Tt[i] = Tt;
+  nxt.flags[i] = fl;
+  nxt.id[i] = cur.id[s];
+
+  double x = x0, y = y0, u = u0, v = v0, sxx = sxx0, sxy = sxy0, syy = syy0, rho = rho0, T = T0;
+  if (!(fl & MFC_FLAG_BLANKED)) {  // leap_frog.cu:42-66
+    const double hdt = 0.5 * k.dt;
+    x = x0 + hdt * ptx; y = y0 + hdt * pty;
+    u = u0 + hdt * vtx; v = v0 + hdt * vty;
+    sxx = sxx0 + hdt * stxx; sxy = sxy0 + hdt * stxy; syy = syy0 + hdt * styy;
+    rho = rho0 + hdt * rhot;
+    T = T0 + hdt * Tt;
+  }
+  double p = pos.p[i], rxx = pos.rxx[i], rxy = pos.rxy[i], ryy = pos.ryy[i];
+  if (!(fl & MFC_FLAG_TOOL)) {
+    p = k.c0 * k.c0 * (rho - k.rho0);
+    artificial_stress(rho, p, sxx, sxy, syy, k.stress_eps, rxx, rxy, ryy);
+    pos.p[i] = p; pos.rxx[i] = rxx; pos.rxy[i] = rxy; pos.ryy[i] = ryy;
+  }
+  const double r21 = 1. / (rho * rho);
+  double r[REC];
+  r[0] = x; r[1] = y;
+  r[3] = h;
+  r[4] = u; r[5] = v;
+  r[6] = rho;
+  r[7] = k.mass / rho;
+  r[8] = (sxx - p) * r21;
+  r[9] = sxy * r21;
+  r[10] = (syy - p) * r21;
+  r[11] = rxx; r[12] = rxy; r[13] = ryy;
+  r[14] = f32sqrt(k.K / rho);
+  r[15] = T;
+  bool fin = true;
+#pragma unroll
+  for (int q = 0; q < REC; q++)
+    if (q != 2) fin = fin && finite_d(r[q]);
+  r[2] = bits_to_double(fl | (fin ? 0 : FLAG_NONFINITE));
+  double2 *dst = reinterpret_cast<double2 *>(rec + (size_t)REC * i);
+#pragma unroll
+  for (int q = 0; q < REC / 2; q++) dst[q] = make_double2(r[2 * q], r[2 * q + 1]);
+}
+
+// ------------------------------------------------------------------------------- interact
+struct pair_acc {
+  double vd0, vd1, vd2, vd3;   // v_der (vx,x vx,y vy,x vy,y)
+  double sd0, sd1, sd2, sd3;   // S_der
+  double vt0, vt1, xt0, xt1;   // vel_t, pos_t (XSPH)
+  double tl;                   // Brookshaw sum
+};
+
+// the body of the neighbour loop of do_interactions_monaghan for one pair
+// (interactions_gpu.cu:249-326 with cubic_spline kernels.cuh:30-70)
+__device__ inline void pair_terms(const dev_consts &k, pair_acc &a, double xi, double yi, double ui, double vi_,
+                                  double hi, double rhoi, double Axx_i, double Axy_i, double Ayy_i, double Rxx_i,
+                                  double Rxy_i, double Ryy_i, double ci, double Ti, bool tool_i, double ta,
+                                  double h1, double fourh2, double fac, const double *rj, double xj, double yj,
+                                  bool tool_j) {
+  const double2 *r2 = reinterpret_cast<const double2 *>(rj);
+  double2 q1 = r2[1], q2 = r2[2], q3 = r2[3], q4 = r2[4], q5 = r2[5], q6 = r2[6], q7 = r2[7];
+  const double hj = q1.y, uj = q2.x, vj = q2.y, rhoj = q3.x, volj = q3.y;
+  const double Axx_j = q4.x, Axy_j = q4.y, Ayy_j = q5.x, Rxx_j = q5.y, Rxy_j = q6.x, Ryy_j = q6.y;
+  const double cj = q7.x, Tj = (ta != 0.) ? q7.y : 0.;
+  const double mass = k.mass;
+  // cubic spline with h_i
+  double w = 0., w_x = 0., w_y = 0.;
+  const double xij = xi - xj, yij = yi - yj;
+  const double rr2 = xij * xij + yij * yij;
+  if (!(rr2 >= fourh2 || rr2 < 1e-8)) {
+    double der, val;
+    double rad = f32sqrt(rr2);
+    double q = rad * h1;
+    if (q > 1) {
+      double m = 2 - q, m2 = m * m;
+      val = 0.25 * m2 * m2 * m2;
+      der = -0.75 * m2 * h1 / rad;
+    } else {
+      val = 1 - 1.5 * q * q * (1 - 0.5 * q);
+      der = -3.0 * q * (1 - 0.75 * q) * h1 / rad;
+    }
+    w = val * fac;
+    w_x = der * xij * fac;
+    w_y = der * yij * fac;
+  }
+  if (!tool_i && !tool_j) {
+    a.vd0 += (uj - ui) * w_x * volj;
+    a.vd1 += (uj - ui) * w_y * volj;
+    a.vd2 += (vj - vi_) * w_x * volj;
+    a.vd3 += (vj - vi_) * w_y * volj;
+    double fab = w / k.wdeltap;
+    fab *= fab;
+    fab *= fab;
+    const double Rx = fab * (Rxx_i + Rxx_j), Ry = fab * (Rxy_i + Rxy_j), Rz = fab * (Ryy_i + Ryy_j);
+    a.sd0 += mass * (Axx_i + Axx_j + Rx) * w_x;
+    a.sd1 += mass * (Axy_i + Axy_j + Ry) * w_y;
+    a.sd2 += mass * (Axy_i + Axy_j + Ry) * w_x;
+    a.sd3 += mass * (Ayy_i + Ayy_j + Rz) * w_y;
+    const double vijx = ui - uj, vijy = vi_ - vj;
+    const double vijposij = vijx * xij + vijy * yij;
+    const double rhoij = 0.5 * (rhoi + rhoj);
+    if (vijposij < 0.) {
+      const double cij = 0.5 * (ci + cj);
+      const double hij = 0.5 * (hi + hj);
+      const double muij = (hij * vijposij) / (rr2 + k.av_eta * k.av_eta * hij * hij);
+      const double piij = (-k.av_alpha * cij * muij + k.av_beta * muij * muij) / rhoij;
+      a.vt0 += -mass * piij * w_x;
+      a.vt1 += -mass * piij * w_y;
+    }
+    a.xt0 += -k.xsph_eps * w * mass / rhoij * vijx;
+    a.xt1 += -k.xsph_eps * w * mass / rhoij * vijy;
+  }
+  if (ta != 0.) {  // Brookshaw
+    const double rij = sqrt(rr2);
+    if (rij > 1e-8) {
+      const double eijx = xij / rij, eijy = yij / rij;
+      const double rij1 = 1. / rij;
+      a.tl += 2.0 * volj * (Ti - Tj) * rij1 * (eijx * w_x + eijy * w_y);
+    }
+  }
+}
+
+__global__ __launch_bounds__(TILE) void k_interact(dev_consts k, int ntiles, const double *__restrict__ rec,
+                                                   const int *__restrict__ key_sorted,
+                                                   const int *__restrict__ offsets, dstate st, dpositional pos,
+                                                   const mfc_tool_state *__restrict__ tool_now,
+                                                   double *__restrict__ partials) {
+  const int t = xcd_tile(blockIdx.x, ntiles);
+  if (t < 0) return;
+  const int i = t * TILE + threadIdx.x;
+  double fsum0 = 0., fsum1 = 0., fsum2 = 0., fsum3 = 0.;
+  if (i < k.N) {
+    const bool pse = k.pse != 0;
+    const double *ri = rec + (size_t)REC * i;
+    const double2 *r2 = reinterpret_cast<const double2 *>(ri);
+    double2 a0 = r2[0], a1 = r2[1], a2 = r2[2], a3 = r2[3], a4 = r2[4], a5 = r2[5], a6 = r2[6], a7 = r2[7];
+    const double xi = a0.x, yi = a0.y;
+    const int rfl = double_to_bits(a1.x);
+    const double hi = a1.y, ui = a2.x, vi_ = a2.y, rhoi = a3.x;
+    const double Axx_i = a4.x, Axy_i = a4.y, Ayy_i = a5.x, Rxx_i = a5.y, Rxy_i = a6.x, Ryy_i = a6.y;
+    const double ci = a7.x, Tpred = a7.y;
+    int fl = st.flags[i];
+    const bool blank = fl & MFC_FLAG_BLANKED, tool = fl & MFC_FLAG_TOOL, fixed = fl & MFC_FLAG_FIXED;
+    const bool nonfin_i = rfl & FLAG_NONFINITE;
+
+    pair_acc acc = {0., 0., 0., 0., 0., 0., 0., 0., 0., 0., 0., 0., 0.};
+    double tpse = 0.;
+    const double ta = pse ? 0. : (tool ? k.alpha_tool : k.alpha_wp);
+    const bool run_mech = !blank && (!pse || !tool);
+    const bool run_pse = !blank && pse && k.thermal_wp;
+    if (run_mech || run_pse) {
+      const double Ti = (ta != 0.) ? Tpred : 0.;
+      const double h1 = 1 / hi;
+      const double fourh2 = 4 * hi * hi;
+      const double fac = PI_FAC * h1 * h1;
+      const double h2 = hi * hi, h4 = h2 * h2;
+      const double pse_c = 4. / (h4 * M_PI);
+      const int hsh = key_sorted[i];
+      const int gi = hsh / k.gny, gj = hsh - gi * k.gny;
+      const int li = gi - 2 < 0 ? 0 : gi - 2;
+      const int lj = gj - 2 < 0 ? 0 : gj - 2;
+      const int hi_ = gi + 3 > k.gnx ? k.gnx : gi + 3;
+      const int hj = gj + 3 > k.gny ? k.gny : gj + 3;
+      for (int ii = li; ii < hi_; ii++) {
+        const int jb = offsets[ii * k.gny + lj], je = offsets[ii * k.gny + hj];
+        for (int j = jb; j < je; j++) {
+          const double *rj = rec + (size_t)REC * j;
+          const double2 b0 = reinterpret_cast<const double2 *>(rj)[0];
+          const double2 b1 = reinterpret_cast<const double2 *>(rj)[1];
+          const double xj = b0.x, yj = b0.y;
+          const int fj = double_to_bits(b1.x);
+          if (run_pse) {  // do_interactions_heat_pse (interactions_gpu.cu:130-139), all cell members
+            const double2 b7 = reinterpret_cast<const double2 *>(rj)[7];
+            const double2 b3 = reinterpret_cast<const double2 *>(rj)[3];
+            const double dx = xi - xj, dy = yi - yj;
+            const double xx = sqrt(dx * dx + dy * dy);
+            const double w2 = pse_c * mfcm::exp(-xx * xx / (h2));
+            tpse += (b7.y - Tpred) * w2 * k.mass / b3.x;
+          }
+          if (!run_mech) continue;
+          if (fj & MFC_FLAG_BLANKED) continue;
+          const double dx = xi - xj, dy = yi - yj;
+          const double rr2 = dx * dx + dy * dy;
+          // outside the support the reference adds exact zeros, unless a value is non-finite
+          if ((rr2 >= fourh2 || rr2 < 1e-8) && !nonfin_i && !(fj & FLAG_NONFINITE)) continue;
+          pair_terms(k, acc, xi, yi, ui, vi_, hi, rhoi, Axx_i, Axy_i, Ayy_i, Rxx_i, Rxy_i, Ryy_i, ci, Ti, tool, ta,
+                     h1, fourh2, fac, rj, xj, yj, (fj & MFC_FLAG_TOOL) != 0);
+        }
+      }
+    }
+    // state at the start of the step (sorted) and previous derivatives
+    const double x0 = st.px[i], y0 = st.py[i], u0 = st.vx[i], v0 = st.vy[i];
+    const double sxx0 = st.sxx[i], sxy0 = st.sxy[i], syy0 = st.syy[i], szz0 = st.szz[i];
+    const double rho0 = st.rho[i], T0 = st.T[i];
+    double ptx = st.ptx[i], pty = st.pty[i], vtx = st.vtx[i], vty = st.vty[i];
+    double rhot = st.rhot[i], stxx = st.stxx[i], stxy = st.stxy[i], styy = st.styy[i], stzz = st.stzz[i];
+    double Tt = st.Tt[i];
+    double fcx = st.fcx[i], fcy = st.fcy[i], ftx = st.ftx[i], fty = st.fty[i];
+    double eps = st.eps[i], epsd = st.epsd[i];
+    // predicted state (same expressions as k_predict)
+    double pu = u0, pv = v0, psxx = sxx0, psxy = sxy0, psyy = syy0, pszz = szz0, prho = rho0;
+    if (!blank) {
+      const double hdt = 0.5 * k.dt;
+      pu = u0 + hdt * vtx; pv = v0 + hdt * vty;
+      psxx = sxx0 + hdt * stxx; psxy = sxy0 + hdt * stxy; psyy = syy0 + hdt * styy; pszz = szz0 + hdt * stzz;
+      prho = rho0 + hdt * rhot;
+    }
+    // interaction outputs (interactions_gpu.cu:334-344, :143)
+    if (run_mech) {
+      ptx = acc.xt0; pty = acc.xt1; vtx = acc.vt0; vty = acc.vt1;
+      if (ta != 0.) Tt = ta * acc.tl;
+    }
+    if (run_pse) {
+      double tflag = tool ? 1. : 0.;
+      if (k.pse_quirk && i >= k.N / 2) tflag = 0.;
+      Tt = ((tflag == 1.) ? k.alpha_tool : k.alpha_wp) * tpse;
+    }
+    if (!tool) {
+      // Jaumann rate, continuity, momentum, advection (actions_gpu.cu:99-224)
+      jaumann(k.G, acc.vd0, acc.vd1, acc.vd2, acc.vd3, psxx, psxy, psyy, pszz, stxx, stxy, styy, stzz);
+      rhot = -prho * (acc.vd0 + acc.vd3);
+      vtx += acc.sd0 + acc.sd1 + fcx / k.mass + ftx / k.mass;
+      vty += acc.sd2 + acc.sd3 + fcy / k.mass + fty / k.mass;
+      ptx = ptx + pu;
+      pty = pty + pv;
+    }
+    // leap-frog corrector (leap_frog.cu:68-92)
+    double x = x0, y = y0, u = u0, v = v0, sxx = sxx0, sxy = sxy0, syy = syy0, szz = szz0, rho = rho0, T = T0;
+    if (!blank) {
+      x = x0 + k.dt * ptx; y = y0 + k.dt * pty;
+      u = u0 + k.dt * vtx; v = v0 + k.dt * vty;
+      sxx = sxx0 + k.dt * stxx; sxy = sxy0 + k.dt * stxy; syy = syy0 + k.dt * styy; szz = szz0 + k.dt * stzz;
+      rho = rho0 + k.dt * rhot;
+      T = T0 + k.dt * Tt;
+    }
+    // frictional heating with the positional normal of the previous step (actions_gpu.cu:135-162)
+    if (k.fric_heat && !tool) {
+      const double fmag = f32sqrt(ftx * ftx + fty * fty);
+      if (fmag != 0.) {
+        const double nx = pos.nx[i], ny = pos.ny[i];
+        const double dvx = u - k.tool_vx, dvy = v - k.tool_vy;
+        const double dvn = dvx * nx + dvy * ny;
+        const double rx = dvx - dvn, ry = dvy - dvn;
+        const double vmag = f32sqrt(rx * rx + ry * ry);
+        T += k.eta_heat * k.dt * fmag * vmag / (k.cp * k.mass);
+      }
+    }
+    if (!tool) plasticity(k, pos.p[i], rho, sxx, sxy, syy, szz, eps, epsd, T);
+    if (tool) {  // update_tool_particles (tool_gpu.cu:210-221)
+      x += k.dt * k.tool_vx;
+      y += k.dt * k.tool_vy;
+    }
+    if (!tool) {
+      double nx, ny;
+      contact_force(k, *tool_now, x, y, u, v, fcx, fcy, ftx, fty, nx, ny);
+      pos.nx[i] = nx; pos.ny[i] = ny;
+      if (fixed) {  // actions_gpu.cu:296-311
+        u = 0.; v = 0.; fcx = 0.; fcy = 0.; ptx = 0.; pty = 0.; vtx = 0.; vty = 0.;
+      }
+    }
+    if (k.thermal_bc && fixed) T = k.T_init;  // actions_gpu.cu:287-294
+    if (!blank) {  // actions_gpu.cu:317-349
+      bool bad = !finite_d(ptx) || !finite_d(pty) || !finite_d(vtx) || !finite_d(vty) || !finite_d(stxx) ||
+                 !finite_d(stxy) || !finite_d(styy) || !finite_d(stzz) || !finite_d(rhot) || !finite_d(Tt);
+      if (bad) fl |= MFC_FLAG_BLANKED;
+    }
+    st.px[i] = x; st.py[i] = y; st.vx[i] = u; st.vy[i] = v;
+    st.sxx[i] = sxx; st.sxy[i] = sxy; st.syy[i] = syy; st.szz[i] = szz;
+    st.rho[i] = rho; st.T[i] = T; st.eps[i] = eps; st.epsd[i] = epsd;
+    st.fcx[i] = fcx; st.fcy[i] = fcy; st.ftx[i] = ftx; st.fty[i] = fty;
+    st.ptx[i] = ptx; st.pty[i] = pty; st.vtx[i] = vtx; st.vty[i] = vty;
+    st.rhot[i] = rhot; st.stxx[i] = stxx; st.stxy[i] = stxy; st.styy[i] = styy; st.stzz[i] = stzz;
+    st.Tt[i] = Tt;
+    st.flags[i] = fl;
+    fsum0 = fcx; fsum1 = fcy; fsum2 = ftx; fsum3 = fty;
+  }
+  // tile partial sums of the tool force by halving strides
+  __shared__ double red[4][TILE];
+  red[0][threadIdx.x] = fsum0; red[1][threadIdx.x] = fsum1;
+  red[2][threadIdx.x] = fsum2; red[3][threadIdx.x] = fsum3;
+  for (int s = TILE / 2; s > 0; s >>= 1) {
+    __syncthreads();
+    if (threadIdx.x < s) {
+#pragma unroll
+      for (int c = 0; c < 4; c++) red[c][threadIdx.x] = red[c][threadIdx.x] + red[c][threadIdx.x + s];
+    }
+  }
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int c = 0; c < 4; c++) partials[4 * t + c] = red[c][0];
+  }
+}
+
+// tile partials summed in ascending tile order; F = sum(fc) + sum(ft)
+__global__ __launch_bounds__(64) void k_force_final(int ntiles, const double *__restrict__ partials,
+                                                    double *__restrict__ force_last, double *__restrict__ hist,
+                                                    int64_t cap, int64_t *__restrict__ ctr) {
+  __shared__ double buf[4][64];
+  double acc[4] = {0., 0., 0., 0.};
+  for (int t0 = 0; t0 < ntiles; t0 += 64) {
+    int t = t0 + threadIdx.x;
+#pragma unroll
+    for (int c = 0; c < 4; c++) buf[c][threadIdx.x] = (t < ntiles) ? partials[4 * t + c] : 0.;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      int m = ntiles - t0 < 64 ? ntiles - t0 : 64;
+      for (int q = 0; q < m; q++)
+#pragma unroll
+        for (int c = 0; c < 4; c++) acc[c] = acc[c] + buf[c][q];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    double F0 = acc[0] + acc[2], F1 = acc[1] + acc[3];
+    force_last[0] = F0;
+    force_last[1] = F1;
+    int64_t s = *ctr;
+    if (s < cap) { hist[2 * s] = F0; hist[2 * s + 1] = F1; }
+    *ctr = s + 1;
+  }
+}
+
+__global__ void k_device_math(int fn, const double *__restrict__ x, const double *__restrict__ y,
+                              double *__restrict__ out, int64_t n) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double a = x[i], b = y ? y[i] : 0., r = 0.;
+  switch (fn) {
+    case 0: r = mfcm::exp(a); break;
+    case 1: r = mfcm::log(a); break;
+    case 2: r = mfcm::pow(a, b); break;
+    case 3: r = mfcm::atan2(a, b); break;
+    case 4: r = mfcm::sin(a); break;
+    case 5: r = mfcm::cos(a); break;
+    case 6: r = (double)mfcm::atan2f(a, b); break;
+    case 7: r = (double)mfcm::cosf(a); break;
+    case 8: r = (double)mfcm::sinf(a); break;
+    case 9: r = sqrt(a); break;
+    case 10: r = f32sqrt(a); break;
+    case 11: r = a / b; break;
+    case 12: r = mfcm::tanh(a); break;
+    default: r = 0.;
+  }
+  out[i] = r;
+}
+
+}  // namespace
+
+int scan_blocks(int num_cell) { return (num_cell + SCAN_CHUNK - 1) / SCAN_CHUNK; }
+
+void launch_binning(const dev_consts &k, const dstate &cur, dbins &b, hipStream_t s) {
+  const int g = (k.N + 255) / 256;
+  const int nb = scan_blocks(k.num_cell);
+  hipLaunchKernelGGL(k_hash_count, dim3(g), dim3(256), 0, s, k, cur.px, cur.py, cur.flags, b.key_sorted, b.key,
+                     b.slot, b.count, b.clamped);
+  hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(SCAN_THREADS), 0, s, b.count, k.num_cell, b.bsum);
+  hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(SCAN_THREADS), 0, s, b.bsum, nb);
+  hipLaunchKernelGGL(k_scan_down, dim3(nb), dim3(SCAN_THREADS), 0, s, b.count, k.num_cell, b.bsum, b.offsets);
+  hipLaunchKernelGGL(k_place, dim3(g), dim3(256), 0, s, k.N, b.key, b.slot, b.offsets, b.bucket);
+  hipLaunchKernelGGL(k_rank, dim3(g), dim3(256), 0, s, k.N, b.key, b.bucket, b.offsets, b.perm, b.key_sorted);
+}
+
+static int padded_tiles(int ntiles) { return ((ntiles + 7) / 8) * 8; }
+
+void launch_predict(const dev_consts &k, const dstate &cur, const dstate &nxt, const dbins &b, const dpositional &pos,
+                    double *rec, const mfc_tool_state *tool_in, mfc_tool_state *tool_out, hipStream_t s) {
+  const int nt = (k.N + TILE - 1) / TILE;
+  hipLaunchKernelGGL(k_predict, dim3(padded_tiles(nt)), dim3(TILE), 0, s, k, nt, b.perm, cur, nxt, pos, rec, tool_in,
+                     tool_out);
+}
+
+void launch_interact(const dev_consts &k, const dstate &st, const dbins &b, const dpositional &pos, const double *rec,
+                     const mfc_tool_state *tool_now, double *partials, hipStream_t s) {
+  const int nt = (k.N + TILE - 1) / TILE;
+  hipLaunchKernelGGL(k_interact, dim3(padded_tiles(nt)), dim3(TILE), 0, s, k, nt, rec, b.key_sorted, b.offsets, st,
+                     pos, tool_now, partials);
+}
+
+void launch_force_final(int ntiles, const double *partials, double *force_last, double *hist, int64_t cap,
+                        int64_t *ctr, hipStream_t s) {
+  hipLaunchKernelGGL(k_force_final, dim3(1), dim3(64), 0, s, ntiles, partials, force_last, hist, cap, ctr);
+}
+
+void launch_device_math(int fn, const double *x, const double *y, double *out, int64_t n, hipStream_t s) {
+  const int64_t g = (n + 255) / 256;
+  hipLaunchKernelGGL(k_device_math, dim3((unsigned)g), dim3(256), 0, s, fn, x, y, out, n);
+}
+
+}  // namespace mfc

@@ -1,0 +1,49 @@
+// mfc_setup.h — host-side scenario generation of the orthogonal cut (setup_ref_cut).
+#pragma once
+
+#include <string>
+#include <vector>
+
+#include "../../include/mfc.h"
+
+namespace mfc {
+
+// constants of one simulation (phys/corr/joco/trml/geom bundles, constants_structs.h:27-85)
+struct sim_constants {
+  int N, n_tool;
+  // grid (grid.cu:139-158)
+  int gnx, gny, num_cell;
+  double bbmin_x, bbmin_y, cell;
+  // time step (benchmarks.cpp:530-531)
+  double dt, t_final;
+  // material
+  double mass, rho0, K, G, c0;  // c0 = sqrtf(K/rho0) as float (actions_gpu.cu:47)
+  // correctors
+  double wdeltap, av_alpha, av_beta, av_eta, xsph_eps, stress_eps;
+  // thermal
+  double alpha_wp, alpha_tool, cp, tq, eta_heat, T_init;
+  // Johnson-Cook
+  double jc_A, jc_B, jc_C, jc_n, jc_m, jc_Tmelt, jc_Tref, jc_eps_dot_ref;
+  // contact
+  double contact_alpha, mu, tool_vx, tool_vy;
+  // switches derived from the constants (actions_gpu.cu:448-468, interactions_gpu.cu:443)
+  int thermal_bc, fric_heat, thermal_wp, pse, pse_quirk;
+};
+
+struct scenario {
+  sim_constants k;
+  std::vector<double> px, py, rho, h, T;
+  std::vector<int> flags;
+  mfc_tool_state tool;
+};
+
+// fills a scenario from a configuration; returns MFC_OK or an MFC_E_* code with err set
+int build_scenario(const mfc_config &cfg, scenario &out, std::string &err);
+
+// one incremental tool move by dt (tool.cpp:211-221, geometric_primitives.cpp:70-85)
+void tool_advance(mfc_tool_state &t, double dt);
+
+// the literal constants of setup_ref_cut (benchmarks.cpp:255-332,384-421)
+void config_ref_cut(mfc_config &c, int ny, double rake, double clear, double chamfer, double speed, double feed);
+
+}  // namespace mfc

@@ -1,0 +1,353 @@
+"""ctypes binding of the MI355X SPH cutting solver (C ABI in include/mfc.h).
+
+The Python side mirrors the reference's host interface for the cutting step:
+
+    reference (src/)                                   here
+    setup_ref_cut(ny, &grid, rake, clear, ...)          config_ref_cut(...) + Sim(cfg)
+        benchmarks.cpp:255-537
+    leap_frog::step(particles, grid)                    Sim.step(n)
+        leap_frog.cu:94-155
+    report_force(particles)  output.cu:31-39            Sim.tool_force()
+    cudaMemcpy of particle_gpu fields                   Sim.field(name)
+
+Loading fails loudly when lib/libmfc.so is missing: there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+PKG_DIR = os.path.dirname(_HERE)
+LIB_PATH = os.path.join(PKG_DIR, "lib", "libmfc.so")
+
+MFC_OK = 0
+THERMAL_BROOKSHAW, THERMAL_PSE = 0, 1
+PREC_FP64, PREC_FP32C = 0, 1
+LATTICE_REF, LATTICE_EXPLICIT = 0, 1
+FLAG_FIXED, FLAG_BLANKED, FLAG_TOOL = 1, 2, 4
+
+FIELDS = {
+    "pos": (0, 2), "vel": (1, 2), "S": (2, 4), "rho": (3, 1), "p": (4, 1), "T": (5, 1),
+    "eps_pl": (6, 1), "eps_pl_dot": (7, 1), "fc": (8, 2), "ft": (9, 2), "n": (10, 2), "h": (11, 1),
+    "flags": (12, 1), "id": (13, 1), "hash": (14, 1), "pos_t": (15, 2), "vel_t": (16, 2),
+    "rho_t": (17, 1), "S_t": (18, 4), "T_t": (19, 1), "R": (20, 3),
+}
+INT_FIELDS = {"flags", "id", "hash"}
+
+
+class Config(C.Structure):
+    """mfc_config (include/mfc.h)."""
+    _fields_ = [
+        ("lattice", C.c_int32), ("nx", C.c_int32), ("ny", C.c_int32), ("dx", C.c_double),
+        ("lo_x", C.c_double), ("hi_x", C.c_double), ("hi_y", C.c_double), ("hdx", C.c_double),
+        ("jitter", C.c_double), ("seed", C.c_uint64),
+        ("fixed_bottom_rows", C.c_int32), ("fix_left", C.c_int32), ("fix_right", C.c_int32),
+        ("E", C.c_double), ("nu", C.c_double), ("rho0", C.c_double),
+        ("jc_A", C.c_double), ("jc_B", C.c_double), ("jc_C", C.c_double), ("jc_n", C.c_double),
+        ("jc_m", C.c_double), ("jc_Tmelt", C.c_double), ("jc_Tref", C.c_double), ("jc_eps_dot_ref", C.c_double),
+        ("cp", C.c_double), ("tq", C.c_double), ("k_wp", C.c_double), ("eta", C.c_double), ("T_init", C.c_double),
+        ("av_alpha", C.c_double), ("av_beta", C.c_double), ("av_eta", C.c_double), ("xsph_eps", C.c_double),
+        ("stress_eps", C.c_double),
+        ("tool_tl_x", C.c_double), ("tool_tl_y", C.c_double), ("tool_length", C.c_double), ("tool_height", C.c_double),
+        ("rake_deg", C.c_double), ("clear_deg", C.c_double), ("edge_radius", C.c_double),
+        ("feed", C.c_double), ("speed", C.c_double), ("mu", C.c_double), ("contact_alpha", C.c_double),
+        ("sample_tool", C.c_int32), ("rho_tool", C.c_double), ("cp_tool", C.c_double), ("k_tool", C.c_double),
+        ("thermal_solver", C.c_int32), ("precision", C.c_int32), ("cfl", C.c_double),
+        ("pse_flag_quirk", C.c_int32), ("reserved", C.c_int32 * 7),
+    ]
+
+    def copy(self) -> "Config":
+        c = Config()
+        C.memmove(C.byref(c), C.byref(self), C.sizeof(Config))
+        return c
+
+    def as_dict(self) -> dict:
+        return {n: getattr(self, n) for n, _ in self._fields_ if n != "reserved"}
+
+
+class Info(C.Structure):
+    """mfc_info (include/mfc.h)."""
+    _fields_ = [
+        ("n_particles", C.c_int32), ("n_tool", C.c_int32), ("grid_nx", C.c_int32), ("grid_ny", C.c_int32),
+        ("num_cell", C.c_int32), ("precision", C.c_int32), ("dt", C.c_double), ("t_final", C.c_double),
+        ("mass", C.c_double), ("bbmin_x", C.c_double), ("bbmin_y", C.c_double), ("cell_size", C.c_double),
+        ("step", C.c_int64),
+    ]
+
+
+class ToolState(C.Structure):
+    """mfc_tool_state (include/mfc.h)."""
+    _fields_ = [
+        ("left", (C.c_double * 2) * 5), ("right", (C.c_double * 2) * 5), ("n", (C.c_double * 2) * 5),
+        ("line_a", C.c_double * 5), ("line_b", C.c_double * 5), ("vertical", C.c_int32 * 5),
+        ("num_seg", C.c_int32), ("has_fillet", C.c_int32),
+        ("fillet_p", C.c_double * 2), ("fillet_r", C.c_double), ("fillet_t1", C.c_double),
+        ("fillet_t2", C.c_double), ("vel", C.c_double * 2),
+    ]
+
+    def as_arrays(self) -> dict:
+        ns = self.num_seg
+        return {
+            "left": np.array(self.left)[:ns], "right": np.array(self.right)[:ns], "n": np.array(self.n)[:ns],
+            "line_a": np.array(self.line_a)[:ns], "line_b": np.array(self.line_b)[:ns],
+            "vertical": np.array(self.vertical)[:ns], "has_fillet": self.has_fillet,
+            "fillet": np.array([self.fillet_p[0], self.fillet_p[1], self.fillet_r, self.fillet_t1, self.fillet_t2]),
+            "vel": np.array(self.vel),
+        }
+
+
+_lib = None
+
+
+def lib() -> C.CDLL:
+    """The solver library; raises if it has not been built (no fallback path exists)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: build it first (python -c 'import __graft_entry__ as g; g.build()')")
+        L = C.CDLL(LIB_PATH)
+        vp, i32, i64, dp, sz = C.c_void_p, C.c_int, C.c_int64, C.POINTER(C.c_double), C.c_size_t
+        L.mfc_last_error.restype = C.c_char_p
+        L.mfc_config_ref_cut.argtypes = [C.POINTER(Config), i32, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double]
+        L.mfc_create.argtypes = [C.POINTER(Config), C.POINTER(vp)]
+        L.mfc_destroy.argtypes = [vp]
+        L.mfc_destroy.restype = None
+        L.mfc_step.argtypes = [vp, i32]
+        L.mfc_sync.argtypes = [vp]
+        L.mfc_profile.argtypes = [vp, i32, dp]
+        L.mfc_tool_force.argtypes = [vp, dp]
+        L.mfc_force_history.argtypes = [vp, dp, i64, C.POINTER(C.c_int64)]
+        L.mfc_force_history_reset.argtypes = [vp]
+        L.mfc_get_info.argtypes = [vp, C.POINTER(Info)]
+        L.mfc_get_field.argtypes = [vp, i32, vp, sz]
+        L.mfc_set_field.argtypes = [vp, i32, vp, sz]
+        L.mfc_get_cells.argtypes = [vp, C.POINTER(C.c_int32), sz]
+        L.mfc_get_tool.argtypes = [vp, C.POINTER(ToolState)]
+        L.mfc_device_math.argtypes = [i32, dp, dp, dp, i64]
+        L.mfc_scenario.argtypes = [C.POINTER(Config), C.POINTER(Info), dp, dp, dp, dp, C.POINTER(C.c_int32),
+                                   C.POINTER(ToolState)]
+        L.mfc_field_components.argtypes = [i32]
+        _lib = L
+    return _lib
+
+
+class MfcError(RuntimeError):
+    pass
+
+
+def _check(rc: int, what: str) -> None:
+    if rc != MFC_OK:
+        raise MfcError(f"{what} failed ({rc}): {lib().mfc_last_error().decode()}")
+
+
+def config_ref_cut(ny: int = 30, rake: float = 10.0, clear: float = 8.0, chamfer: float = 0.002,
+                   speed: float = 83.333328e-5 / 500.0 * 70.0, feed: float = 0.01) -> Config:
+    """setup_ref_cut's constants; defaults are main()'s (mfree_gpu.cpp:160-165)."""
+    c = Config()
+    _check(lib().mfc_config_ref_cut(C.byref(c), ny, rake, clear, chamfer, speed, feed), "mfc_config_ref_cut")
+    return c
+
+
+# ---------------------------------------------------------------- SI -> bomb units (cm, us, g, Mbar)
+def um(x): return x * 1e-4                   # micrometre -> cm
+def m_per_min(v): return v / 60.0 * 1e-4     # m/min -> cm/us
+def gpa(x): return x * 1e-2                  # GPa -> Mbar
+def mpa(x): return x * 1e-5                  # MPa -> Mbar
+def kg_m3(x): return x * 1e-3                # kg/m^3 -> g/cm^3
+def j_kgk(x): return x * 1e-8                # J/(kg K) -> Mbar cm^3/(g K)
+def w_mk(x): return x * 1e-13                # W/(m K)
+def per_s(x): return x * 1e-6                # 1/s -> 1/us
+
+
+def cut_config(nx: int, ny: int, dx_um: float, *, thermal: str = "adiabatic", speed_m_min: float = 70.0,
+               feed_um: float = 100.0, jitter: float = 0.0, seed: int = 0x355, precision: int = PREC_FP64,
+               cfl: float = 0.3, k_wp: float = 7.1, k_tool: float = 50.0, mu: float = 0.35,
+               jc: tuple | None = None, fixed_rows: int = 3) -> Config:
+    """An orthogonal cut of an nx x ny Ti6Al4V lattice (Johnson-Cook after Johnson 1985 in SI
+    units converted to the solver's units) with a 0 deg rake / 7 deg clearance tool of 5 um edge
+    radius, placed like setup_ref_cut places its tool (benchmarks.cpp:389-418).
+
+    thermal: "adiabatic" (no conduction, no tool particles), "contact" (Brookshaw conduction in
+    the workpiece and into sampled tool particles), or "pse" (PSE conduction, no tool particles).
+    jc: (A MPa, B MPa, n, C, m) overrides the Johnson-Cook constants."""
+    c = config_ref_cut(ny=30, rake=0.0, clear=7.0, chamfer=um(5.0), speed=m_per_min(speed_m_min), feed=um(feed_um))
+    c.lattice = LATTICE_EXPLICIT
+    c.nx, c.ny, c.dx = nx, ny, um(dx_um)
+    c.lo_x, c.hi_y = 0.005, 0.060
+    # front of the tool 10 um beyond the workpiece edge, so that the placement rule of
+    # setup_ref_cut (benchmarks.cpp:400-405) moves its rake face back onto the edge: the cut
+    # starts at step 0 instead of after the ~1 mm approach of the reference's placement
+    c.tool_tl_x = c.lo_x + um(10.0) - c.tool_length
+    c.jitter, c.seed = jitter, seed
+    c.fixed_bottom_rows = fixed_rows
+    c.E, c.nu, c.rho0 = gpa(113.8), 0.342, kg_m3(4430.0)
+    A, B, n, Cc, m = jc if jc is not None else (1098.0, 1092.0, 0.93, 0.014, 1.1)
+    c.jc_A, c.jc_B, c.jc_n, c.jc_C, c.jc_m = mpa(A), mpa(B), n, Cc, m
+    c.jc_Tref, c.jc_Tmelt, c.jc_eps_dot_ref = 300.0, 1836.0, per_s(1.0)
+    c.cp, c.tq, c.eta, c.T_init = j_kgk(553.0), 0.9, 0.9, 300.0
+    c.mu = mu
+    c.cfl = cfl
+    c.precision = precision
+    if thermal == "adiabatic":
+        c.k_wp, c.k_tool, c.sample_tool, c.thermal_solver = 0.0, 0.0, 0, THERMAL_BROOKSHAW
+    elif thermal == "contact":
+        c.k_wp, c.k_tool, c.sample_tool, c.thermal_solver = w_mk(k_wp), w_mk(k_tool), 1, THERMAL_BROOKSHAW
+    elif thermal == "pse":
+        c.k_wp, c.k_tool, c.sample_tool, c.thermal_solver = w_mk(k_wp), 0.0, 0, THERMAL_PSE
+    else:
+        raise ValueError(thermal)
+    return c
+
+
+# named workloads (sizes and physics of the benchmark cases)
+def workload(name: str, precision: int = PREC_FP64) -> Config:
+    if name == "adiabatic_6k":        # 150 x 40 at 10 um, FP64 adiabatic, 2000 steps
+        return cut_config(150, 40, 10.0, thermal="adiabatic", precision=precision)
+    if name == "thermal_contact_150k":  # 750 x 200 at 2 um, Brookshaw + tool thermal contact, 20000 steps
+        return cut_config(750, 200, 2.0, thermal="contact", precision=precision)
+    if name == "pse_500k":            # 1250 x 400 at 1.2 um, PSE, 120 m/min, jittered, 10000 steps
+        return cut_config(1250, 400, 1.2, thermal="pse", speed_m_min=120.0, jitter=0.01, precision=precision)
+    if name == "adjoint_6k":
+        return cut_config(150, 40, 10.0, thermal="adiabatic", precision=precision)
+    if name == "adjoint_40k":
+        return cut_config(375, 100, 4.0, thermal="adiabatic", precision=precision)
+    raise KeyError(name)
+
+
+@dataclass
+class SimInfo:
+    n_particles: int
+    n_tool: int
+    grid_nx: int
+    grid_ny: int
+    num_cell: int
+    precision: int
+    dt: float
+    t_final: float
+    mass: float
+    bbmin_x: float
+    bbmin_y: float
+    cell_size: float
+    step: int
+
+
+class Sim:
+    """One simulation handle: setup_ref_cut + leap_frog (+ report_force)."""
+
+    def __init__(self, cfg: Config):
+        self._h = C.c_void_p()
+        _check(lib().mfc_create(C.byref(cfg), C.byref(self._h)), "mfc_create")
+        self.cfg = cfg.copy()
+
+    def close(self) -> None:
+        if self._h:
+            lib().mfc_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    @property
+    def info(self) -> SimInfo:
+        i = Info()
+        _check(lib().mfc_get_info(self._h, C.byref(i)), "mfc_get_info")
+        return SimInfo(*[getattr(i, n) for n, _ in Info._fields_])
+
+    @property
+    def N(self) -> int:
+        return self.info.n_particles
+
+    def step(self, n: int = 1) -> None:
+        _check(lib().mfc_step(self._h, int(n)), "mfc_step")
+
+    def sync(self) -> None:
+        _check(lib().mfc_sync(self._h), "mfc_sync")
+
+    PHASES = ("binning", "predict", "interact", "force")
+
+    def profile(self, nsteps: int) -> dict:
+        """Mean ms per step of each phase, timed with HIP events on the handle's stream."""
+        ms = np.zeros(len(self.PHASES))
+        _check(lib().mfc_profile(self._h, int(nsteps), ms.ctypes.data_as(C.POINTER(C.c_double))), "mfc_profile")
+        return dict(zip(self.PHASES, ms.tolist()))
+
+    def tool_force(self) -> np.ndarray:
+        F = np.zeros(2)
+        _check(lib().mfc_tool_force(self._h, F.ctypes.data_as(C.POINTER(C.c_double))), "mfc_tool_force")
+        return F
+
+    def force_history(self, cap: int = 1 << 20) -> np.ndarray:
+        n = C.c_int64()
+        _check(lib().mfc_force_history(self._h, None, 0, C.byref(n)), "mfc_force_history")
+        buf = np.zeros((max(1, min(cap, n.value)), 2))
+        _check(lib().mfc_force_history(self._h, buf.ctypes.data_as(C.POINTER(C.c_double)), buf.shape[0], C.byref(n)),
+               "mfc_force_history")
+        return buf[: n.value].copy()
+
+    def reset_force_history(self) -> None:
+        _check(lib().mfc_force_history_reset(self._h), "mfc_force_history_reset")
+
+    def field(self, name: str) -> np.ndarray:
+        fid, nc = FIELDS[name]
+        N = self.N
+        dt = np.int32 if name in INT_FIELDS else np.float64
+        out = np.zeros((N, nc) if nc > 1 else (N,), dtype=dt)
+        _check(lib().mfc_get_field(self._h, fid, out.ctypes.data_as(C.c_void_p), out.nbytes), f"mfc_get_field({name})")
+        return out
+
+    def set_field(self, name: str, values: np.ndarray) -> None:
+        fid, nc = FIELDS[name]
+        dt = np.int32 if name in INT_FIELDS else np.float64
+        v = np.ascontiguousarray(values, dtype=dt)
+        _check(lib().mfc_set_field(self._h, fid, v.ctypes.data_as(C.c_void_p), v.nbytes), f"mfc_set_field({name})")
+
+    def cells(self) -> np.ndarray:
+        nc = self.info.num_cell
+        out = np.zeros(nc + 1, dtype=np.int32)
+        _check(lib().mfc_get_cells(self._h, out.ctypes.data_as(C.POINTER(C.c_int32)), nc + 1), "mfc_get_cells")
+        return out
+
+    def tool(self) -> dict:
+        t = ToolState()
+        _check(lib().mfc_get_tool(self._h, C.byref(t)), "mfc_get_tool")
+        return t.as_arrays()
+
+
+def device_math(fn: int, x: np.ndarray, y: np.ndarray | None = None) -> np.ndarray:
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    out = np.zeros_like(x)
+    yp = None
+    if y is not None:
+        y = np.ascontiguousarray(y, dtype=np.float64)
+        yp = y.ctypes.data_as(C.POINTER(C.c_double))
+    _check(lib().mfc_device_math(fn, x.ctypes.data_as(C.POINTER(C.c_double)), yp,
+                                 out.ctypes.data_as(C.POINTER(C.c_double)), x.size), "mfc_device_math")
+    return out
+
+
+def scenario(cfg: Config) -> dict:
+    """Host-side setup of a configuration (no device work): initial particles and tool."""
+    info = Info()
+    _check(lib().mfc_scenario(C.byref(cfg), C.byref(info), None, None, None, None, None, None), "mfc_scenario")
+    N = info.n_particles
+    pos = np.zeros((N, 2)); rho = np.zeros(N); h = np.zeros(N); T = np.zeros(N)
+    flags = np.zeros(N, dtype=np.int32)
+    t = ToolState()
+    dp = lambda a: a.ctypes.data_as(C.POINTER(C.c_double))
+    _check(lib().mfc_scenario(C.byref(cfg), C.byref(info), dp(pos), dp(rho), dp(h), dp(T),
+                              flags.ctypes.data_as(C.POINTER(C.c_int32)), C.byref(t)), "mfc_scenario")
+    return {"info": SimInfo(*[getattr(info, n) for n, _ in Info._fields_]), "pos": pos, "rho": rho, "h": h,
+            "T": T, "flags": flags, "tool": t.as_arrays()}
