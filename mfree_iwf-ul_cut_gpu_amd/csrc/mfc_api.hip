@@ -56,7 +56,8 @@ struct mfc_sim {
   mfc::dstate set[2];
   mfc::dbins bins;
   mfc::dpositional pos;
-  double *rec = nullptr;
+  mfc::drecords recs = {false, nullptr, nullptr, nullptr};
+  int precision = MFC_PREC_FP64;
   mfc_tool_state *tool = nullptr;  // two states, alternating each step
   double *partials = nullptr, *force_last = nullptr, *hist = nullptr;
   int64_t *hist_ctr = nullptr;
@@ -96,8 +97,8 @@ int alloc_state(mfc_sim *s, mfc::dstate &d, int N) {
 void enqueue_step(mfc_sim *s, int c) {
   const int n = 1 - c;
   mfc::launch_binning(s->k, s->set[c], s->bins, s->stream);
-  mfc::launch_predict(s->k, s->set[c], s->set[n], s->bins, s->pos, s->rec, s->tool + c, s->tool + n, s->stream);
-  mfc::launch_interact(s->k, s->set[n], s->bins, s->pos, s->rec, s->tool + n, s->partials, s->stream);
+  mfc::launch_predict(s->k, s->set[c], s->set[n], s->bins, s->pos, s->recs, s->tool + c, s->tool + n, s->stream);
+  mfc::launch_interact(s->k, s->set[n], s->bins, s->pos, s->recs, s->tool + n, s->partials, s->stream);
   mfc::launch_force_final(s->ntiles, s->partials, s->force_last, s->hist, HIST_CAP, s->hist_ctr, s->stream);
 }
 
@@ -146,7 +147,6 @@ void mfc_destroy(mfc_sim *s) {
 int mfc_create(const mfc_config *cfg, mfc_sim **out) {
   if (!cfg || !out) return fail(MFC_E_INVALID, "null argument");
   *out = nullptr;
-  if (cfg->precision != MFC_PREC_FP64) return fail(MFC_E_INVALID, "only MFC_PREC_FP64 is available in this build");
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(MFC_E_HIP, "no HIP device");
   mfc_sim *s = new mfc_sim();
@@ -170,6 +170,8 @@ int mfc_create(const mfc_config *cfg, mfc_sim **out) {
   k.pse_quirk = c.pse_quirk;
   const int N = c.N;
   s->ntiles = (N + mfc::TILE - 1) / mfc::TILE;
+  s->precision = cfg->precision;
+  s->recs.f32 = cfg->precision == MFC_PREC_FP32C;
 
   auto bail = [&](int code) { mfc_destroy(s); return code; };
   if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess) return bail(fail(MFC_E_HIP, "stream"));
@@ -185,7 +187,12 @@ int mfc_create(const mfc_config *cfg, mfc_sim **out) {
   if ((rc = s->alloc(&p.nx, N)) || (rc = s->alloc(&p.ny, N)) || (rc = s->alloc(&p.p, N)) ||
       (rc = s->alloc(&p.rxx, N)) || (rc = s->alloc(&p.rxy, N)) || (rc = s->alloc(&p.ryy, N)))
     return bail(rc);
-  if ((rc = s->alloc(&s->rec, (size_t)mfc::REC * N)) || (rc = s->alloc(&s->tool, 2)) ||
+  if (s->recs.f32) {
+    if ((rc = s->alloc(&s->recs.posd, N)) || (rc = s->alloc(&s->recs.rec32, (size_t)16 * N))) return bail(rc);
+  } else if ((rc = s->alloc(&s->recs.rec, (size_t)mfc::REC * N))) {
+    return bail(rc);
+  }
+  if ((rc = s->alloc(&s->tool, 2)) ||
       (rc = s->alloc(&s->partials, (size_t)4 * s->ntiles)) || (rc = s->alloc(&s->force_last, 2)) ||
       (rc = s->alloc(&s->hist, (size_t)2 * HIST_CAP)) || (rc = s->alloc(&s->hist_ctr, 1)))
     return bail(rc);
@@ -222,7 +229,9 @@ int mfc_create(const mfc_config *cfg, mfc_sim **out) {
         hipMemsetAsync(b.clamped, 0, sizeof(unsigned long long), s->stream) != hipSuccess ||
         hipMemsetAsync(s->hist_ctr, 0, sizeof(int64_t), s->stream) != hipSuccess ||
         hipMemsetAsync(s->force_last, 0, 2 * sizeof(double), s->stream) != hipSuccess ||
-        hipMemsetAsync(s->rec, 0, sizeof(double) * mfc::REC * (size_t)N, s->stream) != hipSuccess)
+        (s->recs.rec && hipMemsetAsync(s->recs.rec, 0, sizeof(double) * mfc::REC * (size_t)N, s->stream) != hipSuccess) ||
+        (s->recs.posd && hipMemsetAsync(s->recs.posd, 0, sizeof(double2) * (size_t)N, s->stream) != hipSuccess) ||
+        (s->recs.rec32 && hipMemsetAsync(s->recs.rec32, 0, sizeof(float) * 16 * (size_t)N, s->stream) != hipSuccess))
       return bail(fail(MFC_E_HIP, "memset"));
     mfc_tool_state t2[2] = {S.tool, S.tool};
     if (up(s->tool, t2, sizeof(t2)) != hipSuccess) return bail(fail(MFC_E_HIP, "upload tool"));
@@ -267,9 +276,9 @@ int mfc_profile(mfc_sim *s, int nsteps, double *ms) {
     (void)hipEventRecord(ev[0], s->stream);
     mfc::launch_binning(s->k, s->set[c], s->bins, s->stream);
     (void)hipEventRecord(ev[1], s->stream);
-    mfc::launch_predict(s->k, s->set[c], s->set[n], s->bins, s->pos, s->rec, s->tool + c, s->tool + n, s->stream);
+    mfc::launch_predict(s->k, s->set[c], s->set[n], s->bins, s->pos, s->recs, s->tool + c, s->tool + n, s->stream);
     (void)hipEventRecord(ev[2], s->stream);
-    mfc::launch_interact(s->k, s->set[n], s->bins, s->pos, s->rec, s->tool + n, s->partials, s->stream);
+    mfc::launch_interact(s->k, s->set[n], s->bins, s->pos, s->recs, s->tool + n, s->partials, s->stream);
     (void)hipEventRecord(ev[3], s->stream);
     mfc::launch_force_final(s->ntiles, s->partials, s->force_last, s->hist, HIST_CAP, s->hist_ctr, s->stream);
     (void)hipEventRecord(ev[4], s->stream);
@@ -330,7 +339,7 @@ int mfc_get_info(const mfc_sim *s, mfc_info *info) {
   std::memset(info, 0, sizeof(*info));
   info->n_particles = c.N; info->n_tool = c.n_tool;
   info->grid_nx = c.gnx; info->grid_ny = c.gny; info->num_cell = c.num_cell;
-  info->precision = MFC_PREC_FP64;
+  info->precision = s->precision;
   info->dt = c.dt; info->t_final = c.t_final; info->mass = c.mass;
   info->bbmin_x = c.bbmin_x; info->bbmin_y = c.bbmin_y; info->cell_size = c.cell;
   info->step = s->step;

@@ -275,7 +275,9 @@ __device__ inline bool contact_force(const dev_consts &k, const mfc_tool_state &
 // (tool.cpp:211-221, geometric_primitives.cpp:70-85)
 __device__ inline void tool_move(mfc_tool_state &t, double dt) {
   double sx = dt * t.vel[0], sy = dt * t.vel[1];
-  for (int i = 0; i < t.num_seg; i++) {
+#pragma unroll
+  for (int i = 0; i < 5; i++) {
+    if (i >= t.num_seg) break;
     double lx = t.left[i][0] + sx, ly = t.left[i][1] + sy;
     double rx = t.right[i][0] + sx, ry = t.right[i][1] + sy;
     t.left[i][0] = lx; t.left[i][1] = ly; t.right[i][0] = rx; t.right[i][1] = ry;

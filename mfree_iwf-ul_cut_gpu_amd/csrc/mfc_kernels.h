@@ -39,15 +39,25 @@ struct dpositional {
 
 constexpr int REC = 16;  // doubles per neighbour record
 
+// neighbour records written by the predict kernel and read by the interaction kernel:
+// FP64 path: rec, 16 doubles per particle; FP32C path: posd (FP64 predicted position) and
+// rec32, 16 floats per particle
+struct drecords {
+  bool f32;
+  double *rec;
+  double2 *posd;
+  float *rec32;
+};
+
 // internal flag: some value of the particle's neighbour record is NaN or Inf
 constexpr int FLAG_NONFINITE = 8;
 
 // step kernels
 void launch_binning(const dev_consts &k, const dstate &cur, dbins &b, hipStream_t s);
 void launch_predict(const dev_consts &k, const dstate &cur, const dstate &nxt, const dbins &b, const dpositional &pos,
-                    double *rec, const mfc_tool_state *tool_in, mfc_tool_state *tool_out, hipStream_t s);
-void launch_interact(const dev_consts &k, const dstate &st, const dbins &b, const dpositional &pos, const double *rec,
-                     const mfc_tool_state *tool_now, double *partials, hipStream_t s);
+                    const drecords &r, const mfc_tool_state *tool_in, mfc_tool_state *tool_out, hipStream_t s);
+void launch_interact(const dev_consts &k, const dstate &st, const dbins &b, const dpositional &pos,
+                     const drecords &r, const mfc_tool_state *tool_now, double *partials, hipStream_t s);
 void launch_force_final(int ntiles, const double *partials, double *force_last, double *hist, int64_t cap,
                         int64_t *ctr, hipStream_t s);
 

@@ -158,9 +158,33 @@ __global__ __launch_bounds__(256) void k_rank(int N, const int *__restrict__ key
 }
 
 // ------------------------------------------------------------------------------- predict
-// gather into sorted order, leap-frog init + predict, EOS, artificial stress, neighbour record
+// artificial stress in single precision without trigonometry: with d = sqrt((sxx-syy)^2 + 4 sxy^2),
+// cos 2theta = (sxx-syy)/d and sin 2theta = 2 sxy/d for theta = atan2(2 sxy, sxx-syy)/2, and the
+// rotated normal stresses are the principal stresses (sxx+syy)/2 +- d/2 (actions_gpu.cu:53-97)
+__device__ inline void artificial_stress_f32(float rho, float p, float Sxx, float Sxy, float Syy, float eps, float &Rxx,
+                                             float &Rxy, float &Ryy) {
+  const float sxx = Sxx - p, syy = Syy - p, sxy = Sxy;
+  const float r21 = 1.0f / (rho * rho);
+  const float a = sxx - syy, b = 2.0f * sxy;
+  const float d = sqrtf(a * a + b * b);
+  float c2 = 1.0f, s2 = 0.0f;
+  if (d > 0.0f) { c2 = a / d; s2 = b / d; }
+  const float ct2 = 0.5f * (1.0f + c2), st2 = 0.5f * (1.0f - c2), cs = 0.5f * s2;
+  const float mid = 0.5f * (sxx + syy);
+  const float r1 = mid + 0.5f * d, r2 = mid - 0.5f * d;
+  const float q1 = r1 > 0.0f ? -eps * r1 * r21 : 0.0f;
+  const float q2 = r2 > 0.0f ? -eps * r2 * r21 : 0.0f;
+  Rxx = ct2 * q1 + st2 * q2;
+  Rxy = cs * (q1 - q2);
+  Ryy = st2 * q1 + ct2 * q2;
+}
+
+// gather into sorted order, leap-frog init + predict, EOS, artificial stress, neighbour record.
+// F32: the record is 16 floats (64 bytes) plus the FP64 predicted position.
+template <bool F32>
 __global__ __launch_bounds__(TILE) void k_predict(dev_consts k, int ntiles, const int *__restrict__ perm, dstate cur,
                                                   dstate nxt, dpositional pos, double *__restrict__ rec,
+                                                  double2 *__restrict__ posd, float *__restrict__ rec32,
                                                   const mfc_tool_state *__restrict__ tool_in,
                                                   mfc_tool_state *__restrict__ tool_out) {
   const int t = xcd_tile(blockIdx.x, ntiles);
@@ -202,10 +226,29 @@ __global__ __launch_bounds__(TILE) void k_predict(dev_consts k, int ntiles, cons
     T = T0 + hdt * Tt;
   }
   double p = pos.p[i], rxx = pos.rxx[i], rxy = pos.rxy[i], ryy = pos.ryy[i];
-  if (!(fl & MFC_FLAG_TOOL)) {
-    p = k.c0 * k.c0 * (rho - k.rho0);
-    artificial_stress(rho, p, sxx, sxy, syy, k.stress_eps, rxx, rxy, ryy);
-    pos.p[i] = p; pos.rxx[i] = rxx; pos.rxy[i] = rxy; pos.ryy[i] = ryy;
+  if constexpr (F32) {
+    float fxx = (float)rxx, fxy = (float)rxy, fyy = (float)ryy;
+    if (!(fl & MFC_FLAG_TOOL)) {
+      p = k.c0 * k.c0 * (rho - k.rho0);
+      artificial_stress_f32((float)rho, (float)p, (float)sxx, (float)sxy, (float)syy, (float)k.stress_eps, fxx, fxy, fyy);
+      pos.p[i] = p; pos.rxx[i] = fxx; pos.rxy[i] = fxy; pos.ryy[i] = fyy;
+    }
+    const float rf = (float)rho, pf = (float)p;
+    const float r21 = 1.0f / (rf * rf);
+    float4 q0 = make_float4((float)u, (float)v, (float)h, rf);
+    float4 q1 = make_float4((float)(k.mass / rho), ((float)sxx - pf) * r21, (float)sxy * r21, ((float)syy - pf) * r21);
+    float4 q2 = make_float4(fxx, fxy, fyy, sqrtf((float)(k.K / rho)));
+    float4 q3 = make_float4((float)T, __int_as_float(fl), 0.0f, 0.0f);
+    float4 *dst = reinterpret_cast<float4 *>(rec32 + (size_t)16 * i);
+    dst[0] = q0; dst[1] = q1; dst[2] = q2; dst[3] = q3;
+    posd[i] = make_double2(x, y);
+    return;
+  } else {
+    if (!(fl & MFC_FLAG_TOOL)) {
+      p = k.c0 * k.c0 * (rho - k.rho0);
+      artificial_stress(rho, p, sxx, sxy, syy, k.stress_eps, rxx, rxy, ryy);
+      pos.p[i] = p; pos.rxx[i] = rxx; pos.rxy[i] = rxy; pos.ryy[i] = ryy;
+    }
   }
   const double r21 = 1. / (rho * rho);
   double r[REC];
@@ -308,7 +351,173 @@ __device__ inline void pair_terms(const dev_consts &k, pair_acc &a, double xi, d
   }
 }
 
+// FP64 neighbour sums of particle i: the reference's loop over the 5 x 5 cell window
+// (interactions_gpu.cu:219-331), walked as 5 contiguous column ranges of the sorted order
+__device__ inline void sums_f64(const dev_consts &k, int i, const double *__restrict__ rec,
+                                const int *__restrict__ key_sorted, const int *__restrict__ offsets, bool tool,
+                                double ta, bool run_mech, bool run_pse, pair_acc &acc, double &tpse) {
+  const double2 *r2 = reinterpret_cast<const double2 *>(rec + (size_t)REC * i);
+  double2 a0 = r2[0], a1 = r2[1], a2 = r2[2], a3 = r2[3], a4 = r2[4], a5 = r2[5], a6 = r2[6], a7 = r2[7];
+  const double xi = a0.x, yi = a0.y;
+  const int rfl = double_to_bits(a1.x);
+  const double hi = a1.y, ui = a2.x, vi_ = a2.y, rhoi = a3.x;
+  const double Axx_i = a4.x, Axy_i = a4.y, Ayy_i = a5.x, Rxx_i = a5.y, Rxy_i = a6.x, Ryy_i = a6.y;
+  const double ci = a7.x, Tpred = a7.y;
+  const bool nonfin_i = rfl & FLAG_NONFINITE;
+  const double Ti = (ta != 0.) ? Tpred : 0.;
+  const double h1 = 1 / hi;
+  const double fourh2 = 4 * hi * hi;
+  const double fac = PI_FAC * h1 * h1;
+  const double h2 = hi * hi, h4 = h2 * h2;
+  const double pse_c = 4. / (h4 * M_PI);
+  const int hsh = key_sorted[i];
+  const int gi = hsh / k.gny, gj = hsh - gi * k.gny;
+  const int li = gi - 2 < 0 ? 0 : gi - 2;
+  const int lj = gj - 2 < 0 ? 0 : gj - 2;
+  const int hi_ = gi + 3 > k.gnx ? k.gnx : gi + 3;
+  const int hj = gj + 3 > k.gny ? k.gny : gj + 3;
+  for (int ii = li; ii < hi_; ii++) {
+    const int jb = offsets[ii * k.gny + lj], je = offsets[ii * k.gny + hj];
+    for (int j = jb; j < je; j++) {
+      const double *rj = rec + (size_t)REC * j;
+      const double2 b0 = reinterpret_cast<const double2 *>(rj)[0];
+      const double2 b1 = reinterpret_cast<const double2 *>(rj)[1];
+      const double xj = b0.x, yj = b0.y;
+      const int fj = double_to_bits(b1.x);
+      if (run_pse) {  // do_interactions_heat_pse (interactions_gpu.cu:130-139), all cell members
+        const double2 b7 = reinterpret_cast<const double2 *>(rj)[7];
+        const double2 b3 = reinterpret_cast<const double2 *>(rj)[3];
+        const double dx = xi - xj, dy = yi - yj;
+        const double xx = sqrt(dx * dx + dy * dy);
+        const double w2 = pse_c * mfcm::exp(-xx * xx / (h2));
+        tpse += (b7.y - Tpred) * w2 * k.mass / b3.x;
+      }
+      if (!run_mech) continue;
+      if (fj & MFC_FLAG_BLANKED) continue;
+      const double dx = xi - xj, dy = yi - yj;
+      const double rr2 = dx * dx + dy * dy;
+      // outside the support the reference adds exact zeros, unless a value is non-finite
+      if ((rr2 >= fourh2 || rr2 < 1e-8) && !nonfin_i && !(fj & FLAG_NONFINITE)) continue;
+      pair_terms(k, acc, xi, yi, ui, vi_, hi, rhoi, Axx_i, Axy_i, Ayy_i, Rxx_i, Rxy_i, Ryy_i, ci, Ti, tool, ta, h1,
+                 fourh2, fac, rj, xj, yj, (fj & MFC_FLAG_TOOL) != 0);
+    }
+  }
+}
+
+// compensated (Kahan) accumulation in single precision
+struct kahan {
+  float s = 0.0f, c = 0.0f;
+  __device__ inline void add(float x) {
+    const float y = x - c;
+    const float t = s + y;
+    c = (t - s) - y;
+    s = t;
+  }
+  __device__ inline double value() const { return (double)s - (double)c; }
+};
+
+// FP32 neighbour sums with Kahan compensation (the same pair terms as pair_terms, rearranged
+// for single precision: reciprocal square roots instead of divisions, the particle mass and the
+// factor 2 of Brookshaw taken out of the sums, and e.gradW/r of Brookshaw written as der*fac,
+// which is its exact value). Pair distances are formed in FP64 and then rounded.
+__device__ inline void sums_f32(const dev_consts &k, int i, const double2 *__restrict__ posd,
+                                const float *__restrict__ rec32, const int *__restrict__ key_sorted,
+                                const int *__restrict__ offsets, bool tool, double ta_d, bool run_mech, bool run_pse,
+                                pair_acc &acc, double &tpse) {
+  const float4 *ri = reinterpret_cast<const float4 *>(rec32 + (size_t)16 * i);
+  const float4 a0 = ri[0], a1 = ri[1], a2 = ri[2], a3 = ri[3];
+  const double2 pi = posd[i];
+  const float ui = a0.x, vi_ = a0.y, hi = a0.z, rhoi = a0.w;
+  const float Axx_i = a1.y, Axy_i = a1.z, Ayy_i = a1.w, Rxx_i = a2.x, Rxy_i = a2.y, Ryy_i = a2.z, ci = a2.w;
+  const float Tpred = a3.x;
+  const float ta = (float)ta_d;
+  const float Ti = ta != 0.0f ? Tpred : 0.0f;
+  const float h1 = 1.0f / hi;
+  const float fourh2 = 4.0f * hi * hi;
+  const float fac = (float)PI_FAC * h1 * h1;
+  const float inv_wd = (float)(1.0 / k.wdeltap);
+  const float eta2 = (float)(k.av_eta * k.av_eta);
+  const float alpha = (float)k.av_alpha, beta = (float)k.av_beta, xeps = (float)k.xsph_eps;
+  const float inv_h2 = h1 * h1;
+  const float pse_c = (float)(4. / M_PI) * inv_h2 * inv_h2;
+  const float massf = (float)k.mass;
+  kahan vd0, vd1, vd2, vd3, sd0, sd1, sd2, sd3, vt0, vt1, xt0, xt1, tl, tp;
+  const int hsh = key_sorted[i];
+  const int gi = hsh / k.gny, gj = hsh - gi * k.gny;
+  const int li = gi - 2 < 0 ? 0 : gi - 2;
+  const int lj = gj - 2 < 0 ? 0 : gj - 2;
+  const int hi_ = gi + 3 > k.gnx ? k.gnx : gi + 3;
+  const int hj = gj + 3 > k.gny ? k.gny : gj + 3;
+  for (int ii = li; ii < hi_; ii++) {
+    const int jb = offsets[ii * k.gny + lj], je = offsets[ii * k.gny + hj];
+    for (int j = jb; j < je; j++) {
+      const double2 pj = posd[j];
+      const float dx = (float)(pi.x - pj.x), dy = (float)(pi.y - pj.y);
+      const float rr2 = __builtin_fmaf(dx, dx, dy * dy);
+      const float4 *rj = reinterpret_cast<const float4 *>(rec32 + (size_t)16 * j);
+      if (run_pse) {
+        const float4 b0 = rj[0], b3 = rj[3];
+        const float w2 = pse_c * __expf(-rr2 * inv_h2);
+        tp.add((b3.x - Tpred) * w2 * (massf / b0.w));
+      }
+      if (!run_mech) continue;
+      if (rr2 >= fourh2 || rr2 < 1e-8f) continue;
+      const float4 b0 = rj[0], b1 = rj[1], b2 = rj[2], b3 = rj[3];
+      const int fj = __float_as_int(b3.y);
+      if (fj & MFC_FLAG_BLANKED) continue;
+      const float uj = b0.x, vj = b0.y, hj_ = b0.z, rhoj = b0.w, volj = b1.x;
+      const float inv_r = __builtin_amdgcn_rsqf(rr2);
+      const float q = rr2 * inv_r * h1;
+      float val, der;  // der = dW/dr / r without the factor fac
+      if (q > 1.0f) {
+        const float m = 2.0f - q, m2 = m * m;
+        val = 0.25f * m2 * m2 * m2;
+        der = -0.75f * m2 * h1 * inv_r;
+      } else {
+        val = 1.0f - 1.5f * q * q * (1.0f - 0.5f * q);
+        der = -3.0f * q * (1.0f - 0.75f * q) * h1 * inv_r;
+      }
+      const float w = val * fac, gr = der * fac;
+      const float w_x = gr * dx, w_y = gr * dy;
+      if (!tool && !(fj & MFC_FLAG_TOOL)) {
+        const float tx = w_x * volj, ty = w_y * volj;
+        const float du = uj - ui, dv = vj - vi_;
+        vd0.add(du * tx); vd1.add(du * ty); vd2.add(dv * tx); vd3.add(dv * ty);
+        float fab = w * inv_wd;
+        fab *= fab;
+        fab *= fab;
+        const float sx = Axx_i + b1.y + fab * (Rxx_i + b2.x);
+        const float sy = Axy_i + b1.z + fab * (Rxy_i + b2.y);
+        const float sz = Ayy_i + b1.w + fab * (Ryy_i + b2.z);
+        sd0.add(sx * w_x); sd1.add(sy * w_y); sd2.add(sy * w_x); sd3.add(sz * w_y);
+        const float vijx = -du, vijy = -dv;
+        const float vdx = __builtin_fmaf(vijx, dx, vijy * dy);
+        const float inv_rhoij = 2.0f / (rhoi + rhoj);
+        if (vdx < 0.0f) {
+          const float cij = 0.5f * (ci + b2.w);
+          const float hij = 0.5f * (hi + hj_);
+          const float muij = hij * vdx / __builtin_fmaf(eta2 * hij, hij, rr2);
+          const float piij = (-alpha * cij * muij + beta * muij * muij) * inv_rhoij;
+          vt0.add(-piij * w_x); vt1.add(-piij * w_y);
+        }
+        const float xs = -xeps * w * inv_rhoij;
+        xt0.add(xs * vijx); xt1.add(xs * vijy);
+      }
+      if (ta != 0.0f) tl.add(volj * (Ti - b3.x) * gr);
+    }
+  }
+  acc.vd0 = vd0.value(); acc.vd1 = vd1.value(); acc.vd2 = vd2.value(); acc.vd3 = vd3.value();
+  acc.sd0 = k.mass * sd0.value(); acc.sd1 = k.mass * sd1.value();
+  acc.sd2 = k.mass * sd2.value(); acc.sd3 = k.mass * sd3.value();
+  acc.vt0 = k.mass * vt0.value(); acc.vt1 = k.mass * vt1.value();
+  acc.xt0 = k.mass * xt0.value(); acc.xt1 = k.mass * xt1.value();
+  acc.tl = 2.0 * tl.value();
+  tpse = tp.value();
+}
+
+template <bool F32>
 __global__ __launch_bounds__(TILE) void k_interact(dev_consts k, int ntiles, const double *__restrict__ rec,
+                                                   const double2 *__restrict__ posd, const float *__restrict__ rec32,
                                                    const int *__restrict__ key_sorted,
                                                    const int *__restrict__ offsets, dstate st, dpositional pos,
                                                    const mfc_tool_state *__restrict__ tool_now,
@@ -319,62 +528,16 @@ __global__ __launch_bounds__(TILE) void k_interact(dev_consts k, int ntiles, con
   double fsum0 = 0., fsum1 = 0., fsum2 = 0., fsum3 = 0.;
   if (i < k.N) {
     const bool pse = k.pse != 0;
-    const double *ri = rec + (size_t)REC * i;
-    const double2 *r2 = reinterpret_cast<const double2 *>(ri);
-    double2 a0 = r2[0], a1 = r2[1], a2 = r2[2], a3 = r2[3], a4 = r2[4], a5 = r2[5], a6 = r2[6], a7 = r2[7];
-    const double xi = a0.x, yi = a0.y;
-    const int rfl = double_to_bits(a1.x);
-    const double hi = a1.y, ui = a2.x, vi_ = a2.y, rhoi = a3.x;
-    const double Axx_i = a4.x, Axy_i = a4.y, Ayy_i = a5.x, Rxx_i = a5.y, Rxy_i = a6.x, Ryy_i = a6.y;
-    const double ci = a7.x, Tpred = a7.y;
     int fl = st.flags[i];
     const bool blank = fl & MFC_FLAG_BLANKED, tool = fl & MFC_FLAG_TOOL, fixed = fl & MFC_FLAG_FIXED;
-    const bool nonfin_i = rfl & FLAG_NONFINITE;
-
     pair_acc acc = {0., 0., 0., 0., 0., 0., 0., 0., 0., 0., 0., 0., 0.};
     double tpse = 0.;
     const double ta = pse ? 0. : (tool ? k.alpha_tool : k.alpha_wp);
     const bool run_mech = !blank && (!pse || !tool);
     const bool run_pse = !blank && pse && k.thermal_wp;
     if (run_mech || run_pse) {
-      const double Ti = (ta != 0.) ? Tpred : 0.;
-      const double h1 = 1 / hi;
-      const double fourh2 = 4 * hi * hi;
-      const double fac = PI_FAC * h1 * h1;
-      const double h2 = hi * hi, h4 = h2 * h2;
-      const double pse_c = 4. / (h4 * M_PI);
-      const int hsh = key_sorted[i];
-      const int gi = hsh / k.gny, gj = hsh - gi * k.gny;
-      const int li = gi - 2 < 0 ? 0 : gi - 2;
-      const int lj = gj - 2 < 0 ? 0 : gj - 2;
-      const int hi_ = gi + 3 > k.gnx ? k.gnx : gi + 3;
-      const int hj = gj + 3 > k.gny ? k.gny : gj + 3;
-      for (int ii = li; ii < hi_; ii++) {
-        const int jb = offsets[ii * k.gny + lj], je = offsets[ii * k.gny + hj];
-        for (int j = jb; j < je; j++) {
-          const double *rj = rec + (size_t)REC * j;
-          const double2 b0 = reinterpret_cast<const double2 *>(rj)[0];
-          const double2 b1 = reinterpret_cast<const double2 *>(rj)[1];
-          const double xj = b0.x, yj = b0.y;
-          const int fj = double_to_bits(b1.x);
-          if (run_pse) {  // do_interactions_heat_pse (interactions_gpu.cu:130-139), all cell members
-            const double2 b7 = reinterpret_cast<const double2 *>(rj)[7];
-            const double2 b3 = reinterpret_cast<const double2 *>(rj)[3];
-            const double dx = xi - xj, dy = yi - yj;
-            const double xx = sqrt(dx * dx + dy * dy);
-            const double w2 = pse_c * mfcm::exp(-xx * xx / (h2));
-            tpse += (b7.y - Tpred) * w2 * k.mass / b3.x;
-          }
-          if (!run_mech) continue;
-          if (fj & MFC_FLAG_BLANKED) continue;
-          const double dx = xi - xj, dy = yi - yj;
-          const double rr2 = dx * dx + dy * dy;
-          // outside the support the reference adds exact zeros, unless a value is non-finite
-          if ((rr2 >= fourh2 || rr2 < 1e-8) && !nonfin_i && !(fj & FLAG_NONFINITE)) continue;
-          pair_terms(k, acc, xi, yi, ui, vi_, hi, rhoi, Axx_i, Axy_i, Ayy_i, Rxx_i, Rxy_i, Ryy_i, ci, Ti, tool, ta,
-                     h1, fourh2, fac, rj, xj, yj, (fj & MFC_FLAG_TOOL) != 0);
-        }
-      }
+      if constexpr (F32) sums_f32(k, i, posd, rec32, key_sorted, offsets, tool, ta, run_mech, run_pse, acc, tpse);
+      else sums_f64(k, i, rec, key_sorted, offsets, tool, ta, run_mech, run_pse, acc, tpse);
     }
     // state at the start of the step (sorted) and previous derivatives
     const double x0 = st.px[i], y0 = st.py[i], u0 = st.vx[i], v0 = st.vy[i];
@@ -551,17 +714,25 @@ void launch_binning(const dev_consts &k, const dstate &cur, dbins &b, hipStream_
 static int padded_tiles(int ntiles) { return ((ntiles + 7) / 8) * 8; }
 
 void launch_predict(const dev_consts &k, const dstate &cur, const dstate &nxt, const dbins &b, const dpositional &pos,
-                    double *rec, const mfc_tool_state *tool_in, mfc_tool_state *tool_out, hipStream_t s) {
+                    const drecords &r, const mfc_tool_state *tool_in, mfc_tool_state *tool_out, hipStream_t s) {
   const int nt = (k.N + TILE - 1) / TILE;
-  hipLaunchKernelGGL(k_predict, dim3(padded_tiles(nt)), dim3(TILE), 0, s, k, nt, b.perm, cur, nxt, pos, rec, tool_in,
-                     tool_out);
+  if (r.f32)
+    hipLaunchKernelGGL(k_predict<true>, dim3(padded_tiles(nt)), dim3(TILE), 0, s, k, nt, b.perm, cur, nxt, pos, r.rec,
+                       r.posd, r.rec32, tool_in, tool_out);
+  else
+    hipLaunchKernelGGL(k_predict<false>, dim3(padded_tiles(nt)), dim3(TILE), 0, s, k, nt, b.perm, cur, nxt, pos, r.rec,
+                       r.posd, r.rec32, tool_in, tool_out);
 }
 
-void launch_interact(const dev_consts &k, const dstate &st, const dbins &b, const dpositional &pos, const double *rec,
-                     const mfc_tool_state *tool_now, double *partials, hipStream_t s) {
+void launch_interact(const dev_consts &k, const dstate &st, const dbins &b, const dpositional &pos,
+                     const drecords &r, const mfc_tool_state *tool_now, double *partials, hipStream_t s) {
   const int nt = (k.N + TILE - 1) / TILE;
-  hipLaunchKernelGGL(k_interact, dim3(padded_tiles(nt)), dim3(TILE), 0, s, k, nt, rec, b.key_sorted, b.offsets, st,
-                     pos, tool_now, partials);
+  if (r.f32)
+    hipLaunchKernelGGL(k_interact<true>, dim3(padded_tiles(nt)), dim3(TILE), 0, s, k, nt, r.rec, r.posd, r.rec32,
+                       b.key_sorted, b.offsets, st, pos, tool_now, partials);
+  else
+    hipLaunchKernelGGL(k_interact<false>, dim3(padded_tiles(nt)), dim3(TILE), 0, s, k, nt, r.rec, r.posd, r.rec32,
+                       b.key_sorted, b.offsets, st, pos, tool_now, partials);
 }
 
 void launch_force_final(int ntiles, const double *partials, double *force_last, double *hist, int64_t cap,

@@ -127,3 +127,42 @@ def test_adiabatic_6k_full_run():
     bad = {f: r for f, r in res.items() if not r[0]}
     assert not bad, bad
     assert equal(g.force_history(), o.force_history())
+
+
+# ----------------------------------------------------------------------------- FP32C accuracy
+def fp32c_errors(cfg, steps):
+    """Run the FP32-compensated path and the FP64 oracle side by side; return error measures."""
+    c32 = cfg.copy()
+    c32.precision = pymfc.PREC_FP32C
+    g = pymfc.Sim(c32)
+    o = ob.OracleSim(cfg)
+    g.step(steps)
+    o.step(steps)
+    # compare per particle id (the FP32C run may sort ties differently once positions differ)
+    gi, oi = np.argsort(g.field("id")), np.argsort(o.field("id"))
+    dx = cfg.dx if cfg.dx > 0 else o.info.cell_size / cfg.hdx
+    out = {}
+    out["pos"] = np.abs(g.field("pos")[gi] - o.field("pos")[oi]).max() / dx
+    S_o = o.field("S")[oi]
+    out["S"] = np.abs(g.field("S")[gi] - S_o).max() / max(np.abs(S_o).max(), 1e-30)
+    out["T"] = np.abs(g.field("T")[gi] - o.field("T")[oi]).max()
+    out["Tpeak"] = abs(g.field("T").max() - o.field("T").max())
+    out["eps_pl"] = np.abs(g.field("eps_pl")[gi] - o.field("eps_pl")[oi]).max()
+    Fg, Fo = g.force_history(), o.force_history()
+    w = max(1, steps // 2)
+    mg, mo = Fg[-w:].mean(axis=0), Fo[-w:].mean(axis=0)
+    out["F_mean_rel"] = np.abs(mg - mo) / np.maximum(np.abs(mo), 1e-30)
+    out["blanked"] = int(np.count_nonzero(g.field("flags") & pymfc.FLAG_BLANKED))
+    return out
+
+
+@pytest.mark.parametrize("thermal", ["adiabatic", "contact", "pse"])
+def test_fp32c_tracks_the_fp64_oracle(thermal):
+    cfg = small_cut(thermal=thermal)
+    e = fp32c_errors(cfg, 400)
+    print("fp32c errors", thermal, e)
+    assert e["blanked"] == 0
+    assert e["pos"] < 1e-3          # positions within 1e-3 dx
+    assert e["S"] < 2e-3            # stress within 0.2 % of the largest stress
+    assert e["Tpeak"] < 1.0 and e["T"] < 1.0
+    assert np.all(e["F_mean_rel"][0] < 5e-3)   # cutting force (time mean over the second half) within 0.5 %
