@@ -94,7 +94,11 @@ typedef struct mfc_config {
   int32_t precision;      /* MFC_PREC_* */
   double cfl;             /* dt = cfl h / (c0 + sqrt(speed)) (benchmarks.cpp:530 uses 0.5) */
   int32_t pse_flag_quirk; /* 1: PSE tool flag reads 0 for index >= N/2 (interactions_gpu.cu:405) */
-  int32_t reserved[7];
+  int32_t promote_fp64;   /* 1: evaluate the reference's single-precision truncations of its FP64
+                             build (sqrtf, fmaxf, atan2f/cosf/sinf) in FP64; 0: as the reference */
+  double secant_tol;      /* absolute tolerance of the radial-return secant; 0 selects the
+                             reference's 1e-6 (actions_gpu.cu:261) */
+  int32_t reserved[6];
 } mfc_config;
 
 typedef struct mfc_info {

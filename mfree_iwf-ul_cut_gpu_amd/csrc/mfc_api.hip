@@ -168,6 +168,8 @@ int mfc_create(const mfc_config *cfg, mfc_sim **out) {
   k.contact_alpha = c.contact_alpha; k.mu = c.mu; k.tool_vx = c.tool_vx; k.tool_vy = c.tool_vy;
   k.thermal_bc = c.thermal_bc; k.fric_heat = c.fric_heat; k.thermal_wp = c.thermal_wp; k.pse = c.pse;
   k.pse_quirk = c.pse_quirk;
+  k.promote = c.promote;
+  k.secant_tol = c.secant_tol;
   const int N = c.N;
   s->ntiles = (N + mfc::TILE - 1) / mfc::TILE;
   s->precision = cfg->precision;

@@ -28,9 +28,13 @@ struct dev_consts {
   double jc_A, jc_B, jc_C, jc_n, jc_m, jc_Tmelt, jc_Tref, jc_eps_dot_ref;
   double contact_alpha, mu, tool_vx, tool_vy;
   int thermal_bc, fric_heat, thermal_wp, pse, pse_quirk;
+  int promote;        // reference float truncations evaluated in FP64
+  double secant_tol;  // absolute stop tolerance of the radial-return secant
 };
 
 __device__ inline double f32sqrt(double x) { return (double)sqrtf((float)x); }
+// sqrtf of the reference's FP64 build, or the FP64 square root when its truncations are promoted
+__device__ inline double tsqrt(const dev_consts &k, double x) { return k.promote ? sqrt(x) : f32sqrt(x); }
 
 // Work-groups are dispatched round-robin over the 8 XCDs of MI355X; map block b so that
 // each XCD owns one contiguous range of tiles (its neighbour data then stays in that XCD's
@@ -62,11 +66,17 @@ __device__ inline int cell_of(const dev_consts &k, double x, double y, bool &cla
 
 // artificial stress R of Gray & Monaghan in the principal frame (actions_gpu.cu:35-97)
 __device__ inline void artificial_stress(double rhoi, double pi, double Sxx, double Sxy, double Syy, double eps,
-                                         double &Rxx, double &Rxy, double &Ryy) {
+                                         double &Rxx, double &Rxy, double &Ryy, bool promote) {
   double sxx = Sxx - pi, sxy = Sxy, syy = Syy - pi;
   double rhoi21 = 1. / (rhoi * rhoi);
-  double theta = 0.5 * (double)mfcm::atan2f(2. * sxy, sxx - syy + 0.);
-  double ct = (double)mfcm::cosf(theta), st = (double)mfcm::sinf(theta);
+  double theta, ct, st;
+  if (promote) {
+    theta = 0.5 * mfcm::atan2(2. * sxy, sxx - syy + 0.);
+    ct = mfcm::cos(theta); st = mfcm::sin(theta);
+  } else {
+    theta = 0.5 * (double)mfcm::atan2f(2. * sxy, sxx - syy + 0.);
+    ct = (double)mfcm::cosf(theta); st = (double)mfcm::sinf(theta);
+  }
   double ct2 = ct * ct, st2 = st * st;
   double rsxx = ct2 * sxx + 2.0 * ct * st * sxy + st2 * syy;
   double rsyy = st2 * sxx - 2.0 * ct * st * sxy + ct2 * syy;
@@ -118,7 +128,7 @@ __device__ inline double sigma_yield(const dev_consts &k, double eps_pl, double 
 
 // residual of the radial return (plasticity.cuh:74-83)
 __device__ inline double rr_residual(const dev_consts &k, double dl, double norm, double eps0, double t) {
-  const double s23 = 0.816496580927726;  // sqrt(2/3), correctly rounded
+  const double s23 = sqrt(2.0 / 3.0);
   double e = eps0 + s23 * fmax(dl, 0.);
   double ed = s23 * fmax(dl, 0.) / k.dt;
   double sY = sigma_yield(k, e, ed, t);
@@ -137,7 +147,7 @@ __device__ inline double rr_secant(const dev_consts &k, double init, double norm
     dl_old = dl;
     dl = dl - g / slope;
     if (dl < 0.) dl = -0.1 * dl;
-    if (fabs(dl - dl_old) < 1e-6) return dl;
+    if (fabs(dl - dl_old) < k.secant_tol) return dl;
     if (iter > 100) return dl;
     iter++;
   }
@@ -146,16 +156,18 @@ __device__ inline double rr_secant(const dev_consts &k, double init, double norm
 // Johnson-Cook plasticity by radial return on one particle (actions_gpu.cu:226-285)
 __device__ inline void plasticity(const dev_consts &k, double p, double rho, double &Sxx, double &Sxy, double &Syy,
                                   double &Szz, double &eps_pl, double &eps_pl_dot, double &T) {
-  double norm = f32sqrt(Sxx * Sxx + Syy * Syy + Szz * Szz + Sxy * Sxy);
+  double norm = tsqrt(k, Sxx * Sxx + Syy * Syy + Szz * Szz + Sxy * Sxy);
   double cxx = Sxx - p, cyy = Syy - p, czz = Szz - p, cxy = Sxy;
-  double svm = f32sqrt((cxx * cxx + cyy * cyy + czz * czz) - cxx * cyy - cxx * czz - cyy * czz + 3.0 * cxy * cxy);
+  double svm = tsqrt(k, (cxx * cxx + cyy * cyy + czz * czz) - cxx * cyy - cxx * czz - cyy * czz + 3.0 * cxy * cxy);
   double sY = sigma_yield(k, eps_pl, eps_pl_dot, T);
   if (svm < sY) { eps_pl_dot = 0.; return; }
-  const double s23 = 0.816496580927726;
+  const double s23 = sqrt(2.0 / 3.0);
   double dl = rr_secant(k, fmax(eps_pl_dot * k.dt * s23, 1e-8), norm, eps_pl, T);
-  float inc = sqrtf((float)(2.0 / 3.0)) * fmaxf((float)dl, 0.f);
-  double e_new = eps_pl + (double)inc;
-  double ed_new = (double)inc / k.dt;
+  double inc;
+  if (k.promote) inc = s23 * fmax(dl, 0.);
+  else inc = (double)(sqrtf((float)(2.0 / 3.0)) * fmaxf((float)dl, 0.f));
+  double e_new = eps_pl + inc;
+  double ed_new = inc / k.dt;
   double mu = k.G;
   double nxx = Sxx - Sxx / norm * dl * 2. * mu;
   double nyy = Syy - Syy / norm * dl * 2. * mu;

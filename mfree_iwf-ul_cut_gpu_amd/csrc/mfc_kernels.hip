@@ -246,7 +246,7 @@ __global__ __launch_bounds__(TILE) void k_predict(dev_consts k, int ntiles, cons
   } else {
     if (!(fl & MFC_FLAG_TOOL)) {
       p = k.c0 * k.c0 * (rho - k.rho0);
-      artificial_stress(rho, p, sxx, sxy, syy, k.stress_eps, rxx, rxy, ryy);
+      artificial_stress(rho, p, sxx, sxy, syy, k.stress_eps, rxx, rxy, ryy, k.promote != 0);
       pos.p[i] = p; pos.rxx[i] = rxx; pos.rxy[i] = rxy; pos.ryy[i] = ryy;
     }
   }
@@ -261,7 +261,7 @@ __global__ __launch_bounds__(TILE) void k_predict(dev_consts k, int ntiles, cons
   r[9] = sxy * r21;
   r[10] = (syy - p) * r21;
   r[11] = rxx; r[12] = rxy; r[13] = ryy;
-  r[14] = f32sqrt(k.K / rho);
+  r[14] = tsqrt(k, k.K / rho);
   r[15] = T;
   bool fin = true;
 #pragma unroll
@@ -300,7 +300,7 @@ __device__ inline void pair_terms(const dev_consts &k, pair_acc &a, double xi, d
   const double rr2 = xij * xij + yij * yij;
   if (!(rr2 >= fourh2 || rr2 < 1e-8)) {
     double der, val;
-    double rad = f32sqrt(rr2);
+    double rad = tsqrt(k, rr2);
     double q = rad * h1;
     if (q > 1) {
       double m = 2 - q, m2 = m * m;
@@ -586,13 +586,13 @@ __global__ __launch_bounds__(TILE) void k_interact(dev_consts k, int ntiles, con
     }
     // frictional heating with the positional normal of the previous step (actions_gpu.cu:135-162)
     if (k.fric_heat && !tool) {
-      const double fmag = f32sqrt(ftx * ftx + fty * fty);
+      const double fmag = tsqrt(k, ftx * ftx + fty * fty);
       if (fmag != 0.) {
         const double nx = pos.nx[i], ny = pos.ny[i];
         const double dvx = u - k.tool_vx, dvy = v - k.tool_vy;
         const double dvn = dvx * nx + dvy * ny;
         const double rx = dvx - dvn, ry = dvy - dvn;
-        const double vmag = f32sqrt(rx * rx + ry * ry);
+        const double vmag = tsqrt(k, rx * rx + ry * ry);
         T += k.eta_heat * k.dt * fmag * vmag / (k.cp * k.mass);
       }
     }

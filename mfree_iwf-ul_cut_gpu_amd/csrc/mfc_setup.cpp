@@ -228,7 +228,9 @@ int build_scenario(const mfc_config &c, scenario &S, std::string &err) {
   k.G = c.E / (2. * (1. + c.nu));
   k.K = 2.0 * k.G * (1 + c.nu) / (3 * (1 - 2 * c.nu));
   k.mass = dx * dx * c.rho0;
-  k.c0 = (double)sqrtf((float)(k.K / k.rho0));
+  k.c0 = c.promote_fp64 ? std::sqrt(k.K / k.rho0) : (double)sqrtf((float)(k.K / k.rho0));
+  k.promote = c.promote_fp64 != 0;
+  k.secant_tol = (c.secant_tol > 0.) ? c.secant_tol : 1e-6;
   k.jc_A = c.jc_A; k.jc_B = c.jc_B; k.jc_C = c.jc_C; k.jc_n = c.jc_n; k.jc_m = c.jc_m;
   k.jc_Tmelt = c.jc_Tmelt; k.jc_Tref = c.jc_Tref; k.jc_eps_dot_ref = c.jc_eps_dot_ref;
   k.cp = c.cp; k.tq = c.tq; k.eta_heat = c.eta; k.T_init = c.T_init;

@@ -28,6 +28,8 @@ struct sim_constants {
   double contact_alpha, mu, tool_vx, tool_vy;
   // switches derived from the constants (actions_gpu.cu:448-468, interactions_gpu.cu:443)
   int thermal_bc, fric_heat, thermal_wp, pse, pse_quirk;
+  int promote;        // float truncations of the reference evaluated in FP64
+  double secant_tol;  // radial-return secant tolerance
 };
 
 struct scenario {

@@ -57,7 +57,8 @@ class Config(C.Structure):
         ("feed", C.c_double), ("speed", C.c_double), ("mu", C.c_double), ("contact_alpha", C.c_double),
         ("sample_tool", C.c_int32), ("rho_tool", C.c_double), ("cp_tool", C.c_double), ("k_tool", C.c_double),
         ("thermal_solver", C.c_int32), ("precision", C.c_int32), ("cfl", C.c_double),
-        ("pse_flag_quirk", C.c_int32), ("reserved", C.c_int32 * 7),
+        ("pse_flag_quirk", C.c_int32), ("promote_fp64", C.c_int32), ("secant_tol", C.c_double),
+        ("reserved", C.c_int32 * 6),
     ]
 
     def copy(self) -> "Config":

@@ -59,6 +59,8 @@ inline double length(V2 a) { return std::sqrt(dot(a, a)); }
 inline V2 normalize(V2 a) { double is = 1.0 / std::sqrt(dot(a, a)); return v2(a.x * is, a.y * is); }
 
 inline double f32sqrt(double x) { return (double)sqrtf((float)x); }
+// sqrtf of the reference's FP64 build, or the FP64 square root when its truncations are promoted
+inline double tsqrt(double x, bool promote) { return promote ? std::sqrt(x) : f32sqrt(x); }
 
 // ---------------------------------------------------------------- geometric primitives
 // geometric_primitives.h:26-63, geometric_primitives.cpp:20-85,112-122
@@ -278,7 +280,7 @@ bool establish_contact(const tool_t &T, V2 qp, V2 &cp, V2 &n) {
 // ---------------------------------------------------------------- kernels.cuh
 constexpr double PI_FAC = 0.454728408833987;
 
-inline void cubic_spline(V2 pi, V2 pj, double hi, double w[3]) {  // kernels.cuh:30-70
+inline void cubic_spline(V2 pi, V2 pj, double hi, double w[3], bool promote = false) {  // kernels.cuh:30-70
   w[0] = w[1] = w[2] = 0.;
   double xij = pi.x - pj.x, yij = pi.y - pj.y;
   double rr2 = xij * xij + yij * yij;
@@ -286,7 +288,7 @@ inline void cubic_spline(V2 pi, V2 pj, double hi, double w[3]) {  // kernels.cuh
   double fourh2 = 4 * hi * hi;
   if (rr2 >= fourh2 || rr2 < 1e-8) return;
   double der, val;
-  double rad = f32sqrt(rr2);
+  double rad = tsqrt(rr2, promote);
   double q = rad * h1;
   double fac = PI_FAC * h1 * h1;
   if (q > 1) {
@@ -360,6 +362,8 @@ struct sim_t {
   double wdeltap, c_alpha, c_beta, c_eta, xspheps, stresseps;
   joco_t jc;
   bool thermal_bc = false, fric_heat = false, thermal_wp = false, pse = false;
+  bool promote = false;     // reference float truncations evaluated in FP64
+  double secant_tol = 1e-6; // actions_gpu.cu:261
   // grid (grid.cu:139-158)
   int gnx = 0, gny = 0, num_cell = 0;
   double bbmin_x = 0, bbmin_y = 0, gdx = 0;
@@ -529,6 +533,8 @@ int setup(sim_t &s, const mfc_config &c) {
   s.fric_heat = s.eta_heat != 0.;
   s.thermal_wp = s.alpha_wp != 0.;
   s.pse = c.thermal_solver == MFC_THERMAL_PSE;
+  s.promote = c.promote_fp64 != 0;
+  s.secant_tol = (c.secant_tol > 0.) ? c.secant_tol : 1e-6;
   s.dt = c.cfl * hdx * dx / (std::sqrt(s.K / s.rho0) + std::sqrt(vel_tool));
   s.t_final = 0.1 / vel_tool;
   // particle_gpu(pos, vel, rho, T, h, fixed, tool_p, N) (particle_gpu.cu:20-125)
@@ -649,7 +655,7 @@ void lf_correct(sim_t &s) {  // leap_frog.cu:68-92
 }
 
 void material_eos(sim_t &s) {  // actions_gpu.cu:41-51
-  double c0 = f32sqrt(s.K / s.rho0);
+  double c0 = tsqrt(s.K / s.rho0, s.promote);
 #pragma omp parallel for schedule(static)
   for (int i = 0; i < s.N; i++) {
     if (s.tool_p[i] != 0.) continue;
@@ -657,13 +663,19 @@ void material_eos(sim_t &s) {  // actions_gpu.cu:41-51
   }
 }
 
-inline V4 artificial_stress(double rhoi, double pi, V4 Si, double eps) {  // actions_gpu.cu:53-97
+inline V4 artificial_stress(double rhoi, double pi, V4 Si, double eps, bool promote = false) {  // actions_gpu.cu:53-97
   double sxx = Si.x, sxy = Si.y, syy = Si.z;
   sxx -= pi; syy -= pi;
   double rhoi21 = 1. / (rhoi * rhoi);
   double numer = 2. * sxy, denom = sxx - syy + 0.;
-  double theta = 0.5 * (double)mfcm::atan2f(numer, denom);
-  double ct = (double)mfcm::cosf(theta), st = (double)mfcm::sinf(theta);
+  double theta, ct, st;
+  if (promote) {
+    theta = 0.5 * mfcm::atan2(numer, denom);
+    ct = mfcm::cos(theta); st = mfcm::sin(theta);
+  } else {
+    theta = 0.5 * (double)mfcm::atan2f(numer, denom);
+    ct = (double)mfcm::cosf(theta); st = (double)mfcm::sinf(theta);
+  }
   double ct2 = ct * ct, st2 = st * st;
   double rsxx = ct2 * sxx + 2.0 * ct * st * sxy + st2 * syy;
   double rsyy = st2 * sxx - 2.0 * ct * st * sxy + ct2 * syy;
@@ -678,7 +690,7 @@ void corrector_artificial_stress(sim_t &s) {
 #pragma omp parallel for schedule(static)
   for (int i = 0; i < s.N; i++) {
     if (s.tool_p[i] != 0.) continue;
-    s.R[i] = artificial_stress(s.rho[i], s.p[i], s.S[i], s.stresseps);
+    s.R[i] = artificial_stress(s.rho[i], s.p[i], s.S[i], s.stresseps, s.promote);
   }
 }
 
@@ -729,7 +741,7 @@ void interactions_monaghan(sim_t &s) {  // interactions_gpu.cu:146-345
           double volj = mass / rhoj;
           double rhoj21 = 1. / (rhoj * rhoj);
           double ww[3];
-          cubic_spline(pi, pj, hi, ww);
+          cubic_spline(pi, pj, hi, ww, s.promote);
           double w = ww[0], w_x = ww[1], w_y = ww[2];
           if (!(tool_i == 1.) && !(tool_j == 1.)) {
             vd.x += (vj.x - vi.x) * w_x * volj;
@@ -749,7 +761,7 @@ void interactions_monaghan(sim_t &s) {  // interactions_gpu.cu:146-345
             double vijposij = vijx * xij + vijy * yij;
             double rhoij = 0.5 * (rhoi + rhoj);
             if (vijposij < 0.) {
-              double ci = f32sqrt(K / rhoi), cj = f32sqrt(K / rhoj);
+              double ci = tsqrt(K / rhoi, s.promote), cj = tsqrt(K / rhoj, s.promote);
               double cij = 0.5 * (ci + cj);
               double hij = 0.5 * (hi + hj_);
               double r2ij = xij * xij + yij * yij;
@@ -869,13 +881,13 @@ void fric_heat_gen(sim_t &s) {  // actions_gpu.cu:135-162
   for (int i = 0; i < s.N; i++) {
     if (s.tool_p[i] != 0.) continue;
     V2 fT = s.ft[i];
-    double fmag = f32sqrt(fT.x * fT.x + fT.y * fT.y);
+    double fmag = tsqrt(fT.x * fT.x + fT.y * fT.y, s.promote);
     if (fmag == 0.) continue;
     V2 nrm = s.n[i], vp = s.vel[i];
     V2 vd = v2(vp.x - vt.x, vp.y - vt.y);
     double vdn = vd.x * nrm.x + vd.y * nrm.y;
     V2 vr = v2(vd.x - vdn, vd.y - vdn);
-    double vmag = f32sqrt(vr.x * vr.x + vr.y * vr.y);
+    double vmag = tsqrt(vr.x * vr.x + vr.y * vr.y, s.promote);
     s.T[i] += s.eta_heat * dt * fmag * vmag / (s.cp * s.mass);
   }
 }
@@ -888,21 +900,27 @@ void plasticity_johnson_cook(sim_t &s, int64_t *iter_total) {  // actions_gpu.cu
     if (s.tool_p[i] != 0.) continue;
     V4 Sv = s.S[i];
     double Sxx = Sv.x, Syy = Sv.z, Szz = Sv.w, Sxy = Sv.y;
-    double norm_Strial = f32sqrt(Sxx * Sxx + Syy * Syy + Szz * Szz + Sxy * Sxy);
+    double norm_Strial = tsqrt(Sxx * Sxx + Syy * Syy + Szz * Szz + Sxy * Sxy, s.promote);
     double p = s.p[i];
     double cxx = Sxx - p, cyy = Syy - p, czz = Szz - p, cxy = Sxy;
     double eps_pl = s.eps_pl[i], eps_pl_dot = s.eps_pl_dot[i], T = s.T[i];
-    double svm = f32sqrt((cxx * cxx + cyy * cyy + czz * czz) - cxx * cyy - cxx * czz - cyy * czz + 3.0 * cxy * cxy);
+    double svm = tsqrt((cxx * cxx + cyy * cyy + czz * czz) - cxx * cyy - cxx * czz - cyy * czz + 3.0 * cxy * cxy,
+                       s.promote);
     double sigma_Y = sigma_yield(s.jc, eps_pl, eps_pl_dot, T);
     if (svm < sigma_Y) { s.eps_pl_dot[i] = 0.; continue; }
     int its = 0;
-    double dl = solve_secant(s.jc, std::fmax(eps_pl_dot * dt * std::sqrt(2. / 3.), 1e-8), 1e-6, norm_Strial,
+    double dl = solve_secant(s.jc, std::fmax(eps_pl_dot * dt * std::sqrt(2. / 3.), 1e-8), s.secant_tol, norm_Strial,
                              eps_pl, T, dt, s.G, &its);
     itot += its + 1;
-    float sq23 = sqrtf((float)(2.0 / 3.0));
-    float inc = sq23 * fmaxf((float)dl, 0.f);
-    double eps_pl_new = eps_pl + (double)inc;
-    double eps_pl_dot_new = (double)inc / dt;
+    double inc;
+    if (s.promote) {
+      inc = std::sqrt(2.0 / 3.0) * std::fmax(dl, 0.);
+    } else {
+      float sq23 = sqrtf((float)(2.0 / 3.0));
+      inc = (double)(sq23 * fmaxf((float)dl, 0.f));
+    }
+    double eps_pl_new = eps_pl + inc;
+    double eps_pl_dot_new = inc / dt;
     s.eps_pl[i] = eps_pl_new;
     s.eps_pl_dot[i] = eps_pl_dot_new;
     V4 Sn;

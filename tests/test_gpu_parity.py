@@ -68,9 +68,14 @@ def test_binning_jittered_lattice():
 
 
 @pytest.mark.parametrize("case", ["ref_cut_thermal_tool", "adiabatic_edge", "pse_jitter", "sharp_tool",
-                                  "contact_thermal"])
+                                  "contact_thermal", "promoted_tight_secant"])
 def test_step_parity(case):
-    if case == "ref_cut_thermal_tool":
+    if case == "promoted_tight_secant":
+        cfg = small_cut(thermal="contact")
+        cfg.promote_fp64 = 1
+        cfg.secant_tol = 1e-13
+        steps = 200
+    elif case == "ref_cut_thermal_tool":
         cfg = pymfc.config_ref_cut(ny=12)
         steps = 40
     elif case == "adiabatic_edge":
