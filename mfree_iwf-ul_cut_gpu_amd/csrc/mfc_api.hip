@@ -155,21 +155,7 @@ int mfc_create(const mfc_config *cfg, mfc_sim **out) {
   if (rc) { delete s; return fail(rc, err); }
   const mfc::sim_constants &c = s->sc.k;
   mfc::dev_consts &k = s->k;
-  std::memset(&k, 0, sizeof(k));
-  k.N = c.N; k.gnx = c.gnx; k.gny = c.gny; k.num_cell = c.num_cell;
-  k.bbmin_x = c.bbmin_x; k.bbmin_y = c.bbmin_y; k.cell = c.cell;
-  k.dt = c.dt; k.mass = c.mass; k.K = c.K; k.G = c.G; k.rho0 = c.rho0; k.c0 = c.c0;
-  k.wdeltap = c.wdeltap; k.av_alpha = c.av_alpha; k.av_beta = c.av_beta; k.av_eta = c.av_eta;
-  k.xsph_eps = c.xsph_eps; k.stress_eps = c.stress_eps;
-  k.alpha_wp = c.alpha_wp; k.alpha_tool = c.alpha_tool; k.cp = c.cp; k.tq = c.tq; k.eta_heat = c.eta_heat;
-  k.T_init = c.T_init;
-  k.jc_A = c.jc_A; k.jc_B = c.jc_B; k.jc_C = c.jc_C; k.jc_n = c.jc_n; k.jc_m = c.jc_m;
-  k.jc_Tmelt = c.jc_Tmelt; k.jc_Tref = c.jc_Tref; k.jc_eps_dot_ref = c.jc_eps_dot_ref;
-  k.contact_alpha = c.contact_alpha; k.mu = c.mu; k.tool_vx = c.tool_vx; k.tool_vy = c.tool_vy;
-  k.thermal_bc = c.thermal_bc; k.fric_heat = c.fric_heat; k.thermal_wp = c.thermal_wp; k.pse = c.pse;
-  k.pse_quirk = c.pse_quirk;
-  k.promote = c.promote;
-  k.secant_tol = c.secant_tol;
+  mfc::make_dev_consts(c, k);
   const int N = c.N;
   s->ntiles = (N + mfc::TILE - 1) / mfc::TILE;
   s->precision = cfg->precision;

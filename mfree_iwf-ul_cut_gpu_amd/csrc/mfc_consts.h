@@ -1,0 +1,26 @@
+// mfc_consts.h — constants of one simulation as seen by the kernels (plain struct, usable from
+// host and device code).
+#pragma once
+
+namespace mfc {
+
+constexpr int TILE = 256;                      // particles per work-group of the particle kernels
+constexpr double PI_FAC = 0.454728408833987;   // 10/(7 pi), kernels.cuh:28
+constexpr double FLT_MAX_D = 3.4028234663852886e38;
+
+// physical, corrector, Johnson-Cook, thermal, contact and grid constants
+// (phys/corr/joco/trml/geom bundles of constants_structs.h:27-85), passed by value to every kernel
+struct dev_consts {
+  int N, gnx, gny, num_cell;
+  double bbmin_x, bbmin_y, cell;
+  double dt, mass, K, G, rho0, c0;
+  double wdeltap, av_alpha, av_beta, av_eta, xsph_eps, stress_eps;
+  double alpha_wp, alpha_tool, cp, tq, eta_heat, T_init;
+  double jc_A, jc_B, jc_C, jc_n, jc_m, jc_Tmelt, jc_Tref, jc_eps_dot_ref;
+  double contact_alpha, mu, tool_vx, tool_vy;
+  int thermal_bc, fric_heat, thermal_wp, pse, pse_quirk;
+  int promote;        // reference float truncations evaluated in FP64
+  double secant_tol;  // absolute stop tolerance of the radial-return secant
+};
+
+}  // namespace mfc

@@ -10,28 +10,13 @@
 #include <stdint.h>
 
 #include "../../include/mfc.h"
+#include "mfc_consts.h"
 #include "mfc_math.h"
 
 namespace mfc {
 
-constexpr int TILE = 256;           // particles per work-group of the particle kernels
-constexpr double PI_FAC = 0.454728408833987;   // 10/(7 pi), kernels.cuh:28
-constexpr double FLT_MAX_D = 3.4028234663852886e38;
 
 // constants of one simulation, passed by value to every kernel
-struct dev_consts {
-  int N, gnx, gny, num_cell;
-  double bbmin_x, bbmin_y, cell;
-  double dt, mass, K, G, rho0, c0;
-  double wdeltap, av_alpha, av_beta, av_eta, xsph_eps, stress_eps;
-  double alpha_wp, alpha_tool, cp, tq, eta_heat, T_init;
-  double jc_A, jc_B, jc_C, jc_n, jc_m, jc_Tmelt, jc_Tref, jc_eps_dot_ref;
-  double contact_alpha, mu, tool_vx, tool_vy;
-  int thermal_bc, fric_heat, thermal_wp, pse, pse_quirk;
-  int promote;        // reference float truncations evaluated in FP64
-  double secant_tol;  // absolute stop tolerance of the radial-return secant
-};
-
 __device__ inline double f32sqrt(double x) { return (double)sqrtf((float)x); }
 // sqrtf of the reference's FP64 build, or the FP64 square root when its truncations are promoted
 __device__ inline double tsqrt(const dev_consts &k, double x) { return k.promote ? sqrt(x) : f32sqrt(x); }

@@ -162,6 +162,24 @@ uint64_t splitmix64(uint64_t &s) {
 
 }  // namespace
 
+void make_dev_consts(const sim_constants &c, dev_consts &k) {
+  std::memset(&k, 0, sizeof(k));
+  k.N = c.N; k.gnx = c.gnx; k.gny = c.gny; k.num_cell = c.num_cell;
+  k.bbmin_x = c.bbmin_x; k.bbmin_y = c.bbmin_y; k.cell = c.cell;
+  k.dt = c.dt; k.mass = c.mass; k.K = c.K; k.G = c.G; k.rho0 = c.rho0; k.c0 = c.c0;
+  k.wdeltap = c.wdeltap; k.av_alpha = c.av_alpha; k.av_beta = c.av_beta; k.av_eta = c.av_eta;
+  k.xsph_eps = c.xsph_eps; k.stress_eps = c.stress_eps;
+  k.alpha_wp = c.alpha_wp; k.alpha_tool = c.alpha_tool; k.cp = c.cp; k.tq = c.tq; k.eta_heat = c.eta_heat;
+  k.T_init = c.T_init;
+  k.jc_A = c.jc_A; k.jc_B = c.jc_B; k.jc_C = c.jc_C; k.jc_n = c.jc_n; k.jc_m = c.jc_m;
+  k.jc_Tmelt = c.jc_Tmelt; k.jc_Tref = c.jc_Tref; k.jc_eps_dot_ref = c.jc_eps_dot_ref;
+  k.contact_alpha = c.contact_alpha; k.mu = c.mu; k.tool_vx = c.tool_vx; k.tool_vy = c.tool_vy;
+  k.thermal_bc = c.thermal_bc; k.fric_heat = c.fric_heat; k.thermal_wp = c.thermal_wp; k.pse = c.pse;
+  k.pse_quirk = c.pse_quirk;
+  k.promote = c.promote;
+  k.secant_tol = c.secant_tol;
+}
+
 void tool_advance(mfc_tool_state &t, double dt) {
   double sx = dt * t.vel[0], sy = dt * t.vel[1];
   for (int i = 0; i < t.num_seg; i++) {

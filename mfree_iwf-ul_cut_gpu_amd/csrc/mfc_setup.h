@@ -5,6 +5,7 @@
 #include <vector>
 
 #include "../../include/mfc.h"
+#include "mfc_consts.h"
 
 namespace mfc {
 
@@ -41,6 +42,9 @@ struct scenario {
 
 // fills a scenario from a configuration; returns MFC_OK or an MFC_E_* code with err set
 int build_scenario(const mfc_config &cfg, scenario &out, std::string &err);
+
+// the constants as passed to the kernels
+void make_dev_consts(const sim_constants &c, dev_consts &k);
 
 // one incremental tool move by dt (tool.cpp:211-221, geometric_primitives.cpp:70-85)
 void tool_advance(mfc_tool_state &t, double dt);
