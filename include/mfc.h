@@ -176,19 +176,47 @@ int mfc_sync(mfc_sim *sim);
 /* run nsteps steps without graphs, timing each phase with HIP events on the handle's
  * stream; ms[phase] receives the mean milliseconds per step of: 0 binning (hash + stable
  * counting sort + cell offsets), 1 gather/predict/EOS/artificial stress, 2 neighbour
- * interactions + per-particle update + contact, 3 tool-force reduction */
-#define MFC_PROFILE_PHASES 4
+ * interactions + per-particle update + contact + tool-force reduction */
+#define MFC_PROFILE_PHASES 3
 int mfc_profile(mfc_sim *sim, int nsteps, double *ms);
 
 /* report_force: F = sum_i fc_i + sum_i ft_i after the last step, summed in a fixed
- * order (tiles of 256 particles in storage order reduced by halving strides, tiles
- * summed in ascending order; sum of fc and sum of ft added last) */
+ * order: tiles of 256 particles in storage order, each reduced by halving strides; lane l
+ * of 256 adds the tile results l, l + 256, ... in ascending order; the 256 lane sums
+ * reduced by halving strides; sum of fc and sum of ft added last */
 int mfc_tool_force(mfc_sim *sim, double F[2]);
 /* forces of every step since creation or the last reset, as F[step][2];
  * *n receives the number of steps copied (at most cap); with F == NULL, *n receives the
  * number of recorded steps (at most 2^20 are kept) */
 int mfc_force_history(mfc_sim *sim, double *F, int64_t cap, int64_t *n);
 int mfc_force_history_reset(mfc_sim *sim);
+
+/* ---- reverse mode ----
+ * Gradient of the mean tool force over a window of steps with respect to
+ * theta = (JC A, JC B, JC n, JC C, JC m, friction mu), in the solver's units. The reference
+ * has no derivatives; this is the calibration interface of the solver. The handle must use
+ * precision MFC_PREC_FP64 with promote_fp64 = 1 (the smooth form of the step) and should use
+ * a tight secant_tol (1e-14): the radial return is differentiated at its converged root.
+ * Branches (elastic/plastic, contact, stick/slip, kernel support) are held at the ones the
+ * forward run took.
+ *
+ * mfc_vjp_forward: nsteps steps from the current state (as mfc_step), keeping the state at
+ * the start of every checkpoint_every-th step on the device; Fmean receives the mean of F over
+ * the steps [w0, w1) of the run (0 <= w0 < w1 <= nsteps).
+ * mfc_vjp_backward: grad[j] = cot . dFmean/dtheta_j for the last mfc_vjp_forward; may be
+ * called again with other cotangents; leaves the handle's state as mfc_vjp_forward left it.
+ * Memory: about 290 B per particle per checkpoint plus checkpoint_every snapshots more.
+ * mfc_vjp: both calls for a cotangent known in advance. */
+typedef struct mfc_vjp_req {
+  int nsteps;
+  int checkpoint_every;
+  int w0, w1;
+  double cot[2];
+  double Fmean[2]; /* out */
+} mfc_vjp_req;
+int mfc_vjp_forward(mfc_sim *sim, int nsteps, int checkpoint_every, int w0, int w1, double Fmean[2]);
+int mfc_vjp_backward(mfc_sim *sim, const double cot[2], double grad[6]);
+int mfc_vjp(mfc_sim *sim, mfc_vjp_req *req, double grad[6]);
 
 int mfc_get_info(const mfc_sim *sim, mfc_info *info);
 /* copy a field to host memory: bytes must be N * components * sizeof(double)

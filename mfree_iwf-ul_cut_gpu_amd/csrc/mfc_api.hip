@@ -10,24 +10,21 @@
 #include <string>
 #include <vector>
 
-#include "../../include/mfc.h"
-#include "mfc_kernels.h"
-#include "mfc_setup.h"
+#include "mfc_sim.h"
 
 namespace {
 
 thread_local std::string g_err;
 
-int fail(int code, const std::string &msg) {
+}  // namespace
+
+int mfc::fail(int code, const std::string &msg) {
   g_err = msg;
   return code;
 }
 
-#define HIP_TRY(expr)                                                                         \
-  do {                                                                                        \
-    hipError_t e_ = (expr);                                                                   \
-    if (e_ != hipSuccess) return fail(MFC_E_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
-  } while (0)
+namespace {
+using mfc::fail;
 
 int field_components(int f) {
   switch (f) {
@@ -43,63 +40,47 @@ int field_components(int f) {
 }
 bool field_is_int(int f) { return f == MFC_FIELD_FLAGS || f == MFC_FIELD_ID || f == MFC_FIELD_HASH; }
 
-constexpr int64_t HIST_CAP = 1 << 20;
-
 }  // namespace
 
-struct mfc_sim {
-  mfc::scenario sc;
-  mfc::dev_consts k;
-  int ntiles = 0;
-  hipStream_t stream = nullptr;
-  std::vector<void *> allocs;
-  mfc::dstate set[2];
-  mfc::dbins bins;
-  mfc::dpositional pos;
-  mfc::drecords recs = {false, nullptr, nullptr, nullptr};
-  int precision = MFC_PREC_FP64;
-  mfc_tool_state *tool = nullptr;  // two states, alternating each step
-  double *partials = nullptr, *force_last = nullptr, *hist = nullptr;
-  int64_t *hist_ctr = nullptr;
-  int64_t hist_base = 0;           // steps recorded before the last history reset
-  int cur = 0;                     // set holding the current state, and index of the current tool state
-  int64_t step = 0;
-  hipGraphExec_t graph[2] = {nullptr, nullptr};  // two steps starting from parity 0 / 1
-  bool use_graphs = true;
-
-  template <class T>
-  int alloc(T **p, size_t n) {
-    void *q = nullptr;
-    if (n == 0) n = 1;
-    hipError_t e = hipMalloc(&q, n * sizeof(T));
-    if (e != hipSuccess) return fail(MFC_E_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
-    allocs.push_back(q);
-    *p = (T *)q;
-    return MFC_OK;
-  }
-};
-
-namespace {
-
-int alloc_state(mfc_sim *s, mfc::dstate &d, int N) {
+static double **state_fields(mfc::dstate &d, int q) {
   double **dp[mfc::DSTATE_DOUBLES] = {&d.px, &d.py, &d.vx, &d.vy, &d.sxx, &d.sxy, &d.syy, &d.szz, &d.rho,
                                       &d.T, &d.eps, &d.epsd, &d.h, &d.fcx, &d.fcy, &d.ftx, &d.fty, &d.ptx,
                                       &d.pty, &d.vtx, &d.vty, &d.rhot, &d.stxx, &d.stxy, &d.styy, &d.stzz, &d.Tt};
-  for (int q = 0; q < mfc::DSTATE_DOUBLES; q++) {
-    int rc = s->alloc(dp[q], N);
-    if (rc) return rc;
-  }
-  int rc = s->alloc(&d.flags, N);
-  if (rc) return rc;
-  return s->alloc(&d.id, N);
+  return dp[q];
+}
+
+size_t state_block_bytes(int N) { return (size_t)N * (mfc::DSTATE_DOUBLES * sizeof(double) + 2 * sizeof(int)); }
+
+void carve_state(void *base, int N, mfc::dstate &d) {
+  double *b = (double *)base;
+  for (int q = 0; q < mfc::DSTATE_DOUBLES; q++) *state_fields(d, q) = b + (size_t)q * N;
+  d.flags = (int *)(b + (size_t)mfc::DSTATE_DOUBLES * N);
+  d.id = d.flags + N;
+}
+
+size_t positional_block_bytes(int N) { return (size_t)N * 6 * sizeof(double); }
+
+void carve_positional(void *base, int N, mfc::dpositional &p) {
+  double *b = (double *)base;
+  p.nx = b; p.ny = b + N; p.p = b + 2 * (size_t)N;
+  p.rxx = b + 3 * (size_t)N; p.rxy = b + 4 * (size_t)N; p.ryy = b + 5 * (size_t)N;
 }
 
 void enqueue_step(mfc_sim *s, int c) {
   const int n = 1 - c;
   mfc::launch_binning(s->k, s->set[c], s->bins, s->stream);
   mfc::launch_predict(s->k, s->set[c], s->set[n], s->bins, s->pos, s->recs, s->tool + c, s->tool + n, s->stream);
-  mfc::launch_interact(s->k, s->set[n], s->bins, s->pos, s->recs, s->tool + n, s->partials, s->stream);
-  mfc::launch_force_final(s->ntiles, s->partials, s->force_last, s->hist, HIST_CAP, s->hist_ctr, s->stream);
+  mfc::launch_interact(s->k, s->set[n], s->bins, s->pos, s->recs, s->tool + n, s->force(), s->stream);
+}
+
+namespace {
+
+int alloc_state(mfc_sim *s, mfc::dstate &d, int N) {
+  char *base = nullptr;
+  int rc = s->alloc(&base, state_block_bytes(N));
+  if (rc) return rc;
+  carve_state(base, N, d);
+  return MFC_OK;
 }
 
 int build_graph(mfc_sim *s, int parity) {
@@ -139,6 +120,7 @@ void mfc_destroy(mfc_sim *s) {
   if (s->stream) (void)hipStreamSynchronize(s->stream);
   for (int p = 0; p < 2; p++)
     if (s->graph[p]) (void)hipGraphExecDestroy(s->graph[p]);
+  mfc::adj_free(s->adj);
   for (void *q : s->allocs) (void)hipFree(q);
   if (s->stream) (void)hipStreamDestroy(s->stream);
   delete s;
@@ -172,9 +154,11 @@ int mfc_create(const mfc_config *cfg, mfc_sim **out) {
       (rc = s->alloc(&b.bsum, mfc::scan_blocks(c.num_cell))) || (rc = s->alloc(&b.clamped, 1)))
     return bail(rc);
   mfc::dpositional &p = s->pos;
-  if ((rc = s->alloc(&p.nx, N)) || (rc = s->alloc(&p.ny, N)) || (rc = s->alloc(&p.p, N)) ||
-      (rc = s->alloc(&p.rxx, N)) || (rc = s->alloc(&p.rxy, N)) || (rc = s->alloc(&p.ryy, N)))
-    return bail(rc);
+  {
+    char *pb = nullptr;
+    if ((rc = s->alloc(&pb, positional_block_bytes(N)))) return bail(rc);
+    carve_positional(pb, N, p);
+  }
   if (s->recs.f32) {
     if ((rc = s->alloc(&s->recs.posd, N)) || (rc = s->alloc(&s->recs.rec32, (size_t)16 * N))) return bail(rc);
   } else if ((rc = s->alloc(&s->recs.rec, (size_t)mfc::REC * N))) {
@@ -255,9 +239,10 @@ int mfc_step(mfc_sim *s, int nsteps) {
 
 int mfc_profile(mfc_sim *s, int nsteps, double *ms) {
   if (!s || !ms || nsteps < 1) return fail(MFC_E_INVALID, "bad argument");
-  hipEvent_t ev[5];
-  for (int q = 0; q < 5; q++) HIP_TRY(hipEventCreate(&ev[q]));
-  double acc[MFC_PROFILE_PHASES] = {0., 0., 0., 0.};
+  constexpr int NE = MFC_PROFILE_PHASES + 1;
+  hipEvent_t ev[NE];
+  for (int q = 0; q < NE; q++) HIP_TRY(hipEventCreate(&ev[q]));
+  double acc[MFC_PROFILE_PHASES] = {0., 0., 0.};
   int rc = MFC_OK;
   for (int it = 0; it < nsteps && rc == MFC_OK; it++) {
     const int c = s->cur, n = 1 - c;
@@ -266,20 +251,18 @@ int mfc_profile(mfc_sim *s, int nsteps, double *ms) {
     (void)hipEventRecord(ev[1], s->stream);
     mfc::launch_predict(s->k, s->set[c], s->set[n], s->bins, s->pos, s->recs, s->tool + c, s->tool + n, s->stream);
     (void)hipEventRecord(ev[2], s->stream);
-    mfc::launch_interact(s->k, s->set[n], s->bins, s->pos, s->recs, s->tool + n, s->partials, s->stream);
+    mfc::launch_interact(s->k, s->set[n], s->bins, s->pos, s->recs, s->tool + n, s->force(), s->stream);
     (void)hipEventRecord(ev[3], s->stream);
-    mfc::launch_force_final(s->ntiles, s->partials, s->force_last, s->hist, HIST_CAP, s->hist_ctr, s->stream);
-    (void)hipEventRecord(ev[4], s->stream);
     s->cur = n;
     s->step++;
-    if (hipEventSynchronize(ev[4]) != hipSuccess) { rc = fail(MFC_E_HIP, "event sync"); break; }
+    if (hipEventSynchronize(ev[NE - 1]) != hipSuccess) { rc = fail(MFC_E_HIP, "event sync"); break; }
     for (int q = 0; q < MFC_PROFILE_PHASES; q++) {
       float t = 0.f;
       if (hipEventElapsedTime(&t, ev[q], ev[q + 1]) != hipSuccess) { rc = fail(MFC_E_HIP, "event time"); break; }
       acc[q] += t;
     }
   }
-  for (int q = 0; q < 5; q++) (void)hipEventDestroy(ev[q]);
+  for (int q = 0; q < NE; q++) (void)hipEventDestroy(ev[q]);
   if (rc) return rc;
   for (int q = 0; q < MFC_PROFILE_PHASES; q++) ms[q] = acc[q] / nsteps;
   return MFC_OK;

@@ -49,6 +49,16 @@ struct drecords {
   float *rec32;
 };
 
+// tool-force reduction: per-tile partials written by the interaction kernel, reduced by
+// k_force_final, which appends F to the history
+struct dforce {
+  double *partials;      // 4 per tile: sum fc.x, fc.y, ft.x, ft.y
+  double *force_last;    // F of the last step
+  double *hist;          // force history, 2 per step, cap entries
+  int64_t cap;
+  int64_t *ctr;          // steps recorded
+};
+
 // internal flag: some value of the particle's neighbour record is NaN or Inf
 constexpr int FLAG_NONFINITE = 8;
 
@@ -56,10 +66,9 @@ constexpr int FLAG_NONFINITE = 8;
 void launch_binning(const dev_consts &k, const dstate &cur, dbins &b, hipStream_t s);
 void launch_predict(const dev_consts &k, const dstate &cur, const dstate &nxt, const dbins &b, const dpositional &pos,
                     const drecords &r, const mfc_tool_state *tool_in, mfc_tool_state *tool_out, hipStream_t s);
+// neighbour sums + per-particle update + contact, then the tool-force reduction (two kernels)
 void launch_interact(const dev_consts &k, const dstate &st, const dbins &b, const dpositional &pos,
-                     const drecords &r, const mfc_tool_state *tool_now, double *partials, hipStream_t s);
-void launch_force_final(int ntiles, const double *partials, double *force_last, double *hist, int64_t cap,
-                        int64_t *ctr, hipStream_t s);
+                     const drecords &r, const mfc_tool_state *tool_now, const dforce &f, hipStream_t s);
 
 int scan_blocks(int num_cell);
 void launch_device_math(int fn, const double *x, const double *y, double *out, int64_t n, hipStream_t s);

@@ -1,6 +1,6 @@
 // mfc_kernels.hip — gfx950 kernels of one FP64 SPH cutting step.
 //
-// Per step (the reference's ~20 launches of leap_frog::step, leap_frog.cu:94-155, become 8):
+// Per step (the reference's ~20 launches of leap_frog::step, leap_frog.cu:94-155, become 9):
 //   binning      k_hash_count, k_scan_reduce, k_scan_top, k_scan_down, k_place, k_rank
 //                cell ids (grid.cu:34-51) and a stable counting sort by cell
 //                (thrust::sort_by_key + reorder_data_and_find_cell_start, grid_gpu_green.cu:22-116,177-201)
@@ -10,7 +10,7 @@
 //                that follows them in the step: Jaumann, continuity, momentum, advection, correct,
 //                friction heat, Johnson-Cook return, tool-particle move, contact, boundary
 //                conditions, NaN blanking, and the per-tile partial sums of the tool force
-//   k_force_final  tool force of the step from the tile partials (output.cu:31-39)
+//   k_force_final  tool force of the step from the tile partials (output.cu:30-39)
 //
 // Memory layout: the persistent particle state is a structure of arrays in cell-sorted order,
 // double-buffered across the gather. Neighbour data are packed per particle into one 128-byte
@@ -388,8 +388,7 @@ __global__ __launch_bounds__(TILE) void k_interact(dev_consts k, int ntiles, con
                                                    const double2 *__restrict__ posd, const float *__restrict__ rec32,
                                                    const int *__restrict__ key_sorted,
                                                    const int *__restrict__ offsets, dstate st, dpositional pos,
-                                                   const mfc_tool_state *__restrict__ tool_now,
-                                                   double *__restrict__ partials) {
+                                                   const mfc_tool_state *__restrict__ tool_now, dforce fo) {
   const int t = xcd_tile(blockIdx.x, ntiles);
   if (t < 0) return;
   const int i = t * TILE + threadIdx.x;
@@ -506,36 +505,35 @@ __global__ __launch_bounds__(TILE) void k_interact(dev_consts k, int ntiles, con
   }
   if (threadIdx.x == 0) {
 #pragma unroll
-    for (int c = 0; c < 4; c++) partials[4 * t + c] = red[c][0];
+    for (int c = 0; c < 4; c++) fo.partials[4 * t + c] = red[c][0];
   }
 }
 
-// tile partials summed in ascending tile order; F = sum(fc) + sum(ft)
-__global__ __launch_bounds__(64) void k_force_final(int ntiles, const double *__restrict__ partials,
-                                                    double *__restrict__ force_last, double *__restrict__ hist,
-                                                    int64_t cap, int64_t *__restrict__ ctr) {
-  __shared__ double buf[4][64];
-  double acc[4] = {0., 0., 0., 0.};
-  for (int t0 = 0; t0 < ntiles; t0 += 64) {
-    int t = t0 + threadIdx.x;
-#pragma unroll
-    for (int c = 0; c < 4; c++) buf[c][threadIdx.x] = (t < ntiles) ? partials[4 * t + c] : 0.;
+// tool force of the step from the tile partials: lane l adds the tile results l, l + TILE, ...
+// in ascending order, then the lane sums are reduced by halving strides; F = sum(fc) + sum(ft)
+__global__ __launch_bounds__(TILE) void k_force_final(int ntiles, dforce fo) {
+  __shared__ double red[4][TILE];
+  double a0 = 0., a1 = 0., a2 = 0., a3 = 0.;
+  for (int q = threadIdx.x; q < ntiles; q += TILE) {
+    const double4 v = reinterpret_cast<const double4 *>(fo.partials)[q];
+    a0 = a0 + v.x; a1 = a1 + v.y; a2 = a2 + v.z; a3 = a3 + v.w;
+  }
+  red[0][threadIdx.x] = a0; red[1][threadIdx.x] = a1;
+  red[2][threadIdx.x] = a2; red[3][threadIdx.x] = a3;
+  for (int s = TILE / 2; s > 0; s >>= 1) {
     __syncthreads();
-    if (threadIdx.x == 0) {
-      int m = ntiles - t0 < 64 ? ntiles - t0 : 64;
-      for (int q = 0; q < m; q++)
+    if (threadIdx.x < s) {
 #pragma unroll
-        for (int c = 0; c < 4; c++) acc[c] = acc[c] + buf[c][q];
+      for (int c = 0; c < 4; c++) red[c][threadIdx.x] = red[c][threadIdx.x] + red[c][threadIdx.x + s];
     }
-    __syncthreads();
   }
   if (threadIdx.x == 0) {
-    double F0 = acc[0] + acc[2], F1 = acc[1] + acc[3];
-    force_last[0] = F0;
-    force_last[1] = F1;
-    int64_t s = *ctr;
-    if (s < cap) { hist[2 * s] = F0; hist[2 * s + 1] = F1; }
-    *ctr = s + 1;
+    const double F0 = red[0][0] + red[2][0], F1 = red[1][0] + red[3][0];
+    fo.force_last[0] = F0;
+    fo.force_last[1] = F1;
+    const int64_t n = *fo.ctr;
+    if (n < fo.cap) { fo.hist[2 * n] = F0; fo.hist[2 * n + 1] = F1; }
+    *fo.ctr = n + 1;
   }
 }
 
@@ -593,19 +591,15 @@ void launch_predict(const dev_consts &k, const dstate &cur, const dstate &nxt, c
 }
 
 void launch_interact(const dev_consts &k, const dstate &st, const dbins &b, const dpositional &pos,
-                     const drecords &r, const mfc_tool_state *tool_now, double *partials, hipStream_t s) {
+                     const drecords &r, const mfc_tool_state *tool_now, const dforce &f, hipStream_t s) {
   const int nt = (k.N + TILE - 1) / TILE;
   if (r.f32)
     hipLaunchKernelGGL(k_interact<true>, dim3(padded_tiles(nt)), dim3(TILE), 0, s, k, nt, r.rec, r.posd, r.rec32,
-                       b.key_sorted, b.offsets, st, pos, tool_now, partials);
+                       b.key_sorted, b.offsets, st, pos, tool_now, f);
   else
     hipLaunchKernelGGL(k_interact<false>, dim3(padded_tiles(nt)), dim3(TILE), 0, s, k, nt, r.rec, r.posd, r.rec32,
-                       b.key_sorted, b.offsets, st, pos, tool_now, partials);
-}
-
-void launch_force_final(int ntiles, const double *partials, double *force_last, double *hist, int64_t cap,
-                        int64_t *ctr, hipStream_t s) {
-  hipLaunchKernelGGL(k_force_final, dim3(1), dim3(64), 0, s, ntiles, partials, force_last, hist, cap, ctr);
+                       b.key_sorted, b.offsets, st, pos, tool_now, f);
+  hipLaunchKernelGGL(k_force_final, dim3(1), dim3(TILE), 0, s, nt, f);
 }
 
 void launch_device_math(int fn, const double *x, const double *y, double *out, int64_t n, hipStream_t s) {

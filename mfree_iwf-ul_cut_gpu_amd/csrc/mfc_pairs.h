@@ -2,7 +2,7 @@
 // adjoint's forward recomputation.
 #pragma once
 
-#include "mfc_device.h"
+#include "mfc_kernels.h"
 
 namespace mfc {
 

@@ -1,0 +1,75 @@
+// mfc_sim.h — the simulation handle behind the C ABI (internal to the library).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/mfc.h"
+#include "mfc_kernels.h"
+#include "mfc_setup.h"
+
+namespace mfc {
+
+// records the message of the last failing call of this thread; returns code
+int fail(int code, const std::string &msg);
+
+}  // namespace mfc
+
+#define HIP_TRY(expr)                                                                         \
+  do {                                                                                        \
+    hipError_t e_ = (expr);                                                                   \
+    if (e_ != hipSuccess) return mfc::fail(MFC_E_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+constexpr int64_t HIST_CAP = 1 << 20;
+
+namespace mfc {
+struct adj_work;  // reverse-mode workspace (mfc_adjoint.hip)
+void adj_free(adj_work *w);
+}  // namespace mfc
+
+struct mfc_sim {
+  mfc::scenario sc;
+  mfc::dev_consts k;
+  int ntiles = 0;
+  hipStream_t stream = nullptr;
+  std::vector<void *> allocs;
+  mfc::dstate set[2];
+  mfc::dbins bins;
+  mfc::dpositional pos;
+  mfc::drecords recs = {false, nullptr, nullptr, nullptr};
+  int precision = MFC_PREC_FP64;
+  mfc_tool_state *tool = nullptr;  // two states, alternating each step
+  double *partials = nullptr, *force_last = nullptr, *hist = nullptr;
+  int64_t *hist_ctr = nullptr;
+  mfc::dforce force() const { return {partials, force_last, hist, HIST_CAP, hist_ctr}; }
+  int64_t hist_base = 0;           // steps recorded before the last history reset
+  int cur = 0;                     // set holding the current state, and index of the current tool state
+  int64_t step = 0;
+  hipGraphExec_t graph[2] = {nullptr, nullptr};  // two steps starting from parity 0 / 1
+  bool use_graphs = true;
+  mfc::adj_work *adj = nullptr;    // reverse-mode tape of the last mfc_vjp_forward
+
+  template <class T>
+  int alloc(T **p, size_t n) {
+    void *q = nullptr;
+    if (n == 0) n = 1;
+    hipError_t e = hipMalloc(&q, n * sizeof(T));
+    if (e != hipSuccess) return mfc::fail(MFC_E_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+    allocs.push_back(q);
+    *p = (T *)q;
+    return MFC_OK;
+  }
+};
+
+// the state arrays of one set, carved from one allocation in the order of dstate
+// (DSTATE_DOUBLES doubles, then flags and id), so that a set is one contiguous block
+size_t state_block_bytes(int N);
+void carve_state(void *base, int N, mfc::dstate &d);
+// the positional arrays (nx, ny, p, rxx, rxy, ryy), one contiguous block
+size_t positional_block_bytes(int N);
+void carve_positional(void *base, int N, mfc::dpositional &p);
+// queue one step: set[c] -> set[1-c]
+void enqueue_step(mfc_sim *s, int c);

@@ -101,6 +101,14 @@ class ToolState(C.Structure):
         }
 
 
+class VjpReq(C.Structure):
+    """mfc_vjp_req (include/mfc.h)."""
+    _fields_ = [("nsteps", C.c_int32), ("checkpoint_every", C.c_int32), ("w0", C.c_int32), ("w1", C.c_int32),
+                ("cot", C.c_double * 2), ("Fmean", C.c_double * 2)]
+
+
+THETA = ("jc_A", "jc_B", "jc_n", "jc_C", "jc_m", "mu")  # order of the adjoint's parameters
+
 _lib = None
 
 
@@ -132,6 +140,9 @@ def lib() -> C.CDLL:
         L.mfc_scenario.argtypes = [C.POINTER(Config), C.POINTER(Info), dp, dp, dp, dp, C.POINTER(C.c_int32),
                                    C.POINTER(ToolState)]
         L.mfc_field_components.argtypes = [i32]
+        L.mfc_vjp_forward.argtypes = [vp, i32, i32, i32, i32, dp]
+        L.mfc_vjp_backward.argtypes = [vp, dp, dp]
+        L.mfc_vjp.argtypes = [vp, C.POINTER(VjpReq), dp]
         _lib = L
     return _lib
 
@@ -277,7 +288,7 @@ class Sim:
     def sync(self) -> None:
         _check(lib().mfc_sync(self._h), "mfc_sync")
 
-    PHASES = ("binning", "predict", "interact", "force")
+    PHASES = ("binning", "predict", "interact")  # interact includes the tool-force reduction
 
     def profile(self, nsteps: int) -> dict:
         """Mean ms per step of each phase, timed with HIP events on the handle's stream."""
@@ -289,6 +300,22 @@ class Sim:
         F = np.zeros(2)
         _check(lib().mfc_tool_force(self._h, F.ctypes.data_as(C.POINTER(C.c_double))), "mfc_tool_force")
         return F
+
+    def vjp_forward(self, nsteps: int, checkpoint_every: int, w0: int, w1: int) -> np.ndarray:
+        """Run nsteps steps keeping a checkpoint every checkpoint_every steps; returns the mean
+        tool force over the steps [w0, w1) of the run."""
+        F = np.zeros(2)
+        _check(lib().mfc_vjp_forward(self._h, int(nsteps), int(checkpoint_every), int(w0), int(w1),
+                                     F.ctypes.data_as(C.POINTER(C.c_double))), "mfc_vjp_forward")
+        return F
+
+    def vjp_backward(self, cot) -> np.ndarray:
+        """cot . d(mean force)/d(theta) for theta = THETA, of the last vjp_forward."""
+        c = np.ascontiguousarray(cot, dtype=np.float64)
+        g = np.zeros(6)
+        _check(lib().mfc_vjp_backward(self._h, c.ctypes.data_as(C.POINTER(C.c_double)),
+                                      g.ctypes.data_as(C.POINTER(C.c_double))), "mfc_vjp_backward")
+        return g
 
     def force_history(self, cap: int = 1 << 20) -> np.ndarray:
         n = C.c_int64()

@@ -1014,19 +1014,23 @@ void invalidate(sim_t &s) {  // actions_gpu.cu:317-349
   }
 }
 
-// report_force with the solver's fixed summation order: tiles of 256 in storage order reduced
-// by halving strides, tile results added in ascending order; sum(fc) + sum(ft)
+// report_force (output.cu:30-39) with the solver's fixed summation order (the reference reduces
+// with thrust::reduce, whose order is unspecified): tiles of 256 particles in storage order, each reduced by halving
+// strides; then lane l of 256 adds the tile results l, l + 256, l + 512, ... in ascending order
+// starting from 0, and the 256 lane sums are reduced by halving strides; sum(fc) + sum(ft)
 double tree_sum(const std::vector<V2> &a, bool y) {
   int N = (int)a.size();
-  double total = 0.;
-  double v[256];
-  for (int t0 = 0; t0 < N; t0 += 256) {
+  double v[256], lane[256];
+  for (int k = 0; k < 256; k++) lane[k] = 0.;
+  for (int t0 = 0, t = 0; t0 < N; t0 += 256, t++) {
     for (int k = 0; k < 256; k++) v[k] = (t0 + k < N) ? (y ? a[t0 + k].y : a[t0 + k].x) : 0.;
     for (int st = 128; st > 0; st >>= 1)
       for (int k = 0; k < st; k++) v[k] = v[k] + v[k + st];
-    total = total + v[0];
+    lane[t % 256] = lane[t % 256] + v[0];
   }
-  return total;
+  for (int st = 128; st > 0; st >>= 1)
+    for (int k = 0; k < st; k++) lane[k] = lane[k] + lane[k + st];
+  return lane[0];
 }
 
 void report_force(const sim_t &s, double F[2]) {

@@ -1,0 +1,142 @@
+"""Reverse mode on the GPU: d(mean tool force)/d(theta), theta = (JC A, B, n, C, m, friction mu),
+against central differences of the FP64 oracle (step 1e-6 relative, the smooth promoted map
+with a tight secant tolerance), plus the torch.autograd.Function wrapper used for calibration."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import oracle_binding as ob
+import pymfc
+
+pytestmark = pytest.mark.gpu
+
+
+def adj_config(thermal="adiabatic", nx=40, ny=12, dx=10.0):
+    cfg = pymfc.cut_config(nx, ny, dx, thermal=thermal)
+    cfg.promote_fp64 = 1
+    cfg.secant_tol = 1e-14
+    return cfg
+
+
+def theta_of(cfg) -> np.ndarray:
+    return np.array([getattr(cfg, n) for n in pymfc.THETA])
+
+
+def with_theta(cfg, th):
+    c = cfg.copy()
+    for n, v in zip(pymfc.THETA, th):
+        setattr(c, n, float(v))
+    return c
+
+
+def oracle_mean_force(cfg, nsteps, w0, w1) -> np.ndarray:
+    o = ob.OracleSim(cfg)
+    o.step(nsteps)
+    return o.force_history()[w0:w1].mean(axis=0)
+
+
+def oracle_fd(cfg, nsteps, w0, w1, rel=1e-6) -> np.ndarray:
+    """J[c, j] = d Fmean_c / d theta_j by central differences of the oracle"""
+    th = theta_of(cfg)
+    J = np.zeros((2, 6))
+    for j in range(6):
+        h = rel * abs(th[j])
+        tp, tm = th.copy(), th.copy()
+        tp[j] += h
+        tm[j] -= h
+        J[:, j] = (oracle_mean_force(with_theta(cfg, tp), nsteps, w0, w1) -
+                   oracle_mean_force(with_theta(cfg, tm), nsteps, w0, w1)) / (2 * h)
+    return J
+
+
+def gpu_jacobian(cfg, nsteps, K, w0, w1):
+    sim = pymfc.Sim(cfg)
+    F = sim.vjp_forward(nsteps, K, w0, w1)
+    J = np.stack([sim.vjp_backward([1.0, 0.0]), sim.vjp_backward([0.0, 1.0])])
+    return F, J, sim
+
+
+def assert_jacobian(J, fd, tol=1e-4):
+    # per force component, relative to the largest sensitivity of that component (parameters
+    # the force barely depends on are held to the same absolute accuracy)
+    for c in range(2):
+        scale = np.abs(fd[c]).max()
+        err = np.abs(J[c] - fd[c]) / scale
+        assert (err < tol).all(), f"component {c}: rel err {err}\nvjp {J[c]}\nfd  {fd[c]}"
+
+
+@pytest.mark.parametrize("thermal", ["adiabatic", "contact"])
+def test_vjp_matches_oracle_differences(thermal):
+    cfg = adj_config(thermal)
+    nsteps, K, w0, w1 = 40, 8, 10, 40
+    F, J, _ = gpu_jacobian(cfg, nsteps, K, w0, w1)
+    Fo = oracle_mean_force(cfg, nsteps, w0, w1)
+    assert np.array_equal(F, Fo)  # the promoted FP64 forward is the oracle's, bit for bit
+    fd = oracle_fd(cfg, nsteps, w0, w1)
+    print("thermal", thermal, "F", F, "\nvjp", J, "\nfd ", fd)
+    assert_jacobian(J, fd)
+
+
+def test_vjp_independent_of_checkpoint_interval():
+    cfg = adj_config("contact")
+    J = [gpu_jacobian(cfg, 24, K, 4, 24)[1] for K in (1, 5, 24)]
+    assert np.array_equal(J[0], J[1]) and np.array_equal(J[0], J[2])
+
+
+def test_vjp_leaves_the_run_state():
+    cfg = adj_config("contact")
+    _, _, sim = gpu_jacobian(cfg, 20, 4, 0, 20)
+    ref = pymfc.Sim(cfg)
+    ref.step(20)
+    sim.step(10)
+    ref.step(10)
+    for f in ("pos", "vel", "S", "T", "eps_pl", "fc", "ft", "flags", "hash"):
+        assert np.array_equal(sim.field(f), ref.field(f)), f
+    assert np.array_equal(sim.force_history(), ref.force_history())
+
+
+def test_autograd_calibration_loss():
+    """torch.autograd.Function around mfc_vjp_forward/backward: the calibration loss
+    |Fmean(theta) - F_target|^2 of a 30-step window, target from perturbed parameters."""
+    torch = pytest.importorskip("torch")
+
+    cfg = adj_config("adiabatic")
+    nsteps, K, w0, w1 = 40, 10, 10, 40
+
+    class MeanForce(torch.autograd.Function):
+        @staticmethod
+        def forward(ctx, theta):
+            sim = pymfc.Sim(with_theta(cfg, theta.detach().cpu().numpy()))
+            F = sim.vjp_forward(nsteps, K, w0, w1)
+            ctx.sim = sim
+            return torch.tensor(F, dtype=torch.float64)
+
+        @staticmethod
+        def backward(ctx, gF):
+            return torch.tensor(ctx.sim.vjp_backward(gF.detach().cpu().numpy()), dtype=torch.float64)
+
+    th0 = theta_of(cfg)
+    rng = np.random.default_rng(7)
+    th_target = th0 * (1 + rng.uniform(-0.05, 0.05, 6))
+    th_target[5] = 0.30
+    F_target = oracle_mean_force(with_theta(cfg, th_target), nsteps, w0, w1)
+
+    theta = torch.tensor(th0, dtype=torch.float64, requires_grad=True)
+    loss = ((MeanForce.apply(theta) - torch.tensor(F_target)) ** 2).sum()
+    loss.backward()
+    g = theta.grad.numpy()
+
+    def L(th):
+        return float(((oracle_mean_force(with_theta(cfg, th), nsteps, w0, w1) - F_target) ** 2).sum())
+
+    fd = np.zeros(6)
+    for j in range(6):
+        h = 1e-6 * abs(th0[j])
+        tp, tm = th0.copy(), th0.copy()
+        tp[j] += h
+        tm[j] -= h
+        fd[j] = (L(tp) - L(tm)) / (2 * h)
+    print("loss", float(loss), "\ngrad", g, "\nfd  ", fd)
+    err = np.abs(g - fd) / np.abs(fd).max()
+    assert (err < 1e-4).all(), err
