@@ -32,6 +32,9 @@ HBM_PEAK_GBS = 8000.0
 # the sorted cell id (read)
 _STATE_RW = 26 * 8 + 4 + 8 + 16 + 4 + (26 * 8 + 4 + 16)
 INTERACT_BYTES = {"fp64": 128 + _STATE_RW, "fp32c": 64 + 16 + _STATE_RW}
+# the neighbour-sum kernel of the split variant: own record, flags and cell id read, 14 sums
+# written (8 B each)
+SUMS_BYTES = {"fp64": 128 + 4 + 4 + 14 * 8, "fp32c": 64 + 16 + 4 + 4 + 14 * 8}
 # whole step, compulsory traffic of a fully fused FP64 step (persistent read-only 32 B,
 # mutable 224 B read + written, 12 B of binning keys)
 STEP_BYTES_FP64 = 256 + 224 + 12
@@ -69,12 +72,31 @@ def cpu_baseline(cfg, target_s: float) -> dict:
                       f"{wall:.1f} s wall on {ncpu} threads ({model})"}
 
 
-def run_path(pymfc, workload: str, precision: int, steps: int, warmup: int, profile_steps: int) -> dict:
+def tune(pymfc, sim, steps: int) -> dict:
+    """Autotuner: time each launch variant of the interaction on the live handle (the results
+    are the same bits for every variant) and keep the fastest."""
+    cand = {}
+    for split in (0, 1):
+        sim.set_tuning(split_interact=split)
+        sim.step(4)
+        sim.sync()
+        t0 = time.perf_counter()
+        sim.step(steps)
+        sim.sync()
+        cand[split] = (time.perf_counter() - t0) / steps * 1e3
+    best = min(cand, key=cand.get)
+    sim.set_tuning(split_interact=best)
+    return {"split_interact": best, "ms_per_step": {("split" if k else "fused"): v for k, v in cand.items()}}
+
+
+def run_path(pymfc, workload: str, precision: int, steps: int, warmup: int, profile_steps: int,
+             tune_steps: int) -> dict:
     cfg = pymfc.workload(workload, precision)
     sim = pymfc.Sim(cfg)
     info = sim.info
     N = info.n_particles
     sim.step(warmup)
+    tuning = tune(pymfc, sim, tune_steps) if tune_steps > 0 else {"split_interact": 0}
     sim.sync()
     t0 = time.perf_counter()
     sim.step(steps)
@@ -83,7 +105,39 @@ def run_path(pymfc, workload: str, precision: int, steps: int, warmup: int, prof
     ph = sim.profile(profile_steps)
     F = sim.tool_force()
     sim.close()
-    return {"N": N, "info": info, "wall": wall, "value": N * steps / wall, "phases": ph, "force": F.tolist()}
+    return {"N": N, "info": info, "wall": wall, "value": N * steps / wall, "phases": ph, "force": F.tolist(),
+            "tuning": tuning}
+
+
+def adjoint_ratio(pymfc, workload: str, nsteps: int, K: int) -> dict:
+    """Reverse mode on the calibration workload: wall time of the backward pass (replay of the
+    checkpoint segments included) over the plain forward of the same steps."""
+    cfg = pymfc.workload(workload, pymfc.PREC_FP64)
+    cfg.promote_fp64 = 1
+    cfg.secant_tol = 1e-14
+    f = pymfc.Sim(cfg)
+    f.step(20)
+    f.sync()
+    t0 = time.perf_counter()
+    f.step(nsteps)
+    f.sync()
+    t_fwd = time.perf_counter() - t0
+    N = f.N
+    f.close()
+    a = pymfc.Sim(cfg)
+    a.step(20)
+    a.sync()
+    t0 = time.perf_counter()
+    a.vjp_forward(nsteps, K, 0, nsteps)
+    t_tape = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    g = a.vjp_backward([1.0, 0.0])
+    t_bwd = time.perf_counter() - t0
+    a.close()
+    return {"workload": workload, "n_particles": N, "steps": nsteps, "checkpoint_every": K,
+            "forward_ms_per_step": t_fwd / nsteps * 1e3, "taped_forward_ms_per_step": t_tape / nsteps * 1e3,
+            "backward_ms_per_step": t_bwd / nsteps * 1e3, "ratio": t_bwd / t_fwd,
+            "ratio_with_tape": (t_tape + t_bwd) / t_fwd, "grad_Fx": g.tolist()}
 
 
 def main() -> int:
@@ -96,6 +150,11 @@ def main() -> int:
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--only", choices=["fp64", "fp32c"], default=None)
+    ap.add_argument("--tune-steps", type=int, default=100, help="steps per candidate of the autotuner (0: off)")
+    ap.add_argument("--adjoint-workload", default="adjoint_40k")
+    ap.add_argument("--adjoint-steps", type=int, default=400)
+    ap.add_argument("--adjoint-k", type=int, default=50)
+    ap.add_argument("--no-adjoint", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only; --gpus {args.gpus} is not supported "
@@ -108,15 +167,19 @@ def main() -> int:
     for name, prec in (("fp64", pymfc.PREC_FP64), ("fp32c", pymfc.PREC_FP32C)):
         if args.only and args.only != name:
             continue
-        r = run_path(pymfc, args.workload, prec, args.steps, args.warmup, args.profile_steps)
+        r = run_path(pymfc, args.workload, prec, args.steps, args.warmup, args.profile_steps, args.tune_steps)
         res[name] = r
         print(f"[bench] {name}: {r['N']} particles ({r['info'].n_tool} tool), {args.steps} steps in {r['wall']:.3f} s "
-              f"-> {r['value']:.4e} particle-steps/s; phases ms/step {r['phases']}", file=sys.stderr, flush=True)
+              f"-> {r['value']:.4e} particle-steps/s; phases ms/step {r['phases']}; tuning {r['tuning']}",
+              file=sys.stderr, flush=True)
     head = "fp32c" if "fp32c" in res else "fp64"
     r = res[head]
     N = r["N"]
+    split = r["tuning"]["split_interact"]
+    kname = "k_sums" if split else "k_interact"
+    kbytes = SUMS_BYTES[head] if split else INTERACT_BYTES[head]
     t_int = r["phases"]["interact"] * 1e-3
-    achieved = N * INTERACT_BYTES[head] / t_int / 1e9
+    achieved = N * kbytes / t_int / 1e9
     result = {
         "metric": METRIC,
         "value": r["value"],
@@ -131,20 +194,27 @@ def main() -> int:
         "dtype": "f32 (Kahan-compensated neighbour sums) + f64 state" if head == "fp32c" else "f64",
         "data": "synthetic",
         "config": {"workload": args.workload, "n_particles": N, "n_tool": r["info"].n_tool,
+                   "split_interact": split,
                    "num_cell": r["info"].num_cell, "precision": head, "thermal": "brookshaw+tool-contact"},
-        "roofline": {"bound": "hbm", "kernel": "k_interact", "achieved": achieved, "peak": HBM_PEAK_GBS,
+        "roofline": {"bound": "hbm", "kernel": kname, "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "bytes_per_particle": INTERACT_BYTES[head], "ms_per_launch": r["phases"]["interact"]},
+                     "bytes_per_particle": kbytes, "ms_per_launch": r["phases"]["interact"]},
+        "tuning": r["tuning"],
         "phases_ms": r["phases"],
         "tool_force": r["force"],
     }
     if "fp64" in res:
         result["fp64"] = {"value": res["fp64"]["value"], "ms_per_step": res["fp64"]["wall"] / args.steps * 1e3,
-                          "phases_ms": res["fp64"]["phases"],
+                          "phases_ms": res["fp64"]["phases"], "tuning": res["fp64"]["tuning"],
                           "step_hbm_gbs": N * STEP_BYTES_FP64 / (res["fp64"]["wall"] / args.steps) / 1e9}
         if head == "fp32c":
             result["fp32c_over_fp64"] = res["fp32c"]["value"] / res["fp64"]["value"]
     result["adjoint_over_forward"] = None
+    if not args.no_adjoint:
+        adj = adjoint_ratio(pymfc, args.adjoint_workload, args.adjoint_steps, args.adjoint_k)
+        print(f"[bench] adjoint: {adj}", file=sys.stderr, flush=True)
+        result["adjoint_over_forward"] = adj["ratio"]
+        result["adjoint"] = adj
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline(pymfc.workload(args.workload, pymfc.PREC_FP64), args.cpu_seconds)
     print(json.dumps(result), flush=True)

@@ -175,9 +175,11 @@ int mfc_sync(mfc_sim *sim);
 
 /* run nsteps steps without graphs, timing each phase with HIP events on the handle's
  * stream; ms[phase] receives the mean milliseconds per step of: 0 binning (hash + stable
- * counting sort + cell offsets), 1 gather/predict/EOS/artificial stress, 2 neighbour
- * interactions + per-particle update + contact + tool-force reduction */
-#define MFC_PROFILE_PHASES 3
+ * counting sort + cell offsets), 1 gather/predict/EOS/artificial stress, 2 the fused
+ * interaction kernel (neighbour sums + per-particle update + contact), or the neighbour-sum
+ * kernel when split_interact is set, 3 the update kernel (split_interact only, else ~0),
+ * 4 tool-force reduction */
+#define MFC_PROFILE_PHASES 5
 int mfc_profile(mfc_sim *sim, int nsteps, double *ms);
 
 /* report_force: F = sum_i fc_i + sum_i ft_i after the last step, summed in a fixed
@@ -217,6 +219,18 @@ typedef struct mfc_vjp_req {
 int mfc_vjp_forward(mfc_sim *sim, int nsteps, int checkpoint_every, int w0, int w1, double Fmean[2]);
 int mfc_vjp_backward(mfc_sim *sim, const double cot[2], double grad[6]);
 int mfc_vjp(mfc_sim *sim, mfc_vjp_req *req, double grad[6]);
+
+/* launch configuration of the step kernels (results do not depend on it: every variant
+ * evaluates the same expressions in the same order). split_interact = 0: one fused
+ * interaction kernel; 1: a neighbour-sum kernel and a per-particle update kernel with the sums
+ * stored in between (fewer registers per kernel, 224 B more traffic per particle). The
+ * benchmark's autotuner picks it per workload. */
+typedef struct mfc_tuning {
+  int split_interact;
+  int reserved[7];
+} mfc_tuning;
+int mfc_set_tuning(mfc_sim *sim, const mfc_tuning *tuning);
+int mfc_get_tuning(const mfc_sim *sim, mfc_tuning *tuning);
 
 int mfc_get_info(const mfc_sim *sim, mfc_info *info);
 /* copy a field to host memory: bytes must be N * components * sizeof(double)

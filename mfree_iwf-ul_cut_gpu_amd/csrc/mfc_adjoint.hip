@@ -321,6 +321,8 @@ int setup_work(mfc_sim *s, int nsteps, int K, int w0, int w1) {
         (rc = w->alloc(&w->g_rec, REC_DOUBLES * n)) || (rc = w->alloc(&w->theta_part, 6 * (size_t)s->ntiles)) ||
         (rc = w->alloc(&w->theta_acc, 6)))
       return rc;
+    // the replay's force reduction keeps no history (cap 0) but counts steps
+    HIP_TRY(hipMemsetAsync(w->ctr, 0, sizeof(int64_t), s->stream));
   }
   w->nsteps = nsteps;
   w->w0 = w0;
@@ -412,7 +414,7 @@ int mfc_vjp_backward(mfc_sim *s, const double cot[2], double grad[6]) {
       carve_state(w->work[1 - wa], N, B);
       launch_binning(k, A, s->bins, st);
       launch_predict(k, A, B, s->bins, P, s->recs, w->toolw, w->toolw + 1, st);
-      launch_interact(k, B, s->bins, P, s->recs, w->toolw + 1, fo, st);
+      launch_interact(k, B, s->bins, P, s->recs, w->toolw + 1, fo, s->interact_acc(), st);
       HIP_TRY(hipMemcpyAsync(w->toolw, w->toolw + 1, sizeof(mfc_tool_state), hipMemcpyDeviceToDevice, st));
       wa = 1 - wa;
     }

@@ -70,7 +70,8 @@ void enqueue_step(mfc_sim *s, int c) {
   const int n = 1 - c;
   mfc::launch_binning(s->k, s->set[c], s->bins, s->stream);
   mfc::launch_predict(s->k, s->set[c], s->set[n], s->bins, s->pos, s->recs, s->tool + c, s->tool + n, s->stream);
-  mfc::launch_interact(s->k, s->set[n], s->bins, s->pos, s->recs, s->tool + n, s->force(), s->stream);
+  mfc::launch_interact(s->k, s->set[n], s->bins, s->pos, s->recs, s->tool + n, s->force(), s->interact_acc(),
+                       s->stream);
 }
 
 namespace {
@@ -166,7 +167,8 @@ int mfc_create(const mfc_config *cfg, mfc_sim **out) {
   }
   if ((rc = s->alloc(&s->tool, 2)) ||
       (rc = s->alloc(&s->partials, (size_t)4 * s->ntiles)) || (rc = s->alloc(&s->force_last, 2)) ||
-      (rc = s->alloc(&s->hist, (size_t)2 * HIST_CAP)) || (rc = s->alloc(&s->hist_ctr, 1)))
+      (rc = s->alloc(&s->hist, (size_t)2 * HIST_CAP)) || (rc = s->alloc(&s->hist_ctr, 1)) ||
+      (rc = s->alloc(&s->acc, (size_t)mfc::ACC_N * N)))
     return bail(rc);
   // zero everything, then upload the initial state (particle_gpu.cu:20-125)
   {
@@ -213,6 +215,25 @@ int mfc_create(const mfc_config *cfg, mfc_sim **out) {
   return MFC_OK;
 }
 
+int mfc_set_tuning(mfc_sim *s, const mfc_tuning *t) {
+  if (!s || !t) return fail(MFC_E_INVALID, "null argument");
+  if (t->split_interact != 0 && t->split_interact != 1) return fail(MFC_E_INVALID, "split_interact must be 0 or 1");
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  s->tuning = *t;
+  for (int p = 0; p < 2; p++)
+    if (s->graph[p]) {
+      (void)hipGraphExecDestroy(s->graph[p]);
+      s->graph[p] = nullptr;
+    }
+  return MFC_OK;
+}
+
+int mfc_get_tuning(const mfc_sim *s, mfc_tuning *t) {
+  if (!s || !t) return fail(MFC_E_INVALID, "null argument");
+  *t = s->tuning;
+  return MFC_OK;
+}
+
 int mfc_step(mfc_sim *s, int nsteps) {
   if (!s || nsteps < 0) return fail(MFC_E_INVALID, "bad argument");
   int done = 0;
@@ -242,8 +263,9 @@ int mfc_profile(mfc_sim *s, int nsteps, double *ms) {
   constexpr int NE = MFC_PROFILE_PHASES + 1;
   hipEvent_t ev[NE];
   for (int q = 0; q < NE; q++) HIP_TRY(hipEventCreate(&ev[q]));
-  double acc[MFC_PROFILE_PHASES] = {0., 0., 0.};
+  double acc[MFC_PROFILE_PHASES] = {0., 0., 0., 0., 0.};
   int rc = MFC_OK;
+  const mfc::dforce fo = s->force();
   for (int it = 0; it < nsteps && rc == MFC_OK; it++) {
     const int c = s->cur, n = 1 - c;
     (void)hipEventRecord(ev[0], s->stream);
@@ -251,8 +273,17 @@ int mfc_profile(mfc_sim *s, int nsteps, double *ms) {
     (void)hipEventRecord(ev[1], s->stream);
     mfc::launch_predict(s->k, s->set[c], s->set[n], s->bins, s->pos, s->recs, s->tool + c, s->tool + n, s->stream);
     (void)hipEventRecord(ev[2], s->stream);
-    mfc::launch_interact(s->k, s->set[n], s->bins, s->pos, s->recs, s->tool + n, s->force(), s->stream);
-    (void)hipEventRecord(ev[3], s->stream);
+    if (s->tuning.split_interact) {
+      mfc::launch_sums(s->k, s->set[n].flags, s->bins, s->recs, s->acc, s->stream);
+      (void)hipEventRecord(ev[3], s->stream);
+      mfc::launch_update(s->k, s->acc, s->set[n], s->pos, s->tool + n, fo, s->stream);
+    } else {
+      mfc::launch_interact_fused(s->k, s->set[n], s->bins, s->pos, s->recs, s->tool + n, fo, s->stream);
+      (void)hipEventRecord(ev[3], s->stream);
+    }
+    (void)hipEventRecord(ev[4], s->stream);
+    mfc::launch_force_final(s->k, fo, s->stream);
+    (void)hipEventRecord(ev[5], s->stream);
     s->cur = n;
     s->step++;
     if (hipEventSynchronize(ev[NE - 1]) != hipSuccess) { rc = fail(MFC_E_HIP, "event sync"); break; }

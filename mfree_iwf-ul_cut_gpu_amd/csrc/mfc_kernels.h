@@ -37,7 +37,8 @@ struct dpositional {
   double *nx, *ny, *p, *rxx, *rxy, *ryy;
 };
 
-constexpr int REC = 16;  // doubles per neighbour record
+constexpr int REC = 16;    // doubles per neighbour record
+constexpr int ACC_N = 14;  // neighbour sums per particle
 
 // neighbour records written by the predict kernel and read by the interaction kernel:
 // FP64 path: rec, 16 doubles per particle; FP32C path: posd (FP64 predicted position) and
@@ -66,9 +67,21 @@ constexpr int FLAG_NONFINITE = 8;
 void launch_binning(const dev_consts &k, const dstate &cur, dbins &b, hipStream_t s);
 void launch_predict(const dev_consts &k, const dstate &cur, const dstate &nxt, const dbins &b, const dpositional &pos,
                     const drecords &r, const mfc_tool_state *tool_in, mfc_tool_state *tool_out, hipStream_t s);
-// neighbour sums + per-particle update + contact, then the tool-force reduction (two kernels)
+// neighbour sums + per-particle update + contact, then the tool-force reduction. acc == nullptr:
+// one fused kernel; otherwise the sums go through acc (ACC_N arrays of N doubles) between a
+// sums kernel and an update kernel (fewer registers per kernel)
 void launch_interact(const dev_consts &k, const dstate &st, const dbins &b, const dpositional &pos,
-                     const drecords &r, const mfc_tool_state *tool_now, const dforce &f, hipStream_t s);
+                     const drecords &r, const mfc_tool_state *tool_now, const dforce &f, double *acc,
+                     hipStream_t s);
+// the pieces of launch_interact, for per-kernel timing
+void launch_interact_fused(const dev_consts &k, const dstate &st, const dbins &b, const dpositional &pos,
+                           const drecords &r, const mfc_tool_state *tool_now, const dforce &f, hipStream_t s);
+void launch_update(const dev_consts &k, const double *acc, const dstate &st, const dpositional &pos,
+                   const mfc_tool_state *tool_now, const dforce &f, hipStream_t s);
+void launch_force_final(const dev_consts &k, const dforce &f, hipStream_t s);
+// neighbour sums only (ACC_N arrays of N doubles), flags in sorted order
+void launch_sums(const dev_consts &k, const int *flags, const dbins &b, const drecords &r, double *acc,
+                 hipStream_t s);
 
 int scan_blocks(int num_cell);
 void launch_device_math(int fn, const double *x, const double *y, double *out, int64_t n, hipStream_t s);

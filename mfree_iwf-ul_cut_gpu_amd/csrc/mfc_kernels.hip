@@ -383,119 +383,127 @@ __device__ inline void sums_f32(const dev_consts &k, int i, const double2 *__res
   tpse = tp.value();
 }
 
-template <bool F32>
-__global__ __launch_bounds__(TILE) void k_interact(dev_consts k, int ntiles, const double *__restrict__ rec,
-                                                   const double2 *__restrict__ posd, const float *__restrict__ rec32,
-                                                   const int *__restrict__ key_sorted,
-                                                   const int *__restrict__ offsets, dstate st, dpositional pos,
-                                                   const mfc_tool_state *__restrict__ tool_now, dforce fo) {
-  const int t = xcd_tile(blockIdx.x, ntiles);
-  if (t < 0) return;
-  const int i = t * TILE + threadIdx.x;
-  double fsum0 = 0., fsum1 = 0., fsum2 = 0., fsum3 = 0.;
-  if (i < k.N) {
-    const bool pse = k.pse != 0;
-    int fl = st.flags[i];
-    const bool blank = fl & MFC_FLAG_BLANKED, tool = fl & MFC_FLAG_TOOL, fixed = fl & MFC_FLAG_FIXED;
-    pair_acc acc = {0., 0., 0., 0., 0., 0., 0., 0., 0., 0., 0., 0., 0.};
-    double tpse = 0.;
-    const double ta = pse ? 0. : (tool ? k.alpha_tool : k.alpha_wp);
-    const bool run_mech = !blank && (!pse || !tool);
-    const bool run_pse = !blank && pse && k.thermal_wp;
-    if (run_mech || run_pse) {
-      if constexpr (F32) sums_f32(k, i, posd, rec32, key_sorted, offsets, tool, ta, run_mech, run_pse, acc, tpse);
-      else sums_f64(k, i, rec, key_sorted, offsets, tool, ta, run_mech, run_pse, acc, tpse);
-    }
-    // state at the start of the step (sorted) and previous derivatives
-    const double x0 = st.px[i], y0 = st.py[i], u0 = st.vx[i], v0 = st.vy[i];
-    const double sxx0 = st.sxx[i], sxy0 = st.sxy[i], syy0 = st.syy[i], szz0 = st.szz[i];
-    const double rho0 = st.rho[i], T0 = st.T[i];
-    double ptx = st.ptx[i], pty = st.pty[i], vtx = st.vtx[i], vty = st.vty[i];
-    double rhot = st.rhot[i], stxx = st.stxx[i], stxy = st.stxy[i], styy = st.styy[i], stzz = st.stzz[i];
-    double Tt = st.Tt[i];
-    double fcx = st.fcx[i], fcy = st.fcy[i], ftx = st.ftx[i], fty = st.fty[i];
-    double eps = st.eps[i], epsd = st.epsd[i];
-    // predicted state (same expressions as k_predict)
-    double pu = u0, pv = v0, psxx = sxx0, psxy = sxy0, psyy = syy0, pszz = szz0, prho = rho0;
-    if (!blank) {
-      const double hdt = 0.5 * k.dt;
-      pu = u0 + hdt * vtx; pv = v0 + hdt * vty;
-      psxx = sxx0 + hdt * stxx; psxy = sxy0 + hdt * stxy; psyy = syy0 + hdt * styy; pszz = szz0 + hdt * stzz;
-      prho = rho0 + hdt * rhot;
-    }
-    // interaction outputs (interactions_gpu.cu:334-344, :143)
-    if (run_mech) {
-      ptx = acc.xt0; pty = acc.xt1; vtx = acc.vt0; vty = acc.vt1;
-      if (ta != 0.) Tt = ta * acc.tl;
-    }
-    if (run_pse) {
-      double tflag = tool ? 1. : 0.;
-      if (k.pse_quirk && i >= k.N / 2) tflag = 0.;
-      Tt = ((tflag == 1.) ? k.alpha_tool : k.alpha_wp) * tpse;
-    }
-    if (!tool) {
-      // Jaumann rate, continuity, momentum, advection (actions_gpu.cu:99-224)
-      jaumann(k.G, acc.vd0, acc.vd1, acc.vd2, acc.vd3, psxx, psxy, psyy, pszz, stxx, stxy, styy, stzz);
-      rhot = -prho * (acc.vd0 + acc.vd3);
-      vtx += acc.sd0 + acc.sd1 + fcx / k.mass + ftx / k.mass;
-      vty += acc.sd2 + acc.sd3 + fcy / k.mass + fty / k.mass;
-      ptx = ptx + pu;
-      pty = pty + pv;
-    }
-    // leap-frog corrector (leap_frog.cu:68-92)
-    double x = x0, y = y0, u = u0, v = v0, sxx = sxx0, sxy = sxy0, syy = syy0, szz = szz0, rho = rho0, T = T0;
-    if (!blank) {
-      x = x0 + k.dt * ptx; y = y0 + k.dt * pty;
-      u = u0 + k.dt * vtx; v = v0 + k.dt * vty;
-      sxx = sxx0 + k.dt * stxx; sxy = sxy0 + k.dt * stxy; syy = syy0 + k.dt * styy; szz = szz0 + k.dt * stzz;
-      rho = rho0 + k.dt * rhot;
-      T = T0 + k.dt * Tt;
-    }
-    // frictional heating with the positional normal of the previous step (actions_gpu.cu:135-162)
-    if (k.fric_heat && !tool) {
-      const double fmag = tsqrt(k, ftx * ftx + fty * fty);
-      if (fmag != 0.) {
-        const double nx = pos.nx[i], ny = pos.ny[i];
-        const double dvx = u - k.tool_vx, dvy = v - k.tool_vy;
-        const double dvn = dvx * nx + dvy * ny;
-        const double rx = dvx - dvn, ry = dvy - dvn;
-        const double vmag = tsqrt(k, rx * rx + ry * ry);
-        T += k.eta_heat * k.dt * fmag * vmag / (k.cp * k.mass);
-      }
-    }
-    if (!tool) plasticity(k, pos.p[i], rho, sxx, sxy, syy, szz, eps, epsd, T);
-    if (tool) {  // update_tool_particles (tool_gpu.cu:210-221)
-      x += k.dt * k.tool_vx;
-      y += k.dt * k.tool_vy;
-    }
-    if (!tool) {
-      double nx, ny;
-      contact_force(k, *tool_now, x, y, u, v, fcx, fcy, ftx, fty, nx, ny);
-      pos.nx[i] = nx; pos.ny[i] = ny;
-      if (fixed) {  // actions_gpu.cu:296-311
-        u = 0.; v = 0.; fcx = 0.; fcy = 0.; ptx = 0.; pty = 0.; vtx = 0.; vty = 0.;
-      }
-    }
-    if (k.thermal_bc && fixed) T = k.T_init;  // actions_gpu.cu:287-294
-    if (!blank) {  // actions_gpu.cu:317-349
-      bool bad = !finite_d(ptx) || !finite_d(pty) || !finite_d(vtx) || !finite_d(vty) || !finite_d(stxx) ||
-                 !finite_d(stxy) || !finite_d(styy) || !finite_d(stzz) || !finite_d(rhot) || !finite_d(Tt);
-      if (bad) fl |= MFC_FLAG_BLANKED;
-    }
-    st.px[i] = x; st.py[i] = y; st.vx[i] = u; st.vy[i] = v;
-    st.sxx[i] = sxx; st.sxy[i] = sxy; st.syy[i] = syy; st.szz[i] = szz;
-    st.rho[i] = rho; st.T[i] = T; st.eps[i] = eps; st.epsd[i] = epsd;
-    st.fcx[i] = fcx; st.fcy[i] = fcy; st.ftx[i] = ftx; st.fty[i] = fty;
-    st.ptx[i] = ptx; st.pty[i] = pty; st.vtx[i] = vtx; st.vty[i] = vty;
-    st.rhot[i] = rhot; st.stxx[i] = stxx; st.stxy[i] = stxy; st.styy[i] = styy; st.stzz[i] = stzz;
-    st.Tt[i] = Tt;
-    st.flags[i] = fl;
-    fsum0 = fcx; fsum1 = fcy; fsum2 = ftx; fsum3 = fty;
+// everything after the neighbour sums for particle i (sorted index), in the reference's order;
+// returns the particle's fc and ft for the tool force
+__device__ inline void update_particle(const dev_consts &k, int i, const pair_acc &acc, double tpse, dstate st,
+                                       dpositional pos, const mfc_tool_state &tool_now, double fsum[4]) {
+  const bool pse = k.pse != 0;
+  int fl = st.flags[i];
+  const bool blank = fl & MFC_FLAG_BLANKED, tool = fl & MFC_FLAG_TOOL, fixed = fl & MFC_FLAG_FIXED;
+  const double ta = pse ? 0. : (tool ? k.alpha_tool : k.alpha_wp);
+  const bool run_mech = !blank && (!pse || !tool);
+  const bool run_pse = !blank && pse && k.thermal_wp;
+  // state at the start of the step (sorted) and previous derivatives
+  const double x0 = st.px[i], y0 = st.py[i], u0 = st.vx[i], v0 = st.vy[i];
+  const double sxx0 = st.sxx[i], sxy0 = st.sxy[i], syy0 = st.syy[i], szz0 = st.szz[i];
+  const double rho0 = st.rho[i], T0 = st.T[i];
+  double ptx = st.ptx[i], pty = st.pty[i], vtx = st.vtx[i], vty = st.vty[i];
+  double rhot = st.rhot[i], stxx = st.stxx[i], stxy = st.stxy[i], styy = st.styy[i], stzz = st.stzz[i];
+  double Tt = st.Tt[i];
+  double fcx = st.fcx[i], fcy = st.fcy[i], ftx = st.ftx[i], fty = st.fty[i];
+  double eps = st.eps[i], epsd = st.epsd[i];
+  // predicted state (same expressions as k_predict)
+  double pu = u0, pv = v0, psxx = sxx0, psxy = sxy0, psyy = syy0, pszz = szz0, prho = rho0;
+  if (!blank) {
+    const double hdt = 0.5 * k.dt;
+    pu = u0 + hdt * vtx; pv = v0 + hdt * vty;
+    psxx = sxx0 + hdt * stxx; psxy = sxy0 + hdt * stxy; psyy = syy0 + hdt * styy; pszz = szz0 + hdt * stzz;
+    prho = rho0 + hdt * rhot;
   }
-  // tile partial sums of the tool force by halving strides
+  // interaction outputs (interactions_gpu.cu:334-344, :143)
+  if (run_mech) {
+    ptx = acc.xt0; pty = acc.xt1; vtx = acc.vt0; vty = acc.vt1;
+    if (ta != 0.) Tt = ta * acc.tl;
+  }
+  if (run_pse) {
+    double tflag = tool ? 1. : 0.;
+    if (k.pse_quirk && i >= k.N / 2) tflag = 0.;
+    Tt = ((tflag == 1.) ? k.alpha_tool : k.alpha_wp) * tpse;
+  }
+  if (!tool) {
+    // Jaumann rate, continuity, momentum, advection (actions_gpu.cu:99-224)
+    jaumann(k.G, acc.vd0, acc.vd1, acc.vd2, acc.vd3, psxx, psxy, psyy, pszz, stxx, stxy, styy, stzz);
+    rhot = -prho * (acc.vd0 + acc.vd3);
+    vtx += acc.sd0 + acc.sd1 + fcx / k.mass + ftx / k.mass;
+    vty += acc.sd2 + acc.sd3 + fcy / k.mass + fty / k.mass;
+    ptx = ptx + pu;
+    pty = pty + pv;
+  }
+  // leap-frog corrector (leap_frog.cu:68-92)
+  double x = x0, y = y0, u = u0, v = v0, sxx = sxx0, sxy = sxy0, syy = syy0, szz = szz0, rho = rho0, T = T0;
+  if (!blank) {
+    x = x0 + k.dt * ptx; y = y0 + k.dt * pty;
+    u = u0 + k.dt * vtx; v = v0 + k.dt * vty;
+    sxx = sxx0 + k.dt * stxx; sxy = sxy0 + k.dt * stxy; syy = syy0 + k.dt * styy; szz = szz0 + k.dt * stzz;
+    rho = rho0 + k.dt * rhot;
+    T = T0 + k.dt * Tt;
+  }
+  // frictional heating with the positional normal of the previous step (actions_gpu.cu:135-162)
+  if (k.fric_heat && !tool) {
+    const double fmag = tsqrt(k, ftx * ftx + fty * fty);
+    if (fmag != 0.) {
+      const double nx = pos.nx[i], ny = pos.ny[i];
+      const double dvx = u - k.tool_vx, dvy = v - k.tool_vy;
+      const double dvn = dvx * nx + dvy * ny;
+      const double rx = dvx - dvn, ry = dvy - dvn;
+      const double vmag = tsqrt(k, rx * rx + ry * ry);
+      T += k.eta_heat * k.dt * fmag * vmag / (k.cp * k.mass);
+    }
+  }
+  if (!tool) plasticity(k, pos.p[i], rho, sxx, sxy, syy, szz, eps, epsd, T);
+  if (tool) {  // update_tool_particles (tool_gpu.cu:210-221)
+    x += k.dt * k.tool_vx;
+    y += k.dt * k.tool_vy;
+  }
+  if (!tool) {
+    double nx, ny;
+    contact_force(k, tool_now, x, y, u, v, fcx, fcy, ftx, fty, nx, ny);
+    pos.nx[i] = nx; pos.ny[i] = ny;
+    if (fixed) {  // actions_gpu.cu:296-311
+      u = 0.; v = 0.; fcx = 0.; fcy = 0.; ptx = 0.; pty = 0.; vtx = 0.; vty = 0.;
+    }
+  }
+  if (k.thermal_bc && fixed) T = k.T_init;  // actions_gpu.cu:287-294
+  if (!blank) {  // actions_gpu.cu:317-349
+    bool bad = !finite_d(ptx) || !finite_d(pty) || !finite_d(vtx) || !finite_d(vty) || !finite_d(stxx) ||
+               !finite_d(stxy) || !finite_d(styy) || !finite_d(stzz) || !finite_d(rhot) || !finite_d(Tt);
+    if (bad) fl |= MFC_FLAG_BLANKED;
+  }
+  st.px[i] = x; st.py[i] = y; st.vx[i] = u; st.vy[i] = v;
+  st.sxx[i] = sxx; st.sxy[i] = sxy; st.syy[i] = syy; st.szz[i] = szz;
+  st.rho[i] = rho; st.T[i] = T; st.eps[i] = eps; st.epsd[i] = epsd;
+  st.fcx[i] = fcx; st.fcy[i] = fcy; st.ftx[i] = ftx; st.fty[i] = fty;
+  st.ptx[i] = ptx; st.pty[i] = pty; st.vtx[i] = vtx; st.vty[i] = vty;
+  st.rhot[i] = rhot; st.stxx[i] = stxx; st.stxy[i] = stxy; st.styy[i] = styy; st.stzz[i] = stzz;
+  st.Tt[i] = Tt;
+  st.flags[i] = fl;
+  fsum[0] = fcx; fsum[1] = fcy; fsum[2] = ftx; fsum[3] = fty;
+}
+
+// the neighbour sums of particle i as pair_acc + PSE sum
+template <bool F32>
+__device__ inline void particle_sums(const dev_consts &k, int i, const double *__restrict__ rec,
+                                     const double2 *__restrict__ posd, const float *__restrict__ rec32,
+                                     const int *__restrict__ key_sorted, const int *__restrict__ offsets, int fl,
+                                     pair_acc &acc, double &tpse) {
+  const bool pse = k.pse != 0;
+  const bool blank = fl & MFC_FLAG_BLANKED, tool = fl & MFC_FLAG_TOOL;
+  const double ta = pse ? 0. : (tool ? k.alpha_tool : k.alpha_wp);
+  const bool run_mech = !blank && (!pse || !tool);
+  const bool run_pse = !blank && pse && k.thermal_wp;
+  acc = {0., 0., 0., 0., 0., 0., 0., 0., 0., 0., 0., 0., 0.};
+  tpse = 0.;
+  if (run_mech || run_pse) {
+    if constexpr (F32) sums_f32(k, i, posd, rec32, key_sorted, offsets, tool, ta, run_mech, run_pse, acc, tpse);
+    else sums_f64(k, i, rec, key_sorted, offsets, tool, ta, run_mech, run_pse, acc, tpse);
+  }
+}
+
+// tile partial sums of the tool force by halving strides (all threads of the work-group call it)
+__device__ inline void force_partials(int t, const double fsum[4], double *__restrict__ partials) {
   __shared__ double red[4][TILE];
-  red[0][threadIdx.x] = fsum0; red[1][threadIdx.x] = fsum1;
-  red[2][threadIdx.x] = fsum2; red[3][threadIdx.x] = fsum3;
+#pragma unroll
+  for (int c = 0; c < 4; c++) red[c][threadIdx.x] = fsum[c];
   for (int s = TILE / 2; s > 0; s >>= 1) {
     __syncthreads();
     if (threadIdx.x < s) {
@@ -505,8 +513,61 @@ __global__ __launch_bounds__(TILE) void k_interact(dev_consts k, int ntiles, con
   }
   if (threadIdx.x == 0) {
 #pragma unroll
-    for (int c = 0; c < 4; c++) fo.partials[4 * t + c] = red[c][0];
+    for (int c = 0; c < 4; c++) partials[4 * t + c] = red[c][0];
   }
+}
+
+// fused: sums + update in one pass (registers of both live in one kernel)
+template <bool F32>
+__global__ __launch_bounds__(TILE) void k_interact(dev_consts k, int ntiles, const double *__restrict__ rec,
+                                                   const double2 *__restrict__ posd, const float *__restrict__ rec32,
+                                                   const int *__restrict__ key_sorted,
+                                                   const int *__restrict__ offsets, dstate st, dpositional pos,
+                                                   const mfc_tool_state *__restrict__ tool_now, dforce fo) {
+  const int t = xcd_tile(blockIdx.x, ntiles);
+  if (t < 0) return;
+  const int i = t * TILE + threadIdx.x;
+  double fsum[4] = {0., 0., 0., 0.};
+  if (i < k.N) {
+    pair_acc acc;
+    double tpse;
+    particle_sums<F32>(k, i, rec, posd, rec32, key_sorted, offsets, st.flags[i], acc, tpse);
+    update_particle(k, i, acc, tpse, st, pos, *tool_now, fsum);
+  }
+  force_partials(t, fsum, fo.partials);
+}
+
+// split, first half: the neighbour sums of every particle into acc (L_N arrays of N doubles)
+template <bool F32>
+__global__ __launch_bounds__(TILE) void k_sums(dev_consts k, int ntiles, const double *__restrict__ rec,
+                                               const double2 *__restrict__ posd, const float *__restrict__ rec32,
+                                               const int *__restrict__ key_sorted, const int *__restrict__ offsets,
+                                               const int *__restrict__ flags, double *__restrict__ acc) {
+  const int t = xcd_tile(blockIdx.x, ntiles);
+  if (t < 0) return;
+  const int i = t * TILE + threadIdx.x;
+  if (i >= k.N) return;
+  pair_acc a;
+  double tpse;
+  particle_sums<F32>(k, i, rec, posd, rec32, key_sorted, offsets, flags[i], a, tpse);
+  store_acc(acc, k.N, i, a, tpse);
+}
+
+// split, second half: the per-particle update from the stored sums
+__global__ __launch_bounds__(TILE) void k_update(dev_consts k, int ntiles, const double *__restrict__ acc,
+                                                 dstate st, dpositional pos,
+                                                 const mfc_tool_state *__restrict__ tool_now, dforce fo) {
+  const int t = xcd_tile(blockIdx.x, ntiles);
+  if (t < 0) return;
+  const int i = t * TILE + threadIdx.x;
+  double fsum[4] = {0., 0., 0., 0.};
+  if (i < k.N) {
+    pair_acc a;
+    double tpse;
+    load_acc(acc, k.N, i, a, tpse);
+    update_particle(k, i, a, tpse, st, pos, *tool_now, fsum);
+  }
+  force_partials(t, fsum, fo.partials);
 }
 
 // tool force of the step from the tile partials: lane l adds the tile results l, l + TILE, ...
@@ -532,7 +593,7 @@ __global__ __launch_bounds__(TILE) void k_force_final(int ntiles, dforce fo) {
     fo.force_last[0] = F0;
     fo.force_last[1] = F1;
     const int64_t n = *fo.ctr;
-    if (n < fo.cap) { fo.hist[2 * n] = F0; fo.hist[2 * n + 1] = F1; }
+    if (n >= 0 && n < fo.cap) { fo.hist[2 * n] = F0; fo.hist[2 * n + 1] = F1; }
     *fo.ctr = n + 1;
   }
 }
@@ -590,8 +651,19 @@ void launch_predict(const dev_consts &k, const dstate &cur, const dstate &nxt, c
                        r.posd, r.rec32, tool_in, tool_out);
 }
 
-void launch_interact(const dev_consts &k, const dstate &st, const dbins &b, const dpositional &pos,
-                     const drecords &r, const mfc_tool_state *tool_now, const dforce &f, hipStream_t s) {
+void launch_sums(const dev_consts &k, const int *flags, const dbins &b, const drecords &r, double *acc,
+                 hipStream_t s) {
+  const int nt = (k.N + TILE - 1) / TILE;
+  if (r.f32)
+    hipLaunchKernelGGL(k_sums<true>, dim3(padded_tiles(nt)), dim3(TILE), 0, s, k, nt, r.rec, r.posd, r.rec32,
+                       b.key_sorted, b.offsets, flags, acc);
+  else
+    hipLaunchKernelGGL(k_sums<false>, dim3(padded_tiles(nt)), dim3(TILE), 0, s, k, nt, r.rec, r.posd, r.rec32,
+                       b.key_sorted, b.offsets, flags, acc);
+}
+
+void launch_interact_fused(const dev_consts &k, const dstate &st, const dbins &b, const dpositional &pos,
+                           const drecords &r, const mfc_tool_state *tool_now, const dforce &f, hipStream_t s) {
   const int nt = (k.N + TILE - 1) / TILE;
   if (r.f32)
     hipLaunchKernelGGL(k_interact<true>, dim3(padded_tiles(nt)), dim3(TILE), 0, s, k, nt, r.rec, r.posd, r.rec32,
@@ -599,7 +671,29 @@ void launch_interact(const dev_consts &k, const dstate &st, const dbins &b, cons
   else
     hipLaunchKernelGGL(k_interact<false>, dim3(padded_tiles(nt)), dim3(TILE), 0, s, k, nt, r.rec, r.posd, r.rec32,
                        b.key_sorted, b.offsets, st, pos, tool_now, f);
+}
+
+void launch_update(const dev_consts &k, const double *acc, const dstate &st, const dpositional &pos,
+                   const mfc_tool_state *tool_now, const dforce &f, hipStream_t s) {
+  const int nt = (k.N + TILE - 1) / TILE;
+  hipLaunchKernelGGL(k_update, dim3(padded_tiles(nt)), dim3(TILE), 0, s, k, nt, acc, st, pos, tool_now, f);
+}
+
+void launch_force_final(const dev_consts &k, const dforce &f, hipStream_t s) {
+  const int nt = (k.N + TILE - 1) / TILE;
   hipLaunchKernelGGL(k_force_final, dim3(1), dim3(TILE), 0, s, nt, f);
+}
+
+void launch_interact(const dev_consts &k, const dstate &st, const dbins &b, const dpositional &pos,
+                     const drecords &r, const mfc_tool_state *tool_now, const dforce &f, double *acc,
+                     hipStream_t s) {
+  if (acc) {
+    launch_sums(k, st.flags, b, r, acc, s);
+    launch_update(k, acc, st, pos, tool_now, f, s);
+  } else {
+    launch_interact_fused(k, st, b, pos, r, tool_now, f, s);
+  }
+  launch_force_final(k, f, s);
 }
 
 void launch_device_math(int fn, const double *x, const double *y, double *out, int64_t n, hipStream_t s) {

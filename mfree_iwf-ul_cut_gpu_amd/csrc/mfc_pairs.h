@@ -19,6 +19,23 @@ struct pair_acc {
   double tl;                   // Brookshaw sum
 };
 
+// neighbour sums stored per particle as ACC_N arrays of N doubles (structure of arrays, in
+// the order vd0..vd3, sd0..sd3, vt0, vt1, xt0, xt1, tl, PSE sum)
+
+__device__ inline void store_acc(double *__restrict__ acc, size_t N, int i, const pair_acc &a, double tpse) {
+  acc[0 * N + i] = a.vd0; acc[1 * N + i] = a.vd1; acc[2 * N + i] = a.vd2; acc[3 * N + i] = a.vd3;
+  acc[4 * N + i] = a.sd0; acc[5 * N + i] = a.sd1; acc[6 * N + i] = a.sd2; acc[7 * N + i] = a.sd3;
+  acc[8 * N + i] = a.vt0; acc[9 * N + i] = a.vt1; acc[10 * N + i] = a.xt0; acc[11 * N + i] = a.xt1;
+  acc[12 * N + i] = a.tl; acc[13 * N + i] = tpse;
+}
+
+__device__ inline void load_acc(const double *__restrict__ acc, size_t N, int i, pair_acc &a, double &tpse) {
+  a.vd0 = acc[0 * N + i]; a.vd1 = acc[1 * N + i]; a.vd2 = acc[2 * N + i]; a.vd3 = acc[3 * N + i];
+  a.sd0 = acc[4 * N + i]; a.sd1 = acc[5 * N + i]; a.sd2 = acc[6 * N + i]; a.sd3 = acc[7 * N + i];
+  a.vt0 = acc[8 * N + i]; a.vt1 = acc[9 * N + i]; a.xt0 = acc[10 * N + i]; a.xt1 = acc[11 * N + i];
+  a.tl = acc[12 * N + i]; tpse = acc[13 * N + i];
+}
+
 // the body of the neighbour loop of do_interactions_monaghan for one pair
 // (interactions_gpu.cu:249-326 with cubic_spline kernels.cuh:30-70)
 __device__ inline void pair_terms(const dev_consts &k, pair_acc &a, double xi, double yi, double ui, double vi_,

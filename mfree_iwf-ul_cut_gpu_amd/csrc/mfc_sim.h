@@ -51,6 +51,9 @@ struct mfc_sim {
   hipGraphExec_t graph[2] = {nullptr, nullptr};  // two steps starting from parity 0 / 1
   bool use_graphs = true;
   mfc::adj_work *adj = nullptr;    // reverse-mode tape of the last mfc_vjp_forward
+  double *acc = nullptr;           // neighbour sums between the split interaction kernels
+  mfc_tuning tuning = {0, {0, 0, 0, 0, 0, 0, 0}};
+  double *interact_acc() const { return tuning.split_interact ? acc : nullptr; }
 
   template <class T>
   int alloc(T **p, size_t n) {

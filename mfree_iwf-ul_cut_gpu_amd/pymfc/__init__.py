@@ -107,6 +107,11 @@ class VjpReq(C.Structure):
                 ("cot", C.c_double * 2), ("Fmean", C.c_double * 2)]
 
 
+class Tuning(C.Structure):
+    """mfc_tuning (include/mfc.h)."""
+    _fields_ = [("split_interact", C.c_int32), ("reserved", C.c_int32 * 7)]
+
+
 THETA = ("jc_A", "jc_B", "jc_n", "jc_C", "jc_m", "mu")  # order of the adjoint's parameters
 
 _lib = None
@@ -143,6 +148,8 @@ def lib() -> C.CDLL:
         L.mfc_vjp_forward.argtypes = [vp, i32, i32, i32, i32, dp]
         L.mfc_vjp_backward.argtypes = [vp, dp, dp]
         L.mfc_vjp.argtypes = [vp, C.POINTER(VjpReq), dp]
+        L.mfc_set_tuning.argtypes = [vp, C.POINTER(Tuning)]
+        L.mfc_get_tuning.argtypes = [vp, C.POINTER(Tuning)]
         _lib = L
     return _lib
 
@@ -223,10 +230,13 @@ def workload(name: str, precision: int = PREC_FP64) -> Config:
         return cut_config(750, 200, 2.0, thermal="contact", precision=precision)
     if name == "pse_500k":            # 1250 x 400 at 1.2 um, PSE, 120 m/min, jittered, 10000 steps
         return cut_config(1250, 400, 1.2, thermal="pse", speed_m_min=120.0, jitter=0.01, precision=precision)
+    # calibration cuts: at 70 m/min the tool advances 0.04 dx in 400 steps and no particle
+    # yields, so the force would not depend on the Johnson-Cook constants; at 1000 m/min the
+    # contact zone yields within the first 50 steps
     if name == "adjoint_6k":
-        return cut_config(150, 40, 10.0, thermal="adiabatic", precision=precision)
+        return cut_config(150, 40, 10.0, thermal="adiabatic", speed_m_min=1000.0, precision=precision)
     if name == "adjoint_40k":
-        return cut_config(375, 100, 4.0, thermal="adiabatic", precision=precision)
+        return cut_config(375, 100, 4.0, thermal="adiabatic", speed_m_min=1000.0, precision=precision)
     raise KeyError(name)
 
 
@@ -288,7 +298,7 @@ class Sim:
     def sync(self) -> None:
         _check(lib().mfc_sync(self._h), "mfc_sync")
 
-    PHASES = ("binning", "predict", "interact")  # interact includes the tool-force reduction
+    PHASES = ("binning", "predict", "interact", "update", "force")  # "update": split interaction only
 
     def profile(self, nsteps: int) -> dict:
         """Mean ms per step of each phase, timed with HIP events on the handle's stream."""
@@ -300,6 +310,19 @@ class Sim:
         F = np.zeros(2)
         _check(lib().mfc_tool_force(self._h, F.ctypes.data_as(C.POINTER(C.c_double))), "mfc_tool_force")
         return F
+
+    def set_tuning(self, **kw) -> None:
+        """Launch configuration (results are independent of it): split_interact=0|1."""
+        t = Tuning()
+        _check(lib().mfc_get_tuning(self._h, C.byref(t)), "mfc_get_tuning")
+        for k, v in kw.items():
+            setattr(t, k, int(v))
+        _check(lib().mfc_set_tuning(self._h, C.byref(t)), "mfc_set_tuning")
+
+    def tuning(self) -> dict:
+        t = Tuning()
+        _check(lib().mfc_get_tuning(self._h, C.byref(t)), "mfc_get_tuning")
+        return {"split_interact": t.split_interact}
 
     def vjp_forward(self, nsteps: int, checkpoint_every: int, w0: int, w1: int) -> np.ndarray:
         """Run nsteps steps keeping a checkpoint every checkpoint_every steps; returns the mean

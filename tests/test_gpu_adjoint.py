@@ -13,7 +13,8 @@ pytestmark = pytest.mark.gpu
 
 
 def adj_config(thermal="adiabatic", nx=40, ny=12, dx=10.0):
-    cfg = pymfc.cut_config(nx, ny, dx, thermal=thermal)
+    # 1000 m/min: the contact zone yields within 50 steps, so every parameter matters
+    cfg = pymfc.cut_config(nx, ny, dx, thermal=thermal, speed_m_min=1000.0)
     cfg.promote_fp64 = 1
     cfg.secant_tol = 1e-14
     return cfg
@@ -50,6 +51,17 @@ def oracle_fd(cfg, nsteps, w0, w1, rel=1e-6) -> np.ndarray:
     return J
 
 
+def oracle_fd_bracket(cfg, nsteps, w0, w1):
+    """FD at steps 1e-6 (the reference value) and its uncertainty: the spread against steps 1e-5
+    and 1e-7. Elastic/plastic and stick/slip switches of single particles make the mean force
+    piecewise smooth; where a switch falls inside the difference stencil the three estimates
+    disagree, and that disagreement bounds what central differences can resolve there."""
+    f6 = oracle_fd(cfg, nsteps, w0, w1, 1e-6)
+    f5 = oracle_fd(cfg, nsteps, w0, w1, 1e-5)
+    f7 = oracle_fd(cfg, nsteps, w0, w1, 1e-7)
+    return f6, np.maximum(np.abs(f6 - f5), np.abs(f6 - f7))
+
+
 def gpu_jacobian(cfg, nsteps, K, w0, w1):
     sim = pymfc.Sim(cfg)
     F = sim.vjp_forward(nsteps, K, w0, w1)
@@ -57,36 +69,42 @@ def gpu_jacobian(cfg, nsteps, K, w0, w1):
     return F, J, sim
 
 
-def assert_jacobian(J, fd, tol=1e-4):
+def assert_jacobian(J, fd, unc, tol=1e-4):
     # per force component, relative to the largest sensitivity of that component (parameters
-    # the force barely depends on are held to the same absolute accuracy)
+    # the force barely depends on are held to the same absolute accuracy), widened by twice
+    # the differences' own uncertainty where a switch falls inside the stencil
+    smooth = 0
     for c in range(2):
         scale = np.abs(fd[c]).max()
-        err = np.abs(J[c] - fd[c]) / scale
-        assert (err < tol).all(), f"component {c}: rel err {err}\nvjp {J[c]}\nfd  {fd[c]}"
+        err = np.abs(J[c] - fd[c])
+        print(f"component {c}: rel err {err / scale}, fd uncertainty {unc[c] / scale}")
+        assert (err <= tol * scale + 2 * unc[c]).all(), f"component {c}: rel err {err / scale}\nvjp {J[c]}\nfd  {fd[c]}"
+        smooth += int(((unc[c] < 1e-5 * scale) & (err < tol * scale)).sum())
+    assert smooth >= 6  # at least half the entries met 1e-4 where the differences are converged
 
 
 @pytest.mark.parametrize("thermal", ["adiabatic", "contact"])
 def test_vjp_matches_oracle_differences(thermal):
     cfg = adj_config(thermal)
-    nsteps, K, w0, w1 = 40, 8, 10, 40
+    nsteps, K, w0, w1 = 150, 25, 50, 150
     F, J, _ = gpu_jacobian(cfg, nsteps, K, w0, w1)
     Fo = oracle_mean_force(cfg, nsteps, w0, w1)
     assert np.array_equal(F, Fo)  # the promoted FP64 forward is the oracle's, bit for bit
-    fd = oracle_fd(cfg, nsteps, w0, w1)
+    fd, unc = oracle_fd_bracket(cfg, nsteps, w0, w1)
     print("thermal", thermal, "F", F, "\nvjp", J, "\nfd ", fd)
-    assert_jacobian(J, fd)
+    assert (np.abs(fd[0, :5]) > 1e-3 * np.abs(fd[0]).max()).sum() >= 3  # the yield law matters here
+    assert_jacobian(J, fd, unc)
 
 
 def test_vjp_independent_of_checkpoint_interval():
     cfg = adj_config("contact")
-    J = [gpu_jacobian(cfg, 24, K, 4, 24)[1] for K in (1, 5, 24)]
+    J = [gpu_jacobian(cfg, 60, K, 10, 60)[1] for K in (1, 7, 60)]
     assert np.array_equal(J[0], J[1]) and np.array_equal(J[0], J[2])
 
 
 def test_vjp_leaves_the_run_state():
     cfg = adj_config("contact")
-    _, _, sim = gpu_jacobian(cfg, 20, 4, 0, 20)
+    _, _, sim = gpu_jacobian(cfg, 40, 8, 0, 40)
     ref = pymfc.Sim(cfg)
     ref.step(20)
     sim.step(10)
@@ -98,11 +116,11 @@ def test_vjp_leaves_the_run_state():
 
 def test_autograd_calibration_loss():
     """torch.autograd.Function around mfc_vjp_forward/backward: the calibration loss
-    |Fmean(theta) - F_target|^2 of a 30-step window, target from perturbed parameters."""
+    |Fmean(theta) - F_target|^2 of a 100-step window, target from perturbed parameters."""
     torch = pytest.importorskip("torch")
 
     cfg = adj_config("adiabatic")
-    nsteps, K, w0, w1 = 40, 10, 10, 40
+    nsteps, K, w0, w1 = 150, 30, 50, 150
 
     class MeanForce(torch.autograd.Function):
         @staticmethod
@@ -130,13 +148,19 @@ def test_autograd_calibration_loss():
     def L(th):
         return float(((oracle_mean_force(with_theta(cfg, th), nsteps, w0, w1) - F_target) ** 2).sum())
 
-    fd = np.zeros(6)
-    for j in range(6):
-        h = 1e-6 * abs(th0[j])
-        tp, tm = th0.copy(), th0.copy()
-        tp[j] += h
-        tm[j] -= h
-        fd[j] = (L(tp) - L(tm)) / (2 * h)
-    print("loss", float(loss), "\ngrad", g, "\nfd  ", fd)
-    err = np.abs(g - fd) / np.abs(fd).max()
-    assert (err < 1e-4).all(), err
+    def fd_at(rel):
+        out = np.zeros(6)
+        for j in range(6):
+            h = rel * abs(th0[j])
+            tp, tm = th0.copy(), th0.copy()
+            tp[j] += h
+            tm[j] -= h
+            out[j] = (L(tp) - L(tm)) / (2 * h)
+        return out
+
+    fd = fd_at(1e-6)
+    unc = np.maximum(np.abs(fd - fd_at(1e-5)), np.abs(fd - fd_at(1e-7)))
+    scale = np.abs(fd).max()
+    err = np.abs(g - fd)
+    print("loss", float(loss), "\ngrad", g, "\nfd  ", fd, "\nrel err", err / scale, "\nfd uncertainty", unc / scale)
+    assert (err <= 1e-4 * scale + 2 * unc).all(), err / scale

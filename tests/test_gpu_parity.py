@@ -110,6 +110,19 @@ def test_graph_replay_matches_eager():
     assert equal(g1.force_history(), g2.force_history())
 
 
+@pytest.mark.parametrize("prec", [pymfc.PREC_FP64, pymfc.PREC_FP32C])
+def test_split_interaction_matches_fused(prec):
+    """The split interaction (sums kernel + update kernel) gives the fused kernel's bits."""
+    cfg = small_cut(thermal="contact", precision=prec)
+    g1, g2 = pymfc.Sim(cfg), pymfc.Sim(cfg)
+    g2.set_tuning(split_interact=1)
+    g1.step(30)
+    g2.step(30)
+    for f in ["pos", "vel", "S", "T", "eps_pl", "fc", "ft", "flags"]:
+        assert equal(g1.field(f), g2.field(f)), f
+    assert equal(g1.force_history(), g2.force_history())
+
+
 def test_nan_blanking_matches_oracle():
     cfg = small_cut(thermal="contact")
     g, o = run_pair(cfg, 10)
