@@ -140,6 +140,76 @@ def adjoint_ratio(pymfc, workload: str, nsteps: int, K: int) -> dict:
             "ratio_with_tape": (t_tape + t_bwd) / t_fwd, "grad_Fx": g.tolist()}
 
 
+def batch_bench(pymfc, n: int, steps: int, tune_steps: int) -> dict:
+    """C5: n independent uneven cuts in one batch (one launch per phase for all instances),
+    against the same instances run one after another as single handles."""
+    out = {"instances": n}
+    for name, prec in (("fp64", pymfc.PREC_FP64), ("fp32c", pymfc.PREC_FP32C)):
+        cfgs = pymfc.batch_configs(n, precision=prec)
+        b = pymfc.Batch(cfgs)
+        N = sum(inst.N for inst in b.instances)
+        b.step(10)
+        cand = {}
+        for split in ((0, 1) if tune_steps > 0 else (0,)):
+            b.set_tuning(split_interact=split)
+            b.step(4)
+            b.sync()
+            t0 = time.perf_counter()
+            b.step(max(tune_steps, 2))
+            b.sync()
+            cand[split] = (time.perf_counter() - t0) / max(tune_steps, 2)
+        best = min(cand, key=cand.get)
+        b.set_tuning(split_interact=best)
+        b.step(4)
+        b.sync()
+        t0 = time.perf_counter()
+        b.step(steps)
+        b.sync()
+        wall = time.perf_counter() - t0
+        sizes = [inst.N for inst in b.instances]
+        b.close()
+        seq = 0.0
+        for c in cfgs:
+            s = pymfc.Sim(c)
+            s.set_tuning(split_interact=best)
+            s.step(10)
+            s.sync()
+            t0 = time.perf_counter()
+            s.step(steps)
+            s.sync()
+            seq += time.perf_counter() - t0
+            s.close()
+        out[name] = {"particles": N, "min_instance": min(sizes), "max_instance": max(sizes),
+                     "particle_steps_per_s": N * steps / wall, "ms_per_step": wall / steps * 1e3,
+                     "sequential_particle_steps_per_s": N * steps / seq, "batch_speedup": seq / wall,
+                     "tuning": {"split_interact": best,
+                                "ms_per_step": {("split" if k else "fused"): v * 1e3 for k, v in cand.items()}}}
+    return out
+
+
+COPY_PROBE_BYTES = 512 << 20
+
+
+def counter_probe(pymfc, workload: str, precision: int, steps: int, split: int) -> None:
+    """A short run for hardware-counter passes (rocprofv3 --pmc): `steps` steps of one path,
+    then a plain device copy of COPY_PROBE_BYTES read + COPY_PROBE_BYTES written, whose counter
+    readings calibrate the bytes per counted unit."""
+    import torch
+    sim = pymfc.Sim(pymfc.workload(workload, precision))
+    sim.set_tuning(split_interact=split)
+    for _ in range(steps):
+        sim.step(1)
+    sim.sync()
+    a = torch.empty(COPY_PROBE_BYTES // 4, dtype=torch.float32, device="cuda").fill_(1.0)
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"probe": workload, "precision": precision, "steps": steps, "split_interact": split,
+                      "copy_bytes": COPY_PROBE_BYTES, "N": sim.N}), flush=True)
+    sim.close()
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -155,6 +225,11 @@ def main() -> int:
     ap.add_argument("--adjoint-steps", type=int, default=400)
     ap.add_argument("--adjoint-k", type=int, default=50)
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--batch-instances", type=int, default=32)
+    ap.add_argument("--batch-steps", type=int, default=200)
+    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--probe", action="store_true", help="counter-pass mode: short run + calibration copy")
+    ap.add_argument("--split", type=int, default=0, help="split_interact for --probe")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only; --gpus {args.gpus} is not supported "
@@ -162,6 +237,11 @@ def main() -> int:
         return 2
 
     import pymfc
+
+    if args.probe:
+        counter_probe(pymfc, args.workload, pymfc.PREC_FP32C if args.only == "fp32c" else pymfc.PREC_FP64,
+                      args.steps, args.split)
+        return 0
 
     res = {}
     for name, prec in (("fp64", pymfc.PREC_FP64), ("fp32c", pymfc.PREC_FP32C)):
@@ -215,6 +295,10 @@ def main() -> int:
         print(f"[bench] adjoint: {adj}", file=sys.stderr, flush=True)
         result["adjoint_over_forward"] = adj["ratio"]
         result["adjoint"] = adj
+    if not args.no_batch:
+        bb = batch_bench(pymfc, args.batch_instances, args.batch_steps, args.tune_steps)
+        print(f"[bench] batch: {bb}", file=sys.stderr, flush=True)
+        result["batch"] = bb
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline(pymfc.workload(args.workload, pymfc.PREC_FP64), args.cpu_seconds)
     print(json.dumps(result), flush=True)

@@ -232,6 +232,28 @@ typedef struct mfc_tuning {
 int mfc_set_tuning(mfc_sim *sim, const mfc_tuning *tuning);
 int mfc_get_tuning(const mfc_sim *sim, mfc_tuning *tuning);
 
+/* ---- batches of independent instances ----
+ * n cutting instances (any sizes, one precision) stepped together: every phase of a step is
+ * one launch over all instances' tiles, and two steps are one hipGraph launch. Each instance
+ * gives exactly the results of its own single run (same kernels, same order). The reference
+ * runs one simulation per process (module-level __constant__ state); this is the solver's
+ * interface for parameter sweeps and calibration ensembles.
+ * mfc_batch_instance returns instance i's handle for the read-out calls (mfc_get_field,
+ * mfc_tool_force, mfc_force_history, mfc_get_info, ...); it is owned by the batch, must not be
+ * destroyed or stepped on its own, and shares the batch's stream.
+ * mfc_batch_vjp_forward/backward: the reverse mode of every instance (Fmean: 2 n doubles,
+ * cot: 2 n, grad: 6 n); the taped forward runs batched, the backward instance by instance. */
+typedef struct mfc_batch mfc_batch;
+int mfc_batch_create(const mfc_config *cfgs, int n, mfc_batch **batch);
+void mfc_batch_destroy(mfc_batch *batch);
+int mfc_batch_size(const mfc_batch *batch);
+int mfc_batch_instance(mfc_batch *batch, int i, mfc_sim **sim);
+int mfc_batch_set_tuning(mfc_batch *batch, const mfc_tuning *tuning);
+int mfc_batch_step(mfc_batch *batch, int nsteps);
+int mfc_batch_sync(mfc_batch *batch);
+int mfc_batch_vjp_forward(mfc_batch *batch, int nsteps, int checkpoint_every, int w0, int w1, double *Fmean);
+int mfc_batch_vjp_backward(mfc_batch *batch, const double *cot, double *grad);
+
 int mfc_get_info(const mfc_sim *sim, mfc_info *info);
 /* copy a field to host memory: bytes must be N * components * sizeof(double)
  * (sizeof(int32_t) for FLAGS, ID, HASH) */

@@ -334,50 +334,66 @@ int setup_work(mfc_sim *s, int nsteps, int K, int w0, int w1) {
 }  // namespace
 }  // namespace mfc
 
+namespace mfc {
+
+int adj_begin(mfc_sim *s, int nsteps, int checkpoint_every, int w0, int w1) {
+  if (nsteps < 1 || checkpoint_every < 1 || w0 < 0 || w1 <= w0 || w1 > nsteps)
+    return fail(MFC_E_INVALID, "need nsteps >= 1, checkpoint_every >= 1 and 0 <= w0 < w1 <= nsteps");
+  if (s->precision != MFC_PREC_FP64 || !s->k.promote)
+    return fail(MFC_E_INVALID, "the adjoint needs precision FP64 and promote_fp64 = 1");
+  int64_t base = 0;
+  int rc = mfc_force_history(s, nullptr, 0, &base);
+  if (rc) return rc;
+  if (base + nsteps > HIST_CAP) return fail(MFC_E_INVALID, "force history full: reset it before the run");
+  if ((rc = setup_work(s, nsteps, checkpoint_every, w0, w1))) return rc;
+  s->adj->hist0 = base;
+  return MFC_OK;
+}
+
+int adj_checkpoint(mfc_sim *s, int t) {
+  adj_work *w = s->adj;
+  if (t % w->K) return MFC_OK;
+  const int c = s->cur;
+  return save(s->stream, w->ckpt[t / w->K], s->set[c].px, s->pos.nx, s->bins.key_sorted, s->tool + c, s->k.N);
+}
+
+int adj_finish(mfc_sim *s, double Fmean[2]) {
+  adj_work *w = s->adj;
+  const int n = w->w1 - w->w0;
+  std::vector<double> F((size_t)2 * n);
+  HIP_TRY(hipMemcpyAsync(F.data(), s->hist + 2 * (w->hist0 + w->w0), sizeof(double) * F.size(),
+                         hipMemcpyDeviceToHost, s->stream));
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  double sx = 0., sy = 0.;
+  for (int t = 0; t < n; t++) {
+    sx += F[2 * t];
+    sy += F[2 * t + 1];
+  }
+  w->Fmean[0] = Fmean[0] = sx / n;
+  w->Fmean[1] = Fmean[1] = sy / n;
+  w->valid = true;
+  return MFC_OK;
+}
+
+}  // namespace mfc
+
 using namespace mfc;
 
 extern "C" {
 
 int mfc_vjp_forward(mfc_sim *s, int nsteps, int checkpoint_every, int w0, int w1, double Fmean[2]) {
   if (!s || !Fmean) return fail(MFC_E_INVALID, "null argument");
-  if (nsteps < 1 || checkpoint_every < 1 || w0 < 0 || w1 <= w0 || w1 > nsteps)
-    return fail(MFC_E_INVALID, "need nsteps >= 1, checkpoint_every >= 1 and 0 <= w0 < w1 <= nsteps");
-  if (s->precision != MFC_PREC_FP64 || !s->k.promote)
-    return fail(MFC_E_INVALID, "the adjoint needs precision FP64 and promote_fp64 = 1");
-  int64_t base = 0;
-  {
-    int rc = mfc_force_history(s, nullptr, 0, &base);
-    if (rc) return rc;
-  }
-  if (base + nsteps > HIST_CAP) return fail(MFC_E_INVALID, "force history full: reset it before the run");
-  int rc = setup_work(s, nsteps, checkpoint_every, w0, w1);
+  if (s->batched) return fail(MFC_E_INVALID, "instance of a batch: use mfc_batch_vjp_forward");
+  int rc = adj_begin(s, nsteps, checkpoint_every, w0, w1);
   if (rc) return rc;
-  adj_work *w = s->adj;
-  const int N = s->k.N;
   for (int t = 0; t < nsteps; t++) {
-    const int c = s->cur;
-    if (t % w->K == 0 &&
-        (rc = save(s->stream, w->ckpt[t / w->K], s->set[c].px, s->pos.nx, s->bins.key_sorted, s->tool + c, N)))
-      return rc;
-    enqueue_step(s, c);
-    s->cur = 1 - c;
+    if ((rc = adj_checkpoint(s, t))) return rc;
+    enqueue_step(s, s->cur);
+    s->cur = 1 - s->cur;
     s->step++;
   }
   HIP_TRY(hipGetLastError());
-  std::vector<double> F((size_t)2 * (w1 - w0));
-  HIP_TRY(hipMemcpyAsync(F.data(), s->hist + 2 * (base + w0), sizeof(double) * F.size(), hipMemcpyDeviceToHost,
-                         s->stream));
-  HIP_TRY(hipStreamSynchronize(s->stream));
-  double sx = 0., sy = 0.;
-  for (int t = 0; t < w1 - w0; t++) {
-    sx += F[2 * t];
-    sy += F[2 * t + 1];
-  }
-  w->Fmean[0] = Fmean[0] = sx / (w1 - w0);
-  w->Fmean[1] = Fmean[1] = sy / (w1 - w0);
-  w->hist0 = base;
-  w->valid = true;
-  return MFC_OK;
+  return adj_finish(s, Fmean);
 }
 
 int mfc_vjp_backward(mfc_sim *s, const double cot[2], double grad[6]) {

@@ -217,6 +217,7 @@ int mfc_create(const mfc_config *cfg, mfc_sim **out) {
 
 int mfc_set_tuning(mfc_sim *s, const mfc_tuning *t) {
   if (!s || !t) return fail(MFC_E_INVALID, "null argument");
+  if (s->batched) return fail(MFC_E_INVALID, "instance of a batch: use mfc_batch_set_tuning");
   if (t->split_interact != 0 && t->split_interact != 1) return fail(MFC_E_INVALID, "split_interact must be 0 or 1");
   HIP_TRY(hipStreamSynchronize(s->stream));
   s->tuning = *t;
@@ -236,6 +237,7 @@ int mfc_get_tuning(const mfc_sim *s, mfc_tuning *t) {
 
 int mfc_step(mfc_sim *s, int nsteps) {
   if (!s || nsteps < 0) return fail(MFC_E_INVALID, "bad argument");
+  if (s->batched) return fail(MFC_E_INVALID, "instance of a batch: step it with mfc_batch_step");
   int done = 0;
   if (s->use_graphs && nsteps >= 4) {
     while (nsteps - done >= 2) {
@@ -260,6 +262,7 @@ int mfc_step(mfc_sim *s, int nsteps) {
 
 int mfc_profile(mfc_sim *s, int nsteps, double *ms) {
   if (!s || !ms || nsteps < 1) return fail(MFC_E_INVALID, "bad argument");
+  if (s->batched) return fail(MFC_E_INVALID, "instance of a batch: step it with mfc_batch_step");
   constexpr int NE = MFC_PROFILE_PHASES + 1;
   hipEvent_t ev[NE];
   for (int q = 0; q < NE; q++) HIP_TRY(hipEventCreate(&ev[q]));

@@ -1,0 +1,231 @@
+// mfc_batch.hip — many independent cutting instances stepped together (C ABI in include/mfc.h).
+//
+// A batch owns one stream and n instance handles created by mfc_create (each with its own
+// buffers and constants). One batched step is one launch per phase for all instances
+// (launch_batch_step): every block takes an (instance, tile) or (instance, scan chunk) item from
+// a work table and runs the single-instance device body on that instance's descriptor, so each
+// instance computes exactly the bits of its own single run while small instances share the
+// chip. Two consecutive batched steps are captured in a hipGraph per step parity.
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <vector>
+
+#include "mfc_sim.h"
+
+struct mfc_batch {
+  std::vector<mfc_sim *> sims;
+  hipStream_t stream = nullptr;
+  bool f32 = false;
+  mfc_tuning tuning = {0, {0, 0, 0, 0, 0, 0, 0}};
+  mfc::inst_args *args[2] = {nullptr, nullptr};  // descriptors per start parity (device)
+  int2 *tiles = nullptr, *chunks = nullptr;
+  int ntiles = 0, nchunks = 0;
+  int cur = 0;
+  int64_t step = 0;
+  hipGraphExec_t graph[2] = {nullptr, nullptr};
+
+  mfc::batch_work work(int parity) const {
+    return {args[parity], (int)sims.size(), tiles, ntiles, chunks, nchunks, f32, tuning.split_interact != 0};
+  }
+  void drop_graphs() {
+    for (int p = 0; p < 2; p++)
+      if (graph[p]) {
+        (void)hipGraphExecDestroy(graph[p]);
+        graph[p] = nullptr;
+      }
+  }
+};
+
+using mfc::fail;
+
+namespace {
+
+mfc::inst_args make_args(const mfc_sim *s, int c) {
+  mfc::inst_args a;
+  std::memset(&a, 0, sizeof(a));
+  a.k = s->k;
+  a.cur = s->set[c];
+  a.nxt = s->set[1 - c];
+  a.b = s->bins;
+  a.pos = s->pos;
+  a.r = s->recs;
+  a.tool_in = s->tool + c;
+  a.tool_out = s->tool + (1 - c);
+  a.f = s->force();
+  a.acc = s->acc;
+  a.ntiles = s->ntiles;
+  a.nscan = (s->k.num_cell + mfc::scan_chunk_cells() - 1) / mfc::scan_chunk_cells();
+  return a;
+}
+
+void batch_release(mfc_batch *b) {
+  if (!b) return;
+  if (b->stream) (void)hipStreamSynchronize(b->stream);
+  b->drop_graphs();
+  for (mfc_sim *s : b->sims) {
+    if (!s) continue;
+    s->stream = nullptr;  // the batch's stream, destroyed below
+    mfc_destroy(s);
+  }
+  for (void *p : {(void *)b->args[0], (void *)b->args[1], (void *)b->tiles, (void *)b->chunks})
+    if (p) (void)hipFree(p);
+  if (b->stream) (void)hipStreamDestroy(b->stream);
+  delete b;
+}
+
+int build_batch_graph(mfc_batch *b, int parity) {
+  hipGraph_t g = nullptr;
+  HIP_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+  mfc::launch_batch_step(b->work(parity), b->stream);
+  mfc::launch_batch_step(b->work(1 - parity), b->stream);
+  HIP_TRY(hipStreamEndCapture(b->stream, &g));
+  hipError_t e = hipGraphInstantiate(&b->graph[parity], g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  if (e != hipSuccess) return fail(MFC_E_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+  return MFC_OK;
+}
+
+void advance(mfc_batch *b) {
+  b->cur = 1 - b->cur;
+  b->step++;
+  for (mfc_sim *s : b->sims) {
+    s->cur = b->cur;
+    s->step++;
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+int mfc_batch_create(const mfc_config *cfgs, int n, mfc_batch **out) {
+  if (!cfgs || !out || n < 1) return fail(MFC_E_INVALID, "need n >= 1 configurations");
+  *out = nullptr;
+  for (int i = 1; i < n; i++)
+    if (cfgs[i].precision != cfgs[0].precision) return fail(MFC_E_INVALID, "all instances need the same precision");
+  mfc_batch *b = new mfc_batch();
+  b->f32 = cfgs[0].precision == MFC_PREC_FP32C;
+  auto bail = [&](int code) {
+    batch_release(b);
+    return code;
+  };
+  if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) return bail(fail(MFC_E_HIP, "stream"));
+  b->sims.assign(n, nullptr);
+  for (int i = 0; i < n; i++) {
+    mfc_sim *s = nullptr;
+    int rc = mfc_create(&cfgs[i], &s);
+    if (rc) {
+      std::string msg = "instance " + std::to_string(i) + ": " + mfc_last_error();
+      return bail(fail(rc, msg));
+    }
+    b->sims[i] = s;
+    if (hipStreamSynchronize(s->stream) != hipSuccess) return bail(fail(MFC_E_HIP, "instance setup"));
+    (void)hipStreamDestroy(s->stream);
+    s->stream = b->stream;
+    s->batched = true;
+    s->use_graphs = false;
+  }
+  std::vector<int2> tiles, chunks;
+  std::vector<mfc::inst_args> a0(n), a1(n);
+  for (int i = 0; i < n; i++) {
+    const mfc_sim *s = b->sims[i];
+    a0[i] = make_args(s, 0);
+    a1[i] = make_args(s, 1);
+    for (int t = 0; t < a0[i].ntiles; t++) tiles.push_back(make_int2(i, t));
+    for (int c = 0; c < a0[i].nscan; c++) chunks.push_back(make_int2(i, c));
+  }
+  b->ntiles = (int)tiles.size();
+  b->nchunks = (int)chunks.size();
+  if (hipMalloc(&b->args[0], sizeof(mfc::inst_args) * n) != hipSuccess ||
+      hipMalloc(&b->args[1], sizeof(mfc::inst_args) * n) != hipSuccess ||
+      hipMalloc(&b->tiles, sizeof(int2) * tiles.size()) != hipSuccess ||
+      hipMalloc(&b->chunks, sizeof(int2) * chunks.size()) != hipSuccess)
+    return bail(fail(MFC_E_NOMEM, "hipMalloc (batch tables)"));
+  if (hipMemcpyAsync(b->args[0], a0.data(), sizeof(mfc::inst_args) * n, hipMemcpyHostToDevice, b->stream) ||
+      hipMemcpyAsync(b->args[1], a1.data(), sizeof(mfc::inst_args) * n, hipMemcpyHostToDevice, b->stream) ||
+      hipMemcpyAsync(b->tiles, tiles.data(), sizeof(int2) * tiles.size(), hipMemcpyHostToDevice, b->stream) ||
+      hipMemcpyAsync(b->chunks, chunks.data(), sizeof(int2) * chunks.size(), hipMemcpyHostToDevice, b->stream) ||
+      hipStreamSynchronize(b->stream))
+    return bail(fail(MFC_E_HIP, "upload of the batch tables"));
+  *out = b;
+  return MFC_OK;
+}
+
+void mfc_batch_destroy(mfc_batch *b) { batch_release(b); }
+
+int mfc_batch_size(const mfc_batch *b) { return b ? (int)b->sims.size() : 0; }
+
+int mfc_batch_instance(mfc_batch *b, int i, mfc_sim **sim) {
+  if (!b || !sim || i < 0 || i >= (int)b->sims.size()) return fail(MFC_E_INVALID, "bad instance");
+  *sim = b->sims[i];
+  return MFC_OK;
+}
+
+int mfc_batch_set_tuning(mfc_batch *b, const mfc_tuning *t) {
+  if (!b || !t) return fail(MFC_E_INVALID, "null argument");
+  if (t->split_interact != 0 && t->split_interact != 1) return fail(MFC_E_INVALID, "split_interact must be 0 or 1");
+  HIP_TRY(hipStreamSynchronize(b->stream));
+  b->tuning = *t;
+  for (mfc_sim *s : b->sims) s->tuning = *t;
+  b->drop_graphs();
+  return MFC_OK;
+}
+
+int mfc_batch_step(mfc_batch *b, int nsteps) {
+  if (!b || nsteps < 0) return fail(MFC_E_INVALID, "bad argument");
+  int done = 0;
+  if (nsteps >= 4) {
+    while (nsteps - done >= 2) {
+      const int par = b->cur;
+      if (!b->graph[par]) {
+        int rc = build_batch_graph(b, par);
+        if (rc) return rc;
+      }
+      HIP_TRY(hipGraphLaunch(b->graph[par], b->stream));
+      advance(b);
+      advance(b);
+      done += 2;
+    }
+  }
+  for (; done < nsteps; done++) {
+    mfc::launch_batch_step(b->work(b->cur), b->stream);
+    advance(b);
+  }
+  HIP_TRY(hipGetLastError());
+  return MFC_OK;
+}
+
+int mfc_batch_sync(mfc_batch *b) {
+  if (!b) return fail(MFC_E_INVALID, "null handle");
+  HIP_TRY(hipStreamSynchronize(b->stream));
+  return MFC_OK;
+}
+
+int mfc_batch_vjp_forward(mfc_batch *b, int nsteps, int checkpoint_every, int w0, int w1, double *Fmean) {
+  if (!b || !Fmean) return fail(MFC_E_INVALID, "null argument");
+  int rc;
+  for (mfc_sim *s : b->sims)
+    if ((rc = mfc::adj_begin(s, nsteps, checkpoint_every, w0, w1))) return rc;
+  for (int t = 0; t < nsteps; t++) {
+    for (mfc_sim *s : b->sims)
+      if ((rc = mfc::adj_checkpoint(s, t))) return rc;
+    mfc::launch_batch_step(b->work(b->cur), b->stream);
+    advance(b);
+  }
+  HIP_TRY(hipGetLastError());
+  for (size_t i = 0; i < b->sims.size(); i++)
+    if ((rc = mfc::adj_finish(b->sims[i], Fmean + 2 * i))) return rc;
+  return MFC_OK;
+}
+
+int mfc_batch_vjp_backward(mfc_batch *b, const double *cot, double *grad) {
+  if (!b || !cot || !grad) return fail(MFC_E_INVALID, "null argument");
+  for (size_t i = 0; i < b->sims.size(); i++) {
+    int rc = mfc_vjp_backward(b->sims[i], cot + 2 * i, grad + 6 * i);
+    if (rc) return rc;
+  }
+  return MFC_OK;
+}
+
+}  // extern "C"

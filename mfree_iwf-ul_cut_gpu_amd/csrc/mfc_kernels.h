@@ -63,6 +63,37 @@ struct dforce {
 // internal flag: some value of the particle's neighbour record is NaN or Inf
 constexpr int FLAG_NONFINITE = 8;
 
+// one instance of a batch as the batched kernels see it (an array of these lives in device
+// memory, one per instance and step parity)
+struct inst_args {
+  dev_consts k;
+  dstate cur, nxt;  // state at the start of the step, and the sorted copy updated by the step
+  dbins b;
+  dpositional pos;
+  drecords r;
+  const mfc_tool_state *tool_in;
+  mfc_tool_state *tool_out;
+  dforce f;
+  double *acc;  // split interaction only
+  int ntiles;   // particle tiles of TILE
+  int nscan;    // scan chunks of the cell table
+};
+
+// the work of one batched step: instance descriptors and (instance, block) work tables
+struct batch_work {
+  const inst_args *args;  // n descriptors, device memory
+  int n;
+  const int2 *tiles;      // (instance, particle tile) for every tile of every instance
+  int ntiles;
+  const int2 *chunks;     // (instance, scan chunk) for every chunk of every cell table
+  int nchunks;
+  bool f32, split;
+};
+
+// one step of every instance of a batch: 9 (fused) or 10 (split) launches in all
+void launch_batch_step(const batch_work &w, hipStream_t s);
+int scan_chunk_cells();
+
 // step kernels
 void launch_binning(const dev_consts &k, const dstate &cur, dbins &b, hipStream_t s);
 void launch_predict(const dev_consts &k, const dstate &cur, const dstate &nxt, const dbins &b, const dpositional &pos,

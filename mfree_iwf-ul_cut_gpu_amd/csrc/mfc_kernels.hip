@@ -52,14 +52,15 @@ __device__ inline int block_exclusive_scan(int v, int &total) {
 }
 
 // ------------------------------------------------------------------------------- binning
+// Each kernel body is a device function of its work index, called by the single-instance kernel
+// (index from blockIdx) and by the batched kernel (index from the (instance, block) work table).
+
 // cell id of every particle (grid.cu:34-51; blanked particles keep their last sorted id,
 // grid.cu:39-41) and its arrival slot inside the cell
-__global__ __launch_bounds__(256) void k_hash_count(dev_consts k, const double *__restrict__ px,
-                                                    const double *__restrict__ py, const int *__restrict__ flags,
-                                                    const int *__restrict__ key_sorted, int *__restrict__ key,
-                                                    int *__restrict__ slot, int *__restrict__ count,
-                                                    unsigned long long *__restrict__ clamped) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ inline void hash_count(const dev_consts &k, int i, const double *__restrict__ px,
+                                  const double *__restrict__ py, const int *__restrict__ flags,
+                                  const int *__restrict__ key_sorted, int *__restrict__ key, int *__restrict__ slot,
+                                  int *__restrict__ count, unsigned long long *__restrict__ clamped) {
   if (i >= k.N) return;
   int h;
   if (flags[i] & MFC_FLAG_BLANKED) {
@@ -73,19 +74,18 @@ __global__ __launch_bounds__(256) void k_hash_count(dev_consts k, const double *
   slot[i] = atomicAdd(&count[h], 1);
 }
 
-__global__ __launch_bounds__(SCAN_THREADS) void k_scan_reduce(const int *__restrict__ count, int n,
-                                                               int *__restrict__ bsum) {
-  int base = blockIdx.x * SCAN_CHUNK + threadIdx.x * SCAN_ITEMS;
+__device__ inline void scan_reduce(int blk, const int *__restrict__ count, int n, int *__restrict__ bsum) {
+  int base = blk * SCAN_CHUNK + threadIdx.x * SCAN_ITEMS;
   int s = 0;
 #pragma unroll
   for (int q = 0; q < SCAN_ITEMS; q++)
     if (base + q < n) s += count[base + q];
   int total;
   block_exclusive_scan(s, total);
-  if (threadIdx.x == 0) bsum[blockIdx.x] = total;
+  if (threadIdx.x == 0) bsum[blk] = total;
 }
 
-__global__ __launch_bounds__(SCAN_THREADS) void k_scan_top(int *__restrict__ bsum, int nb) {
+__device__ inline void scan_top(int *__restrict__ bsum, int nb) {
   int carry = 0;
   for (int c0 = 0; c0 < nb; c0 += SCAN_CHUNK) {
     int base = c0 + threadIdx.x * SCAN_ITEMS;
@@ -109,9 +109,9 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan_top(int *__restrict__ bsu
 }
 
 // cell offsets from the counts; the counts are cleared for the next step
-__global__ __launch_bounds__(SCAN_THREADS) void k_scan_down(int *__restrict__ count, int n,
-                                                             const int *__restrict__ bsum, int *__restrict__ offsets) {
-  int base = blockIdx.x * SCAN_CHUNK + threadIdx.x * SCAN_ITEMS;
+__device__ inline void scan_down(int blk, int *__restrict__ count, int n, const int *__restrict__ bsum,
+                                 int *__restrict__ offsets) {
+  int base = blk * SCAN_CHUNK + threadIdx.x * SCAN_ITEMS;
   int v[SCAN_ITEMS];
   int s = 0;
 #pragma unroll
@@ -120,7 +120,7 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan_down(int *__restrict__ co
     s += v[q];
   }
   int total;
-  int pre = block_exclusive_scan(s, total) + bsum[blockIdx.x];
+  int pre = block_exclusive_scan(s, total) + bsum[blk];
 #pragma unroll
   for (int q = 0; q < SCAN_ITEMS; q++) {
     int c = base + q;
@@ -133,19 +133,16 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan_down(int *__restrict__ co
   }
 }
 
-__global__ __launch_bounds__(256) void k_place(int N, const int *__restrict__ key, const int *__restrict__ slot,
-                                               const int *__restrict__ offsets, int *__restrict__ bucket) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ inline void place(int i, int N, const int *__restrict__ key, const int *__restrict__ slot,
+                             const int *__restrict__ offsets, int *__restrict__ bucket) {
   if (i >= N) return;
   bucket[offsets[key[i]] + slot[i]] = i;
 }
 
 // stable position: the cell's offset plus the number of particles of the same cell that
 // precede this one in the current order (thrust::sort_by_key on (hash, 0..N-1))
-__global__ __launch_bounds__(256) void k_rank(int N, const int *__restrict__ key, const int *__restrict__ bucket,
-                                              const int *__restrict__ offsets, int *__restrict__ perm,
-                                              int *__restrict__ key_sorted) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ inline void rank(int i, int N, const int *__restrict__ key, const int *__restrict__ bucket,
+                            const int *__restrict__ offsets, int *__restrict__ perm, int *__restrict__ key_sorted) {
   if (i >= N) return;
   int h = key[i];
   int b = offsets[h], e = offsets[h + 1];
@@ -153,6 +150,37 @@ __global__ __launch_bounds__(256) void k_rank(int N, const int *__restrict__ key
   for (int q = b; q < e; q++) r += bucket[q] < i;
   perm[b + r] = i;
   key_sorted[b + r] = h;
+}
+
+__global__ __launch_bounds__(256) void k_hash_count(dev_consts k, const double *__restrict__ px,
+                                                    const double *__restrict__ py, const int *__restrict__ flags,
+                                                    const int *__restrict__ key_sorted, int *__restrict__ key,
+                                                    int *__restrict__ slot, int *__restrict__ count,
+                                                    unsigned long long *__restrict__ clamped) {
+  hash_count(k, blockIdx.x * blockDim.x + threadIdx.x, px, py, flags, key_sorted, key, slot, count, clamped);
+}
+
+__global__ __launch_bounds__(SCAN_THREADS) void k_scan_reduce(const int *__restrict__ count, int n,
+                                                               int *__restrict__ bsum) {
+  scan_reduce(blockIdx.x, count, n, bsum);
+}
+
+__global__ __launch_bounds__(SCAN_THREADS) void k_scan_top(int *__restrict__ bsum, int nb) { scan_top(bsum, nb); }
+
+__global__ __launch_bounds__(SCAN_THREADS) void k_scan_down(int *__restrict__ count, int n,
+                                                             const int *__restrict__ bsum, int *__restrict__ offsets) {
+  scan_down(blockIdx.x, count, n, bsum, offsets);
+}
+
+__global__ __launch_bounds__(256) void k_place(int N, const int *__restrict__ key, const int *__restrict__ slot,
+                                               const int *__restrict__ offsets, int *__restrict__ bucket) {
+  place(blockIdx.x * blockDim.x + threadIdx.x, N, key, slot, offsets, bucket);
+}
+
+__global__ __launch_bounds__(256) void k_rank(int N, const int *__restrict__ key, const int *__restrict__ bucket,
+                                              const int *__restrict__ offsets, int *__restrict__ perm,
+                                              int *__restrict__ key_sorted) {
+  rank(blockIdx.x * blockDim.x + threadIdx.x, N, key, bucket, offsets, perm, key_sorted);
 }
 
 // ------------------------------------------------------------------------------- predict
@@ -180,13 +208,10 @@ __device__ inline void artificial_stress_f32(float rho, float p, float Sxx, floa
 // gather into sorted order, leap-frog init + predict, EOS, artificial stress, neighbour record.
 // F32: the record is 16 floats (64 bytes) plus the FP64 predicted position.
 template <bool F32>
-__global__ __launch_bounds__(TILE) void k_predict(dev_consts k, int ntiles, const int *__restrict__ perm, dstate cur,
-                                                  dstate nxt, dpositional pos, double *__restrict__ rec,
-                                                  double2 *__restrict__ posd, float *__restrict__ rec32,
-                                                  const mfc_tool_state *__restrict__ tool_in,
-                                                  mfc_tool_state *__restrict__ tool_out) {
-  const int t = xcd_tile(blockIdx.x, ntiles);
-  if (t < 0) return;
+__device__ inline void predict(const dev_consts &k, int t, const int *__restrict__ perm, const dstate &cur,
+                               const dstate &nxt, const dpositional &pos, double *__restrict__ rec,
+                               double2 *__restrict__ posd, float *__restrict__ rec32,
+                               const mfc_tool_state *__restrict__ tool_in, mfc_tool_state *__restrict__ tool_out) {
   if (t == 0 && threadIdx.x == 0) {
     mfc_tool_state ts = *tool_in;
     tool_move(ts, k.dt);
@@ -269,6 +294,17 @@ __global__ __launch_bounds__(TILE) void k_predict(dev_consts k, int ntiles, cons
   double2 *dst = reinterpret_cast<double2 *>(rec + (size_t)REC * i);
 #pragma unroll
   for (int q = 0; q < REC / 2; q++) dst[q] = make_double2(r[2 * q], r[2 * q + 1]);
+}
+
+template <bool F32>
+__global__ __launch_bounds__(TILE) void k_predict(dev_consts k, int ntiles, const int *__restrict__ perm, dstate cur,
+                                                  dstate nxt, dpositional pos, double *__restrict__ rec,
+                                                  double2 *__restrict__ posd, float *__restrict__ rec32,
+                                                  const mfc_tool_state *__restrict__ tool_in,
+                                                  mfc_tool_state *__restrict__ tool_out) {
+  const int t = xcd_tile(blockIdx.x, ntiles);
+  if (t < 0) return;
+  predict<F32>(k, t, perm, cur, nxt, pos, rec, posd, rec32, tool_in, tool_out);
 }
 
 // ------------------------------------------------------------------------------- interact
@@ -519,13 +555,11 @@ __device__ inline void force_partials(int t, const double fsum[4], double *__res
 
 // fused: sums + update in one pass (registers of both live in one kernel)
 template <bool F32>
-__global__ __launch_bounds__(TILE) void k_interact(dev_consts k, int ntiles, const double *__restrict__ rec,
-                                                   const double2 *__restrict__ posd, const float *__restrict__ rec32,
-                                                   const int *__restrict__ key_sorted,
-                                                   const int *__restrict__ offsets, dstate st, dpositional pos,
-                                                   const mfc_tool_state *__restrict__ tool_now, dforce fo) {
-  const int t = xcd_tile(blockIdx.x, ntiles);
-  if (t < 0) return;
+__device__ inline void interact(const dev_consts &k, int t, const double *__restrict__ rec,
+                                const double2 *__restrict__ posd, const float *__restrict__ rec32,
+                                const int *__restrict__ key_sorted, const int *__restrict__ offsets,
+                                const dstate &st, const dpositional &pos, const mfc_tool_state *__restrict__ tool_now,
+                                const dforce &fo) {
   const int i = t * TILE + threadIdx.x;
   double fsum[4] = {0., 0., 0., 0.};
   if (i < k.N) {
@@ -537,14 +571,12 @@ __global__ __launch_bounds__(TILE) void k_interact(dev_consts k, int ntiles, con
   force_partials(t, fsum, fo.partials);
 }
 
-// split, first half: the neighbour sums of every particle into acc (L_N arrays of N doubles)
+// split, first half: the neighbour sums of every particle into acc (ACC_N arrays of N doubles)
 template <bool F32>
-__global__ __launch_bounds__(TILE) void k_sums(dev_consts k, int ntiles, const double *__restrict__ rec,
-                                               const double2 *__restrict__ posd, const float *__restrict__ rec32,
-                                               const int *__restrict__ key_sorted, const int *__restrict__ offsets,
-                                               const int *__restrict__ flags, double *__restrict__ acc) {
-  const int t = xcd_tile(blockIdx.x, ntiles);
-  if (t < 0) return;
+__device__ inline void sums(const dev_consts &k, int t, const double *__restrict__ rec,
+                            const double2 *__restrict__ posd, const float *__restrict__ rec32,
+                            const int *__restrict__ key_sorted, const int *__restrict__ offsets,
+                            const int *__restrict__ flags, double *__restrict__ acc) {
   const int i = t * TILE + threadIdx.x;
   if (i >= k.N) return;
   pair_acc a;
@@ -554,11 +586,8 @@ __global__ __launch_bounds__(TILE) void k_sums(dev_consts k, int ntiles, const d
 }
 
 // split, second half: the per-particle update from the stored sums
-__global__ __launch_bounds__(TILE) void k_update(dev_consts k, int ntiles, const double *__restrict__ acc,
-                                                 dstate st, dpositional pos,
-                                                 const mfc_tool_state *__restrict__ tool_now, dforce fo) {
-  const int t = xcd_tile(blockIdx.x, ntiles);
-  if (t < 0) return;
+__device__ inline void update(const dev_consts &k, int t, const double *__restrict__ acc, const dstate &st,
+                              const dpositional &pos, const mfc_tool_state *__restrict__ tool_now, const dforce &fo) {
   const int i = t * TILE + threadIdx.x;
   double fsum[4] = {0., 0., 0., 0.};
   if (i < k.N) {
@@ -572,7 +601,7 @@ __global__ __launch_bounds__(TILE) void k_update(dev_consts k, int ntiles, const
 
 // tool force of the step from the tile partials: lane l adds the tile results l, l + TILE, ...
 // in ascending order, then the lane sums are reduced by halving strides; F = sum(fc) + sum(ft)
-__global__ __launch_bounds__(TILE) void k_force_final(int ntiles, dforce fo) {
+__device__ inline void force_final(int ntiles, const dforce &fo) {
   __shared__ double red[4][TILE];
   double a0 = 0., a1 = 0., a2 = 0., a3 = 0.;
   for (int q = threadIdx.x; q < ntiles; q += TILE) {
@@ -596,6 +625,136 @@ __global__ __launch_bounds__(TILE) void k_force_final(int ntiles, dforce fo) {
     if (n >= 0 && n < fo.cap) { fo.hist[2 * n] = F0; fo.hist[2 * n + 1] = F1; }
     *fo.ctr = n + 1;
   }
+}
+
+template <bool F32>
+__global__ __launch_bounds__(TILE) void k_interact(dev_consts k, int ntiles, const double *__restrict__ rec,
+                                                   const double2 *__restrict__ posd, const float *__restrict__ rec32,
+                                                   const int *__restrict__ key_sorted,
+                                                   const int *__restrict__ offsets, dstate st, dpositional pos,
+                                                   const mfc_tool_state *__restrict__ tool_now, dforce fo) {
+  const int t = xcd_tile(blockIdx.x, ntiles);
+  if (t < 0) return;
+  interact<F32>(k, t, rec, posd, rec32, key_sorted, offsets, st, pos, tool_now, fo);
+}
+
+template <bool F32>
+__global__ __launch_bounds__(TILE) void k_sums(dev_consts k, int ntiles, const double *__restrict__ rec,
+                                               const double2 *__restrict__ posd, const float *__restrict__ rec32,
+                                               const int *__restrict__ key_sorted, const int *__restrict__ offsets,
+                                               const int *__restrict__ flags, double *__restrict__ acc) {
+  const int t = xcd_tile(blockIdx.x, ntiles);
+  if (t < 0) return;
+  sums<F32>(k, t, rec, posd, rec32, key_sorted, offsets, flags, acc);
+}
+
+__global__ __launch_bounds__(TILE) void k_update(dev_consts k, int ntiles, const double *__restrict__ acc,
+                                                 dstate st, dpositional pos,
+                                                 const mfc_tool_state *__restrict__ tool_now, dforce fo) {
+  const int t = xcd_tile(blockIdx.x, ntiles);
+  if (t < 0) return;
+  update(k, t, acc, st, pos, tool_now, fo);
+}
+
+__global__ __launch_bounds__(TILE) void k_force_final(int ntiles, dforce fo) { force_final(ntiles, fo); }
+
+// ------------------------------------------------------------------------------- batched
+// One launch per phase for all instances of a batch: block b takes work item xcd_tile(b) of the
+// (instance, block) table, and runs the same device body as the single-instance kernel with
+// that instance's constants and buffers, so every instance computes exactly its own run.
+__global__ __launch_bounds__(256) void kb_hash_count(const inst_args *__restrict__ A, const int2 *__restrict__ work,
+                                                     int nwork) {
+  const int w = xcd_tile(blockIdx.x, nwork);
+  if (w < 0) return;
+  const int2 it = work[w];
+  const inst_args &a = A[it.x];
+  hash_count(a.k, it.y * 256 + threadIdx.x, a.cur.px, a.cur.py, a.cur.flags, a.b.key_sorted, a.b.key, a.b.slot,
+             a.b.count, a.b.clamped);
+}
+
+__global__ __launch_bounds__(SCAN_THREADS) void kb_scan_reduce(const inst_args *__restrict__ A,
+                                                                const int2 *__restrict__ work, int nwork) {
+  const int w = xcd_tile(blockIdx.x, nwork);
+  if (w < 0) return;
+  const int2 it = work[w];
+  const inst_args &a = A[it.x];
+  scan_reduce(it.y, a.b.count, a.k.num_cell, a.b.bsum);
+}
+
+__global__ __launch_bounds__(SCAN_THREADS) void kb_scan_top(const inst_args *__restrict__ A) {
+  const inst_args &a = A[blockIdx.x];
+  scan_top(a.b.bsum, a.nscan);
+}
+
+__global__ __launch_bounds__(SCAN_THREADS) void kb_scan_down(const inst_args *__restrict__ A,
+                                                              const int2 *__restrict__ work, int nwork) {
+  const int w = xcd_tile(blockIdx.x, nwork);
+  if (w < 0) return;
+  const int2 it = work[w];
+  const inst_args &a = A[it.x];
+  scan_down(it.y, a.b.count, a.k.num_cell, a.b.bsum, a.b.offsets);
+}
+
+__global__ __launch_bounds__(256) void kb_place(const inst_args *__restrict__ A, const int2 *__restrict__ work,
+                                                int nwork) {
+  const int w = xcd_tile(blockIdx.x, nwork);
+  if (w < 0) return;
+  const int2 it = work[w];
+  const inst_args &a = A[it.x];
+  place(it.y * 256 + threadIdx.x, a.k.N, a.b.key, a.b.slot, a.b.offsets, a.b.bucket);
+}
+
+__global__ __launch_bounds__(256) void kb_rank(const inst_args *__restrict__ A, const int2 *__restrict__ work,
+                                               int nwork) {
+  const int w = xcd_tile(blockIdx.x, nwork);
+  if (w < 0) return;
+  const int2 it = work[w];
+  const inst_args &a = A[it.x];
+  rank(it.y * 256 + threadIdx.x, a.k.N, a.b.key, a.b.bucket, a.b.offsets, a.b.perm, a.b.key_sorted);
+}
+
+template <bool F32>
+__global__ __launch_bounds__(TILE) void kb_predict(const inst_args *__restrict__ A, const int2 *__restrict__ work,
+                                                   int nwork) {
+  const int w = xcd_tile(blockIdx.x, nwork);
+  if (w < 0) return;
+  const int2 it = work[w];
+  const inst_args &a = A[it.x];
+  predict<F32>(a.k, it.y, a.b.perm, a.cur, a.nxt, a.pos, a.r.rec, a.r.posd, a.r.rec32, a.tool_in, a.tool_out);
+}
+
+template <bool F32>
+__global__ __launch_bounds__(TILE) void kb_interact(const inst_args *__restrict__ A, const int2 *__restrict__ work,
+                                                    int nwork) {
+  const int w = xcd_tile(blockIdx.x, nwork);
+  if (w < 0) return;
+  const int2 it = work[w];
+  const inst_args &a = A[it.x];
+  interact<F32>(a.k, it.y, a.r.rec, a.r.posd, a.r.rec32, a.b.key_sorted, a.b.offsets, a.nxt, a.pos, a.tool_out, a.f);
+}
+
+template <bool F32>
+__global__ __launch_bounds__(TILE) void kb_sums(const inst_args *__restrict__ A, const int2 *__restrict__ work,
+                                                int nwork) {
+  const int w = xcd_tile(blockIdx.x, nwork);
+  if (w < 0) return;
+  const int2 it = work[w];
+  const inst_args &a = A[it.x];
+  sums<F32>(a.k, it.y, a.r.rec, a.r.posd, a.r.rec32, a.b.key_sorted, a.b.offsets, a.nxt.flags, a.acc);
+}
+
+__global__ __launch_bounds__(TILE) void kb_update(const inst_args *__restrict__ A, const int2 *__restrict__ work,
+                                                  int nwork) {
+  const int w = xcd_tile(blockIdx.x, nwork);
+  if (w < 0) return;
+  const int2 it = work[w];
+  const inst_args &a = A[it.x];
+  update(a.k, it.y, a.acc, a.nxt, a.pos, a.tool_out, a.f);
+}
+
+__global__ __launch_bounds__(TILE) void kb_force_final(const inst_args *__restrict__ A) {
+  const inst_args &a = A[blockIdx.x];
+  force_final(a.ntiles, a.f);
 }
 
 __global__ void k_device_math(int fn, const double *__restrict__ x, const double *__restrict__ y,
@@ -699,6 +858,30 @@ void launch_interact(const dev_consts &k, const dstate &st, const dbins &b, cons
 void launch_device_math(int fn, const double *x, const double *y, double *out, int64_t n, hipStream_t s) {
   const int64_t g = (n + 255) / 256;
   hipLaunchKernelGGL(k_device_math, dim3((unsigned)g), dim3(256), 0, s, fn, x, y, out, n);
+}
+
+int scan_chunk_cells() { return SCAN_CHUNK; }
+
+void launch_batch_step(const batch_work &w, hipStream_t s) {
+  const dim3 gt(padded_tiles(w.ntiles)), gc(padded_tiles(w.nchunks));
+  hipLaunchKernelGGL(kb_hash_count, gt, dim3(256), 0, s, w.args, w.tiles, w.ntiles);
+  hipLaunchKernelGGL(kb_scan_reduce, gc, dim3(SCAN_THREADS), 0, s, w.args, w.chunks, w.nchunks);
+  hipLaunchKernelGGL(kb_scan_top, dim3(w.n), dim3(SCAN_THREADS), 0, s, w.args);
+  hipLaunchKernelGGL(kb_scan_down, gc, dim3(SCAN_THREADS), 0, s, w.args, w.chunks, w.nchunks);
+  hipLaunchKernelGGL(kb_place, gt, dim3(256), 0, s, w.args, w.tiles, w.ntiles);
+  hipLaunchKernelGGL(kb_rank, gt, dim3(256), 0, s, w.args, w.tiles, w.ntiles);
+  if (w.f32) hipLaunchKernelGGL(kb_predict<true>, gt, dim3(TILE), 0, s, w.args, w.tiles, w.ntiles);
+  else hipLaunchKernelGGL(kb_predict<false>, gt, dim3(TILE), 0, s, w.args, w.tiles, w.ntiles);
+  if (w.split) {
+    if (w.f32) hipLaunchKernelGGL(kb_sums<true>, gt, dim3(TILE), 0, s, w.args, w.tiles, w.ntiles);
+    else hipLaunchKernelGGL(kb_sums<false>, gt, dim3(TILE), 0, s, w.args, w.tiles, w.ntiles);
+    hipLaunchKernelGGL(kb_update, gt, dim3(TILE), 0, s, w.args, w.tiles, w.ntiles);
+  } else if (w.f32) {
+    hipLaunchKernelGGL(kb_interact<true>, gt, dim3(TILE), 0, s, w.args, w.tiles, w.ntiles);
+  } else {
+    hipLaunchKernelGGL(kb_interact<false>, gt, dim3(TILE), 0, s, w.args, w.tiles, w.ntiles);
+  }
+  hipLaunchKernelGGL(kb_force_final, dim3(w.n), dim3(TILE), 0, s, w.args);
 }
 
 }  // namespace mfc

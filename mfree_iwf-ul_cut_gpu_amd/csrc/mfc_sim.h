@@ -25,9 +25,16 @@ int fail(int code, const std::string &msg);
 
 constexpr int64_t HIST_CAP = 1 << 20;
 
+struct mfc_sim;
+
 namespace mfc {
 struct adj_work;  // reverse-mode workspace (mfc_adjoint.hip)
 void adj_free(adj_work *w);
+// taped forward in three parts, so that a batch can interleave its instances: validate and
+// allocate, keep the state at the start of step t when t is a checkpoint, and mean the window
+int adj_begin(mfc_sim *s, int nsteps, int checkpoint_every, int w0, int w1);
+int adj_checkpoint(mfc_sim *s, int t);
+int adj_finish(mfc_sim *s, double Fmean[2]);
 }  // namespace mfc
 
 struct mfc_sim {
@@ -51,6 +58,7 @@ struct mfc_sim {
   hipGraphExec_t graph[2] = {nullptr, nullptr};  // two steps starting from parity 0 / 1
   bool use_graphs = true;
   mfc::adj_work *adj = nullptr;    // reverse-mode tape of the last mfc_vjp_forward
+  bool batched = false;            // instance of a batch: stepped by mfc_batch_step only
   double *acc = nullptr;           // neighbour sums between the split interaction kernels
   mfc_tuning tuning = {0, {0, 0, 0, 0, 0, 0, 0}};
   double *interact_acc() const { return tuning.split_interact ? acc : nullptr; }

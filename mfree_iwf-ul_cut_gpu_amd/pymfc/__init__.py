@@ -148,6 +148,16 @@ def lib() -> C.CDLL:
         L.mfc_vjp_forward.argtypes = [vp, i32, i32, i32, i32, dp]
         L.mfc_vjp_backward.argtypes = [vp, dp, dp]
         L.mfc_vjp.argtypes = [vp, C.POINTER(VjpReq), dp]
+        L.mfc_batch_create.argtypes = [C.POINTER(Config), i32, C.POINTER(vp)]
+        L.mfc_batch_destroy.argtypes = [vp]
+        L.mfc_batch_destroy.restype = None
+        L.mfc_batch_size.argtypes = [vp]
+        L.mfc_batch_instance.argtypes = [vp, i32, C.POINTER(vp)]
+        L.mfc_batch_set_tuning.argtypes = [vp, C.POINTER(Tuning)]
+        L.mfc_batch_step.argtypes = [vp, i32]
+        L.mfc_batch_sync.argtypes = [vp]
+        L.mfc_batch_vjp_forward.argtypes = [vp, i32, i32, i32, i32, dp]
+        L.mfc_batch_vjp_backward.argtypes = [vp, dp, dp]
         L.mfc_set_tuning.argtypes = [vp, C.POINTER(Tuning)]
         L.mfc_get_tuning.argtypes = [vp, C.POINTER(Tuning)]
         _lib = L
@@ -402,3 +412,93 @@ def scenario(cfg: Config) -> dict:
                               flags.ctypes.data_as(C.POINTER(C.c_int32)), C.byref(t)), "mfc_scenario")
     return {"info": SimInfo(*[getattr(info, n) for n, _ in Info._fields_]), "pos": pos, "rho": rho, "h": h,
             "T": T, "flags": flags, "tool": t.as_arrays()}
+
+
+class _Instance(Sim):
+    """Read-out view of one instance of a Batch (owned by the batch)."""
+
+    def __init__(self, handle, cfg, owner):
+        self._h = handle
+        self.cfg = cfg
+        self._owner = owner  # keeps the batch alive
+
+    def close(self) -> None:
+        self._h = C.c_void_p()
+
+
+class Batch:
+    """n independent instances stepped together: one launch per phase for all of them."""
+
+    def __init__(self, cfgs):
+        arr = (Config * len(cfgs))(*cfgs)
+        self._h = C.c_void_p()
+        _check(lib().mfc_batch_create(arr, len(cfgs), C.byref(self._h)), "mfc_batch_create")
+        self.cfgs = [c.copy() for c in cfgs]
+        self.instances = []
+        for i, c in enumerate(self.cfgs):
+            h = C.c_void_p()
+            _check(lib().mfc_batch_instance(self._h, i, C.byref(h)), "mfc_batch_instance")
+            self.instances.append(_Instance(h, c, self))
+
+    def __len__(self):
+        return len(self.instances)
+
+    def __getitem__(self, i) -> Sim:
+        return self.instances[i]
+
+    def close(self) -> None:
+        if self._h:
+            for inst in self.instances:
+                inst.close()
+            lib().mfc_batch_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_tuning(self, **kw) -> None:
+        t = Tuning()
+        for k, v in kw.items():
+            setattr(t, k, int(v))
+        _check(lib().mfc_batch_set_tuning(self._h, C.byref(t)), "mfc_batch_set_tuning")
+
+    def step(self, n: int = 1) -> None:
+        _check(lib().mfc_batch_step(self._h, int(n)), "mfc_batch_step")
+
+    def sync(self) -> None:
+        _check(lib().mfc_batch_sync(self._h), "mfc_batch_sync")
+
+    def vjp_forward(self, nsteps: int, checkpoint_every: int, w0: int, w1: int) -> np.ndarray:
+        F = np.zeros((len(self), 2))
+        _check(lib().mfc_batch_vjp_forward(self._h, int(nsteps), int(checkpoint_every), int(w0), int(w1),
+                                           F.ctypes.data_as(C.POINTER(C.c_double))), "mfc_batch_vjp_forward")
+        return F
+
+    def vjp_backward(self, cot) -> np.ndarray:
+        c = np.ascontiguousarray(cot, dtype=np.float64).reshape(len(self), 2)
+        g = np.zeros((len(self), 6))
+        _check(lib().mfc_batch_vjp_backward(self._h, c.ctypes.data_as(C.POINTER(C.c_double)),
+                                            g.ctypes.data_as(C.POINTER(C.c_double))), "mfc_batch_vjp_backward")
+        return g
+
+
+def batch_configs(n: int = 32, seed: int = 0x355, precision: int = PREC_FP64) -> list:
+    """Independent cuts drawn from a seed: depth of cut U[20, 100] um, speed U[30, 200] m/min,
+    dx in {4, 6, 10} um, Johnson-Cook constants each scaled by U[0.9, 1.1], workpiece
+    U[0.8, 2.0] mm x U[0.25, 0.5] mm (about 2k to 60k particles)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    A, B, nn, Cc, m = 1098.0, 1092.0, 0.93, 0.014, 1.1
+    for _ in range(n):
+        dx = float(rng.choice([4.0, 6.0, 10.0]))
+        nx = int(round(rng.uniform(0.8, 2.0) * 1000.0 / dx))
+        ny = int(round(rng.uniform(0.25, 0.5) * 1000.0 / dx))
+        f = rng.uniform(0.9, 1.1, 5)
+        jc = (A * f[0], B * f[1], nn * f[2], Cc * f[3], m * f[4])
+        out.append(cut_config(nx, ny, dx, thermal="adiabatic", speed_m_min=float(rng.uniform(30.0, 200.0)),
+                              feed_um=float(rng.uniform(20.0, 100.0)), jc=jc, precision=precision))
+    return out
+

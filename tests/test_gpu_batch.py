@@ -1,0 +1,82 @@
+"""Batches of independent cutting instances: every instance of a batched run must equal its own
+single-instance run bit for bit (forward, force history, reverse mode), for uneven sizes,
+both precisions and both interaction variants."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import pymfc
+from parity import equal
+
+pytestmark = pytest.mark.gpu
+
+FIELDS = ("pos", "vel", "S", "T", "eps_pl", "eps_pl_dot", "rho", "fc", "ft", "flags", "id", "hash")
+
+
+def compare_instance(inst, single, what):
+    for f in FIELDS:
+        assert equal(inst.field(f), single.field(f)), f"{what}: field {f}"
+    assert equal(inst.force_history(), single.force_history()), f"{what}: force history"
+    assert equal(inst.tool_force(), single.tool_force()), f"{what}: tool force"
+
+
+@pytest.mark.parametrize("prec,split", [(pymfc.PREC_FP64, 0), (pymfc.PREC_FP64, 1), (pymfc.PREC_FP32C, 0),
+                                        (pymfc.PREC_FP32C, 1)])
+def test_batch_equals_single_runs(prec, split):
+    cfgs = pymfc.batch_configs(8, seed=11, precision=prec)
+    b = pymfc.Batch(cfgs)
+    b.set_tuning(split_interact=split)
+    b.step(7)   # graph pairs plus one eager step
+    b.step(3)
+    b.sync()
+    sizes = [inst.N for inst in b.instances]
+    assert max(sizes) > 4 * min(sizes)  # uneven
+    for i, c in enumerate(cfgs):
+        s = pymfc.Sim(c)
+        s.step(10)
+        compare_instance(b[i], s, f"instance {i} ({sizes[i]} particles)")
+        s.close()
+    b.close()
+
+
+def test_c5_batch_of_32_matches_single_runs():
+    cfgs = pymfc.batch_configs(32)
+    b = pymfc.Batch(cfgs)
+    b.step(20)
+    b.sync()
+    n = [inst.N for inst in b.instances]
+    print("instances", len(n), "particles", sum(n), "min", min(n), "max", max(n))
+    for i in (0, 7, 15, 31):  # a sample of the 32 single runs
+        s = pymfc.Sim(cfgs[i])
+        s.step(20)
+        compare_instance(b[i], s, f"instance {i}")
+        s.close()
+    b.close()
+
+
+def test_batch_reverse_mode_equals_single():
+    cfgs = []
+    for c in pymfc.batch_configs(3, seed=5):
+        c.promote_fp64 = 1
+        c.secant_tol = 1e-14
+        cfgs.append(c)
+    b = pymfc.Batch(cfgs)
+    F = b.vjp_forward(30, 10, 10, 30)
+    cot = np.array([[1.0, 0.5], [0.0, 1.0], [-1.0, 2.0]])
+    G = b.vjp_backward(cot)
+    for i, c in enumerate(cfgs):
+        s = pymfc.Sim(c)
+        Fi = s.vjp_forward(30, 10, 10, 30)
+        Gi = s.vjp_backward(cot[i])
+        assert equal(F[i], Fi), i
+        assert equal(G[i], Gi), i
+        s.close()
+    b.close()
+
+
+def test_batch_instances_refuse_single_steps():
+    b = pymfc.Batch(pymfc.batch_configs(2, seed=3))
+    with pytest.raises(pymfc.MfcError):
+        b[0].step(1)
+    b.close()
