@@ -72,21 +72,25 @@ def cpu_baseline(cfg, target_s: float) -> dict:
                       f"{wall:.1f} s wall on {ncpu} threads ({model})"}
 
 
+VARIANTS = [(0, 0), (0, 1), (1, 0), (1, 1)]  # (split_interact, compact_pairs)
+
+
 def tune(pymfc, sim, steps: int) -> dict:
-    """Autotuner: time each launch variant of the interaction on the live handle (the results
-    are the same bits for every variant) and keep the fastest."""
+    """Autotuner: time each launch variant of the interaction on the live handle (every variant
+    gives the same bits) and keep the fastest."""
     cand = {}
-    for split in (0, 1):
-        sim.set_tuning(split_interact=split)
+    for split, compact in VARIANTS:
+        sim.set_tuning(split_interact=split, compact_pairs=compact)
         sim.step(4)
         sim.sync()
         t0 = time.perf_counter()
         sim.step(steps)
         sim.sync()
-        cand[split] = (time.perf_counter() - t0) / steps * 1e3
+        cand[(split, compact)] = (time.perf_counter() - t0) / steps * 1e3
     best = min(cand, key=cand.get)
-    sim.set_tuning(split_interact=best)
-    return {"split_interact": best, "ms_per_step": {("split" if k else "fused"): v for k, v in cand.items()}}
+    sim.set_tuning(split_interact=best[0], compact_pairs=best[1])
+    return {"split_interact": best[0], "compact_pairs": best[1],
+            "ms_per_step": {f"split{k[0]}_compact{k[1]}": v for k, v in cand.items()}}
 
 
 def run_path(pymfc, workload: str, precision: int, steps: int, warmup: int, profile_steps: int,
@@ -96,7 +100,7 @@ def run_path(pymfc, workload: str, precision: int, steps: int, warmup: int, prof
     info = sim.info
     N = info.n_particles
     sim.step(warmup)
-    tuning = tune(pymfc, sim, tune_steps) if tune_steps > 0 else {"split_interact": 0}
+    tuning = tune(pymfc, sim, tune_steps) if tune_steps > 0 else {"split_interact": 0, "compact_pairs": 0}
     sim.sync()
     t0 = time.perf_counter()
     sim.step(steps)
@@ -150,16 +154,16 @@ def batch_bench(pymfc, n: int, steps: int, tune_steps: int) -> dict:
         N = sum(inst.N for inst in b.instances)
         b.step(10)
         cand = {}
-        for split in ((0, 1) if tune_steps > 0 else (0,)):
-            b.set_tuning(split_interact=split)
+        for split, compact in (VARIANTS if tune_steps > 0 else [(0, 0)]):
+            b.set_tuning(split_interact=split, compact_pairs=compact)
             b.step(4)
             b.sync()
             t0 = time.perf_counter()
             b.step(max(tune_steps, 2))
             b.sync()
-            cand[split] = (time.perf_counter() - t0) / max(tune_steps, 2)
+            cand[(split, compact)] = (time.perf_counter() - t0) / max(tune_steps, 2)
         best = min(cand, key=cand.get)
-        b.set_tuning(split_interact=best)
+        b.set_tuning(split_interact=best[0], compact_pairs=best[1])
         b.step(4)
         b.sync()
         t0 = time.perf_counter()
@@ -171,7 +175,7 @@ def batch_bench(pymfc, n: int, steps: int, tune_steps: int) -> dict:
         seq = 0.0
         for c in cfgs:
             s = pymfc.Sim(c)
-            s.set_tuning(split_interact=best)
+            s.set_tuning(split_interact=best[0], compact_pairs=best[1])
             s.step(10)
             s.sync()
             t0 = time.perf_counter()
@@ -182,21 +186,21 @@ def batch_bench(pymfc, n: int, steps: int, tune_steps: int) -> dict:
         out[name] = {"particles": N, "min_instance": min(sizes), "max_instance": max(sizes),
                      "particle_steps_per_s": N * steps / wall, "ms_per_step": wall / steps * 1e3,
                      "sequential_particle_steps_per_s": N * steps / seq, "batch_speedup": seq / wall,
-                     "tuning": {"split_interact": best,
-                                "ms_per_step": {("split" if k else "fused"): v * 1e3 for k, v in cand.items()}}}
+                     "tuning": {"split_interact": best[0], "compact_pairs": best[1],
+                                "ms_per_step": {f"split{k[0]}_compact{k[1]}": v * 1e3 for k, v in cand.items()}}}
     return out
 
 
 COPY_PROBE_BYTES = 512 << 20
 
 
-def counter_probe(pymfc, workload: str, precision: int, steps: int, split: int) -> None:
+def counter_probe(pymfc, workload: str, precision: int, steps: int, split: int, compact: int) -> None:
     """A short run for hardware-counter passes (rocprofv3 --pmc): `steps` steps of one path,
     then a plain device copy of COPY_PROBE_BYTES read + COPY_PROBE_BYTES written, whose counter
     readings calibrate the bytes per counted unit."""
     import torch
     sim = pymfc.Sim(pymfc.workload(workload, precision))
-    sim.set_tuning(split_interact=split)
+    sim.set_tuning(split_interact=split, compact_pairs=compact)
     for _ in range(steps):
         sim.step(1)
     sim.sync()
@@ -206,6 +210,7 @@ def counter_probe(pymfc, workload: str, precision: int, steps: int, split: int) 
     b.copy_(a)
     torch.cuda.synchronize()
     print(json.dumps({"probe": workload, "precision": precision, "steps": steps, "split_interact": split,
+                      "compact_pairs": compact,
                       "copy_bytes": COPY_PROBE_BYTES, "N": sim.N}), flush=True)
     sim.close()
 
@@ -230,6 +235,7 @@ def main() -> int:
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--probe", action="store_true", help="counter-pass mode: short run + calibration copy")
     ap.add_argument("--split", type=int, default=0, help="split_interact for --probe")
+    ap.add_argument("--compact", type=int, default=0, help="compact_pairs for --probe")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only; --gpus {args.gpus} is not supported "
@@ -240,7 +246,7 @@ def main() -> int:
 
     if args.probe:
         counter_probe(pymfc, args.workload, pymfc.PREC_FP32C if args.only == "fp32c" else pymfc.PREC_FP64,
-                      args.steps, args.split)
+                      args.steps, args.split, args.compact)
         return 0
 
     res = {}
@@ -274,7 +280,7 @@ def main() -> int:
         "dtype": "f32 (Kahan-compensated neighbour sums) + f64 state" if head == "fp32c" else "f64",
         "data": "synthetic",
         "config": {"workload": args.workload, "n_particles": N, "n_tool": r["info"].n_tool,
-                   "split_interact": split,
+                   "split_interact": split, "compact_pairs": r["tuning"]["compact_pairs"],
                    "num_cell": r["info"].num_cell, "precision": head, "thermal": "brookshaw+tool-contact"},
         "roofline": {"bound": "hbm", "kernel": kname, "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,

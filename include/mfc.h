@@ -223,11 +223,14 @@ int mfc_vjp(mfc_sim *sim, mfc_vjp_req *req, double grad[6]);
 /* launch configuration of the step kernels (results do not depend on it: every variant
  * evaluates the same expressions in the same order). split_interact = 0: one fused
  * interaction kernel; 1: a neighbour-sum kernel and a per-particle update kernel with the sums
- * stored in between (fewer registers per kernel, 224 B more traffic per particle). The
- * benchmark's autotuner picks it per workload. */
+ * stored in between (fewer registers per kernel, 224 B more traffic per particle).
+ * compact_pairs = 1 (default): each lane first lists its in-support neighbours, then runs the
+ * pair terms over the list; 0: the pair terms run inside the window walk. The benchmark's
+ * autotuner picks both per workload. */
 typedef struct mfc_tuning {
   int split_interact;
-  int reserved[7];
+  int compact_pairs;
+  int reserved[6];
 } mfc_tuning;
 int mfc_set_tuning(mfc_sim *sim, const mfc_tuning *tuning);
 int mfc_get_tuning(const mfc_sim *sim, mfc_tuning *tuning);

@@ -9,7 +9,7 @@
 // one by one:
 //   F1  binning          launch_binning (cell ids, stable counting sort)
 //   F2  predict          launch_predict (gather, predictor, EOS, artificial stress, records)
-//   F3  k_adj_sums       neighbour sums of every particle (the loop of k_interact, no epilogue)
+//   F3  launch_sums      neighbour sums of every particle (the sums kernel of the step)
 //   B1  k_adj_tail       VJP of everything after the sums, per particle (tail_vjp), and the
 //                        per-tile theta partials of the step (k_adj_theta adds them up)
 //   B2  k_adj_pull       VJP of the neighbour sums with respect to the records: each particle
@@ -36,31 +36,6 @@ namespace mfc {
 namespace {
 
 using namespace adj;
-
-// ------------------------------------------------------------------ F3: neighbour sums
-__global__ __launch_bounds__(TILE) void k_adj_sums(dev_consts k, int ntiles, const double *__restrict__ rec,
-                                                   const int *__restrict__ key_sorted,
-                                                   const int *__restrict__ offsets, const int *__restrict__ flags,
-                                                   double *__restrict__ acc) {
-  const int t = xcd_tile(blockIdx.x, ntiles);
-  if (t < 0) return;
-  const int i = t * TILE + threadIdx.x;
-  if (i >= k.N) return;
-  const int fl = flags[i];
-  const bool blank = fl & MFC_FLAG_BLANKED, tool = fl & MFC_FLAG_TOOL;
-  const bool pse = k.pse != 0;
-  const double ta = pse ? 0. : (tool ? k.alpha_tool : k.alpha_wp);
-  const bool run_mech = !blank && (!pse || !tool);
-  const bool run_pse = !blank && pse && k.thermal_wp;
-  pair_acc a = {0., 0., 0., 0., 0., 0., 0., 0., 0., 0., 0., 0., 0.};
-  double tpse = 0.;
-  if (run_mech || run_pse) sums_f64(k, i, rec, key_sorted, offsets, tool, ta, run_mech, run_pse, a, tpse);
-  const size_t N = k.N;
-  acc[L_VD0 * N + i] = a.vd0; acc[L_VD1 * N + i] = a.vd1; acc[L_VD2 * N + i] = a.vd2; acc[L_VD3 * N + i] = a.vd3;
-  acc[L_SD0 * N + i] = a.sd0; acc[L_SD1 * N + i] = a.sd1; acc[L_SD2 * N + i] = a.sd2; acc[L_SD3 * N + i] = a.sd3;
-  acc[L_VT0 * N + i] = a.vt0; acc[L_VT1 * N + i] = a.vt1; acc[L_XT0 * N + i] = a.xt0; acc[L_XT1 * N + i] = a.xt1;
-  acc[L_TL * N + i] = a.tl; acc[L_TPSE * N + i] = tpse;
-}
 
 // ------------------------------------------------------------------ B1: per-particle tail
 // st: sorted start state (DSTATE_DOUBLES arrays of stride N, TS_* order); lam_s: adjoint of the
@@ -430,7 +405,7 @@ int mfc_vjp_backward(mfc_sim *s, const double cot[2], double grad[6]) {
       carve_state(w->work[1 - wa], N, B);
       launch_binning(k, A, s->bins, st);
       launch_predict(k, A, B, s->bins, P, s->recs, w->toolw, w->toolw + 1, st);
-      launch_interact(k, B, s->bins, P, s->recs, w->toolw + 1, fo, s->interact_acc(), st);
+      launch_interact(k, B, s->bins, P, s->recs, w->toolw + 1, fo, s->acc, s->variant(), st);
       HIP_TRY(hipMemcpyAsync(w->toolw, w->toolw + 1, sizeof(mfc_tool_state), hipMemcpyDeviceToDevice, st));
       wa = 1 - wa;
     }
@@ -444,8 +419,7 @@ int mfc_vjp_backward(mfc_sim *s, const double cot[2], double grad[6]) {
       const double *rec = s->recs.rec;
       const double *sb = B.px;  // sorted start state, TS_* order with stride N
       const bool in_win = t >= w->w0 && t < w->w1;
-      hipLaunchKernelGGL(k_adj_sums, dim3(padded(nt)), dim3(TILE), 0, st, k, nt, rec, s->bins.key_sorted,
-                         s->bins.offsets, B.flags, w->acc);
+      launch_sums(k, B.flags, s->bins, s->recs, w->acc, s->variant(), st);
       hipLaunchKernelGGL(k_adj_tail, dim3(padded(nt)), dim3(TILE), 0, st, k, nt, sb, B.flags, P, w->acc,
                          w->lam_s[cur], w->lam_n, in_win ? Fb[0] : 0., in_win ? Fb[1] : 0., w->toolw + 1, w->g_s,
                          w->lam_acc, w->theta_part);

@@ -70,7 +70,7 @@ void enqueue_step(mfc_sim *s, int c) {
   const int n = 1 - c;
   mfc::launch_binning(s->k, s->set[c], s->bins, s->stream);
   mfc::launch_predict(s->k, s->set[c], s->set[n], s->bins, s->pos, s->recs, s->tool + c, s->tool + n, s->stream);
-  mfc::launch_interact(s->k, s->set[n], s->bins, s->pos, s->recs, s->tool + n, s->force(), s->interact_acc(),
+  mfc::launch_interact(s->k, s->set[n], s->bins, s->pos, s->recs, s->tool + n, s->force(), s->acc, s->variant(),
                        s->stream);
 }
 
@@ -218,7 +218,8 @@ int mfc_create(const mfc_config *cfg, mfc_sim **out) {
 int mfc_set_tuning(mfc_sim *s, const mfc_tuning *t) {
   if (!s || !t) return fail(MFC_E_INVALID, "null argument");
   if (s->batched) return fail(MFC_E_INVALID, "instance of a batch: use mfc_batch_set_tuning");
-  if (t->split_interact != 0 && t->split_interact != 1) return fail(MFC_E_INVALID, "split_interact must be 0 or 1");
+  if ((t->split_interact != 0 && t->split_interact != 1) || (t->compact_pairs != 0 && t->compact_pairs != 1))
+    return fail(MFC_E_INVALID, "split_interact and compact_pairs must be 0 or 1");
   HIP_TRY(hipStreamSynchronize(s->stream));
   s->tuning = *t;
   for (int p = 0; p < 2; p++)
@@ -277,11 +278,12 @@ int mfc_profile(mfc_sim *s, int nsteps, double *ms) {
     mfc::launch_predict(s->k, s->set[c], s->set[n], s->bins, s->pos, s->recs, s->tool + c, s->tool + n, s->stream);
     (void)hipEventRecord(ev[2], s->stream);
     if (s->tuning.split_interact) {
-      mfc::launch_sums(s->k, s->set[n].flags, s->bins, s->recs, s->acc, s->stream);
+      mfc::launch_sums(s->k, s->set[n].flags, s->bins, s->recs, s->acc, s->variant(), s->stream);
       (void)hipEventRecord(ev[3], s->stream);
       mfc::launch_update(s->k, s->acc, s->set[n], s->pos, s->tool + n, fo, s->stream);
     } else {
-      mfc::launch_interact_fused(s->k, s->set[n], s->bins, s->pos, s->recs, s->tool + n, fo, s->stream);
+      mfc::launch_interact_fused(s->k, s->set[n], s->bins, s->pos, s->recs, s->tool + n, fo, s->variant(),
+                                 s->stream);
       (void)hipEventRecord(ev[3], s->stream);
     }
     (void)hipEventRecord(ev[4], s->stream);

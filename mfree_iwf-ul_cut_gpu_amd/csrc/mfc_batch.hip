@@ -17,7 +17,7 @@ struct mfc_batch {
   std::vector<mfc_sim *> sims;
   hipStream_t stream = nullptr;
   bool f32 = false;
-  mfc_tuning tuning = {0, {0, 0, 0, 0, 0, 0, 0}};
+  mfc_tuning tuning = {0, 0, {0, 0, 0, 0, 0, 0}};
   mfc::inst_args *args[2] = {nullptr, nullptr};  // descriptors per start parity (device)
   int2 *tiles = nullptr, *chunks = nullptr;
   int ntiles = 0, nchunks = 0;
@@ -26,7 +26,8 @@ struct mfc_batch {
   hipGraphExec_t graph[2] = {nullptr, nullptr};
 
   mfc::batch_work work(int parity) const {
-    return {args[parity], (int)sims.size(), tiles, ntiles, chunks, nchunks, f32, tuning.split_interact != 0};
+    return {args[parity], (int)sims.size(), tiles, ntiles, chunks, nchunks, f32,
+            {tuning.split_interact != 0, tuning.compact_pairs != 0}};
   }
   void drop_graphs() {
     for (int p = 0; p < 2; p++)
@@ -164,7 +165,8 @@ int mfc_batch_instance(mfc_batch *b, int i, mfc_sim **sim) {
 
 int mfc_batch_set_tuning(mfc_batch *b, const mfc_tuning *t) {
   if (!b || !t) return fail(MFC_E_INVALID, "null argument");
-  if (t->split_interact != 0 && t->split_interact != 1) return fail(MFC_E_INVALID, "split_interact must be 0 or 1");
+  if ((t->split_interact != 0 && t->split_interact != 1) || (t->compact_pairs != 0 && t->compact_pairs != 1))
+    return fail(MFC_E_INVALID, "split_interact and compact_pairs must be 0 or 1");
   HIP_TRY(hipStreamSynchronize(b->stream));
   b->tuning = *t;
   for (mfc_sim *s : b->sims) s->tuning = *t;

@@ -39,6 +39,7 @@ struct dpositional {
 
 constexpr int REC = 16;    // doubles per neighbour record
 constexpr int ACC_N = 14;  // neighbour sums per particle
+constexpr int PAIR_LIST = 32;  // per-lane list of in-support neighbours (compacted pair loop)
 
 // neighbour records written by the predict kernel and read by the interaction kernel:
 // FP64 path: rec, 16 doubles per particle; FP32C path: posd (FP64 predicted position) and
@@ -62,6 +63,12 @@ struct dforce {
 
 // internal flag: some value of the particle's neighbour record is NaN or Inf
 constexpr int FLAG_NONFINITE = 8;
+
+// launch variant of the interaction (results are identical for every variant)
+struct variant {
+  bool split;    // sums kernel + update kernel instead of one fused kernel
+  bool compact;  // per-lane list of in-support neighbours before the pair loop
+};
 
 // one instance of a batch as the batched kernels see it (an array of these lives in device
 // memory, one per instance and step parity)
@@ -87,7 +94,8 @@ struct batch_work {
   int ntiles;
   const int2 *chunks;     // (instance, scan chunk) for every chunk of every cell table
   int nchunks;
-  bool f32, split;
+  bool f32;
+  variant v;
 };
 
 // one step of every instance of a batch: 9 (fused) or 10 (split) launches in all
@@ -98,20 +106,21 @@ int scan_chunk_cells();
 void launch_binning(const dev_consts &k, const dstate &cur, dbins &b, hipStream_t s);
 void launch_predict(const dev_consts &k, const dstate &cur, const dstate &nxt, const dbins &b, const dpositional &pos,
                     const drecords &r, const mfc_tool_state *tool_in, mfc_tool_state *tool_out, hipStream_t s);
-// neighbour sums + per-particle update + contact, then the tool-force reduction. acc == nullptr:
-// one fused kernel; otherwise the sums go through acc (ACC_N arrays of N doubles) between a
-// sums kernel and an update kernel (fewer registers per kernel)
+// neighbour sums + per-particle update + contact, then the tool-force reduction; v.split: the
+// sums go through acc (ACC_N arrays of N doubles) between a sums kernel and an update kernel
+// (fewer registers per kernel), else one fused kernel
 void launch_interact(const dev_consts &k, const dstate &st, const dbins &b, const dpositional &pos,
-                     const drecords &r, const mfc_tool_state *tool_now, const dforce &f, double *acc,
+                     const drecords &r, const mfc_tool_state *tool_now, const dforce &f, double *acc, variant v,
                      hipStream_t s);
 // the pieces of launch_interact, for per-kernel timing
 void launch_interact_fused(const dev_consts &k, const dstate &st, const dbins &b, const dpositional &pos,
-                           const drecords &r, const mfc_tool_state *tool_now, const dforce &f, hipStream_t s);
+                           const drecords &r, const mfc_tool_state *tool_now, const dforce &f, variant v,
+                           hipStream_t s);
 void launch_update(const dev_consts &k, const double *acc, const dstate &st, const dpositional &pos,
                    const mfc_tool_state *tool_now, const dforce &f, hipStream_t s);
 void launch_force_final(const dev_consts &k, const dforce &f, hipStream_t s);
 // neighbour sums only (ACC_N arrays of N doubles), flags in sorted order
-void launch_sums(const dev_consts &k, const int *flags, const dbins &b, const drecords &r, double *acc,
+void launch_sums(const dev_consts &k, const int *flags, const dbins &b, const drecords &r, double *acc, variant v,
                  hipStream_t s);
 
 int scan_blocks(int num_cell);

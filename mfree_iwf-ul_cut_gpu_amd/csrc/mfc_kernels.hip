@@ -324,6 +324,7 @@ struct kahan {
 // for single precision: reciprocal square roots instead of divisions, the particle mass and the
 // factor 2 of Brookshaw taken out of the sums, and e.gradW/r of Brookshaw written as der*fac,
 // which is its exact value). Pair distances are formed in FP64 and then rounded.
+template <bool COMPACT>
 __device__ inline void sums_f32(const dev_consts &k, int i, const double2 *__restrict__ posd,
                                 const float *__restrict__ rec32, const int *__restrict__ key_sorted,
                                 const int *__restrict__ offsets, bool tool, double ta_d, bool run_mech, bool run_pse,
@@ -352,62 +353,101 @@ __device__ inline void sums_f32(const dev_consts &k, int i, const double2 *__res
   const int lj = gj - 2 < 0 ? 0 : gj - 2;
   const int hi_ = gi + 3 > k.gnx ? k.gnx : gi + 3;
   const int hj = gj + 3 > k.gny ? k.gny : gj + 3;
-  for (int ii = li; ii < hi_; ii++) {
-    const int jb = offsets[ii * k.gny + lj], je = offsets[ii * k.gny + hj];
-    for (int j = jb; j < je; j++) {
-      const double2 pj = posd[j];
-      const float dx = (float)(pi.x - pj.x), dy = (float)(pi.y - pj.y);
-      const float rr2 = __builtin_fmaf(dx, dx, dy * dy);
-      const float4 *rj = reinterpret_cast<const float4 *>(rec32 + (size_t)16 * j);
-      if (run_pse) {
-        const float4 b0 = rj[0], b3 = rj[3];
-        const float w2 = pse_c * __expf(-rr2 * inv_h2);
-        tp.add((b3.x - Tpred) * w2 * (massf / b0.w));
+  // the mechanical and Brookshaw terms of one in-support neighbour
+  auto pair = [&](float dx, float dy, float rr2, const float4 *__restrict__ rj) {
+    const float4 b0 = rj[0], b1 = rj[1], b2 = rj[2], b3 = rj[3];
+    const int fj = __float_as_int(b3.y);
+    const float uj = b0.x, vj = b0.y, hj_ = b0.z, rhoj = b0.w, volj = b1.x;
+    const float inv_r = __builtin_amdgcn_rsqf(rr2);
+    const float q = rr2 * inv_r * h1;
+    float val, der;  // der = dW/dr / r without the factor fac
+    if (q > 1.0f) {
+      const float m = 2.0f - q, m2 = m * m;
+      val = 0.25f * m2 * m2 * m2;
+      der = -0.75f * m2 * h1 * inv_r;
+    } else {
+      val = 1.0f - 1.5f * q * q * (1.0f - 0.5f * q);
+      der = -3.0f * q * (1.0f - 0.75f * q) * h1 * inv_r;
+    }
+    const float w = val * fac, gr = der * fac;
+    const float w_x = gr * dx, w_y = gr * dy;
+    if (!tool && !(fj & MFC_FLAG_TOOL)) {
+      const float tx = w_x * volj, ty = w_y * volj;
+      const float du = uj - ui, dv = vj - vi_;
+      vd0.add(du * tx); vd1.add(du * ty); vd2.add(dv * tx); vd3.add(dv * ty);
+      float fab = w * inv_wd;
+      fab *= fab;
+      fab *= fab;
+      const float sx = Axx_i + b1.y + fab * (Rxx_i + b2.x);
+      const float sy = Axy_i + b1.z + fab * (Rxy_i + b2.y);
+      const float sz = Ayy_i + b1.w + fab * (Ryy_i + b2.z);
+      sd0.add(sx * w_x); sd1.add(sy * w_y); sd2.add(sy * w_x); sd3.add(sz * w_y);
+      const float vijx = -du, vijy = -dv;
+      const float vdx = __builtin_fmaf(vijx, dx, vijy * dy);
+      const float inv_rhoij = 2.0f / (rhoi + rhoj);
+      if (vdx < 0.0f) {
+        const float cij = 0.5f * (ci + b2.w);
+        const float hij = 0.5f * (hi + hj_);
+        const float muij = hij * vdx / __builtin_fmaf(eta2 * hij, hij, rr2);
+        const float piij = (-alpha * cij * muij + beta * muij * muij) * inv_rhoij;
+        vt0.add(-piij * w_x); vt1.add(-piij * w_y);
       }
-      if (!run_mech) continue;
-      if (rr2 >= fourh2 || rr2 < 1e-8f) continue;
-      const float4 b0 = rj[0], b1 = rj[1], b2 = rj[2], b3 = rj[3];
-      const int fj = __float_as_int(b3.y);
-      if (fj & MFC_FLAG_BLANKED) continue;
-      const float uj = b0.x, vj = b0.y, hj_ = b0.z, rhoj = b0.w, volj = b1.x;
-      const float inv_r = __builtin_amdgcn_rsqf(rr2);
-      const float q = rr2 * inv_r * h1;
-      float val, der;  // der = dW/dr / r without the factor fac
-      if (q > 1.0f) {
-        const float m = 2.0f - q, m2 = m * m;
-        val = 0.25f * m2 * m2 * m2;
-        der = -0.75f * m2 * h1 * inv_r;
-      } else {
-        val = 1.0f - 1.5f * q * q * (1.0f - 0.5f * q);
-        der = -3.0f * q * (1.0f - 0.75f * q) * h1 * inv_r;
+      const float xs = -xeps * w * inv_rhoij;
+      xt0.add(xs * vijx); xt1.add(xs * vijy);
+    }
+    if (ta != 0.0f) tl.add(volj * (Ti - b3.x) * gr);
+  };
+  // the PSE term of every window member, and the support test; true: j is a mechanical pair
+  auto visit = [&](int j, float &dx, float &dy, float &rr2) -> bool {
+    const double2 pj = posd[j];
+    dx = (float)(pi.x - pj.x);
+    dy = (float)(pi.y - pj.y);
+    rr2 = __builtin_fmaf(dx, dx, dy * dy);
+    const float4 *rj = reinterpret_cast<const float4 *>(rec32 + (size_t)16 * j);
+    if (run_pse) {
+      const float4 b0 = rj[0], b3 = rj[3];
+      const float w2 = pse_c * __expf(-rr2 * inv_h2);
+      tp.add((b3.x - Tpred) * w2 * (massf / b0.w));
+    }
+    if (!run_mech) return false;
+    if (rr2 >= fourh2 || rr2 < 1e-8f) return false;
+    return !(__float_as_int(rj[3].y) & MFC_FLAG_BLANKED);
+  };
+  if constexpr (!COMPACT) {
+    for (int ii = li; ii < hi_; ii++) {
+      const int jb = offsets[ii * k.gny + lj], je = offsets[ii * k.gny + hj];
+      for (int j = jb; j < je; j++) {
+        float dx, dy, rr2;
+        if (visit(j, dx, dy, rr2)) pair(dx, dy, rr2, reinterpret_cast<const float4 *>(rec32 + (size_t)16 * j));
       }
-      const float w = val * fac, gr = der * fac;
-      const float w_x = gr * dx, w_y = gr * dy;
-      if (!tool && !(fj & MFC_FLAG_TOOL)) {
-        const float tx = w_x * volj, ty = w_y * volj;
-        const float du = uj - ui, dv = vj - vi_;
-        vd0.add(du * tx); vd1.add(du * ty); vd2.add(dv * tx); vd3.add(dv * ty);
-        float fab = w * inv_wd;
-        fab *= fab;
-        fab *= fab;
-        const float sx = Axx_i + b1.y + fab * (Rxx_i + b2.x);
-        const float sy = Axy_i + b1.z + fab * (Rxy_i + b2.y);
-        const float sz = Ayy_i + b1.w + fab * (Ryy_i + b2.z);
-        sd0.add(sx * w_x); sd1.add(sy * w_y); sd2.add(sy * w_x); sd3.add(sz * w_y);
-        const float vijx = -du, vijy = -dv;
-        const float vdx = __builtin_fmaf(vijx, dx, vijy * dy);
-        const float inv_rhoij = 2.0f / (rhoi + rhoj);
-        if (vdx < 0.0f) {
-          const float cij = 0.5f * (ci + b2.w);
-          const float hij = 0.5f * (hi + hj_);
-          const float muij = hij * vdx / __builtin_fmaf(eta2 * hij, hij, rr2);
-          const float piij = (-alpha * cij * muij + beta * muij * muij) * inv_rhoij;
-          vt0.add(-piij * w_x); vt1.add(-piij * w_y);
+    }
+  } else {
+    // walk the window collecting the in-support neighbours in a per-lane list, then evaluate
+    // the list: the wave runs the pair body about as often as its busiest lane has pairs,
+    // instead of once per window candidate
+    __shared__ int list[PAIR_LIST][TILE];
+    int ii = li, j = 0, je = 0;
+    bool more = ii < hi_;
+    if (more) { j = offsets[ii * k.gny + lj]; je = offsets[ii * k.gny + hj]; }
+    while (more) {
+      int cnt = 0;
+      while (cnt < PAIR_LIST) {
+        if (j >= je) {
+          if (++ii >= hi_) { more = false; break; }
+          j = offsets[ii * k.gny + lj];
+          je = offsets[ii * k.gny + hj];
+          continue;
         }
-        const float xs = -xeps * w * inv_rhoij;
-        xt0.add(xs * vijx); xt1.add(xs * vijy);
+        float dx, dy, rr2;
+        if (visit(j, dx, dy, rr2)) list[cnt++][threadIdx.x] = j;
+        j++;
       }
-      if (ta != 0.0f) tl.add(volj * (Ti - b3.x) * gr);
+      for (int q = 0; q < cnt; q++) {
+        const int jj = list[q][threadIdx.x];
+        const double2 pj = posd[jj];
+        const float dx = (float)(pi.x - pj.x), dy = (float)(pi.y - pj.y);
+        pair(dx, dy, __builtin_fmaf(dx, dx, dy * dy), reinterpret_cast<const float4 *>(rec32 + (size_t)16 * jj));
+      }
     }
   }
   acc.vd0 = vd0.value(); acc.vd1 = vd1.value(); acc.vd2 = vd2.value(); acc.vd3 = vd3.value();
@@ -517,7 +557,7 @@ __device__ inline void update_particle(const dev_consts &k, int i, const pair_ac
 }
 
 // the neighbour sums of particle i as pair_acc + PSE sum
-template <bool F32>
+template <bool F32, bool COMPACT>
 __device__ inline void particle_sums(const dev_consts &k, int i, const double *__restrict__ rec,
                                      const double2 *__restrict__ posd, const float *__restrict__ rec32,
                                      const int *__restrict__ key_sorted, const int *__restrict__ offsets, int fl,
@@ -530,8 +570,10 @@ __device__ inline void particle_sums(const dev_consts &k, int i, const double *_
   acc = {0., 0., 0., 0., 0., 0., 0., 0., 0., 0., 0., 0., 0.};
   tpse = 0.;
   if (run_mech || run_pse) {
-    if constexpr (F32) sums_f32(k, i, posd, rec32, key_sorted, offsets, tool, ta, run_mech, run_pse, acc, tpse);
-    else sums_f64(k, i, rec, key_sorted, offsets, tool, ta, run_mech, run_pse, acc, tpse);
+    if constexpr (F32)
+      sums_f32<COMPACT>(k, i, posd, rec32, key_sorted, offsets, tool, ta, run_mech, run_pse, acc, tpse);
+    else
+      sums_f64<COMPACT>(k, i, rec, key_sorted, offsets, tool, ta, run_mech, run_pse, acc, tpse);
   }
 }
 
@@ -554,7 +596,7 @@ __device__ inline void force_partials(int t, const double fsum[4], double *__res
 }
 
 // fused: sums + update in one pass (registers of both live in one kernel)
-template <bool F32>
+template <bool F32, bool COMPACT>
 __device__ inline void interact(const dev_consts &k, int t, const double *__restrict__ rec,
                                 const double2 *__restrict__ posd, const float *__restrict__ rec32,
                                 const int *__restrict__ key_sorted, const int *__restrict__ offsets,
@@ -565,14 +607,14 @@ __device__ inline void interact(const dev_consts &k, int t, const double *__rest
   if (i < k.N) {
     pair_acc acc;
     double tpse;
-    particle_sums<F32>(k, i, rec, posd, rec32, key_sorted, offsets, st.flags[i], acc, tpse);
+    particle_sums<F32, COMPACT>(k, i, rec, posd, rec32, key_sorted, offsets, st.flags[i], acc, tpse);
     update_particle(k, i, acc, tpse, st, pos, *tool_now, fsum);
   }
   force_partials(t, fsum, fo.partials);
 }
 
 // split, first half: the neighbour sums of every particle into acc (ACC_N arrays of N doubles)
-template <bool F32>
+template <bool F32, bool COMPACT>
 __device__ inline void sums(const dev_consts &k, int t, const double *__restrict__ rec,
                             const double2 *__restrict__ posd, const float *__restrict__ rec32,
                             const int *__restrict__ key_sorted, const int *__restrict__ offsets,
@@ -581,7 +623,7 @@ __device__ inline void sums(const dev_consts &k, int t, const double *__restrict
   if (i >= k.N) return;
   pair_acc a;
   double tpse;
-  particle_sums<F32>(k, i, rec, posd, rec32, key_sorted, offsets, flags[i], a, tpse);
+  particle_sums<F32, COMPACT>(k, i, rec, posd, rec32, key_sorted, offsets, flags[i], a, tpse);
   store_acc(acc, k.N, i, a, tpse);
 }
 
@@ -627,7 +669,7 @@ __device__ inline void force_final(int ntiles, const dforce &fo) {
   }
 }
 
-template <bool F32>
+template <bool F32, bool COMPACT>
 __global__ __launch_bounds__(TILE) void k_interact(dev_consts k, int ntiles, const double *__restrict__ rec,
                                                    const double2 *__restrict__ posd, const float *__restrict__ rec32,
                                                    const int *__restrict__ key_sorted,
@@ -635,17 +677,17 @@ __global__ __launch_bounds__(TILE) void k_interact(dev_consts k, int ntiles, con
                                                    const mfc_tool_state *__restrict__ tool_now, dforce fo) {
   const int t = xcd_tile(blockIdx.x, ntiles);
   if (t < 0) return;
-  interact<F32>(k, t, rec, posd, rec32, key_sorted, offsets, st, pos, tool_now, fo);
+  interact<F32, COMPACT>(k, t, rec, posd, rec32, key_sorted, offsets, st, pos, tool_now, fo);
 }
 
-template <bool F32>
+template <bool F32, bool COMPACT>
 __global__ __launch_bounds__(TILE) void k_sums(dev_consts k, int ntiles, const double *__restrict__ rec,
                                                const double2 *__restrict__ posd, const float *__restrict__ rec32,
                                                const int *__restrict__ key_sorted, const int *__restrict__ offsets,
                                                const int *__restrict__ flags, double *__restrict__ acc) {
   const int t = xcd_tile(blockIdx.x, ntiles);
   if (t < 0) return;
-  sums<F32>(k, t, rec, posd, rec32, key_sorted, offsets, flags, acc);
+  sums<F32, COMPACT>(k, t, rec, posd, rec32, key_sorted, offsets, flags, acc);
 }
 
 __global__ __launch_bounds__(TILE) void k_update(dev_consts k, int ntiles, const double *__restrict__ acc,
@@ -723,24 +765,24 @@ __global__ __launch_bounds__(TILE) void kb_predict(const inst_args *__restrict__
   predict<F32>(a.k, it.y, a.b.perm, a.cur, a.nxt, a.pos, a.r.rec, a.r.posd, a.r.rec32, a.tool_in, a.tool_out);
 }
 
-template <bool F32>
+template <bool F32, bool COMPACT>
 __global__ __launch_bounds__(TILE) void kb_interact(const inst_args *__restrict__ A, const int2 *__restrict__ work,
                                                     int nwork) {
   const int w = xcd_tile(blockIdx.x, nwork);
   if (w < 0) return;
   const int2 it = work[w];
   const inst_args &a = A[it.x];
-  interact<F32>(a.k, it.y, a.r.rec, a.r.posd, a.r.rec32, a.b.key_sorted, a.b.offsets, a.nxt, a.pos, a.tool_out, a.f);
+  interact<F32, COMPACT>(a.k, it.y, a.r.rec, a.r.posd, a.r.rec32, a.b.key_sorted, a.b.offsets, a.nxt, a.pos, a.tool_out, a.f);
 }
 
-template <bool F32>
+template <bool F32, bool COMPACT>
 __global__ __launch_bounds__(TILE) void kb_sums(const inst_args *__restrict__ A, const int2 *__restrict__ work,
                                                 int nwork) {
   const int w = xcd_tile(blockIdx.x, nwork);
   if (w < 0) return;
   const int2 it = work[w];
   const inst_args &a = A[it.x];
-  sums<F32>(a.k, it.y, a.r.rec, a.r.posd, a.r.rec32, a.b.key_sorted, a.b.offsets, a.nxt.flags, a.acc);
+  sums<F32, COMPACT>(a.k, it.y, a.r.rec, a.r.posd, a.r.rec32, a.b.key_sorted, a.b.offsets, a.nxt.flags, a.acc);
 }
 
 __global__ __launch_bounds__(TILE) void kb_update(const inst_args *__restrict__ A, const int2 *__restrict__ work,
@@ -810,26 +852,23 @@ void launch_predict(const dev_consts &k, const dstate &cur, const dstate &nxt, c
                        r.posd, r.rec32, tool_in, tool_out);
 }
 
-void launch_sums(const dev_consts &k, const int *flags, const dbins &b, const drecords &r, double *acc,
+// the four instantiations of a kernel template <F32, COMPACT>
+#define MFC_PICK(kern, f32, compact) \
+  ((f32) ? ((compact) ? kern<true, true> : kern<true, false>) : ((compact) ? kern<false, true> : kern<false, false>))
+
+void launch_sums(const dev_consts &k, const int *flags, const dbins &b, const drecords &r, double *acc, variant v,
                  hipStream_t s) {
   const int nt = (k.N + TILE - 1) / TILE;
-  if (r.f32)
-    hipLaunchKernelGGL(k_sums<true>, dim3(padded_tiles(nt)), dim3(TILE), 0, s, k, nt, r.rec, r.posd, r.rec32,
-                       b.key_sorted, b.offsets, flags, acc);
-  else
-    hipLaunchKernelGGL(k_sums<false>, dim3(padded_tiles(nt)), dim3(TILE), 0, s, k, nt, r.rec, r.posd, r.rec32,
-                       b.key_sorted, b.offsets, flags, acc);
+  hipLaunchKernelGGL(MFC_PICK(k_sums, r.f32, v.compact), dim3(padded_tiles(nt)), dim3(TILE), 0, s, k, nt, r.rec,
+                     r.posd, r.rec32, b.key_sorted, b.offsets, flags, acc);
 }
 
 void launch_interact_fused(const dev_consts &k, const dstate &st, const dbins &b, const dpositional &pos,
-                           const drecords &r, const mfc_tool_state *tool_now, const dforce &f, hipStream_t s) {
+                           const drecords &r, const mfc_tool_state *tool_now, const dforce &f, variant v,
+                           hipStream_t s) {
   const int nt = (k.N + TILE - 1) / TILE;
-  if (r.f32)
-    hipLaunchKernelGGL(k_interact<true>, dim3(padded_tiles(nt)), dim3(TILE), 0, s, k, nt, r.rec, r.posd, r.rec32,
-                       b.key_sorted, b.offsets, st, pos, tool_now, f);
-  else
-    hipLaunchKernelGGL(k_interact<false>, dim3(padded_tiles(nt)), dim3(TILE), 0, s, k, nt, r.rec, r.posd, r.rec32,
-                       b.key_sorted, b.offsets, st, pos, tool_now, f);
+  hipLaunchKernelGGL(MFC_PICK(k_interact, r.f32, v.compact), dim3(padded_tiles(nt)), dim3(TILE), 0, s, k, nt, r.rec,
+                     r.posd, r.rec32, b.key_sorted, b.offsets, st, pos, tool_now, f);
 }
 
 void launch_update(const dev_consts &k, const double *acc, const dstate &st, const dpositional &pos,
@@ -844,13 +883,13 @@ void launch_force_final(const dev_consts &k, const dforce &f, hipStream_t s) {
 }
 
 void launch_interact(const dev_consts &k, const dstate &st, const dbins &b, const dpositional &pos,
-                     const drecords &r, const mfc_tool_state *tool_now, const dforce &f, double *acc,
+                     const drecords &r, const mfc_tool_state *tool_now, const dforce &f, double *acc, variant v,
                      hipStream_t s) {
-  if (acc) {
-    launch_sums(k, st.flags, b, r, acc, s);
+  if (v.split) {
+    launch_sums(k, st.flags, b, r, acc, v, s);
     launch_update(k, acc, st, pos, tool_now, f, s);
   } else {
-    launch_interact_fused(k, st, b, pos, r, tool_now, f, s);
+    launch_interact_fused(k, st, b, pos, r, tool_now, f, v, s);
   }
   launch_force_final(k, f, s);
 }
@@ -872,14 +911,11 @@ void launch_batch_step(const batch_work &w, hipStream_t s) {
   hipLaunchKernelGGL(kb_rank, gt, dim3(256), 0, s, w.args, w.tiles, w.ntiles);
   if (w.f32) hipLaunchKernelGGL(kb_predict<true>, gt, dim3(TILE), 0, s, w.args, w.tiles, w.ntiles);
   else hipLaunchKernelGGL(kb_predict<false>, gt, dim3(TILE), 0, s, w.args, w.tiles, w.ntiles);
-  if (w.split) {
-    if (w.f32) hipLaunchKernelGGL(kb_sums<true>, gt, dim3(TILE), 0, s, w.args, w.tiles, w.ntiles);
-    else hipLaunchKernelGGL(kb_sums<false>, gt, dim3(TILE), 0, s, w.args, w.tiles, w.ntiles);
+  if (w.v.split) {
+    hipLaunchKernelGGL(MFC_PICK(kb_sums, w.f32, w.v.compact), gt, dim3(TILE), 0, s, w.args, w.tiles, w.ntiles);
     hipLaunchKernelGGL(kb_update, gt, dim3(TILE), 0, s, w.args, w.tiles, w.ntiles);
-  } else if (w.f32) {
-    hipLaunchKernelGGL(kb_interact<true>, gt, dim3(TILE), 0, s, w.args, w.tiles, w.ntiles);
   } else {
-    hipLaunchKernelGGL(kb_interact<false>, gt, dim3(TILE), 0, s, w.args, w.tiles, w.ntiles);
+    hipLaunchKernelGGL(MFC_PICK(kb_interact, w.f32, w.v.compact), gt, dim3(TILE), 0, s, w.args, w.tiles, w.ntiles);
   }
   hipLaunchKernelGGL(kb_force_final, dim3(w.n), dim3(TILE), 0, s, w.args);
 }

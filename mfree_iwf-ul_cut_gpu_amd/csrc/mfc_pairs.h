@@ -107,7 +107,11 @@ __device__ inline void pair_terms(const dev_consts &k, pair_acc &a, double xi, d
 }
 
 // FP64 neighbour sums of particle i: the reference's loop over the 5 x 5 cell window
-// (interactions_gpu.cu:219-331), walked as 5 contiguous column ranges of the sorted order
+// (interactions_gpu.cu:219-331), walked as 5 contiguous column ranges of the sorted order.
+// COMPACT: the window is walked first, collecting the neighbours that contribute in a per-lane
+// list (in the same order), and the list is evaluated afterwards, so that a wave runs the pair
+// body about as often as its busiest lane has pairs instead of once per window candidate.
+template <bool COMPACT>
 __device__ inline void sums_f64(const dev_consts &k, int i, const double *__restrict__ rec,
                                 const int *__restrict__ key_sorted, const int *__restrict__ offsets, bool tool,
                                 double ta, bool run_mech, bool run_pse, pair_acc &acc, double &tpse) {
@@ -131,30 +135,59 @@ __device__ inline void sums_f64(const dev_consts &k, int i, const double *__rest
   const int lj = gj - 2 < 0 ? 0 : gj - 2;
   const int hi_ = gi + 3 > k.gnx ? k.gnx : gi + 3;
   const int hj = gj + 3 > k.gny ? k.gny : gj + 3;
-  for (int ii = li; ii < hi_; ii++) {
-    const int jb = offsets[ii * k.gny + lj], je = offsets[ii * k.gny + hj];
-    for (int j = jb; j < je; j++) {
-      const double *rj = rec + (size_t)REC_DOUBLES * j;
-      const double2 b0 = reinterpret_cast<const double2 *>(rj)[0];
-      const double2 b1 = reinterpret_cast<const double2 *>(rj)[1];
-      const double xj = b0.x, yj = b0.y;
-      const int fj = double_to_bits(b1.x);
-      if (run_pse) {  // do_interactions_heat_pse (interactions_gpu.cu:130-139), all cell members
-        const double2 b7 = reinterpret_cast<const double2 *>(rj)[7];
-        const double2 b3 = reinterpret_cast<const double2 *>(rj)[3];
-        const double dx = xi - xj, dy = yi - yj;
-        const double xx = sqrt(dx * dx + dy * dy);
-        const double w2 = pse_c * mfcm::exp(-xx * xx / (h2));
-        tpse += (b7.y - Tpred) * w2 * k.mass / b3.x;
-      }
-      if (!run_mech) continue;
-      if (fj & MFC_FLAG_BLANKED) continue;
+  // the PSE term of every window member and the test whether j adds to the mechanical sums
+  auto visit = [&](int j) -> bool {
+    const double *rj = rec + (size_t)REC_DOUBLES * j;
+    const double2 b0 = reinterpret_cast<const double2 *>(rj)[0];
+    const double2 b1 = reinterpret_cast<const double2 *>(rj)[1];
+    const double xj = b0.x, yj = b0.y;
+    const int fj = double_to_bits(b1.x);
+    if (run_pse) {  // do_interactions_heat_pse (interactions_gpu.cu:130-139), all cell members
+      const double2 b7 = reinterpret_cast<const double2 *>(rj)[7];
+      const double2 b3 = reinterpret_cast<const double2 *>(rj)[3];
       const double dx = xi - xj, dy = yi - yj;
-      const double rr2 = dx * dx + dy * dy;
-      // outside the support the reference adds exact zeros, unless a value is non-finite
-      if ((rr2 >= fourh2 || rr2 < 1e-8) && !nonfin_i && !(fj & FLAG_NONFINITE)) continue;
-      pair_terms(k, acc, xi, yi, ui, vi_, hi, rhoi, Axx_i, Axy_i, Ayy_i, Rxx_i, Rxy_i, Ryy_i, ci, Ti, tool, ta, h1,
-                 fourh2, fac, rj, xj, yj, (fj & MFC_FLAG_TOOL) != 0);
+      const double xx = sqrt(dx * dx + dy * dy);
+      const double w2 = pse_c * mfcm::exp(-xx * xx / (h2));
+      tpse += (b7.y - Tpred) * w2 * k.mass / b3.x;
+    }
+    if (!run_mech) return false;
+    if (fj & MFC_FLAG_BLANKED) return false;
+    const double dx = xi - xj, dy = yi - yj;
+    const double rr2 = dx * dx + dy * dy;
+    // outside the support the reference adds exact zeros, unless a value is non-finite
+    return !((rr2 >= fourh2 || rr2 < 1e-8) && !nonfin_i && !(fj & FLAG_NONFINITE));
+  };
+  auto pair = [&](int j) {
+    const double *rj = rec + (size_t)REC_DOUBLES * j;
+    const double2 b0 = reinterpret_cast<const double2 *>(rj)[0];
+    const double2 b1 = reinterpret_cast<const double2 *>(rj)[1];
+    pair_terms(k, acc, xi, yi, ui, vi_, hi, rhoi, Axx_i, Axy_i, Ayy_i, Rxx_i, Rxy_i, Ryy_i, ci, Ti, tool, ta, h1,
+               fourh2, fac, rj, b0.x, b0.y, (double_to_bits(b1.x) & MFC_FLAG_TOOL) != 0);
+  };
+  if constexpr (!COMPACT) {
+    for (int ii = li; ii < hi_; ii++) {
+      const int jb = offsets[ii * k.gny + lj], je = offsets[ii * k.gny + hj];
+      for (int j = jb; j < je; j++)
+        if (visit(j)) pair(j);
+    }
+  } else {
+    __shared__ int list[PAIR_LIST][TILE];
+    int ii = li, j = 0, je = 0;
+    bool more = ii < hi_;
+    if (more) { j = offsets[ii * k.gny + lj]; je = offsets[ii * k.gny + hj]; }
+    while (more) {
+      int cnt = 0;
+      while (cnt < PAIR_LIST) {
+        if (j >= je) {
+          if (++ii >= hi_) { more = false; break; }
+          j = offsets[ii * k.gny + lj];
+          je = offsets[ii * k.gny + hj];
+          continue;
+        }
+        if (visit(j)) list[cnt++][threadIdx.x] = j;
+        j++;
+      }
+      for (int q = 0; q < cnt; q++) pair(list[q][threadIdx.x]);
     }
   }
 }

@@ -60,8 +60,8 @@ struct mfc_sim {
   mfc::adj_work *adj = nullptr;    // reverse-mode tape of the last mfc_vjp_forward
   bool batched = false;            // instance of a batch: stepped by mfc_batch_step only
   double *acc = nullptr;           // neighbour sums between the split interaction kernels
-  mfc_tuning tuning = {0, {0, 0, 0, 0, 0, 0, 0}};
-  double *interact_acc() const { return tuning.split_interact ? acc : nullptr; }
+  mfc_tuning tuning = {0, 0, {0, 0, 0, 0, 0, 0}};
+  mfc::variant variant() const { return {tuning.split_interact != 0, tuning.compact_pairs != 0}; }
 
   template <class T>
   int alloc(T **p, size_t n) {

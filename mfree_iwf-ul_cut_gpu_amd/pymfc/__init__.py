@@ -109,7 +109,7 @@ class VjpReq(C.Structure):
 
 class Tuning(C.Structure):
     """mfc_tuning (include/mfc.h)."""
-    _fields_ = [("split_interact", C.c_int32), ("reserved", C.c_int32 * 7)]
+    _fields_ = [("split_interact", C.c_int32), ("compact_pairs", C.c_int32), ("reserved", C.c_int32 * 6)]
 
 
 THETA = ("jc_A", "jc_B", "jc_n", "jc_C", "jc_m", "mu")  # order of the adjoint's parameters
@@ -322,7 +322,8 @@ class Sim:
         return F
 
     def set_tuning(self, **kw) -> None:
-        """Launch configuration (results are independent of it): split_interact=0|1."""
+        """Launch configuration (results are independent of it): split_interact=0|1,
+        compact_pairs=0|1."""
         t = Tuning()
         _check(lib().mfc_get_tuning(self._h, C.byref(t)), "mfc_get_tuning")
         for k, v in kw.items():
@@ -332,7 +333,7 @@ class Sim:
     def tuning(self) -> dict:
         t = Tuning()
         _check(lib().mfc_get_tuning(self._h, C.byref(t)), "mfc_get_tuning")
-        return {"split_interact": t.split_interact}
+        return {"split_interact": t.split_interact, "compact_pairs": t.compact_pairs}
 
     def vjp_forward(self, nsteps: int, checkpoint_every: int, w0: int, w1: int) -> np.ndarray:
         """Run nsteps steps keeping a checkpoint every checkpoint_every steps; returns the mean
@@ -434,6 +435,7 @@ class Batch:
         self._h = C.c_void_p()
         _check(lib().mfc_batch_create(arr, len(cfgs), C.byref(self._h)), "mfc_batch_create")
         self.cfgs = [c.copy() for c in cfgs]
+        self._tuning = {"split_interact": 0, "compact_pairs": 0}
         self.instances = []
         for i, c in enumerate(self.cfgs):
             h = C.c_void_p()
@@ -460,9 +462,11 @@ class Batch:
             pass
 
     def set_tuning(self, **kw) -> None:
+        """Launch configuration of every instance: split_interact=0|1, compact_pairs=0|1."""
+        self._tuning.update({k: int(v) for k, v in kw.items()})
         t = Tuning()
-        for k, v in kw.items():
-            setattr(t, k, int(v))
+        for k, v in self._tuning.items():
+            setattr(t, k, v)
         _check(lib().mfc_batch_set_tuning(self._h, C.byref(t)), "mfc_batch_set_tuning")
 
     def step(self, n: int = 1) -> None:

@@ -21,12 +21,12 @@ def compare_instance(inst, single, what):
     assert equal(inst.tool_force(), single.tool_force()), f"{what}: tool force"
 
 
-@pytest.mark.parametrize("prec,split", [(pymfc.PREC_FP64, 0), (pymfc.PREC_FP64, 1), (pymfc.PREC_FP32C, 0),
-                                        (pymfc.PREC_FP32C, 1)])
-def test_batch_equals_single_runs(prec, split):
+@pytest.mark.parametrize("prec", [pymfc.PREC_FP64, pymfc.PREC_FP32C])
+@pytest.mark.parametrize("split,compact", [(0, 0), (1, 0), (0, 1), (1, 1)])
+def test_batch_equals_single_runs(prec, split, compact):
     cfgs = pymfc.batch_configs(8, seed=11, precision=prec)
     b = pymfc.Batch(cfgs)
-    b.set_tuning(split_interact=split)
+    b.set_tuning(split_interact=split, compact_pairs=compact)
     b.step(7)   # graph pairs plus one eager step
     b.step(3)
     b.sync()

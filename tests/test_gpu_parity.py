@@ -111,11 +111,15 @@ def test_graph_replay_matches_eager():
 
 
 @pytest.mark.parametrize("prec", [pymfc.PREC_FP64, pymfc.PREC_FP32C])
-def test_split_interaction_matches_fused(prec):
-    """The split interaction (sums kernel + update kernel) gives the fused kernel's bits."""
-    cfg = small_cut(thermal="contact", precision=prec)
+@pytest.mark.parametrize("thermal", ["contact", "pse"])
+@pytest.mark.parametrize("split,compact", [(1, 0), (0, 1), (1, 1)])
+def test_interaction_variants_match(prec, thermal, split, compact):
+    """Every launch variant of the interaction (split sums/update kernels, compacted pair
+    lists) gives the bits of the fused kernel that walks the window directly."""
+    cfg = small_cut(thermal=thermal, precision=prec)
     g1, g2 = pymfc.Sim(cfg), pymfc.Sim(cfg)
-    g2.set_tuning(split_interact=1)
+    g1.set_tuning(split_interact=0, compact_pairs=0)
+    g2.set_tuning(split_interact=split, compact_pairs=compact)
     g1.step(30)
     g2.step(30)
     for f in ["pos", "vel", "S", "T", "eps_pl", "fc", "ft", "flags"]:
