@@ -1,0 +1,47 @@
+"""Accuracy of the FP32-compensated path on the full thermal-contact cut at ~150k particles
+(750 x 200 at 2 um + 46,979 tool particles, Brookshaw + tool thermal contact), 20,000 steps,
+against the FP64 path. The FP64 GPU path is the oracle's arithmetic bit for bit (the parity
+tests show it on every kernel and on full runs of the smaller cuts); running the oracle itself
+for 20,000 steps of 197k particles would take about an hour of host time."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import pymfc
+
+pytestmark = pytest.mark.gpu
+
+
+def test_fp32c_full_thermal_contact_cut():
+    steps = 20000
+    out = {}
+    for name, prec in (("fp64", pymfc.PREC_FP64), ("fp32c", pymfc.PREC_FP32C)):
+        s = pymfc.Sim(pymfc.workload("thermal_contact_150k", prec))
+        s.step(steps)
+        s.sync()
+        out[name] = {f: s.field(f) for f in ("id", "pos", "T", "eps_pl", "S", "flags")}
+        out[name]["F"] = s.force_history()
+        s.close()
+    a, b = out["fp64"], out["fp32c"]
+    ia, ib = np.argsort(a["id"]), np.argsort(b["id"])
+    dx = 2e-4
+    w = steps // 2
+    Fa, Fb = a["F"][-w:].mean(axis=0), b["F"][-w:].mean(axis=0)
+    err = {
+        "force_rel": (np.abs(Fb - Fa) / np.abs(Fa)).tolist(),
+        "Tpeak_K": float(abs(b["T"].max() - a["T"].max())),
+        "T_max_abs_K": float(np.abs(b["T"][ib] - a["T"][ia]).max()),
+        "pos_max_dx": float(np.abs(b["pos"][ib] - a["pos"][ia]).max() / dx),
+        "eps_pl_max_abs": float(np.abs(b["eps_pl"][ib] - a["eps_pl"][ia]).max()),
+        "eps_pl_max": float(a["eps_pl"].max()),
+        "S_max_rel": float(np.abs(b["S"][ib] - a["S"][ia]).max() / np.abs(a["S"]).max()),
+        "blanked": [int(np.count_nonzero(a["flags"] & pymfc.FLAG_BLANKED)),
+                    int(np.count_nonzero(b["flags"] & pymfc.FLAG_BLANKED))],
+        "F_fp64": Fa.tolist(),
+        "T_peak_fp64": float(a["T"].max()),
+    }
+    print("fp32c vs fp64, thermal_contact_150k, 20000 steps:", err)
+    assert err["force_rel"][0] < 5e-3      # cutting force (mean over the second half) within 0.5 %
+    assert err["Tpeak_K"] < 1.0            # peak temperature within 1 K
+    assert err["blanked"][1] == err["blanked"][0]
