@@ -26,15 +26,15 @@ sys.path.insert(0, os.path.join(REPO, "mfree_iwf-ul_cut_gpu_amd"))
 METRIC = "particle-steps/s at 150k particles (FP32-comp vs FP64); adjoint/forward ratio"
 HBM_PEAK_GBS = 8000.0
 
-# compulsory bytes per particle of one launch of the interaction kernel: the particle's own
-# neighbour record (neighbour re-reads are cache traffic), 26 state doubles read and written,
+# compulsory bytes per particle of one launch of the fused interaction kernel: the particle's
+# own neighbour record (neighbour re-reads are cache traffic), 26 state doubles read and written,
 # flags read and written, the positional pressure (read) and contact normal (read, written),
-# the sorted cell id (read)
-_STATE_RW = 26 * 8 + 4 + 8 + 16 + 4 + (26 * 8 + 4 + 16)
+# the sorted cell id and the permutation entry (read)
+_STATE_RW = 26 * 8 + 4 + 8 + 16 + 4 + 4 + (26 * 8 + 4 + 16)
 INTERACT_BYTES = {"fp64": 128 + _STATE_RW, "fp32c": 64 + 16 + _STATE_RW}
-# the neighbour-sum kernel of the split variant: own record, flags and cell id read, 14 sums
-# written (8 B each)
-SUMS_BYTES = {"fp64": 128 + 4 + 4 + 14 * 8, "fp32c": 64 + 16 + 4 + 4 + 14 * 8}
+# the neighbour-sum kernel of the split variant: own record, flags, cell id and permutation entry
+# read, 14 sums written (8 B each)
+SUMS_BYTES = {"fp64": 128 + 4 + 4 + 4 + 14 * 8, "fp32c": 64 + 16 + 4 + 4 + 4 + 14 * 8}
 # whole step, compulsory traffic of a fully fused FP64 step (persistent read-only 32 B,
 # mutable 224 B read + written, 12 B of binning keys)
 STEP_BYTES_FP64 = 256 + 224 + 12
@@ -262,7 +262,7 @@ def main() -> int:
     r = res[head]
     N = r["N"]
     split = r["tuning"]["split_interact"]
-    kname = "k_sums" if split else "k_interact"
+    kname = "kb_sums" if split else "kb_interact"
     kbytes = SUMS_BYTES[head] if split else INTERACT_BYTES[head]
     t_int = r["phases"]["interact"] * 1e-3
     achieved = N * kbytes / t_int / 1e9

@@ -8,7 +8,8 @@
 // kernels to recover the start state of each of its steps, and the steps are then reversed
 // one by one:
 //   F1  binning          launch_binning (cell ids, stable counting sort)
-//   F2  predict          launch_predict (gather, predictor, EOS, artificial stress, records)
+//   F2  predict          launch_predict with the sorted copy of the start state (gather,
+//                        predictor, EOS, artificial stress, records)
 //   F3  launch_sums      neighbour sums of every particle (the sums kernel of the step)
 //   B1  k_adj_tail       VJP of everything after the sums, per particle (tail_vjp), and the
 //                        per-tile theta partials of the step (k_adj_theta adds them up)
@@ -404,8 +405,8 @@ int mfc_vjp_backward(mfc_sim *s, const double cot[2], double grad[6]) {
       carve_state(w->work[wa], N, A);
       carve_state(w->work[1 - wa], N, B);
       launch_binning(k, A, s->bins, st);
-      launch_predict(k, A, B, s->bins, P, s->recs, w->toolw, w->toolw + 1, st);
-      launch_interact(k, B, s->bins, P, s->recs, w->toolw + 1, fo, s->acc, s->variant(), st);
+      launch_predict(k, A, B, s->bins, P, s->recs, w->toolw, w->toolw + 1, false, st);
+      launch_interact(k, A, B, s->bins, P, s->recs, w->toolw + 1, fo, s->acc, s->variant(), st);
       HIP_TRY(hipMemcpyAsync(w->toolw, w->toolw + 1, sizeof(mfc_tool_state), hipMemcpyDeviceToDevice, st));
       wa = 1 - wa;
     }
@@ -415,11 +416,11 @@ int mfc_vjp_backward(mfc_sim *s, const double cot[2], double grad[6]) {
       carve_state(w->work[0], N, A);
       carve_state(w->work[1], N, B);
       launch_binning(k, A, s->bins, st);
-      launch_predict(k, A, B, s->bins, P, s->recs, w->toolw, w->toolw + 1, st);
+      launch_predict(k, A, B, s->bins, P, s->recs, w->toolw, w->toolw + 1, true, st);
       const double *rec = s->recs.rec;
       const double *sb = B.px;  // sorted start state, TS_* order with stride N
       const bool in_win = t >= w->w0 && t < w->w1;
-      launch_sums(k, B.flags, s->bins, s->recs, w->acc, s->variant(), st);
+      launch_sums(k, A.flags, s->bins, s->recs, w->acc, s->variant(), st);
       hipLaunchKernelGGL(k_adj_tail, dim3(padded(nt)), dim3(TILE), 0, st, k, nt, sb, B.flags, P, w->acc,
                          w->lam_s[cur], w->lam_n, in_win ? Fb[0] : 0., in_win ? Fb[1] : 0., w->toolw + 1, w->g_s,
                          w->lam_acc, w->theta_part);

@@ -66,12 +66,24 @@ void carve_positional(void *base, int N, mfc::dpositional &p) {
   p.rxx = b + 3 * (size_t)N; p.rxy = b + 4 * (size_t)N; p.ryy = b + 5 * (size_t)N;
 }
 
-void enqueue_step(mfc_sim *s, int c) {
-  const int n = 1 - c;
-  mfc::launch_binning(s->k, s->set[c], s->bins, s->stream);
-  mfc::launch_predict(s->k, s->set[c], s->set[n], s->bins, s->pos, s->recs, s->tool + c, s->tool + n, s->stream);
-  mfc::launch_interact(s->k, s->set[n], s->bins, s->pos, s->recs, s->tool + n, s->force(), s->acc, s->variant(),
-                       s->stream);
+void enqueue_step(mfc_sim *s, int c) { mfc::launch_batch_step(s->work(c), s->stream); }
+
+mfc::inst_args instance_args(const mfc_sim *s, int c) {
+  mfc::inst_args a;
+  std::memset(&a, 0, sizeof(a));
+  a.k = s->k;
+  a.cur = s->set[c];
+  a.nxt = s->set[1 - c];
+  a.b = s->bins;
+  a.pos = s->pos;
+  a.r = s->recs;
+  a.tool_in = s->tool + c;
+  a.tool_out = s->tool + (1 - c);
+  a.f = s->force();
+  a.acc = s->acc;
+  a.ntiles = s->ntiles;
+  a.nscan = mfc::scan_blocks(s->k.num_cell);
+  return a;
 }
 
 namespace {
@@ -209,6 +221,20 @@ int mfc_create(const mfc_config *cfg, mfc_sim **out) {
       return bail(fail(MFC_E_HIP, "memset"));
     mfc_tool_state t2[2] = {S.tool, S.tool};
     if (up(s->tool, t2, sizeof(t2)) != hipSuccess) return bail(fail(MFC_E_HIP, "upload tool"));
+    // descriptors and work tables of the step kernels (a batch of one)
+    const int nscan = mfc::scan_blocks(c.num_cell);
+    std::vector<int2> tiles(s->ntiles), chunks(nscan);
+    for (int t = 0; t < s->ntiles; t++) tiles[t] = make_int2(0, t);
+    for (int q = 0; q < nscan; q++) chunks[q] = make_int2(0, q);
+    s->nchunks = nscan;
+    mfc::inst_args a2[2] = {instance_args(s, 0), instance_args(s, 1)};
+    if ((rc = s->alloc(&s->dargs, 2)) || (rc = s->alloc(&s->dtiles, (size_t)s->ntiles)) ||
+        (rc = s->alloc(&s->dchunks, (size_t)nscan)))
+      return bail(rc);
+    if (up(s->dargs, a2, sizeof(a2)) != hipSuccess ||
+        up(s->dtiles, tiles.data(), sizeof(int2) * tiles.size()) != hipSuccess ||
+        up(s->dchunks, chunks.data(), sizeof(int2) * chunks.size()) != hipSuccess)
+      return bail(fail(MFC_E_HIP, "upload of the step tables"));
     if (hipStreamSynchronize(s->stream) != hipSuccess) return bail(fail(MFC_E_HIP, "sync"));
   }
   *out = s;
@@ -269,25 +295,24 @@ int mfc_profile(mfc_sim *s, int nsteps, double *ms) {
   for (int q = 0; q < NE; q++) HIP_TRY(hipEventCreate(&ev[q]));
   double acc[MFC_PROFILE_PHASES] = {0., 0., 0., 0., 0.};
   int rc = MFC_OK;
-  const mfc::dforce fo = s->force();
   for (int it = 0; it < nsteps && rc == MFC_OK; it++) {
     const int c = s->cur, n = 1 - c;
+    const mfc::batch_work w = s->work(c);
     (void)hipEventRecord(ev[0], s->stream);
-    mfc::launch_binning(s->k, s->set[c], s->bins, s->stream);
+    mfc::launch_batch_binning(w, s->stream);
     (void)hipEventRecord(ev[1], s->stream);
-    mfc::launch_predict(s->k, s->set[c], s->set[n], s->bins, s->pos, s->recs, s->tool + c, s->tool + n, s->stream);
+    mfc::launch_batch_predict(w, s->stream);
     (void)hipEventRecord(ev[2], s->stream);
-    if (s->tuning.split_interact) {
-      mfc::launch_sums(s->k, s->set[n].flags, s->bins, s->recs, s->acc, s->variant(), s->stream);
+    if (w.v.split) {
+      mfc::launch_batch_sums(w, s->stream);
       (void)hipEventRecord(ev[3], s->stream);
-      mfc::launch_update(s->k, s->acc, s->set[n], s->pos, s->tool + n, fo, s->stream);
+      mfc::launch_batch_update(w, s->stream);
     } else {
-      mfc::launch_interact_fused(s->k, s->set[n], s->bins, s->pos, s->recs, s->tool + n, fo, s->variant(),
-                                 s->stream);
+      mfc::launch_batch_interact_fused(w, s->stream);
       (void)hipEventRecord(ev[3], s->stream);
     }
     (void)hipEventRecord(ev[4], s->stream);
-    mfc::launch_force_final(s->k, fo, s->stream);
+    mfc::launch_batch_force(w, s->stream);
     (void)hipEventRecord(ev[5], s->stream);
     s->cur = n;
     s->step++;

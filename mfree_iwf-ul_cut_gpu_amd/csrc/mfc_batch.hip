@@ -42,24 +42,6 @@ using mfc::fail;
 
 namespace {
 
-mfc::inst_args make_args(const mfc_sim *s, int c) {
-  mfc::inst_args a;
-  std::memset(&a, 0, sizeof(a));
-  a.k = s->k;
-  a.cur = s->set[c];
-  a.nxt = s->set[1 - c];
-  a.b = s->bins;
-  a.pos = s->pos;
-  a.r = s->recs;
-  a.tool_in = s->tool + c;
-  a.tool_out = s->tool + (1 - c);
-  a.f = s->force();
-  a.acc = s->acc;
-  a.ntiles = s->ntiles;
-  a.nscan = (s->k.num_cell + mfc::scan_chunk_cells() - 1) / mfc::scan_chunk_cells();
-  return a;
-}
-
 void batch_release(mfc_batch *b) {
   if (!b) return;
   if (b->stream) (void)hipStreamSynchronize(b->stream);
@@ -131,8 +113,8 @@ int mfc_batch_create(const mfc_config *cfgs, int n, mfc_batch **out) {
   std::vector<mfc::inst_args> a0(n), a1(n);
   for (int i = 0; i < n; i++) {
     const mfc_sim *s = b->sims[i];
-    a0[i] = make_args(s, 0);
-    a1[i] = make_args(s, 1);
+    a0[i] = instance_args(s, 0);
+    a1[i] = instance_args(s, 1);
     for (int t = 0; t < a0[i].ntiles; t++) tiles.push_back(make_int2(i, t));
     for (int c = 0; c < a0[i].nscan; c++) chunks.push_back(make_int2(i, c));
   }

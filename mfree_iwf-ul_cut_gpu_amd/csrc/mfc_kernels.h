@@ -61,6 +61,9 @@ struct dforce {
   int64_t *ctr;          // steps recorded
 };
 
+// the particle flags of the state (FIXED, BLANKED, TOOL)
+constexpr int FLAG_MASK = MFC_FLAG_FIXED | MFC_FLAG_BLANKED | MFC_FLAG_TOOL;
+
 // internal flag: some value of the particle's neighbour record is NaN or Inf
 constexpr int FLAG_NONFINITE = 8;
 
@@ -98,29 +101,39 @@ struct batch_work {
   variant v;
 };
 
-// one step of every instance of a batch: 9 (fused) or 10 (split) launches in all
+// one step of every instance of a batch: 9 (fused) or 10 (split) launches in all, and its phases
 void launch_batch_step(const batch_work &w, hipStream_t s);
+void launch_batch_binning(const batch_work &w, hipStream_t s);
+void launch_batch_predict(const batch_work &w, hipStream_t s);
+void launch_batch_interact_fused(const batch_work &w, hipStream_t s);
+void launch_batch_sums(const batch_work &w, hipStream_t s);
+void launch_batch_update(const batch_work &w, hipStream_t s);
+void launch_batch_force(const batch_work &w, hipStream_t s);
 int scan_chunk_cells();
 
-// step kernels
+// step kernels. The start state cur is in the storage order of the previous step; predict reads
+// it through the sort permutation and writes the neighbour records (and, with gather, the
+// sorted copy of cur into nxt); the interaction reads cur through the permutation and writes the
+// updated state to nxt in sorted order.
 void launch_binning(const dev_consts &k, const dstate &cur, dbins &b, hipStream_t s);
 void launch_predict(const dev_consts &k, const dstate &cur, const dstate &nxt, const dbins &b, const dpositional &pos,
-                    const drecords &r, const mfc_tool_state *tool_in, mfc_tool_state *tool_out, hipStream_t s);
+                    const drecords &r, const mfc_tool_state *tool_in, mfc_tool_state *tool_out, bool gather,
+                    hipStream_t s);
 // neighbour sums + per-particle update + contact, then the tool-force reduction; v.split: the
 // sums go through acc (ACC_N arrays of N doubles) between a sums kernel and an update kernel
 // (fewer registers per kernel), else one fused kernel
-void launch_interact(const dev_consts &k, const dstate &st, const dbins &b, const dpositional &pos,
-                     const drecords &r, const mfc_tool_state *tool_now, const dforce &f, double *acc, variant v,
-                     hipStream_t s);
+void launch_interact(const dev_consts &k, const dstate &cur, const dstate &nxt, const dbins &b,
+                     const dpositional &pos, const drecords &r, const mfc_tool_state *tool_now, const dforce &f,
+                     double *acc, variant v, hipStream_t s);
 // the pieces of launch_interact, for per-kernel timing
-void launch_interact_fused(const dev_consts &k, const dstate &st, const dbins &b, const dpositional &pos,
-                           const drecords &r, const mfc_tool_state *tool_now, const dforce &f, variant v,
-                           hipStream_t s);
-void launch_update(const dev_consts &k, const double *acc, const dstate &st, const dpositional &pos,
-                   const mfc_tool_state *tool_now, const dforce &f, hipStream_t s);
+void launch_interact_fused(const dev_consts &k, const dstate &cur, const dstate &nxt, const dbins &b,
+                           const dpositional &pos, const drecords &r, const mfc_tool_state *tool_now, const dforce &f,
+                           variant v, hipStream_t s);
+void launch_update(const dev_consts &k, const double *acc, const dbins &b, const dstate &cur, const dstate &nxt,
+                   const dpositional &pos, const mfc_tool_state *tool_now, const dforce &f, hipStream_t s);
 void launch_force_final(const dev_consts &k, const dforce &f, hipStream_t s);
-// neighbour sums only (ACC_N arrays of N doubles), flags in sorted order
-void launch_sums(const dev_consts &k, const int *flags, const dbins &b, const drecords &r, double *acc, variant v,
+// neighbour sums only (ACC_N arrays of N doubles); src_flags in the storage order of cur
+void launch_sums(const dev_consts &k, const int *src_flags, const dbins &b, const drecords &r, double *acc, variant v,
                  hipStream_t s);
 
 int scan_blocks(int num_cell);

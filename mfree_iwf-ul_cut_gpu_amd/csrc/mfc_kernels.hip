@@ -207,7 +207,7 @@ __device__ inline void artificial_stress_f32(float rho, float p, float Sxx, floa
 
 // gather into sorted order, leap-frog init + predict, EOS, artificial stress, neighbour record.
 // F32: the record is 16 floats (64 bytes) plus the FP64 predicted position.
-template <bool F32>
+template <bool F32, bool GATHER>
 __device__ inline void predict(const dev_consts &k, int t, const int *__restrict__ perm, const dstate &cur,
                                const dstate &nxt, const dpositional &pos, double *__restrict__ rec,
                                double2 *__restrict__ posd, float *__restrict__ rec32,
@@ -221,23 +221,25 @@ __device__ inline void predict(const dev_consts &k, int t, const int *__restrict
   if (i >= k.N) return;
   const int s = perm[i];
   const double x0 = cur.px[s], y0 = cur.py[s], u0 = cur.vx[s], v0 = cur.vy[s];
-  const double sxx0 = cur.sxx[s], sxy0 = cur.sxy[s], syy0 = cur.syy[s], szz0 = cur.szz[s];
+  const double sxx0 = cur.sxx[s], sxy0 = cur.sxy[s], syy0 = cur.syy[s];
   const double rho0 = cur.rho[s], T0 = cur.T[s];
   const double ptx = cur.ptx[s], pty = cur.pty[s], vtx = cur.vtx[s], vty = cur.vty[s];
-  const double rhot = cur.rhot[s], stxx = cur.stxx[s], stxy = cur.stxy[s], styy = cur.styy[s], stzz = cur.stzz[s];
+  const double rhot = cur.rhot[s], stxx = cur.stxx[s], stxy = cur.stxy[s], styy = cur.styy[s];
   const double Tt = cur.Tt[s];
-  const int fl = cur.flags[s] & (MFC_FLAG_FIXED | MFC_FLAG_BLANKED | MFC_FLAG_TOOL);
+  const int fl = cur.flags[s] & FLAG_MASK;
   const double h = cur.h[s];
-  nxt.px[i] = x0; nxt.py[i] = y0; nxt.vx[i] = u0; nxt.vy[i] = v0;
-  nxt.sxx[i] = sxx0; nxt.sxy[i] = sxy0; nxt.syy[i] = syy0; nxt.szz[i] = szz0;
-  nxt.rho[i] = rho0; nxt.T[i] = T0; nxt.h[i] = h;
-  nxt.eps[i] = cur.eps[s]; nxt.epsd[i] = cur.epsd[s];
-  nxt.fcx[i] = cur.fcx[s]; nxt.fcy[i] = cur.fcy[s]; nxt.ftx[i] = cur.ftx[s]; nxt.fty[i] = cur.fty[s];
-  nxt.ptx[i] = ptx; nxt.pty[i] = pty; nxt.vtx[i] = vtx; nxt.vty[i] = vty;
-  nxt.rhot[i] = rhot; nxt.stxx[i] = stxx; nxt.stxy[i] = stxy; nxt.styy[i] = styy; nxt.stzz[i] = stzz;
-  nxt.Tt[i] = Tt;
-  nxt.flags[i] = fl;
-  nxt.id[i] = cur.id[s];
+  if constexpr (GATHER) {  // the sorted copy of the start state (the reverse mode reads it)
+    nxt.px[i] = x0; nxt.py[i] = y0; nxt.vx[i] = u0; nxt.vy[i] = v0;
+    nxt.sxx[i] = sxx0; nxt.sxy[i] = sxy0; nxt.syy[i] = syy0; nxt.szz[i] = cur.szz[s];
+    nxt.rho[i] = rho0; nxt.T[i] = T0; nxt.h[i] = h;
+    nxt.eps[i] = cur.eps[s]; nxt.epsd[i] = cur.epsd[s];
+    nxt.fcx[i] = cur.fcx[s]; nxt.fcy[i] = cur.fcy[s]; nxt.ftx[i] = cur.ftx[s]; nxt.fty[i] = cur.fty[s];
+    nxt.ptx[i] = ptx; nxt.pty[i] = pty; nxt.vtx[i] = vtx; nxt.vty[i] = vty;
+    nxt.rhot[i] = rhot; nxt.stxx[i] = stxx; nxt.stxy[i] = stxy; nxt.styy[i] = styy; nxt.stzz[i] = cur.stzz[s];
+    nxt.Tt[i] = Tt;
+    nxt.flags[i] = fl;
+    nxt.id[i] = cur.id[s];
+  }
 
   double x = x0, y = y0, u = u0, v = v0, sxx = sxx0, sxy = sxy0, syy = syy0, rho = rho0, T = T0;
   if (!(fl & MFC_FLAG_BLANKED)) {  // leap_frog.cu:42-66
@@ -296,7 +298,7 @@ __device__ inline void predict(const dev_consts &k, int t, const int *__restrict
   for (int q = 0; q < REC / 2; q++) dst[q] = make_double2(r[2 * q], r[2 * q + 1]);
 }
 
-template <bool F32>
+template <bool F32, bool GATHER>
 __global__ __launch_bounds__(TILE) void k_predict(dev_consts k, int ntiles, const int *__restrict__ perm, dstate cur,
                                                   dstate nxt, dpositional pos, double *__restrict__ rec,
                                                   double2 *__restrict__ posd, float *__restrict__ rec32,
@@ -304,7 +306,7 @@ __global__ __launch_bounds__(TILE) void k_predict(dev_consts k, int ntiles, cons
                                                   mfc_tool_state *__restrict__ tool_out) {
   const int t = xcd_tile(blockIdx.x, ntiles);
   if (t < 0) return;
-  predict<F32>(k, t, perm, cur, nxt, pos, rec, posd, rec32, tool_in, tool_out);
+  predict<F32, GATHER>(k, t, perm, cur, nxt, pos, rec, posd, rec32, tool_in, tool_out);
 }
 
 // ------------------------------------------------------------------------------- interact
@@ -461,23 +463,26 @@ __device__ inline void sums_f32(const dev_consts &k, int i, const double2 *__res
 
 // everything after the neighbour sums for particle i (sorted index), in the reference's order;
 // returns the particle's fc and ft for the tool force
-__device__ inline void update_particle(const dev_consts &k, int i, const pair_acc &acc, double tpse, dstate st,
-                                       dpositional pos, const mfc_tool_state &tool_now, double fsum[4]) {
+// reads particle s of the unsorted start state src (s = perm[i]) and writes the updated
+// particle to position i of dst: the gather into sorted order happens here, once
+__device__ inline void update_particle(const dev_consts &k, int i, int s, const pair_acc &acc, double tpse,
+                                       const dstate &src, const dstate &dst, const dpositional &pos,
+                                       const mfc_tool_state &tool_now, double fsum[4]) {
   const bool pse = k.pse != 0;
-  int fl = st.flags[i];
+  int fl = src.flags[s] & FLAG_MASK;
   const bool blank = fl & MFC_FLAG_BLANKED, tool = fl & MFC_FLAG_TOOL, fixed = fl & MFC_FLAG_FIXED;
   const double ta = pse ? 0. : (tool ? k.alpha_tool : k.alpha_wp);
   const bool run_mech = !blank && (!pse || !tool);
   const bool run_pse = !blank && pse && k.thermal_wp;
   // state at the start of the step (sorted) and previous derivatives
-  const double x0 = st.px[i], y0 = st.py[i], u0 = st.vx[i], v0 = st.vy[i];
-  const double sxx0 = st.sxx[i], sxy0 = st.sxy[i], syy0 = st.syy[i], szz0 = st.szz[i];
-  const double rho0 = st.rho[i], T0 = st.T[i];
-  double ptx = st.ptx[i], pty = st.pty[i], vtx = st.vtx[i], vty = st.vty[i];
-  double rhot = st.rhot[i], stxx = st.stxx[i], stxy = st.stxy[i], styy = st.styy[i], stzz = st.stzz[i];
-  double Tt = st.Tt[i];
-  double fcx = st.fcx[i], fcy = st.fcy[i], ftx = st.ftx[i], fty = st.fty[i];
-  double eps = st.eps[i], epsd = st.epsd[i];
+  const double x0 = src.px[s], y0 = src.py[s], u0 = src.vx[s], v0 = src.vy[s];
+  const double sxx0 = src.sxx[s], sxy0 = src.sxy[s], syy0 = src.syy[s], szz0 = src.szz[s];
+  const double rho0 = src.rho[s], T0 = src.T[s];
+  double ptx = src.ptx[s], pty = src.pty[s], vtx = src.vtx[s], vty = src.vty[s];
+  double rhot = src.rhot[s], stxx = src.stxx[s], stxy = src.stxy[s], styy = src.styy[s], stzz = src.stzz[s];
+  double Tt = src.Tt[s];
+  double fcx = src.fcx[s], fcy = src.fcy[s], ftx = src.ftx[s], fty = src.fty[s];
+  double eps = src.eps[s], epsd = src.epsd[s];
   // predicted state (same expressions as k_predict)
   double pu = u0, pv = v0, psxx = sxx0, psxy = sxy0, psyy = syy0, pszz = szz0, prho = rho0;
   if (!blank) {
@@ -545,14 +550,15 @@ __device__ inline void update_particle(const dev_consts &k, int i, const pair_ac
                !finite_d(stxy) || !finite_d(styy) || !finite_d(stzz) || !finite_d(rhot) || !finite_d(Tt);
     if (bad) fl |= MFC_FLAG_BLANKED;
   }
-  st.px[i] = x; st.py[i] = y; st.vx[i] = u; st.vy[i] = v;
-  st.sxx[i] = sxx; st.sxy[i] = sxy; st.syy[i] = syy; st.szz[i] = szz;
-  st.rho[i] = rho; st.T[i] = T; st.eps[i] = eps; st.epsd[i] = epsd;
-  st.fcx[i] = fcx; st.fcy[i] = fcy; st.ftx[i] = ftx; st.fty[i] = fty;
-  st.ptx[i] = ptx; st.pty[i] = pty; st.vtx[i] = vtx; st.vty[i] = vty;
-  st.rhot[i] = rhot; st.stxx[i] = stxx; st.stxy[i] = stxy; st.styy[i] = styy; st.stzz[i] = stzz;
-  st.Tt[i] = Tt;
-  st.flags[i] = fl;
+  dst.px[i] = x; dst.py[i] = y; dst.vx[i] = u; dst.vy[i] = v;
+  dst.sxx[i] = sxx; dst.sxy[i] = sxy; dst.syy[i] = syy; dst.szz[i] = szz;
+  dst.rho[i] = rho; dst.T[i] = T; dst.eps[i] = eps; dst.epsd[i] = epsd; dst.h[i] = src.h[s];
+  dst.fcx[i] = fcx; dst.fcy[i] = fcy; dst.ftx[i] = ftx; dst.fty[i] = fty;
+  dst.ptx[i] = ptx; dst.pty[i] = pty; dst.vtx[i] = vtx; dst.vty[i] = vty;
+  dst.rhot[i] = rhot; dst.stxx[i] = stxx; dst.stxy[i] = stxy; dst.styy[i] = styy; dst.stzz[i] = stzz;
+  dst.Tt[i] = Tt;
+  dst.flags[i] = fl;
+  dst.id[i] = src.id[s];
   fsum[0] = fcx; fsum[1] = fcy; fsum[2] = ftx; fsum[3] = fty;
 }
 
@@ -600,15 +606,17 @@ template <bool F32, bool COMPACT>
 __device__ inline void interact(const dev_consts &k, int t, const double *__restrict__ rec,
                                 const double2 *__restrict__ posd, const float *__restrict__ rec32,
                                 const int *__restrict__ key_sorted, const int *__restrict__ offsets,
-                                const dstate &st, const dpositional &pos, const mfc_tool_state *__restrict__ tool_now,
+                                const int *__restrict__ perm, const dstate &src, const dstate &dst,
+                                const dpositional &pos, const mfc_tool_state *__restrict__ tool_now,
                                 const dforce &fo) {
   const int i = t * TILE + threadIdx.x;
   double fsum[4] = {0., 0., 0., 0.};
   if (i < k.N) {
+    const int s = perm[i];
     pair_acc acc;
     double tpse;
-    particle_sums<F32, COMPACT>(k, i, rec, posd, rec32, key_sorted, offsets, st.flags[i], acc, tpse);
-    update_particle(k, i, acc, tpse, st, pos, *tool_now, fsum);
+    particle_sums<F32, COMPACT>(k, i, rec, posd, rec32, key_sorted, offsets, src.flags[s] & FLAG_MASK, acc, tpse);
+    update_particle(k, i, s, acc, tpse, src, dst, pos, *tool_now, fsum);
   }
   force_partials(t, fsum, fo.partials);
 }
@@ -618,17 +626,19 @@ template <bool F32, bool COMPACT>
 __device__ inline void sums(const dev_consts &k, int t, const double *__restrict__ rec,
                             const double2 *__restrict__ posd, const float *__restrict__ rec32,
                             const int *__restrict__ key_sorted, const int *__restrict__ offsets,
-                            const int *__restrict__ flags, double *__restrict__ acc) {
+                            const int *__restrict__ perm, const int *__restrict__ src_flags,
+                            double *__restrict__ acc) {
   const int i = t * TILE + threadIdx.x;
   if (i >= k.N) return;
   pair_acc a;
   double tpse;
-  particle_sums<F32, COMPACT>(k, i, rec, posd, rec32, key_sorted, offsets, flags[i], a, tpse);
+  particle_sums<F32, COMPACT>(k, i, rec, posd, rec32, key_sorted, offsets, src_flags[perm[i]] & FLAG_MASK, a, tpse);
   store_acc(acc, k.N, i, a, tpse);
 }
 
 // split, second half: the per-particle update from the stored sums
-__device__ inline void update(const dev_consts &k, int t, const double *__restrict__ acc, const dstate &st,
+__device__ inline void update(const dev_consts &k, int t, const double *__restrict__ acc,
+                              const int *__restrict__ perm, const dstate &src, const dstate &dst,
                               const dpositional &pos, const mfc_tool_state *__restrict__ tool_now, const dforce &fo) {
   const int i = t * TILE + threadIdx.x;
   double fsum[4] = {0., 0., 0., 0.};
@@ -636,7 +646,7 @@ __device__ inline void update(const dev_consts &k, int t, const double *__restri
     pair_acc a;
     double tpse;
     load_acc(acc, k.N, i, a, tpse);
-    update_particle(k, i, a, tpse, st, pos, *tool_now, fsum);
+    update_particle(k, i, perm[i], a, tpse, src, dst, pos, *tool_now, fsum);
   }
   force_partials(t, fsum, fo.partials);
 }
@@ -673,29 +683,32 @@ template <bool F32, bool COMPACT>
 __global__ __launch_bounds__(TILE) void k_interact(dev_consts k, int ntiles, const double *__restrict__ rec,
                                                    const double2 *__restrict__ posd, const float *__restrict__ rec32,
                                                    const int *__restrict__ key_sorted,
-                                                   const int *__restrict__ offsets, dstate st, dpositional pos,
+                                                   const int *__restrict__ offsets, const int *__restrict__ perm,
+                                                   dstate src, dstate dst, dpositional pos,
                                                    const mfc_tool_state *__restrict__ tool_now, dforce fo) {
   const int t = xcd_tile(blockIdx.x, ntiles);
   if (t < 0) return;
-  interact<F32, COMPACT>(k, t, rec, posd, rec32, key_sorted, offsets, st, pos, tool_now, fo);
+  interact<F32, COMPACT>(k, t, rec, posd, rec32, key_sorted, offsets, perm, src, dst, pos, tool_now, fo);
 }
 
 template <bool F32, bool COMPACT>
 __global__ __launch_bounds__(TILE) void k_sums(dev_consts k, int ntiles, const double *__restrict__ rec,
                                                const double2 *__restrict__ posd, const float *__restrict__ rec32,
                                                const int *__restrict__ key_sorted, const int *__restrict__ offsets,
-                                               const int *__restrict__ flags, double *__restrict__ acc) {
+                                               const int *__restrict__ perm, const int *__restrict__ src_flags,
+                                               double *__restrict__ acc) {
   const int t = xcd_tile(blockIdx.x, ntiles);
   if (t < 0) return;
-  sums<F32, COMPACT>(k, t, rec, posd, rec32, key_sorted, offsets, flags, acc);
+  sums<F32, COMPACT>(k, t, rec, posd, rec32, key_sorted, offsets, perm, src_flags, acc);
 }
 
 __global__ __launch_bounds__(TILE) void k_update(dev_consts k, int ntiles, const double *__restrict__ acc,
-                                                 dstate st, dpositional pos,
-                                                 const mfc_tool_state *__restrict__ tool_now, dforce fo) {
+                                                 const int *__restrict__ perm, dstate src, dstate dst,
+                                                 dpositional pos, const mfc_tool_state *__restrict__ tool_now,
+                                                 dforce fo) {
   const int t = xcd_tile(blockIdx.x, ntiles);
   if (t < 0) return;
-  update(k, t, acc, st, pos, tool_now, fo);
+  update(k, t, acc, perm, src, dst, pos, tool_now, fo);
 }
 
 __global__ __launch_bounds__(TILE) void k_force_final(int ntiles, dforce fo) { force_final(ntiles, fo); }
@@ -762,7 +775,8 @@ __global__ __launch_bounds__(TILE) void kb_predict(const inst_args *__restrict__
   if (w < 0) return;
   const int2 it = work[w];
   const inst_args &a = A[it.x];
-  predict<F32>(a.k, it.y, a.b.perm, a.cur, a.nxt, a.pos, a.r.rec, a.r.posd, a.r.rec32, a.tool_in, a.tool_out);
+  predict<F32, false>(a.k, it.y, a.b.perm, a.cur, a.nxt, a.pos, a.r.rec, a.r.posd, a.r.rec32, a.tool_in,
+                      a.tool_out);
 }
 
 template <bool F32, bool COMPACT>
@@ -772,7 +786,8 @@ __global__ __launch_bounds__(TILE) void kb_interact(const inst_args *__restrict_
   if (w < 0) return;
   const int2 it = work[w];
   const inst_args &a = A[it.x];
-  interact<F32, COMPACT>(a.k, it.y, a.r.rec, a.r.posd, a.r.rec32, a.b.key_sorted, a.b.offsets, a.nxt, a.pos, a.tool_out, a.f);
+  interact<F32, COMPACT>(a.k, it.y, a.r.rec, a.r.posd, a.r.rec32, a.b.key_sorted, a.b.offsets, a.b.perm, a.cur,
+                         a.nxt, a.pos, a.tool_out, a.f);
 }
 
 template <bool F32, bool COMPACT>
@@ -782,7 +797,8 @@ __global__ __launch_bounds__(TILE) void kb_sums(const inst_args *__restrict__ A,
   if (w < 0) return;
   const int2 it = work[w];
   const inst_args &a = A[it.x];
-  sums<F32, COMPACT>(a.k, it.y, a.r.rec, a.r.posd, a.r.rec32, a.b.key_sorted, a.b.offsets, a.nxt.flags, a.acc);
+  sums<F32, COMPACT>(a.k, it.y, a.r.rec, a.r.posd, a.r.rec32, a.b.key_sorted, a.b.offsets, a.b.perm, a.cur.flags,
+                     a.acc);
 }
 
 __global__ __launch_bounds__(TILE) void kb_update(const inst_args *__restrict__ A, const int2 *__restrict__ work,
@@ -791,7 +807,7 @@ __global__ __launch_bounds__(TILE) void kb_update(const inst_args *__restrict__ 
   if (w < 0) return;
   const int2 it = work[w];
   const inst_args &a = A[it.x];
-  update(a.k, it.y, a.acc, a.nxt, a.pos, a.tool_out, a.f);
+  update(a.k, it.y, a.acc, a.b.perm, a.cur, a.nxt, a.pos, a.tool_out, a.f);
 }
 
 __global__ __launch_bounds__(TILE) void kb_force_final(const inst_args *__restrict__ A) {
@@ -842,39 +858,39 @@ void launch_binning(const dev_consts &k, const dstate &cur, dbins &b, hipStream_
 static int padded_tiles(int ntiles) { return ((ntiles + 7) / 8) * 8; }
 
 void launch_predict(const dev_consts &k, const dstate &cur, const dstate &nxt, const dbins &b, const dpositional &pos,
-                    const drecords &r, const mfc_tool_state *tool_in, mfc_tool_state *tool_out, hipStream_t s) {
+                    const drecords &r, const mfc_tool_state *tool_in, mfc_tool_state *tool_out, bool gather,
+                    hipStream_t s) {
   const int nt = (k.N + TILE - 1) / TILE;
-  if (r.f32)
-    hipLaunchKernelGGL(k_predict<true>, dim3(padded_tiles(nt)), dim3(TILE), 0, s, k, nt, b.perm, cur, nxt, pos, r.rec,
-                       r.posd, r.rec32, tool_in, tool_out);
-  else
-    hipLaunchKernelGGL(k_predict<false>, dim3(padded_tiles(nt)), dim3(TILE), 0, s, k, nt, b.perm, cur, nxt, pos, r.rec,
-                       r.posd, r.rec32, tool_in, tool_out);
+  auto kern = r.f32 ? (gather ? k_predict<true, true> : k_predict<true, false>)
+                    : (gather ? k_predict<false, true> : k_predict<false, false>);
+  hipLaunchKernelGGL(kern, dim3(padded_tiles(nt)), dim3(TILE), 0, s, k, nt, b.perm, cur, nxt, pos, r.rec, r.posd,
+                     r.rec32, tool_in, tool_out);
 }
 
 // the four instantiations of a kernel template <F32, COMPACT>
 #define MFC_PICK(kern, f32, compact) \
   ((f32) ? ((compact) ? kern<true, true> : kern<true, false>) : ((compact) ? kern<false, true> : kern<false, false>))
 
-void launch_sums(const dev_consts &k, const int *flags, const dbins &b, const drecords &r, double *acc, variant v,
+void launch_sums(const dev_consts &k, const int *src_flags, const dbins &b, const drecords &r, double *acc, variant v,
                  hipStream_t s) {
   const int nt = (k.N + TILE - 1) / TILE;
   hipLaunchKernelGGL(MFC_PICK(k_sums, r.f32, v.compact), dim3(padded_tiles(nt)), dim3(TILE), 0, s, k, nt, r.rec,
-                     r.posd, r.rec32, b.key_sorted, b.offsets, flags, acc);
+                     r.posd, r.rec32, b.key_sorted, b.offsets, b.perm, src_flags, acc);
 }
 
-void launch_interact_fused(const dev_consts &k, const dstate &st, const dbins &b, const dpositional &pos,
-                           const drecords &r, const mfc_tool_state *tool_now, const dforce &f, variant v,
-                           hipStream_t s) {
+void launch_interact_fused(const dev_consts &k, const dstate &cur, const dstate &nxt, const dbins &b,
+                           const dpositional &pos, const drecords &r, const mfc_tool_state *tool_now, const dforce &f,
+                           variant v, hipStream_t s) {
   const int nt = (k.N + TILE - 1) / TILE;
   hipLaunchKernelGGL(MFC_PICK(k_interact, r.f32, v.compact), dim3(padded_tiles(nt)), dim3(TILE), 0, s, k, nt, r.rec,
-                     r.posd, r.rec32, b.key_sorted, b.offsets, st, pos, tool_now, f);
+                     r.posd, r.rec32, b.key_sorted, b.offsets, b.perm, cur, nxt, pos, tool_now, f);
 }
 
-void launch_update(const dev_consts &k, const double *acc, const dstate &st, const dpositional &pos,
-                   const mfc_tool_state *tool_now, const dforce &f, hipStream_t s) {
+void launch_update(const dev_consts &k, const double *acc, const dbins &b, const dstate &cur, const dstate &nxt,
+                   const dpositional &pos, const mfc_tool_state *tool_now, const dforce &f, hipStream_t s) {
   const int nt = (k.N + TILE - 1) / TILE;
-  hipLaunchKernelGGL(k_update, dim3(padded_tiles(nt)), dim3(TILE), 0, s, k, nt, acc, st, pos, tool_now, f);
+  hipLaunchKernelGGL(k_update, dim3(padded_tiles(nt)), dim3(TILE), 0, s, k, nt, acc, b.perm, cur, nxt, pos,
+                     tool_now, f);
 }
 
 void launch_force_final(const dev_consts &k, const dforce &f, hipStream_t s) {
@@ -882,14 +898,14 @@ void launch_force_final(const dev_consts &k, const dforce &f, hipStream_t s) {
   hipLaunchKernelGGL(k_force_final, dim3(1), dim3(TILE), 0, s, nt, f);
 }
 
-void launch_interact(const dev_consts &k, const dstate &st, const dbins &b, const dpositional &pos,
-                     const drecords &r, const mfc_tool_state *tool_now, const dforce &f, double *acc, variant v,
-                     hipStream_t s) {
+void launch_interact(const dev_consts &k, const dstate &cur, const dstate &nxt, const dbins &b,
+                     const dpositional &pos, const drecords &r, const mfc_tool_state *tool_now, const dforce &f,
+                     double *acc, variant v, hipStream_t s) {
   if (v.split) {
-    launch_sums(k, st.flags, b, r, acc, v, s);
-    launch_update(k, acc, st, pos, tool_now, f, s);
+    launch_sums(k, cur.flags, b, r, acc, v, s);
+    launch_update(k, acc, b, cur, nxt, pos, tool_now, f, s);
   } else {
-    launch_interact_fused(k, st, b, pos, r, tool_now, f, v, s);
+    launch_interact_fused(k, cur, nxt, b, pos, r, tool_now, f, v, s);
   }
   launch_force_final(k, f, s);
 }
@@ -901,7 +917,7 @@ void launch_device_math(int fn, const double *x, const double *y, double *out, i
 
 int scan_chunk_cells() { return SCAN_CHUNK; }
 
-void launch_batch_step(const batch_work &w, hipStream_t s) {
+void launch_batch_binning(const batch_work &w, hipStream_t s) {
   const dim3 gt(padded_tiles(w.ntiles)), gc(padded_tiles(w.nchunks));
   hipLaunchKernelGGL(kb_hash_count, gt, dim3(256), 0, s, w.args, w.tiles, w.ntiles);
   hipLaunchKernelGGL(kb_scan_reduce, gc, dim3(SCAN_THREADS), 0, s, w.args, w.chunks, w.nchunks);
@@ -909,15 +925,42 @@ void launch_batch_step(const batch_work &w, hipStream_t s) {
   hipLaunchKernelGGL(kb_scan_down, gc, dim3(SCAN_THREADS), 0, s, w.args, w.chunks, w.nchunks);
   hipLaunchKernelGGL(kb_place, gt, dim3(256), 0, s, w.args, w.tiles, w.ntiles);
   hipLaunchKernelGGL(kb_rank, gt, dim3(256), 0, s, w.args, w.tiles, w.ntiles);
+}
+
+void launch_batch_predict(const batch_work &w, hipStream_t s) {
+  const dim3 gt(padded_tiles(w.ntiles));
   if (w.f32) hipLaunchKernelGGL(kb_predict<true>, gt, dim3(TILE), 0, s, w.args, w.tiles, w.ntiles);
   else hipLaunchKernelGGL(kb_predict<false>, gt, dim3(TILE), 0, s, w.args, w.tiles, w.ntiles);
-  if (w.v.split) {
-    hipLaunchKernelGGL(MFC_PICK(kb_sums, w.f32, w.v.compact), gt, dim3(TILE), 0, s, w.args, w.tiles, w.ntiles);
-    hipLaunchKernelGGL(kb_update, gt, dim3(TILE), 0, s, w.args, w.tiles, w.ntiles);
-  } else {
-    hipLaunchKernelGGL(MFC_PICK(kb_interact, w.f32, w.v.compact), gt, dim3(TILE), 0, s, w.args, w.tiles, w.ntiles);
-  }
+}
+
+void launch_batch_sums(const batch_work &w, hipStream_t s) {
+  hipLaunchKernelGGL(MFC_PICK(kb_sums, w.f32, w.v.compact), dim3(padded_tiles(w.ntiles)), dim3(TILE), 0, s, w.args,
+                     w.tiles, w.ntiles);
+}
+
+void launch_batch_update(const batch_work &w, hipStream_t s) {
+  hipLaunchKernelGGL(kb_update, dim3(padded_tiles(w.ntiles)), dim3(TILE), 0, s, w.args, w.tiles, w.ntiles);
+}
+
+void launch_batch_interact_fused(const batch_work &w, hipStream_t s) {
+  hipLaunchKernelGGL(MFC_PICK(kb_interact, w.f32, w.v.compact), dim3(padded_tiles(w.ntiles)), dim3(TILE), 0, s,
+                     w.args, w.tiles, w.ntiles);
+}
+
+void launch_batch_force(const batch_work &w, hipStream_t s) {
   hipLaunchKernelGGL(kb_force_final, dim3(w.n), dim3(TILE), 0, s, w.args);
+}
+
+void launch_batch_step(const batch_work &w, hipStream_t s) {
+  launch_batch_binning(w, s);
+  launch_batch_predict(w, s);
+  if (w.v.split) {
+    launch_batch_sums(w, s);
+    launch_batch_update(w, s);
+  } else {
+    launch_batch_interact_fused(w, s);
+  }
+  launch_batch_force(w, s);
 }
 
 }  // namespace mfc

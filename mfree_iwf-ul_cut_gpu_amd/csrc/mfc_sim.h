@@ -62,6 +62,14 @@ struct mfc_sim {
   double *acc = nullptr;           // neighbour sums between the split interaction kernels
   mfc_tuning tuning = {0, 0, {0, 0, 0, 0, 0, 0}};
   mfc::variant variant() const { return {tuning.split_interact != 0, tuning.compact_pairs != 0}; }
+  // the step runs the batched kernels on a batch of one: descriptors of this handle for start
+  // parity 0 and 1, and its (instance, block) work tables
+  mfc::inst_args *dargs = nullptr;
+  int2 *dtiles = nullptr, *dchunks = nullptr;
+  int nchunks = 0;
+  mfc::batch_work work(int c) const {
+    return {dargs + c, 1, dtiles, ntiles, dchunks, nchunks, recs.f32, variant()};
+  }
 
   template <class T>
   int alloc(T **p, size_t n) {
@@ -84,3 +92,5 @@ size_t positional_block_bytes(int N);
 void carve_positional(void *base, int N, mfc::dpositional &p);
 // queue one step: set[c] -> set[1-c]
 void enqueue_step(mfc_sim *s, int c);
+// the batched kernels' view of a handle for start parity c
+mfc::inst_args instance_args(const mfc_sim *s, int c);
