@@ -323,7 +323,7 @@ struct kahan {
 };
 
 // FP32 neighbour sums with Kahan compensation (the same pair terms as pair_terms, rearranged
-// for single precision: reciprocal square roots instead of divisions, the particle mass and the
+// for single precision: hardware reciprocals and reciprocal square roots instead of divisions, the particle mass and the
 // factor 2 of Brookshaw taken out of the sums, and e.gradW/r of Brookshaw written as der*fac,
 // which is its exact value). Pair distances are formed in FP64 and then rounded.
 template <bool COMPACT>
@@ -386,11 +386,11 @@ __device__ inline void sums_f32(const dev_consts &k, int i, const double2 *__res
       sd0.add(sx * w_x); sd1.add(sy * w_y); sd2.add(sy * w_x); sd3.add(sz * w_y);
       const float vijx = -du, vijy = -dv;
       const float vdx = __builtin_fmaf(vijx, dx, vijy * dy);
-      const float inv_rhoij = 2.0f / (rhoi + rhoj);
+      const float inv_rhoij = 2.0f * __builtin_amdgcn_rcpf(rhoi + rhoj);
       if (vdx < 0.0f) {
         const float cij = 0.5f * (ci + b2.w);
         const float hij = 0.5f * (hi + hj_);
-        const float muij = hij * vdx / __builtin_fmaf(eta2 * hij, hij, rr2);
+        const float muij = hij * vdx * __builtin_amdgcn_rcpf(__builtin_fmaf(eta2 * hij, hij, rr2));
         const float piij = (-alpha * cij * muij + beta * muij * muij) * inv_rhoij;
         vt0.add(-piij * w_x); vt1.add(-piij * w_y);
       }
@@ -409,7 +409,7 @@ __device__ inline void sums_f32(const dev_consts &k, int i, const double2 *__res
     if (run_pse) {
       const float4 b0 = rj[0], b3 = rj[3];
       const float w2 = pse_c * __expf(-rr2 * inv_h2);
-      tp.add((b3.x - Tpred) * w2 * (massf / b0.w));
+      tp.add((b3.x - Tpred) * w2 * (massf * __builtin_amdgcn_rcpf(b0.w)));
     }
     if (!run_mech) return false;
     if (rr2 >= fourh2 || rr2 < 1e-8f) return false;
