@@ -15,7 +15,8 @@
 //                        per-tile theta partials of the step (k_adj_theta adds them up)
 //   B2  k_adj_pull       VJP of the neighbour sums with respect to the records: each particle
 //                        pulls the contributions of the pairs it takes part in over the same
-//                        symmetric 5 x 5 cell window, in a fixed order (no atomics)
+//                        symmetric 5 x 5 cell window, in a fixed order (no atomics); two passes,
+//                        pairs of the neighbours' sums, then of its own sums
 //   B3  k_adj_record     VJP of the record and predictor, then the scatter back to the
 //                        storage order of the previous step (inverse of the sort permutation)
 // The positional contact normal (read by the friction heat of the next step at the same sorted
@@ -119,6 +120,10 @@ __global__ __launch_bounds__(TILE) void k_adj_theta(int ntiles, const double *__
 }
 
 // ------------------------------------------------------------------ B2: pull over pairs
+// OWN = false: the pairs (a, b) of the sums of a window member a that include b, with the
+// adjoint of a's sums; g_rec[b] is written. OWN = true: the pairs (b, a) of b's own sums, with
+// b's adjoint; added to g_rec[b]. Two passes keep fewer records live per lane than one.
+template <bool OWN>
 __global__ __launch_bounds__(TILE) void k_adj_pull(dev_consts k, int ntiles, const double *__restrict__ rec,
                                                    const int *__restrict__ key_sorted,
                                                    const int *__restrict__ offsets, const int *__restrict__ flags,
@@ -129,11 +134,16 @@ __global__ __launch_bounds__(TILE) void k_adj_pull(dev_consts k, int ntiles, con
   if (b >= k.N) return;
   const size_t N = k.N;
   const bool pse = k.pse != 0;
-  double rb[REC_DOUBLES], lb[L_N], gb[REC_DOUBLES], dummy[REC_DOUBLES];
+  double rb[REC_DOUBLES], gb[REC_DOUBLES], lb[L_N], dummy[REC_DOUBLES];
 #pragma unroll
-  for (int q = 0; q < REC_DOUBLES; q++) { rb[q] = rec[REC_DOUBLES * (size_t)b + q]; gb[q] = 0.; }
+  for (int q = 0; q < REC_DOUBLES; q++) {
+    rb[q] = rec[REC_DOUBLES * (size_t)b + q];
+    gb[q] = OWN ? g_rec[q * N + b] : 0.;
+  }
+  if constexpr (OWN) {
 #pragma unroll
-  for (int q = 0; q < L_N; q++) lb[q] = lam_acc[q * N + b];
+    for (int q = 0; q < L_N; q++) lb[q] = lam_acc[q * N + b];
+  }
   const int fb = flags[b];
   const bool blank_b = fb & MFC_FLAG_BLANKED, tool_b = fb & MFC_FLAG_TOOL;
   const double ta_b = pse ? 0. : (tool_b ? k.alpha_tool : k.alpha_wp);
@@ -153,22 +163,31 @@ __global__ __launch_bounds__(TILE) void k_adj_pull(dev_consts k, int ntiles, con
       const bool blank_a = fa & MFC_FLAG_BLANKED, tool_a = fa & MFC_FLAG_TOOL;
       const bool mech_a = !blank_a && (!pse || !tool_a);
       const bool pse_a = !blank_a && pse && k.thermal_wp;
-      const bool a_sums_b = mech_a && !blank_b;  // the sums of a include b
-      const bool b_sums_a = mech_b && !blank_a;  // the sums of b include a
-      if (!a_sums_b && !b_sums_a && !pse_a && !pse_b) continue;
-      double ra[REC_DOUBLES];
+      const double *pa = rec + REC_DOUBLES * (size_t)a;
+      const double dx = pa[R_X] - rb[R_X], dy = pa[R_Y] - rb[R_Y];
+      const double rr2 = dx * dx + dy * dy;
+      if constexpr (!OWN) {
+        const double ha = pa[R_H];
+        const bool mech = mech_a && !blank_b && rr2 < 4 * ha * ha && rr2 >= 1e-8;  // a's sums include b
+        if (!mech && !pse_a) continue;
+        double ra[REC_DOUBLES], la[L_N];
 #pragma unroll
-      for (int q = 0; q < REC_DOUBLES; q++) ra[q] = rec[REC_DOUBLES * (size_t)a + q];
-      if (a_sums_b || pse_a) {
-        double la[L_N];
+        for (int q = 0; q < REC_DOUBLES; q++) ra[q] = pa[q];
 #pragma unroll
         for (int q = 0; q < L_N; q++) la[q] = lam_acc[q * N + a];
         const double ta_a = pse ? 0. : (tool_a ? k.alpha_tool : k.alpha_wp);
-        if (a_sums_b) pair_vjp(k, ra, rb, tool_a, tool_b, ta_a, la, false, true, dummy, gb);
+        if (mech) pair_vjp(k, ra, rb, tool_a, tool_b, ta_a, la, false, true, dummy, gb);
         if (pse_a) pse_vjp(k, ra, rb, la[L_TPSE], false, true, dummy, gb);
+      } else {
+        const double hb = rb[R_H];
+        const bool mech = mech_b && !blank_a && rr2 < 4 * hb * hb && rr2 >= 1e-8;  // b's sums include a
+        if (!mech && !pse_b) continue;
+        double ra[REC_DOUBLES];
+#pragma unroll
+        for (int q = 0; q < REC_DOUBLES; q++) ra[q] = pa[q];
+        if (mech) pair_vjp(k, rb, ra, tool_b, tool_a, ta_b, lb, true, false, gb, dummy);
+        if (pse_b) pse_vjp(k, rb, ra, lb[L_TPSE], true, false, gb, dummy);
       }
-      if (b_sums_a) pair_vjp(k, rb, ra, tool_b, tool_a, ta_b, lb, true, false, gb, dummy);
-      if (pse_b) pse_vjp(k, rb, ra, lb[L_TPSE], true, false, gb, dummy);
     }
   }
 #pragma unroll
@@ -425,7 +444,9 @@ int mfc_vjp_backward(mfc_sim *s, const double cot[2], double grad[6]) {
                          w->lam_s[cur], w->lam_n, in_win ? Fb[0] : 0., in_win ? Fb[1] : 0., w->toolw + 1, w->g_s,
                          w->lam_acc, w->theta_part);
       hipLaunchKernelGGL(k_adj_theta, dim3(1), dim3(TILE), 0, st, nt, w->theta_part, w->theta_acc);
-      hipLaunchKernelGGL(k_adj_pull, dim3(padded(nt)), dim3(TILE), 0, st, k, nt, rec, s->bins.key_sorted,
+      hipLaunchKernelGGL(k_adj_pull<false>, dim3(padded(nt)), dim3(TILE), 0, st, k, nt, rec, s->bins.key_sorted,
+                         s->bins.offsets, B.flags, w->lam_acc, w->g_rec);
+      hipLaunchKernelGGL(k_adj_pull<true>, dim3(padded(nt)), dim3(TILE), 0, st, k, nt, rec, s->bins.key_sorted,
                          s->bins.offsets, B.flags, w->lam_acc, w->g_rec);
       hipLaunchKernelGGL(k_adj_record, dim3(padded(nt)), dim3(TILE), 0, st, k, nt, sb, B.flags, P.p, w->g_rec,
                          w->g_s, s->bins.perm, w->lam_s[1 - cur]);
