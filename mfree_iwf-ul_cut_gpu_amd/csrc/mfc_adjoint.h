@@ -501,6 +501,7 @@ MFC_HD void tail_vjp(const dev_consts &k, const mfc_tool_state &tool, const tail
   const double s23 = sqrt(2.0 / 3.0);
   const double e0 = s[TS_EPS];
   bool plastic = false;
+  bool folded = false;  // the last secant update took the fold branch (x < 0 -> -0.1 x)
   double norm = 0, dl = 0, ratio = 1.0;
   double S2xx = SXX, S2xy = SXY, S2yy = SYY, S2zz = SZZ, E2 = e0, ED2 = s[TS_EPSD], T2 = Tf;
   jc_grad gr = {0, 0, 0, 0, 0, 0, 0, 0, 0};
@@ -523,7 +524,8 @@ MFC_HD void tail_vjp(const dev_consts &k, const mfc_tool_state &tool, const tail
         const double slope = (g0 - g1) / (x - x1);
         xo = x;
         x = x - g0 / slope;
-        if (x < 0.) x = -0.1 * x;
+        folded = false;
+        if (x < 0.) { x = -0.1 * x; folded = true; }
         if (fabs(x - xo) < k.secant_tol || it > 100) break;
         it++;
       }
@@ -586,14 +588,34 @@ MFC_HD void tail_vjp(const dev_consts &k, const mfc_tool_state &tool, const tail
       g.theta[0] += bs * gr.A; g.theta[1] += bs * gr.B; g.theta[2] += bs * gr.n;
       g.theta[3] += bs * gr.C; g.theta[4] += bs * gr.m;
     }
-    // implicit function theorem on g(dl; norm, e0, Tf, theta) = 0
     const double dgd = -2. * G - s23 * (gr.e * s23 + gr.ed * s23 / dt);
-    const double mlt = -bdl / dgd;
-    bnorm += mlt;
-    be0 += mlt * (-s23 * gr.e);
-    bTf += mlt * (-s23 * gr.T);
-    g.theta[0] += mlt * (-s23 * gr.A); g.theta[1] += mlt * (-s23 * gr.B); g.theta[2] += mlt * (-s23 * gr.n);
-    g.theta[3] += mlt * (-s23 * gr.C); g.theta[4] += mlt * (-s23 * gr.m);
+    if (!folded) {
+      // implicit function theorem on g(dl; norm, e0, Tf, theta) = 0
+      const double mlt = -bdl / dgd;
+      bnorm += mlt;
+      be0 += mlt * (-s23 * gr.e);
+      bTf += mlt * (-s23 * gr.T);
+      g.theta[0] += mlt * (-s23 * gr.A); g.theta[1] += mlt * (-s23 * gr.B); g.theta[2] += mlt * (-s23 * gr.n);
+      g.theta[3] += mlt * (-s23 * gr.C); g.theta[4] += mlt * (-s23 * gr.m);
+    } else {
+      // g has no root (g(0) < 0 can happen: the return's norm counts S_xy once while the yield
+      // test uses the von Mises stress), and the secant stops at the fixed point of its fold:
+      // x = -0.1 (x - g(x)/s(x)) with s(x) = (g(x) - g(1.01 x)) / (-0.01 x), that is
+      // H(x) = 1101 g(x) - 1100 g(1.01 x) = 0; implicit function theorem on H
+      const double x1 = 1.01 * dl;
+      const jc_grad g1 = sigma_yield_grad(k, e0 + s23 * x1, s23 * x1 / dt, Tf);
+      const double dgd1 = -2. * G - s23 * (g1.e * s23 + g1.ed * s23 / dt);
+      const double dH = 1101. * dgd - 1100. * 1.01 * dgd1;
+      const double mlt = -bdl / dH;
+      bnorm += mlt * (1101. - 1100.);
+      be0 += mlt * (-s23) * (1101. * gr.e - 1100. * g1.e);
+      bTf += mlt * (-s23) * (1101. * gr.T - 1100. * g1.T);
+      g.theta[0] += mlt * (-s23) * (1101. * gr.A - 1100. * g1.A);
+      g.theta[1] += mlt * (-s23) * (1101. * gr.B - 1100. * g1.B);
+      g.theta[2] += mlt * (-s23) * (1101. * gr.n - 1100. * g1.n);
+      g.theta[3] += mlt * (-s23) * (1101. * gr.C - 1100. * g1.C);
+      g.theta[4] += mlt * (-s23) * (1101. * gr.m - 1100. * g1.m);
+    }
     bSXX += bnorm * SXX / norm; bSXY += bnorm * SXY / norm; bSYY += bnorm * SYY / norm; bSZZ += bnorm * SZZ / norm;
   }
   // friction heat

@@ -443,3 +443,26 @@ def test_record_vjp_matches_differences(flags):
         L.adjh_record_vjp(k.buf, P(s), flags, float(p[0]), P(brec), P(gs))
         fd = fd_grad(J, s, state_scale(s, k))
         assert_grad(gs, fd, tol=1e-5, what=f"record flags {flags} case {case}")
+
+
+def test_tail_vjp_secant_fold_branch():
+    """Plastic states whose return equation g(dl) = |S| - 2 G dl - sqrt(2/3) sigma_Y = 0 has no
+    root: the norm counts S_xy once while the yield test uses the von Mises stress, so a
+    shear-dominated stress between 0.577 and 0.816 of the yield stress is plastic with g(0) < 0.
+    The secant then stops at the fixed point of its fold branch, which the VJP differentiates."""
+    rng = np.random.default_rng(19)
+    k = Consts()
+    n_fold = 0
+    for case in range(8):
+        s = make_state(rng, k, 0.03, 0.01, plastic=False)
+        s[TS_STXX:TS_STZZ + 1] = 0.0
+        jc = np.zeros(9)
+        L.adjh_jc(k.buf, s[TS_EPS], 0.0, s[TS_T], P(jc))  # rate-free yield stress
+        s[TS_SXX] = s[TS_SYY] = s[TS_SZZ] = 0.0
+        s[TS_SXY] = 0.72 * jc[0]
+        acc = make_acc(rng)
+        acc[0:4] *= 1e-3  # keep the stress rate small against the stress
+        fwd = check_tail(rng, k, s, 0, acc, rng.normal(0, 1, 2), what=f"fold case {case}")
+        g0 = np.sqrt(2.0 / 3.0) * jc[0] - 0.72 * jc[0]
+        n_fold += int(fwd[TS_EPS] > s[TS_EPS] and g0 > 0)
+    assert n_fold >= 6

@@ -96,6 +96,22 @@ def test_vjp_matches_oracle_differences(thermal):
     assert_jacobian(J, fd, unc)
 
 
+@pytest.mark.parametrize("thermal", ["adiabatic", "contact"])
+def test_vjp_matches_host_chain(thermal):
+    """The GPU reverse mode against the host restatement of the same chain (same blocks, pairs
+    pushed instead of pulled, plain sums): equal to rounding."""
+    from test_adjoint_chain import host_chain
+    cfg = adj_config(thermal)
+    nsteps, K, w0, w1 = 150, 25, 50, 150
+    F, J, _ = gpu_jacobian(cfg, nsteps, K, w0, w1)
+    for c, cot in enumerate(([1.0, 0.0], [0.0, 1.0])):
+        Fh, gh, _, _ = host_chain(cfg, nsteps, w0, w1, cot)
+        assert np.abs(F - Fh).max() <= 1e-12 * np.abs(F).max()
+        err = np.abs(J[c] - gh) / np.abs(gh).max()
+        print(thermal, c, "gpu", J[c], "\nhost", gh, "\nrel", err)
+        assert (err < 1e-8).all()
+
+
 def test_vjp_independent_of_checkpoint_interval():
     cfg = adj_config("contact")
     J = [gpu_jacobian(cfg, 60, K, 10, 60)[1] for K in (1, 7, 60)]
@@ -106,7 +122,7 @@ def test_vjp_leaves_the_run_state():
     cfg = adj_config("contact")
     _, _, sim = gpu_jacobian(cfg, 40, 8, 0, 40)
     ref = pymfc.Sim(cfg)
-    ref.step(20)
+    ref.step(40)
     sim.step(10)
     ref.step(10)
     for f in ("pos", "vel", "S", "T", "eps_pl", "fc", "ft", "flags", "hash"):
