@@ -13,19 +13,20 @@ import pymfc
 pytestmark = pytest.mark.gpu
 
 
-def test_fp32c_full_thermal_contact_cut():
-    steps = 20000
+def run_both(workload, steps):
     out = {}
     for name, prec in (("fp64", pymfc.PREC_FP64), ("fp32c", pymfc.PREC_FP32C)):
-        s = pymfc.Sim(pymfc.workload("thermal_contact_150k", prec))
+        s = pymfc.Sim(pymfc.workload(workload, prec))
         s.step(steps)
         s.sync()
         out[name] = {f: s.field(f) for f in ("id", "pos", "T", "eps_pl", "S", "flags")}
         out[name]["F"] = s.force_history()
         s.close()
-    a, b = out["fp64"], out["fp32c"]
+    return out["fp64"], out["fp32c"]
+
+
+def errors(a, b, dx, steps):
     ia, ib = np.argsort(a["id"]), np.argsort(b["id"])
-    dx = 2e-4
     w = steps // 2
     Fa, Fb = a["F"][-w:].mean(axis=0), b["F"][-w:].mean(axis=0)
     err = {
@@ -41,7 +42,26 @@ def test_fp32c_full_thermal_contact_cut():
         "F_fp64": Fa.tolist(),
         "T_peak_fp64": float(a["T"].max()),
     }
+    return err
+
+
+def test_fp32c_full_thermal_contact_cut():
+    steps = 20000
+    a, b = run_both("thermal_contact_150k", steps)
+    err = errors(a, b, 2e-4, steps)
     print("fp32c vs fp64, thermal_contact_150k, 20000 steps:", err)
     assert err["force_rel"][0] < 5e-3      # cutting force (mean over the second half) within 0.5 %
     assert err["Tpeak_K"] < 1.0            # peak temperature within 1 K
     assert err["blanked"][1] == err["blanked"][0]
+
+
+def test_fp32c_pse_500k_cut():
+    """The 1250 x 400 jittered PSE cut at 1.2 um (500,000 particles), 10,000 steps."""
+    steps = 10000
+    a, b = run_both("pse_500k", steps)
+    err = errors(a, b, 1.2e-4, steps)
+    print("fp32c vs fp64, pse_500k, 10000 steps:", err)
+    assert err["force_rel"][0] < 5e-3
+    assert err["Tpeak_K"] < 1.0
+    assert err["blanked"][1] == err["blanked"][0]
+
