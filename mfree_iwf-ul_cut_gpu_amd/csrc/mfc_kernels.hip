@@ -1,21 +1,27 @@
-// mfc_kernels.hip — gfx950 kernels of one FP64 SPH cutting step.
+// mfc_kernels.hip — gfx950 kernels of one SPH cutting step (FP64 and FP32C).
 //
-// Per step (the reference's ~20 launches of leap_frog::step, leap_frog.cu:94-155, become 9):
-//   binning      k_hash_count, k_scan_reduce, k_scan_top, k_scan_down, k_place, k_rank
-//                cell ids (grid.cu:34-51) and a stable counting sort by cell
-//                (thrust::sort_by_key + reorder_data_and_find_cell_start, grid_gpu_green.cu:22-116,177-201)
-//   k_predict    gather into sorted order + leap-frog init/predict (leap_frog.cu:30-66) + EOS
-//                (actions_gpu.cu:41-51) + artificial stress (:53-97) + neighbour record + tool move
-//   k_interact   neighbour sums (interactions_gpu.cu:88-345) fused with every per-particle action
+// Per step (the reference's ~20 launches of leap_frog::step, leap_frog.cu:94-155, become 8):
+//   binning      hash_count, scan_reduce, scan_down, place, rank: cell ids (grid.cu:34-51) and
+//                a stable counting sort by cell (thrust::sort_by_key +
+//                reorder_data_and_find_cell_start, grid_gpu_green.cu:22-116,177-201)
+//   predict      leap-frog predict (leap_frog.cu:30-66) + EOS (actions_gpu.cu:41-51) + artificial
+//                stress (:53-97) + neighbour record + tool move, reading the start state
+//                through the sort permutation
+//   interact     neighbour sums (interactions_gpu.cu:88-345) fused with every per-particle action
 //                that follows them in the step: Jaumann, continuity, momentum, advection, correct,
 //                friction heat, Johnson-Cook return, tool-particle move, contact, boundary
-//                conditions, NaN blanking, and the per-tile partial sums of the tool force
-//   k_force_final  tool force of the step from the tile partials (output.cu:30-39)
+//                conditions, NaN blanking, the gather into sorted order, and the per-tile partial
+//                sums of the tool force (or split into sums + update)
+//   force_final  tool force of the step from the tile partials (output.cu:30-39)
+// Each body is a device function of its work index; the kb_* kernels run it over an
+// (instance, tile) work table for any number of instances (a single handle is a batch of one),
+// the k_* kernels over one instance's tiles (used by the reverse mode's replay).
 //
 // Memory layout: the persistent particle state is a structure of arrays in cell-sorted order,
-// double-buffered across the gather. Neighbour data are packed per particle into one 128-byte
-// record (two 64-byte lines) so that the divergent per-lane neighbour reads of k_interact are
-// 16-byte vector loads from one cache line. Work-group to tile mapping is XCD-aware.
+// double-buffered across the step. Neighbour data are packed per particle into one 128-byte
+// record (two 64-byte lines; FP32C: 64 bytes plus the FP64 position) so that the divergent
+// per-lane neighbour reads are 16-byte vector loads from one cache line. Work-group to tile
+// mapping is XCD-aware.
 #include "mfc_kernels.h"
 #include "mfc_pairs.h"
 
@@ -61,17 +67,32 @@ __device__ inline void hash_count(const dev_consts &k, int i, const double *__re
                                   const double *__restrict__ py, const int *__restrict__ flags,
                                   const int *__restrict__ key_sorted, int *__restrict__ key, int *__restrict__ slot,
                                   int *__restrict__ count, unsigned long long *__restrict__ clamped) {
-  if (i >= k.N) return;
-  int h;
-  if (flags[i] & MFC_FLAG_BLANKED) {
-    h = key_sorted[i];
-  } else {
-    bool cl;
-    h = cell_of(k, px[i], py[i], cl);
-    if (cl) atomicAdd(clamped, 1ull);
+  // all 64 lanes of the wave take part (lanes past N carry no cell)
+  int h = -1;
+  if (i < k.N) {
+    if (flags[i] & MFC_FLAG_BLANKED) {
+      h = key_sorted[i];
+    } else {
+      bool cl;
+      h = cell_of(k, px[i], py[i], cl);
+      if (cl) atomicAdd(clamped, 1ull);
+    }
+    key[i] = h;
   }
-  key[i] = h;
-  slot[i] = atomicAdd(&count[h], 1);
+  // the storage order is the previous step's cell order, so neighbouring lanes mostly share a
+  // cell: each run of equal cell ids in the wave takes its slots with one atomic
+  const int lane = threadIdx.x & 63;
+  const int prev = __shfl_up(h, 1, 64);
+  const bool head = lane == 0 || prev != h;
+  const unsigned long long heads = __ballot(head);
+  const unsigned long long upto = (lane == 63) ? ~0ull : ((2ull << lane) - 1ull);
+  const int my_head = 63 - __clzll(heads & upto);
+  const unsigned long long after = (lane == 63) ? 0ull : (heads & ~upto);
+  const int next_head = after ? __ffsll((long long)after) - 1 : 64;
+  int base = 0;
+  if (head && h >= 0) base = atomicAdd(&count[h], next_head - lane);
+  base = __shfl(base, my_head, 64);
+  if (h >= 0) slot[i] = base + (lane - my_head);
 }
 
 __device__ inline void scan_reduce(int blk, const int *__restrict__ count, int n, int *__restrict__ bsum) {
@@ -85,32 +106,15 @@ __device__ inline void scan_reduce(int blk, const int *__restrict__ count, int n
   if (threadIdx.x == 0) bsum[blk] = total;
 }
 
-__device__ inline void scan_top(int *__restrict__ bsum, int nb) {
-  int carry = 0;
-  for (int c0 = 0; c0 < nb; c0 += SCAN_CHUNK) {
-    int base = c0 + threadIdx.x * SCAN_ITEMS;
-    int v[SCAN_ITEMS];
-    int s = 0;
-#pragma unroll
-    for (int q = 0; q < SCAN_ITEMS; q++) {
-      v[q] = (base + q < nb) ? bsum[base + q] : 0;
-      s += v[q];
-    }
-    int total;
-    int pre = block_exclusive_scan(s, total) + carry;
-#pragma unroll
-    for (int q = 0; q < SCAN_ITEMS; q++) {
-      if (base + q < nb) bsum[base + q] = pre;
-      pre += v[q];
-    }
-    carry += total;
-    __syncthreads();
-  }
-}
-
-// cell offsets from the counts; the counts are cleared for the next step
+// cell offsets from the counts; the counts are cleared for the next step. The offset of chunk
+// blk is the sum of the chunk sums before it, added up here (a few hundred values), so no
+// separate scan of the chunk sums is needed.
 __device__ inline void scan_down(int blk, int *__restrict__ count, int n, const int *__restrict__ bsum,
                                  int *__restrict__ offsets) {
+  int carry_part = 0;
+  for (int q = threadIdx.x; q < blk; q += SCAN_THREADS) carry_part += bsum[q];
+  int carry;
+  block_exclusive_scan(carry_part, carry);
   int base = blk * SCAN_CHUNK + threadIdx.x * SCAN_ITEMS;
   int v[SCAN_ITEMS];
   int s = 0;
@@ -120,7 +124,7 @@ __device__ inline void scan_down(int blk, int *__restrict__ count, int n, const 
     s += v[q];
   }
   int total;
-  int pre = block_exclusive_scan(s, total) + bsum[blk];
+  int pre = block_exclusive_scan(s, total) + carry;
 #pragma unroll
   for (int q = 0; q < SCAN_ITEMS; q++) {
     int c = base + q;
@@ -164,8 +168,6 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan_reduce(const int *__restr
                                                                int *__restrict__ bsum) {
   scan_reduce(blockIdx.x, count, n, bsum);
 }
-
-__global__ __launch_bounds__(SCAN_THREADS) void k_scan_top(int *__restrict__ bsum, int nb) { scan_top(bsum, nb); }
 
 __global__ __launch_bounds__(SCAN_THREADS) void k_scan_down(int *__restrict__ count, int n,
                                                              const int *__restrict__ bsum, int *__restrict__ offsets) {
@@ -736,11 +738,6 @@ __global__ __launch_bounds__(SCAN_THREADS) void kb_scan_reduce(const inst_args *
   scan_reduce(it.y, a.b.count, a.k.num_cell, a.b.bsum);
 }
 
-__global__ __launch_bounds__(SCAN_THREADS) void kb_scan_top(const inst_args *__restrict__ A) {
-  const inst_args &a = A[blockIdx.x];
-  scan_top(a.b.bsum, a.nscan);
-}
-
 __global__ __launch_bounds__(SCAN_THREADS) void kb_scan_down(const inst_args *__restrict__ A,
                                                               const int2 *__restrict__ work, int nwork) {
   const int w = xcd_tile(blockIdx.x, nwork);
@@ -849,7 +846,6 @@ void launch_binning(const dev_consts &k, const dstate &cur, dbins &b, hipStream_
   hipLaunchKernelGGL(k_hash_count, dim3(g), dim3(256), 0, s, k, cur.px, cur.py, cur.flags, b.key_sorted, b.key,
                      b.slot, b.count, b.clamped);
   hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(SCAN_THREADS), 0, s, b.count, k.num_cell, b.bsum);
-  hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(SCAN_THREADS), 0, s, b.bsum, nb);
   hipLaunchKernelGGL(k_scan_down, dim3(nb), dim3(SCAN_THREADS), 0, s, b.count, k.num_cell, b.bsum, b.offsets);
   hipLaunchKernelGGL(k_place, dim3(g), dim3(256), 0, s, k.N, b.key, b.slot, b.offsets, b.bucket);
   hipLaunchKernelGGL(k_rank, dim3(g), dim3(256), 0, s, k.N, b.key, b.bucket, b.offsets, b.perm, b.key_sorted);
@@ -921,7 +917,6 @@ void launch_batch_binning(const batch_work &w, hipStream_t s) {
   const dim3 gt(padded_tiles(w.ntiles)), gc(padded_tiles(w.nchunks));
   hipLaunchKernelGGL(kb_hash_count, gt, dim3(256), 0, s, w.args, w.tiles, w.ntiles);
   hipLaunchKernelGGL(kb_scan_reduce, gc, dim3(SCAN_THREADS), 0, s, w.args, w.chunks, w.nchunks);
-  hipLaunchKernelGGL(kb_scan_top, dim3(w.n), dim3(SCAN_THREADS), 0, s, w.args);
   hipLaunchKernelGGL(kb_scan_down, gc, dim3(SCAN_THREADS), 0, s, w.args, w.chunks, w.nchunks);
   hipLaunchKernelGGL(kb_place, gt, dim3(256), 0, s, w.args, w.tiles, w.ntiles);
   hipLaunchKernelGGL(kb_rank, gt, dim3(256), 0, s, w.args, w.tiles, w.ntiles);
