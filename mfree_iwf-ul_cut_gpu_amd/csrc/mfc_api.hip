@@ -172,8 +172,9 @@ int mfc_create(const mfc_config *cfg, mfc_sim **out) {
     if ((rc = s->alloc(&pb, positional_block_bytes(N)))) return bail(rc);
     carve_positional(pb, N, p);
   }
+  if ((rc = s->alloc(&s->recs.posd, N)) || (rc = s->alloc(&s->recs.vfl, N))) return bail(rc);
   if (s->recs.f32) {
-    if ((rc = s->alloc(&s->recs.posd, N)) || (rc = s->alloc(&s->recs.rec32, (size_t)16 * N))) return bail(rc);
+    if ((rc = s->alloc(&s->recs.rec32, (size_t)16 * N))) return bail(rc);
   } else if ((rc = s->alloc(&s->recs.rec, (size_t)mfc::REC * N))) {
     return bail(rc);
   }
@@ -216,7 +217,8 @@ int mfc_create(const mfc_config *cfg, mfc_sim **out) {
         hipMemsetAsync(s->hist_ctr, 0, sizeof(int64_t), s->stream) != hipSuccess ||
         hipMemsetAsync(s->force_last, 0, 2 * sizeof(double), s->stream) != hipSuccess ||
         (s->recs.rec && hipMemsetAsync(s->recs.rec, 0, sizeof(double) * mfc::REC * (size_t)N, s->stream) != hipSuccess) ||
-        (s->recs.posd && hipMemsetAsync(s->recs.posd, 0, sizeof(double2) * (size_t)N, s->stream) != hipSuccess) ||
+        hipMemsetAsync(s->recs.posd, 0, sizeof(double2) * (size_t)N, s->stream) != hipSuccess ||
+        hipMemsetAsync(s->recs.vfl, 0, sizeof(int) * (size_t)N, s->stream) != hipSuccess ||
         (s->recs.rec32 && hipMemsetAsync(s->recs.rec32, 0, sizeof(float) * 16 * (size_t)N, s->stream) != hipSuccess))
       return bail(fail(MFC_E_HIP, "memset"));
     mfc_tool_state t2[2] = {S.tool, S.tool};

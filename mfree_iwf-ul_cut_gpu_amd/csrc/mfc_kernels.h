@@ -41,15 +41,21 @@ constexpr int REC = 16;    // doubles per neighbour record
 constexpr int ACC_N = 14;  // neighbour sums per particle
 constexpr int PAIR_LIST = 32;  // per-lane list of in-support neighbours (compacted pair loop)
 
-// neighbour records written by the predict kernel and read by the interaction kernel:
-// FP64 path: rec, 16 doubles per particle; FP32C path: posd (FP64 predicted position) and
-// rec32, 16 floats per particle
+// neighbour records written by the predict kernel and read by the interaction kernel, in
+// sorted order: posd (FP64 predicted position) and vfl (flags of the record) for the window
+// walk, and the pair record, rec (16 doubles per particle, FP64 path) or rec32 (16 floats,
+// FP32C path)
 struct drecords {
   bool f32;
   double *rec;
   double2 *posd;
+  int *vfl;
   float *rec32;
 };
+
+// window candidates of a tile that the interaction kernels stage in LDS (positions and flags,
+// 20 bytes each); a tile whose candidates span more sorted indices reads them from memory
+constexpr int STAGE_CAP = 2048;
 
 // tool-force reduction: per-tile partials written by the interaction kernel, reduced by
 // k_force_final, which appends F to the history

@@ -211,9 +211,12 @@ __device__ inline void artificial_stress_f32(float rho, float p, float Sxx, floa
 // F32: the record is 16 floats (64 bytes) plus the FP64 predicted position.
 template <bool F32, bool GATHER>
 __device__ inline void predict(const dev_consts &k, int t, const int *__restrict__ perm, const dstate &cur,
-                               const dstate &nxt, const dpositional &pos, double *__restrict__ rec,
-                               double2 *__restrict__ posd, float *__restrict__ rec32,
+                               const dstate &nxt, const dpositional &pos, const drecords &recs,
                                const mfc_tool_state *__restrict__ tool_in, mfc_tool_state *__restrict__ tool_out) {
+  double *__restrict__ rec = recs.rec;
+  double2 *__restrict__ posd = recs.posd;
+  int *__restrict__ vfl = recs.vfl;
+  float *__restrict__ rec32 = recs.rec32;
   if (t == 0 && threadIdx.x == 0) {
     mfc_tool_state ts = *tool_in;
     tool_move(ts, k.dt);
@@ -269,6 +272,7 @@ __device__ inline void predict(const dev_consts &k, int t, const int *__restrict
     float4 *dst = reinterpret_cast<float4 *>(rec32 + (size_t)16 * i);
     dst[0] = q0; dst[1] = q1; dst[2] = q2; dst[3] = q3;
     posd[i] = make_double2(x, y);
+    vfl[i] = fl;
     return;
   } else {
     if (!(fl & MFC_FLAG_TOOL)) {
@@ -294,24 +298,31 @@ __device__ inline void predict(const dev_consts &k, int t, const int *__restrict
 #pragma unroll
   for (int q = 0; q < REC; q++)
     if (q != 2) fin = fin && finite_d(r[q]);
-  r[2] = bits_to_double(fl | (fin ? 0 : FLAG_NONFINITE));
+  const int rfl = fl | (fin ? 0 : FLAG_NONFINITE);
+  r[2] = bits_to_double(rfl);
   double2 *dst = reinterpret_cast<double2 *>(rec + (size_t)REC * i);
 #pragma unroll
   for (int q = 0; q < REC / 2; q++) dst[q] = make_double2(r[2 * q], r[2 * q + 1]);
+  posd[i] = make_double2(x, y);
+  vfl[i] = rfl;
 }
 
 template <bool F32, bool GATHER>
 __global__ __launch_bounds__(TILE) void k_predict(dev_consts k, int ntiles, const int *__restrict__ perm, dstate cur,
-                                                  dstate nxt, dpositional pos, double *__restrict__ rec,
-                                                  double2 *__restrict__ posd, float *__restrict__ rec32,
+                                                  dstate nxt, dpositional pos, drecords r,
                                                   const mfc_tool_state *__restrict__ tool_in,
                                                   mfc_tool_state *__restrict__ tool_out) {
   const int t = xcd_tile(blockIdx.x, ntiles);
   if (t < 0) return;
-  predict<F32, GATHER>(k, t, perm, cur, nxt, pos, rec, posd, rec32, tool_in, tool_out);
+  predict<F32, GATHER>(k, t, perm, cur, nxt, pos, r, tool_in, tool_out);
 }
 
 // ------------------------------------------------------------------------------- interact
+// the sums kernels keep at least 3 waves per SIMD (at most 168 VGPRs): a 150k-particle step
+// has about 3 waves per SIMD in all, so all of them are resident at once. The compacted
+// variant's LDS (candidate stage and pair lists, 72 KB per work-group) allows 2.
+#define SUMS_OCCUPANCY __attribute__((amdgpu_waves_per_eu(COMPACT ? 2 : 3)))
+
 // compensated (Kahan) accumulation in single precision
 struct kahan {
   float s = 0.0f, c = 0.0f;
@@ -331,8 +342,8 @@ struct kahan {
 template <bool COMPACT>
 __device__ inline void sums_f32(const dev_consts &k, int i, const double2 *__restrict__ posd,
                                 const float *__restrict__ rec32, const int *__restrict__ key_sorted,
-                                const int *__restrict__ offsets, bool tool, double ta_d, bool run_mech, bool run_pse,
-                                pair_acc &acc, double &tpse) {
+                                const int *__restrict__ offsets, const cand_src &cs, bool tool, double ta_d,
+                                bool run_mech, bool run_pse, pair_acc &acc, double &tpse) {
   const float4 *ri = reinterpret_cast<const float4 *>(rec32 + (size_t)16 * i);
   const float4 a0 = ri[0], a1 = ri[1], a2 = ri[2], a3 = ri[3];
   const double2 pi = posd[i];
@@ -351,15 +362,17 @@ __device__ inline void sums_f32(const dev_consts &k, int i, const double2 *__res
   const float pse_c = (float)(4. / M_PI) * inv_h2 * inv_h2;
   const float massf = (float)k.mass;
   kahan vd0, vd1, vd2, vd3, sd0, sd1, sd2, sd3, vt0, vt1, xt0, xt1, tl, tp;
-  const int hsh = key_sorted[i];
-  const int gi = hsh / k.gny, gj = hsh - gi * k.gny;
-  const int li = gi - 2 < 0 ? 0 : gi - 2;
-  const int lj = gj - 2 < 0 ? 0 : gj - 2;
-  const int hi_ = gi + 3 > k.gnx ? k.gnx : gi + 3;
-  const int hj = gj + 3 > k.gny ? k.gny : gj + 3;
+  const window w = make_window(k, offsets, key_sorted[i]);
   // the mechanical and Brookshaw terms of one in-support neighbour
-  auto pair = [&](float dx, float dy, float rr2, const float4 *__restrict__ rj) {
-    const float4 b0 = rj[0], b1 = rj[1], b2 = rj[2], b3 = rj[3];
+  struct rec4 {
+    float4 b0, b1, b2, b3;
+  };
+  auto load_rec = [&](int j, rec4 &r) {
+    const float4 *rj = reinterpret_cast<const float4 *>(rec32 + (size_t)16 * j);
+    r.b0 = rj[0]; r.b1 = rj[1]; r.b2 = rj[2]; r.b3 = rj[3];
+  };
+  auto pair = [&](float dx, float dy, float rr2, const rec4 &r) {
+    const float4 b0 = r.b0, b1 = r.b1, b2 = r.b2, b3 = r.b3;
     const int fj = __float_as_int(b3.y);
     const float uj = b0.x, vj = b0.y, hj_ = b0.z, rhoj = b0.w, volj = b1.x;
     const float inv_r = __builtin_amdgcn_rsqf(rr2);
@@ -401,28 +414,53 @@ __device__ inline void sums_f32(const dev_consts &k, int i, const double2 *__res
     }
     if (ta != 0.0f) tl.add(volj * (Ti - b3.x) * gr);
   };
-  // the PSE term of every window member, and the support test; true: j is a mechanical pair
-  auto visit = [&](int j, float &dx, float &dy, float &rr2) -> bool {
-    const double2 pj = posd[j];
-    dx = (float)(pi.x - pj.x);
-    dy = (float)(pi.y - pj.y);
-    rr2 = __builtin_fmaf(dx, dx, dy * dy);
-    const float4 *rj = reinterpret_cast<const float4 *>(rec32 + (size_t)16 * j);
+  // what the walk loads per candidate: FP64 position and flags, and, for PSE, temperature and
+  // density
+  struct cand {
+    double2 p;
+    int f;
+    float T, rho;
+  };
+  auto load = [&](int j, cand &c) {
+    c.p = cs.pos[j - cs.base];
+    c.f = cs.fl[j - cs.base];
     if (run_pse) {
-      const float4 b0 = rj[0], b3 = rj[3];
+      c.T = rec32[(size_t)16 * j + 12];
+      c.rho = rec32[(size_t)16 * j + 3];
+    }
+  };
+  // the PSE term of every window member, and the support test; true: j is a mechanical pair
+  auto visit = [&](const cand &c, float &dx, float &dy, float &rr2) -> bool {
+    dx = (float)(pi.x - c.p.x);
+    dy = (float)(pi.y - c.p.y);
+    rr2 = __builtin_fmaf(dx, dx, dy * dy);
+    if (run_pse) {
       const float w2 = pse_c * __expf(-rr2 * inv_h2);
-      tp.add((b3.x - Tpred) * w2 * (massf * __builtin_amdgcn_rcpf(b0.w)));
+      tp.add((c.T - Tpred) * w2 * (massf * __builtin_amdgcn_rcpf(c.rho)));
     }
     if (!run_mech) return false;
     if (rr2 >= fourh2 || rr2 < 1e-8f) return false;
-    return !(__float_as_int(rj[3].y) & MFC_FLAG_BLANKED);
+    return !(c.f & MFC_FLAG_BLANKED);
   };
+  // the window in groups of WALK_U candidates, the loads of a group issued before its first
+  // candidate is evaluated; the order of evaluation is the candidates' order
   if constexpr (!COMPACT) {
-    for (int ii = li; ii < hi_; ii++) {
-      const int jb = offsets[ii * k.gny + lj], je = offsets[ii * k.gny + hj];
-      for (int j = jb; j < je; j++) {
+    for (int t0 = 0; t0 < w.total; t0 += WALK_U) {
+      int j[WALK_U];
+      cand c[WALK_U];
+#pragma unroll
+      for (int u = 0; u < WALK_U; u++) {
+        j[u] = w.at(t0 + u < w.total ? t0 + u : w.total - 1);
+        load(j[u], c[u]);
+      }
+#pragma unroll
+      for (int u = 0; u < WALK_U; u++) {
         float dx, dy, rr2;
-        if (visit(j, dx, dy, rr2)) pair(dx, dy, rr2, reinterpret_cast<const float4 *>(rec32 + (size_t)16 * j));
+        if (t0 + u < w.total && visit(c[u], dx, dy, rr2)) {
+          rec4 r;
+          load_rec(j[u], r);
+          pair(dx, dy, rr2, r);
+        }
       }
     }
   } else {
@@ -430,28 +468,47 @@ __device__ inline void sums_f32(const dev_consts &k, int i, const double2 *__res
     // the list: the wave runs the pair body about as often as its busiest lane has pairs,
     // instead of once per window candidate
     __shared__ int list[PAIR_LIST][TILE];
-    int ii = li, j = 0, je = 0;
-    bool more = ii < hi_;
-    if (more) { j = offsets[ii * k.gny + lj]; je = offsets[ii * k.gny + hj]; }
-    while (more) {
-      int cnt = 0;
-      while (cnt < PAIR_LIST) {
-        if (j >= je) {
-          if (++ii >= hi_) { more = false; break; }
-          j = offsets[ii * k.gny + lj];
-          je = offsets[ii * k.gny + hj];
-          continue;
+    int cnt = 0, t0 = 0;
+    // collect until the list could overflow or the window is done (each lane on its own),
+    // then evaluate the lists of all lanes together
+    for (;;) {
+      for (; t0 < w.total && cnt <= PAIR_LIST - WALK_U; t0 += WALK_U) {
+        int j[WALK_U];
+        cand c[WALK_U];
+#pragma unroll
+        for (int u = 0; u < WALK_U; u++) {
+          j[u] = w.at(t0 + u < w.total ? t0 + u : w.total - 1);
+          load(j[u], c[u]);
         }
-        float dx, dy, rr2;
-        if (visit(j, dx, dy, rr2)) list[cnt++][threadIdx.x] = j;
-        j++;
+#pragma unroll
+        for (int u = 0; u < WALK_U; u++) {
+          float dx, dy, rr2;
+          if (t0 + u < w.total && visit(c[u], dx, dy, rr2)) list[cnt++][threadIdx.x] = j[u];
+        }
       }
-      for (int q = 0; q < cnt; q++) {
-        const int jj = list[q][threadIdx.x];
-        const double2 pj = posd[jj];
-        const float dx = (float)(pi.x - pj.x), dy = (float)(pi.y - pj.y);
-        pair(dx, dy, __builtin_fmaf(dx, dx, dy * dy), reinterpret_cast<const float4 *>(rec32 + (size_t)16 * jj));
+      {  // evaluate the list
+        // the loads of pair q + 1 are issued before pair q is evaluated
+        double2 pn;
+        rec4 rn;
+        if (cnt > 0) {
+          const int jj = list[0][threadIdx.x];
+          pn = cs.pos[jj - cs.base];
+          load_rec(jj, rn);
+        }
+        for (int q = 0; q < cnt; q++) {
+          const double2 pj = pn;
+          const rec4 r = rn;
+          if (q + 1 < cnt) {
+            const int jj = list[q + 1][threadIdx.x];
+            pn = cs.pos[jj - cs.base];
+            load_rec(jj, rn);
+          }
+          const float dx = (float)(pi.x - pj.x), dy = (float)(pi.y - pj.y);
+          pair(dx, dy, __builtin_fmaf(dx, dx, dy * dy), r);
+        }
+        cnt = 0;
       }
+      if (t0 >= w.total) break;
     }
   }
   acc.vd0 = vd0.value(); acc.vd1 = vd1.value(); acc.vd2 = vd2.value(); acc.vd3 = vd3.value();
@@ -568,8 +625,8 @@ __device__ inline void update_particle(const dev_consts &k, int i, int s, const 
 template <bool F32, bool COMPACT>
 __device__ inline void particle_sums(const dev_consts &k, int i, const double *__restrict__ rec,
                                      const double2 *__restrict__ posd, const float *__restrict__ rec32,
-                                     const int *__restrict__ key_sorted, const int *__restrict__ offsets, int fl,
-                                     pair_acc &acc, double &tpse) {
+                                     const int *__restrict__ key_sorted, const int *__restrict__ offsets,
+                                     const cand_src &cs, int fl, pair_acc &acc, double &tpse) {
   const bool pse = k.pse != 0;
   const bool blank = fl & MFC_FLAG_BLANKED, tool = fl & MFC_FLAG_TOOL;
   const double ta = pse ? 0. : (tool ? k.alpha_tool : k.alpha_wp);
@@ -579,15 +636,16 @@ __device__ inline void particle_sums(const dev_consts &k, int i, const double *_
   tpse = 0.;
   if (run_mech || run_pse) {
     if constexpr (F32)
-      sums_f32<COMPACT>(k, i, posd, rec32, key_sorted, offsets, tool, ta, run_mech, run_pse, acc, tpse);
+      sums_f32<COMPACT>(k, i, posd, rec32, key_sorted, offsets, cs, tool, ta, run_mech, run_pse, acc, tpse);
     else
-      sums_f64<COMPACT>(k, i, rec, key_sorted, offsets, tool, ta, run_mech, run_pse, acc, tpse);
+      sums_f64<COMPACT>(k, i, rec, key_sorted, offsets, cs, tool, ta, run_mech, run_pse, acc, tpse);
   }
 }
 
-// tile partial sums of the tool force by halving strides (all threads of the work-group call it)
-__device__ inline void force_partials(int t, const double fsum[4], double *__restrict__ partials) {
-  __shared__ double red[4][TILE];
+// tile partial sums of the tool force by halving strides in red (LDS, 4 x TILE doubles; all
+// threads of the work-group call it)
+__device__ inline void force_partials(int t, const double fsum[4], double *__restrict__ partials,
+                                      double (*red)[TILE]) {
 #pragma unroll
   for (int c = 0; c < 4; c++) red[c][threadIdx.x] = fsum[c];
   for (int s = TILE / 2; s > 0; s >>= 1) {
@@ -603,38 +661,83 @@ __device__ inline void force_partials(int t, const double fsum[4], double *__res
   }
 }
 
+// LDS copy of the window candidates of a tile (positions and flags), when they fit
+struct stage_buf {
+  double2 pos[STAGE_CAP];
+  int fl[STAGE_CAP];
+};
+static_assert(sizeof(double2) * STAGE_CAP >= sizeof(double) * 4 * TILE, "force partials reuse the stage");
+
+// [lo, hi): the sorted indices of every window candidate of the particles of tile t. The first
+// and the last cell of a particle's window (column-major cell ids) do not decrease along the
+// sorted order, so the first particle's window start and the last particle's window end bound
+// the tile; where a window is clipped at the first or last two columns of the grid the bound
+// is the start or the end of the table instead.
+__device__ inline int2 tile_span(const dev_consts &k, int t, const int *__restrict__ key_sorted,
+                                 const int *__restrict__ offsets) {
+  const int first = t * TILE;
+  const int last = (first + TILE < k.N ? first + TILE : k.N) - 1;
+  const int hf = key_sorted[first], hl = key_sorted[last];
+  const int gif = hf / k.gny, gjf = hf - gif * k.gny;
+  const int gil = hl / k.gny, gjl = hl - gil * k.gny;
+  const int lo = gif - 2 < 0 ? 0 : offsets[(gif - 2) * k.gny + (gjf - 2 < 0 ? 0 : gjf - 2)];
+  const int hi = gil + 3 > k.gnx ? k.N : offsets[(gil + 2) * k.gny + (gjl + 3 > k.gny ? k.gny : gjl + 3)];
+  return make_int2(lo, hi);
+}
+
+// copies the tile's candidates into st when they fit (the same decision in every thread);
+// ends with a work-group barrier
+__device__ inline cand_src stage_tile(const dev_consts &k, int t, const drecords &r, const int *__restrict__ key_sorted,
+                                      const int *__restrict__ offsets, stage_buf &st) {
+  const int2 sp = tile_span(k, t, key_sorted, offsets);
+  const int n = sp.y - sp.x;
+  cand_src cs{r.posd, r.vfl, 0};
+  if (n <= STAGE_CAP) {
+    for (int q = threadIdx.x; q < n; q += TILE) {
+      st.pos[q] = r.posd[sp.x + q];
+      st.fl[q] = r.vfl[sp.x + q];
+    }
+    cs = cand_src{st.pos, st.fl, sp.x};
+  }
+  __syncthreads();
+  return cs;
+}
+
 // fused: sums + update in one pass (registers of both live in one kernel)
 template <bool F32, bool COMPACT>
-__device__ inline void interact(const dev_consts &k, int t, const double *__restrict__ rec,
-                                const double2 *__restrict__ posd, const float *__restrict__ rec32,
-                                const int *__restrict__ key_sorted, const int *__restrict__ offsets,
-                                const int *__restrict__ perm, const dstate &src, const dstate &dst,
-                                const dpositional &pos, const mfc_tool_state *__restrict__ tool_now,
-                                const dforce &fo) {
+__device__ inline void interact(const dev_consts &k, int t, const drecords &r, const int *__restrict__ key_sorted,
+                                const int *__restrict__ offsets, const int *__restrict__ perm, const dstate &src,
+                                const dstate &dst, const dpositional &pos,
+                                const mfc_tool_state *__restrict__ tool_now, const dforce &fo) {
+  __shared__ stage_buf st;
+  const cand_src cs = stage_tile(k, t, r, key_sorted, offsets, st);
   const int i = t * TILE + threadIdx.x;
   double fsum[4] = {0., 0., 0., 0.};
   if (i < k.N) {
     const int s = perm[i];
     pair_acc acc;
     double tpse;
-    particle_sums<F32, COMPACT>(k, i, rec, posd, rec32, key_sorted, offsets, src.flags[s] & FLAG_MASK, acc, tpse);
+    particle_sums<F32, COMPACT>(k, i, r.rec, r.posd, r.rec32, key_sorted, offsets, cs, src.flags[s] & FLAG_MASK, acc,
+                                tpse);
     update_particle(k, i, s, acc, tpse, src, dst, pos, *tool_now, fsum);
   }
-  force_partials(t, fsum, fo.partials);
+  __syncthreads();  // the force partials reuse the stage
+  force_partials(t, fsum, fo.partials, reinterpret_cast<double (*)[TILE]>(st.pos));
 }
 
 // split, first half: the neighbour sums of every particle into acc (ACC_N arrays of N doubles)
 template <bool F32, bool COMPACT>
-__device__ inline void sums(const dev_consts &k, int t, const double *__restrict__ rec,
-                            const double2 *__restrict__ posd, const float *__restrict__ rec32,
-                            const int *__restrict__ key_sorted, const int *__restrict__ offsets,
-                            const int *__restrict__ perm, const int *__restrict__ src_flags,
-                            double *__restrict__ acc) {
+__device__ inline void sums(const dev_consts &k, int t, const drecords &r, const int *__restrict__ key_sorted,
+                            const int *__restrict__ offsets, const int *__restrict__ perm,
+                            const int *__restrict__ src_flags, double *__restrict__ acc) {
+  __shared__ stage_buf st;
+  const cand_src cs = stage_tile(k, t, r, key_sorted, offsets, st);
   const int i = t * TILE + threadIdx.x;
   if (i >= k.N) return;
   pair_acc a;
   double tpse;
-  particle_sums<F32, COMPACT>(k, i, rec, posd, rec32, key_sorted, offsets, src_flags[perm[i]] & FLAG_MASK, a, tpse);
+  particle_sums<F32, COMPACT>(k, i, r.rec, r.posd, r.rec32, key_sorted, offsets, cs, src_flags[perm[i]] & FLAG_MASK,
+                              a, tpse);
   store_acc(acc, k.N, i, a, tpse);
 }
 
@@ -650,7 +753,8 @@ __device__ inline void update(const dev_consts &k, int t, const double *__restri
     load_acc(acc, k.N, i, a, tpse);
     update_particle(k, i, perm[i], a, tpse, src, dst, pos, *tool_now, fsum);
   }
-  force_partials(t, fsum, fo.partials);
+  __shared__ double red[4][TILE];
+  force_partials(t, fsum, fo.partials, red);
 }
 
 // tool force of the step from the tile partials: lane l adds the tile results l, l + TILE, ...
@@ -682,26 +786,24 @@ __device__ inline void force_final(int ntiles, const dforce &fo) {
 }
 
 template <bool F32, bool COMPACT>
-__global__ __launch_bounds__(TILE) void k_interact(dev_consts k, int ntiles, const double *__restrict__ rec,
-                                                   const double2 *__restrict__ posd, const float *__restrict__ rec32,
+__global__ __launch_bounds__(TILE) SUMS_OCCUPANCY void k_interact(dev_consts k, int ntiles, drecords r,
                                                    const int *__restrict__ key_sorted,
                                                    const int *__restrict__ offsets, const int *__restrict__ perm,
                                                    dstate src, dstate dst, dpositional pos,
                                                    const mfc_tool_state *__restrict__ tool_now, dforce fo) {
   const int t = xcd_tile(blockIdx.x, ntiles);
   if (t < 0) return;
-  interact<F32, COMPACT>(k, t, rec, posd, rec32, key_sorted, offsets, perm, src, dst, pos, tool_now, fo);
+  interact<F32, COMPACT>(k, t, r, key_sorted, offsets, perm, src, dst, pos, tool_now, fo);
 }
 
 template <bool F32, bool COMPACT>
-__global__ __launch_bounds__(TILE) void k_sums(dev_consts k, int ntiles, const double *__restrict__ rec,
-                                               const double2 *__restrict__ posd, const float *__restrict__ rec32,
+__global__ __launch_bounds__(TILE) SUMS_OCCUPANCY void k_sums(dev_consts k, int ntiles, drecords r,
                                                const int *__restrict__ key_sorted, const int *__restrict__ offsets,
                                                const int *__restrict__ perm, const int *__restrict__ src_flags,
                                                double *__restrict__ acc) {
   const int t = xcd_tile(blockIdx.x, ntiles);
   if (t < 0) return;
-  sums<F32, COMPACT>(k, t, rec, posd, rec32, key_sorted, offsets, perm, src_flags, acc);
+  sums<F32, COMPACT>(k, t, r, key_sorted, offsets, perm, src_flags, acc);
 }
 
 __global__ __launch_bounds__(TILE) void k_update(dev_consts k, int ntiles, const double *__restrict__ acc,
@@ -772,29 +874,28 @@ __global__ __launch_bounds__(TILE) void kb_predict(const inst_args *__restrict__
   if (w < 0) return;
   const int2 it = work[w];
   const inst_args &a = A[it.x];
-  predict<F32, false>(a.k, it.y, a.b.perm, a.cur, a.nxt, a.pos, a.r.rec, a.r.posd, a.r.rec32, a.tool_in,
-                      a.tool_out);
+  predict<F32, false>(a.k, it.y, a.b.perm, a.cur, a.nxt, a.pos, a.r, a.tool_in, a.tool_out);
 }
 
 template <bool F32, bool COMPACT>
-__global__ __launch_bounds__(TILE) void kb_interact(const inst_args *__restrict__ A, const int2 *__restrict__ work,
+__global__ __launch_bounds__(TILE) SUMS_OCCUPANCY void kb_interact(const inst_args *__restrict__ A, const int2 *__restrict__ work,
                                                     int nwork) {
   const int w = xcd_tile(blockIdx.x, nwork);
   if (w < 0) return;
   const int2 it = work[w];
   const inst_args &a = A[it.x];
-  interact<F32, COMPACT>(a.k, it.y, a.r.rec, a.r.posd, a.r.rec32, a.b.key_sorted, a.b.offsets, a.b.perm, a.cur,
+  interact<F32, COMPACT>(a.k, it.y, a.r, a.b.key_sorted, a.b.offsets, a.b.perm, a.cur,
                          a.nxt, a.pos, a.tool_out, a.f);
 }
 
 template <bool F32, bool COMPACT>
-__global__ __launch_bounds__(TILE) void kb_sums(const inst_args *__restrict__ A, const int2 *__restrict__ work,
+__global__ __launch_bounds__(TILE) SUMS_OCCUPANCY void kb_sums(const inst_args *__restrict__ A, const int2 *__restrict__ work,
                                                 int nwork) {
   const int w = xcd_tile(blockIdx.x, nwork);
   if (w < 0) return;
   const int2 it = work[w];
   const inst_args &a = A[it.x];
-  sums<F32, COMPACT>(a.k, it.y, a.r.rec, a.r.posd, a.r.rec32, a.b.key_sorted, a.b.offsets, a.b.perm, a.cur.flags,
+  sums<F32, COMPACT>(a.k, it.y, a.r, a.b.key_sorted, a.b.offsets, a.b.perm, a.cur.flags,
                      a.acc);
 }
 
@@ -859,8 +960,8 @@ void launch_predict(const dev_consts &k, const dstate &cur, const dstate &nxt, c
   const int nt = (k.N + TILE - 1) / TILE;
   auto kern = r.f32 ? (gather ? k_predict<true, true> : k_predict<true, false>)
                     : (gather ? k_predict<false, true> : k_predict<false, false>);
-  hipLaunchKernelGGL(kern, dim3(padded_tiles(nt)), dim3(TILE), 0, s, k, nt, b.perm, cur, nxt, pos, r.rec, r.posd,
-                     r.rec32, tool_in, tool_out);
+  hipLaunchKernelGGL(kern, dim3(padded_tiles(nt)), dim3(TILE), 0, s, k, nt, b.perm, cur, nxt, pos, r, tool_in,
+                     tool_out);
 }
 
 // the four instantiations of a kernel template <F32, COMPACT>
@@ -870,16 +971,16 @@ void launch_predict(const dev_consts &k, const dstate &cur, const dstate &nxt, c
 void launch_sums(const dev_consts &k, const int *src_flags, const dbins &b, const drecords &r, double *acc, variant v,
                  hipStream_t s) {
   const int nt = (k.N + TILE - 1) / TILE;
-  hipLaunchKernelGGL(MFC_PICK(k_sums, r.f32, v.compact), dim3(padded_tiles(nt)), dim3(TILE), 0, s, k, nt, r.rec,
-                     r.posd, r.rec32, b.key_sorted, b.offsets, b.perm, src_flags, acc);
+  hipLaunchKernelGGL(MFC_PICK(k_sums, r.f32, v.compact), dim3(padded_tiles(nt)), dim3(TILE), 0, s, k, nt, r,
+                     b.key_sorted, b.offsets, b.perm, src_flags, acc);
 }
 
 void launch_interact_fused(const dev_consts &k, const dstate &cur, const dstate &nxt, const dbins &b,
                            const dpositional &pos, const drecords &r, const mfc_tool_state *tool_now, const dforce &f,
                            variant v, hipStream_t s) {
   const int nt = (k.N + TILE - 1) / TILE;
-  hipLaunchKernelGGL(MFC_PICK(k_interact, r.f32, v.compact), dim3(padded_tiles(nt)), dim3(TILE), 0, s, k, nt, r.rec,
-                     r.posd, r.rec32, b.key_sorted, b.offsets, b.perm, cur, nxt, pos, tool_now, f);
+  hipLaunchKernelGGL(MFC_PICK(k_interact, r.f32, v.compact), dim3(padded_tiles(nt)), dim3(TILE), 0, s, k, nt, r,
+                     b.key_sorted, b.offsets, b.perm, cur, nxt, pos, tool_now, f);
 }
 
 void launch_update(const dev_consts &k, const double *acc, const dbins &b, const dstate &cur, const dstate &nxt,

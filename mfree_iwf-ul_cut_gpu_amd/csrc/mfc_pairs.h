@@ -8,6 +8,61 @@ namespace mfc {
 
 constexpr int REC_DOUBLES = 16;  // doubles per FP64 neighbour record
 
+// candidates of the window walk whose loads are issued together
+constexpr int WALK_U = 4;
+
+// where the window walk reads candidate positions and flags: candidate j at [j - base] of pos
+// and fl (the records in memory with base 0, or a tile's copy in LDS)
+struct cand_src {
+  const double2 *pos;
+  const int *fl;
+  int base;
+};
+
+// The 5 x 5 cell window around the cell of a particle (interactions_gpu.cu:219-229): up to 5
+// columns of the cell grid (clipped to the grid), each a contiguous range of the sorted order.
+// The candidates are numbered 0..total-1 column by column in ascending sorted index, which is
+// the reference's visiting order; at(t) is the sorted index of candidate t. The ten range
+// bounds are loaded together, before any candidate.
+struct window {
+  int total;
+  int c1, c2, c3, c4;      // number of the first candidate of columns 1..4
+  int d0;                  // sorted index minus candidate number in column 0
+  int e1, e2, e3, e4;      // its change from column r - 1 to column r
+  __device__ inline int at(int t) const {
+    // sums of selected increments (not a select among the five offsets, which the compiler
+    // may turn into an indexed load from private memory)
+    return t + d0 + (t >= c1 ? e1 : 0) + (t >= c2 ? e2 : 0) + (t >= c3 ? e3 : 0) + (t >= c4 ? e4 : 0);
+  }
+};
+
+__device__ inline window make_window(const dev_consts &k, const int *__restrict__ offsets, int hsh) {
+  const int gi = hsh / k.gny, gj = hsh - gi * k.gny;
+  const int li = gi - 2 < 0 ? 0 : gi - 2;
+  const int lj = gj - 2 < 0 ? 0 : gj - 2;
+  const int hi = gi + 3 > k.gnx ? k.gnx : gi + 3;
+  const int hj = gj + 3 > k.gny ? k.gny : gj + 3;
+  int b[5], e[5];
+#pragma unroll
+  for (int r = 0; r < 5; r++) {
+    const bool in = li + r < hi;
+    const int ii = in ? li + r : hi - 1;
+    b[r] = offsets[ii * k.gny + lj];
+    e[r] = offsets[ii * k.gny + hj];
+    if (!in) e[r] = b[r];
+  }
+  window w;
+  // d_r = b[r] - c_r with c_r the candidates before column r; e_r = d_r - d_(r-1)
+  int c = e[0] - b[0];
+  w.d0 = b[0];
+  w.c1 = c; w.e1 = b[1] - c - w.d0; c += e[1] - b[1];
+  w.c2 = c; w.e2 = b[2] - c - (b[1] - w.c1); c += e[2] - b[2];
+  w.c3 = c; w.e3 = b[3] - c - (b[2] - w.c2); c += e[3] - b[3];
+  w.c4 = c; w.e4 = b[4] - c - (b[3] - w.c3); c += e[4] - b[4];
+  w.total = c;
+  return w;
+}
+
 __device__ inline double bits_to_double(int f) { return __longlong_as_double((long long)f); }
 __device__ inline int double_to_bits(double d) { return (int)__double_as_longlong(d); }
 __device__ inline bool finite_d(double v) { return !mfcm::is_nan(v) && !mfcm::is_inf(v); }
@@ -113,8 +168,9 @@ __device__ inline void pair_terms(const dev_consts &k, pair_acc &a, double xi, d
 // body about as often as its busiest lane has pairs instead of once per window candidate.
 template <bool COMPACT>
 __device__ inline void sums_f64(const dev_consts &k, int i, const double *__restrict__ rec,
-                                const int *__restrict__ key_sorted, const int *__restrict__ offsets, bool tool,
-                                double ta, bool run_mech, bool run_pse, pair_acc &acc, double &tpse) {
+                                const int *__restrict__ key_sorted, const int *__restrict__ offsets,
+                                const cand_src &cs, bool tool, double ta, bool run_mech, bool run_pse, pair_acc &acc,
+                                double &tpse) {
   const double2 *r2 = reinterpret_cast<const double2 *>(rec + (size_t)REC_DOUBLES * i);
   double2 a0 = r2[0], a1 = r2[1], a2 = r2[2], a3 = r2[3], a4 = r2[4], a5 = r2[5], a6 = r2[6], a7 = r2[7];
   const double xi = a0.x, yi = a0.y;
@@ -129,65 +185,81 @@ __device__ inline void sums_f64(const dev_consts &k, int i, const double *__rest
   const double fac = PI_FAC * h1 * h1;
   const double h2 = hi * hi, h4 = h2 * h2;
   const double pse_c = 4. / (h4 * M_PI);
-  const int hsh = key_sorted[i];
-  const int gi = hsh / k.gny, gj = hsh - gi * k.gny;
-  const int li = gi - 2 < 0 ? 0 : gi - 2;
-  const int lj = gj - 2 < 0 ? 0 : gj - 2;
-  const int hi_ = gi + 3 > k.gnx ? k.gnx : gi + 3;
-  const int hj = gj + 3 > k.gny ? k.gny : gj + 3;
+  const window w = make_window(k, offsets, key_sorted[i]);
+  // what the walk loads per candidate ahead of its evaluation: position and flags (density and
+  // temperature for PSE are loaded when the candidate is evaluated, which keeps the registers
+  // of the group at 5 per candidate)
+  struct cand {
+    double2 p;
+    int f;
+  };
+  auto load = [&](int j, cand &c) {
+    c.p = cs.pos[j - cs.base];
+    c.f = cs.fl[j - cs.base];
+  };
   // the PSE term of every window member and the test whether j adds to the mechanical sums
-  auto visit = [&](int j) -> bool {
-    const double *rj = rec + (size_t)REC_DOUBLES * j;
-    const double2 b0 = reinterpret_cast<const double2 *>(rj)[0];
-    const double2 b1 = reinterpret_cast<const double2 *>(rj)[1];
-    const double xj = b0.x, yj = b0.y;
-    const int fj = double_to_bits(b1.x);
+  auto visit = [&](int j, const cand &c) -> bool {
+    const double xj = c.p.x, yj = c.p.y;
     if (run_pse) {  // do_interactions_heat_pse (interactions_gpu.cu:130-139), all cell members
-      const double2 b7 = reinterpret_cast<const double2 *>(rj)[7];
-      const double2 b3 = reinterpret_cast<const double2 *>(rj)[3];
+      const double *rj = rec + (size_t)REC_DOUBLES * j;
       const double dx = xi - xj, dy = yi - yj;
       const double xx = sqrt(dx * dx + dy * dy);
       const double w2 = pse_c * mfcm::exp(-xx * xx / (h2));
-      tpse += (b7.y - Tpred) * w2 * k.mass / b3.x;
+      tpse += (rj[15] - Tpred) * w2 * k.mass / rj[6];
     }
     if (!run_mech) return false;
-    if (fj & MFC_FLAG_BLANKED) return false;
+    if (c.f & MFC_FLAG_BLANKED) return false;
     const double dx = xi - xj, dy = yi - yj;
     const double rr2 = dx * dx + dy * dy;
     // outside the support the reference adds exact zeros, unless a value is non-finite
-    return !((rr2 >= fourh2 || rr2 < 1e-8) && !nonfin_i && !(fj & FLAG_NONFINITE));
+    return !((rr2 >= fourh2 || rr2 < 1e-8) && !nonfin_i && !(c.f & FLAG_NONFINITE));
   };
-  auto pair = [&](int j) {
-    const double *rj = rec + (size_t)REC_DOUBLES * j;
-    const double2 b0 = reinterpret_cast<const double2 *>(rj)[0];
-    const double2 b1 = reinterpret_cast<const double2 *>(rj)[1];
+  auto pair = [&](int j, double xj, double yj, int fj) {
     pair_terms(k, acc, xi, yi, ui, vi_, hi, rhoi, Axx_i, Axy_i, Ayy_i, Rxx_i, Rxy_i, Ryy_i, ci, Ti, tool, ta, h1,
-               fourh2, fac, rj, b0.x, b0.y, (double_to_bits(b1.x) & MFC_FLAG_TOOL) != 0);
+               fourh2, fac, rec + (size_t)REC_DOUBLES * j, xj, yj, (fj & MFC_FLAG_TOOL) != 0);
   };
+  // the window in groups of WALK_U candidates, the loads of a group issued before its first
+  // candidate is evaluated; the order of evaluation is the candidates' order
   if constexpr (!COMPACT) {
-    for (int ii = li; ii < hi_; ii++) {
-      const int jb = offsets[ii * k.gny + lj], je = offsets[ii * k.gny + hj];
-      for (int j = jb; j < je; j++)
-        if (visit(j)) pair(j);
+    for (int t0 = 0; t0 < w.total; t0 += WALK_U) {
+      int j[WALK_U];
+      cand c[WALK_U];
+#pragma unroll
+      for (int u = 0; u < WALK_U; u++) {
+        j[u] = w.at(t0 + u < w.total ? t0 + u : w.total - 1);
+        load(j[u], c[u]);
+      }
+#pragma unroll
+      for (int u = 0; u < WALK_U; u++)
+        if (t0 + u < w.total && visit(j[u], c[u])) pair(j[u], c[u].p.x, c[u].p.y, c[u].f);
     }
   } else {
     __shared__ int list[PAIR_LIST][TILE];
-    int ii = li, j = 0, je = 0;
-    bool more = ii < hi_;
-    if (more) { j = offsets[ii * k.gny + lj]; je = offsets[ii * k.gny + hj]; }
-    while (more) {
-      int cnt = 0;
-      while (cnt < PAIR_LIST) {
-        if (j >= je) {
-          if (++ii >= hi_) { more = false; break; }
-          j = offsets[ii * k.gny + lj];
-          je = offsets[ii * k.gny + hj];
-          continue;
+    int cnt = 0, t0 = 0;
+    // collect until the list could overflow or the window is done (each lane on its own),
+    // then evaluate the lists of all lanes together
+    for (;;) {
+      for (; t0 < w.total && cnt <= PAIR_LIST - WALK_U; t0 += WALK_U) {
+        int j[WALK_U];
+        cand c[WALK_U];
+#pragma unroll
+        for (int u = 0; u < WALK_U; u++) {
+          j[u] = w.at(t0 + u < w.total ? t0 + u : w.total - 1);
+          load(j[u], c[u]);
         }
-        if (visit(j)) list[cnt++][threadIdx.x] = j;
-        j++;
+#pragma unroll
+        for (int u = 0; u < WALK_U; u++)
+          if (t0 + u < w.total && visit(j[u], c[u])) list[cnt++][threadIdx.x] = j[u];
       }
-      for (int q = 0; q < cnt; q++) pair(list[q][threadIdx.x]);
+      {  // evaluate the list
+        for (int q = 0; q < cnt; q++) {
+          const int jj = list[q][threadIdx.x];
+          const double2 p = cs.pos[jj - cs.base];
+          pair(jj, p.x, p.y, cs.fl[jj - cs.base]);
+        }
+        cnt = 0;
+      }
+      if (t0 >= w.total) break;
     }
   }
 }
