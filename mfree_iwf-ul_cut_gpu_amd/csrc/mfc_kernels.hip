@@ -685,22 +685,23 @@ __device__ inline int2 tile_span(const dev_consts &k, int t, const int *__restri
   return make_int2(lo, hi);
 }
 
-// copies the tile's candidates into st when they fit (the same decision in every thread);
-// ends with a work-group barrier
-__device__ inline cand_src stage_tile(const dev_consts &k, int t, const drecords &r, const int *__restrict__ key_sorted,
-                                      const int *__restrict__ offsets, stage_buf &st) {
+// copies the tile's candidates into st when they fit (the same decision in every thread) and
+// returns the sorted index of st's first entry, or -1 when they do not fit; ends with a
+// work-group barrier
+__device__ inline int stage_tile(const dev_consts &k, int t, const drecords &r, const int *__restrict__ key_sorted,
+                                 const int *__restrict__ offsets, stage_buf &st) {
   const int2 sp = tile_span(k, t, key_sorted, offsets);
   const int n = sp.y - sp.x;
-  cand_src cs{r.posd, r.vfl, 0};
+  int lo = -1;
   if (n <= STAGE_CAP) {
     for (int q = threadIdx.x; q < n; q += TILE) {
       st.pos[q] = r.posd[sp.x + q];
       st.fl[q] = r.vfl[sp.x + q];
     }
-    cs = cand_src{st.pos, st.fl, sp.x};
+    lo = sp.x;
   }
   __syncthreads();
-  return cs;
+  return lo;
 }
 
 // fused: sums + update in one pass (registers of both live in one kernel)
@@ -710,15 +711,17 @@ __device__ inline void interact(const dev_consts &k, int t, const drecords &r, c
                                 const dstate &dst, const dpositional &pos,
                                 const mfc_tool_state *__restrict__ tool_now, const dforce &fo) {
   __shared__ stage_buf st;
-  const cand_src cs = stage_tile(k, t, r, key_sorted, offsets, st);
+  const int lo = stage_tile(k, t, r, key_sorted, offsets, st);
   const int i = t * TILE + threadIdx.x;
   double fsum[4] = {0., 0., 0., 0.};
   if (i < k.N) {
     const int s = perm[i];
+    const int fl = src.flags[s] & FLAG_MASK;
     pair_acc acc;
     double tpse;
-    particle_sums<F32, COMPACT>(k, i, r.rec, r.posd, r.rec32, key_sorted, offsets, cs, src.flags[s] & FLAG_MASK, acc,
-                                tpse);
+    // one call site: the candidate loads are flat loads (LDS or global memory)
+    const cand_src cs = lo >= 0 ? cand_src{st.pos, st.fl, lo} : cand_src{r.posd, r.vfl, 0};
+    particle_sums<F32, COMPACT>(k, i, r.rec, r.posd, r.rec32, key_sorted, offsets, cs, fl, acc, tpse);
     update_particle(k, i, s, acc, tpse, src, dst, pos, *tool_now, fsum);
   }
   __syncthreads();  // the force partials reuse the stage
@@ -731,13 +734,14 @@ __device__ inline void sums(const dev_consts &k, int t, const drecords &r, const
                             const int *__restrict__ offsets, const int *__restrict__ perm,
                             const int *__restrict__ src_flags, double *__restrict__ acc) {
   __shared__ stage_buf st;
-  const cand_src cs = stage_tile(k, t, r, key_sorted, offsets, st);
+  const int lo = stage_tile(k, t, r, key_sorted, offsets, st);
   const int i = t * TILE + threadIdx.x;
   if (i >= k.N) return;
+  const int fl = src_flags[perm[i]] & FLAG_MASK;
   pair_acc a;
   double tpse;
-  particle_sums<F32, COMPACT>(k, i, r.rec, r.posd, r.rec32, key_sorted, offsets, cs, src_flags[perm[i]] & FLAG_MASK,
-                              a, tpse);
+  const cand_src cs = lo >= 0 ? cand_src{st.pos, st.fl, lo} : cand_src{r.posd, r.vfl, 0};
+  particle_sums<F32, COMPACT>(k, i, r.rec, r.posd, r.rec32, key_sorted, offsets, cs, fl, a, tpse);
   store_acc(acc, k.N, i, a, tpse);
 }
 
@@ -818,6 +822,61 @@ __global__ __launch_bounds__(TILE) void k_update(dev_consts k, int ntiles, const
 __global__ __launch_bounds__(TILE) void k_force_final(int ntiles, dforce fo) { force_final(ntiles, fo); }
 
 // ------------------------------------------------------------------------------- batched
+// Fields of the instance descriptors read through the constant address space: the compiler
+// then takes the pointers loaded from them for global pointers and emits global loads, which
+// count only against the vector-memory wait counter, instead of flat loads, which also hold up
+// every wait for an LDS or scalar load. (A whole-struct copy loses that, hence the field lists.)
+// The descriptors do not change while a kernel runs.
+#define MFC_CONST(T) const __attribute__((address_space(4))) T
+#define MFC_F(f) o.f = c->f
+__device__ inline dstate load_state(MFC_CONST(dstate) * c) {
+  dstate o;
+  MFC_F(px); MFC_F(py); MFC_F(vx); MFC_F(vy); MFC_F(sxx); MFC_F(sxy); MFC_F(syy); MFC_F(szz); MFC_F(rho);
+  MFC_F(T); MFC_F(eps); MFC_F(epsd); MFC_F(h); MFC_F(fcx); MFC_F(fcy); MFC_F(ftx); MFC_F(fty); MFC_F(ptx);
+  MFC_F(pty); MFC_F(vtx); MFC_F(vty); MFC_F(rhot); MFC_F(stxx); MFC_F(stxy); MFC_F(styy); MFC_F(stzz); MFC_F(Tt);
+  MFC_F(flags); MFC_F(id);
+  return o;
+}
+__device__ inline dbins load_bins(MFC_CONST(dbins) * c) {
+  dbins o;
+  MFC_F(key); MFC_F(key_sorted); MFC_F(slot); MFC_F(count); MFC_F(offsets); MFC_F(bucket); MFC_F(perm);
+  MFC_F(bsum); MFC_F(clamped);
+  return o;
+}
+__device__ inline dpositional load_pos(MFC_CONST(dpositional) * c) {
+  dpositional o;
+  MFC_F(nx); MFC_F(ny); MFC_F(p); MFC_F(rxx); MFC_F(rxy); MFC_F(ryy);
+  return o;
+}
+__device__ inline drecords load_recs(MFC_CONST(drecords) * c) {
+  drecords o;
+  MFC_F(f32); MFC_F(rec); MFC_F(posd); MFC_F(vfl); MFC_F(rec32);
+  return o;
+}
+__device__ inline dforce load_force(MFC_CONST(dforce) * c) {
+  dforce o;
+  MFC_F(partials); MFC_F(force_last); MFC_F(hist); MFC_F(cap); MFC_F(ctr);
+  return o;
+}
+__device__ inline MFC_CONST(inst_args) * const_args(const inst_args *A, int q) {
+  return (MFC_CONST(inst_args) *)(A + q);
+}
+__device__ inline inst_args load_args(const inst_args *A, int q) {
+  MFC_CONST(inst_args) *c = const_args(A, q);
+  inst_args o;
+  o.k = A[q].k;  // no pointers
+  o.cur = load_state(&c->cur);
+  o.nxt = load_state(&c->nxt);
+  o.b = load_bins(&c->b);
+  o.pos = load_pos(&c->pos);
+  o.r = load_recs(&c->r);
+  MFC_F(tool_in); MFC_F(tool_out);
+  o.f = load_force(&c->f);
+  MFC_F(acc); MFC_F(ntiles); MFC_F(nscan);
+  return o;
+}
+#undef MFC_F
+
 // One launch per phase for all instances of a batch: block b takes work item xcd_tile(b) of the
 // (instance, block) table, and runs the same device body as the single-instance kernel with
 // that instance's constants and buffers, so every instance computes exactly its own run.
@@ -826,9 +885,9 @@ __global__ __launch_bounds__(256) void kb_hash_count(const inst_args *__restrict
   const int w = xcd_tile(blockIdx.x, nwork);
   if (w < 0) return;
   const int2 it = work[w];
-  const inst_args &a = A[it.x];
-  hash_count(a.k, it.y * 256 + threadIdx.x, a.cur.px, a.cur.py, a.cur.flags, a.b.key_sorted, a.b.key, a.b.slot,
-             a.b.count, a.b.clamped);
+  const inst_args a = load_args(A, it.x);
+  hash_count(a.k, it.y * 256 + threadIdx.x, a.cur.px, a.cur.py, a.cur.flags, a.b.key_sorted, a.b.key,
+             a.b.slot, a.b.count, a.b.clamped);
 }
 
 __global__ __launch_bounds__(SCAN_THREADS) void kb_scan_reduce(const inst_args *__restrict__ A,
@@ -836,7 +895,7 @@ __global__ __launch_bounds__(SCAN_THREADS) void kb_scan_reduce(const inst_args *
   const int w = xcd_tile(blockIdx.x, nwork);
   if (w < 0) return;
   const int2 it = work[w];
-  const inst_args &a = A[it.x];
+  const inst_args a = load_args(A, it.x);
   scan_reduce(it.y, a.b.count, a.k.num_cell, a.b.bsum);
 }
 
@@ -845,7 +904,7 @@ __global__ __launch_bounds__(SCAN_THREADS) void kb_scan_down(const inst_args *__
   const int w = xcd_tile(blockIdx.x, nwork);
   if (w < 0) return;
   const int2 it = work[w];
-  const inst_args &a = A[it.x];
+  const inst_args a = load_args(A, it.x);
   scan_down(it.y, a.b.count, a.k.num_cell, a.b.bsum, a.b.offsets);
 }
 
@@ -854,7 +913,7 @@ __global__ __launch_bounds__(256) void kb_place(const inst_args *__restrict__ A,
   const int w = xcd_tile(blockIdx.x, nwork);
   if (w < 0) return;
   const int2 it = work[w];
-  const inst_args &a = A[it.x];
+  const inst_args a = load_args(A, it.x);
   place(it.y * 256 + threadIdx.x, a.k.N, a.b.key, a.b.slot, a.b.offsets, a.b.bucket);
 }
 
@@ -863,7 +922,7 @@ __global__ __launch_bounds__(256) void kb_rank(const inst_args *__restrict__ A, 
   const int w = xcd_tile(blockIdx.x, nwork);
   if (w < 0) return;
   const int2 it = work[w];
-  const inst_args &a = A[it.x];
+  const inst_args a = load_args(A, it.x);
   rank(it.y * 256 + threadIdx.x, a.k.N, a.b.key, a.b.bucket, a.b.offsets, a.b.perm, a.b.key_sorted);
 }
 
@@ -873,8 +932,13 @@ __global__ __launch_bounds__(TILE) void kb_predict(const inst_args *__restrict__
   const int w = xcd_tile(blockIdx.x, nwork);
   if (w < 0) return;
   const int2 it = work[w];
+  // records and bins through the constant address space (global loads and stores); the state
+  // pointers through generic loads, where they are used (holding all of them would take more
+  // scalar registers than there are)
   const inst_args &a = A[it.x];
-  predict<F32, false>(a.k, it.y, a.b.perm, a.cur, a.nxt, a.pos, a.r, a.tool_in, a.tool_out);
+  MFC_CONST(inst_args) *c = const_args(A, it.x);
+  predict<F32, false>(a.k, it.y, load_bins(&c->b).perm, a.cur, a.nxt, a.pos, load_recs(&c->r), a.tool_in,
+                      a.tool_out);
 }
 
 template <bool F32, bool COMPACT>
@@ -884,8 +948,10 @@ __global__ __launch_bounds__(TILE) SUMS_OCCUPANCY void kb_interact(const inst_ar
   if (w < 0) return;
   const int2 it = work[w];
   const inst_args &a = A[it.x];
-  interact<F32, COMPACT>(a.k, it.y, a.r, a.b.key_sorted, a.b.offsets, a.b.perm, a.cur,
-                         a.nxt, a.pos, a.tool_out, a.f);
+  MFC_CONST(inst_args) *c = const_args(A, it.x);
+  const dbins b = load_bins(&c->b);
+  interact<F32, COMPACT>(a.k, it.y, load_recs(&c->r), b.key_sorted, b.offsets, b.perm, a.cur, a.nxt, a.pos,
+                         a.tool_out, a.f);
 }
 
 template <bool F32, bool COMPACT>
@@ -895,8 +961,9 @@ __global__ __launch_bounds__(TILE) SUMS_OCCUPANCY void kb_sums(const inst_args *
   if (w < 0) return;
   const int2 it = work[w];
   const inst_args &a = A[it.x];
-  sums<F32, COMPACT>(a.k, it.y, a.r, a.b.key_sorted, a.b.offsets, a.b.perm, a.cur.flags,
-                     a.acc);
+  MFC_CONST(inst_args) *c = const_args(A, it.x);
+  const dbins b = load_bins(&c->b);
+  sums<F32, COMPACT>(a.k, it.y, load_recs(&c->r), b.key_sorted, b.offsets, b.perm, a.cur.flags, c->acc);
 }
 
 __global__ __launch_bounds__(TILE) void kb_update(const inst_args *__restrict__ A, const int2 *__restrict__ work,
@@ -905,11 +972,12 @@ __global__ __launch_bounds__(TILE) void kb_update(const inst_args *__restrict__ 
   if (w < 0) return;
   const int2 it = work[w];
   const inst_args &a = A[it.x];
-  update(a.k, it.y, a.acc, a.b.perm, a.cur, a.nxt, a.pos, a.tool_out, a.f);
+  MFC_CONST(inst_args) *c = const_args(A, it.x);
+  update(a.k, it.y, c->acc, load_bins(&c->b).perm, a.cur, a.nxt, a.pos, a.tool_out, a.f);
 }
 
 __global__ __launch_bounds__(TILE) void kb_force_final(const inst_args *__restrict__ A) {
-  const inst_args &a = A[blockIdx.x];
+  const inst_args a = load_args(A, blockIdx.x);
   force_final(a.ntiles, a.f);
 }
 
