@@ -240,6 +240,7 @@ struct adj_work {
   mfc_tool_state *toolw = nullptr;      // 2
   double *partials = nullptr, *force_last = nullptr, *hist = nullptr;
   int64_t *ctr = nullptr;
+  int *pending = nullptr;
   int *keys_keep = nullptr;  // the handle's cell ids, kept across the replay of the backward pass
   double *acc = nullptr, *lam_acc = nullptr, *lam_s[2] = {nullptr, nullptr}, *lam_n = nullptr, *g_s = nullptr,
          *g_rec = nullptr, *theta_part = nullptr, *theta_acc = nullptr;
@@ -308,7 +309,7 @@ int setup_work(mfc_sim *s, int nsteps, int K, int w0, int w1) {
     if ((rc = w->alloc(&w->work[0], state_block_bytes(N))) || (rc = w->alloc(&w->work[1], state_block_bytes(N))) ||
         (rc = w->alloc(&w->posw, positional_block_bytes(N))) || (rc = w->alloc(&w->toolw, 2)) ||
         (rc = w->alloc(&w->partials, 4 * (size_t)s->ntiles)) || (rc = w->alloc(&w->force_last, 2)) ||
-        (rc = w->alloc(&w->hist, 2)) || (rc = w->alloc(&w->ctr, 1)) ||
+        (rc = w->alloc(&w->hist, 2)) || (rc = w->alloc(&w->ctr, 1)) || (rc = w->alloc(&w->pending, 1)) ||
         (rc = w->alloc(&w->keys_keep, n)) ||
         (rc = w->alloc(&w->acc, L_N * n)) || (rc = w->alloc(&w->lam_acc, L_N * n)) ||
         (rc = w->alloc(&w->lam_s[0], TS_N * n)) || (rc = w->alloc(&w->lam_s[1], TS_N * n)) ||
@@ -318,6 +319,7 @@ int setup_work(mfc_sim *s, int nsteps, int K, int w0, int w1) {
       return rc;
     // the replay's force reduction keeps no history (cap 0) but counts steps
     HIP_TRY(hipMemsetAsync(w->ctr, 0, sizeof(int64_t), s->stream));
+    HIP_TRY(hipMemsetAsync(w->pending, 0, sizeof(int), s->stream));
   }
   w->nsteps = nsteps;
   w->w0 = w0;
@@ -387,6 +389,7 @@ int mfc_vjp_forward(mfc_sim *s, int nsteps, int checkpoint_every, int w0, int w1
     s->cur = 1 - s->cur;
     s->step++;
   }
+  finish_steps(s);
   HIP_TRY(hipGetLastError());
   return adj_finish(s, Fmean);
 }
@@ -410,7 +413,7 @@ int mfc_vjp_backward(mfc_sim *s, const double cot[2], double grad[6]) {
   dstate A, B;
   dpositional P;
   carve_positional(w->posw, N, P);
-  dforce fo = {w->partials, w->force_last, w->hist, 0, w->ctr};
+  dforce fo = {w->partials, w->force_last, w->hist, 0, w->ctr, w->pending};
   const int nseg = (w->nsteps + K - 1) / K;
   int rc;
   for (int sg = nseg - 1; sg >= 0; sg--) {

@@ -1,7 +1,7 @@
 // mfc_api.hip — C ABI of the solver (declared in include/mfc.h).
 //
 // A handle owns every device buffer of one simulation, its own HIP stream (default priority)
-// and the hipGraph of two consecutive steps. Steps are queued asynchronously; calls that
+// and the hipGraphs of GRAPH_STEPS consecutive steps. Steps are queued asynchronously; calls that
 // return data synchronise the handle's stream.
 #include <hip/hip_runtime.h>
 
@@ -68,6 +68,9 @@ void carve_positional(void *base, int N, mfc::dpositional &p) {
 
 void enqueue_step(mfc_sim *s, int c) { mfc::launch_batch_step(s->work(c), s->stream); }
 
+// each step's tool force is reduced by the next step's first kernel; this reduces the last one
+void finish_steps(mfc_sim *s) { mfc::launch_batch_force(s->work(s->cur), s->stream); }
+
 mfc::inst_args instance_args(const mfc_sim *s, int c) {
   mfc::inst_args a;
   std::memset(&a, 0, sizeof(a));
@@ -99,8 +102,7 @@ int alloc_state(mfc_sim *s, mfc::dstate &d, int N) {
 int build_graph(mfc_sim *s, int parity) {
   hipGraph_t g = nullptr;
   HIP_TRY(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
-  enqueue_step(s, parity);
-  enqueue_step(s, 1 - parity);
+  for (int q = 0; q < GRAPH_STEPS; q++) enqueue_step(s, (parity + q) & 1);
   HIP_TRY(hipStreamEndCapture(s->stream, &g));
   hipError_t e = hipGraphInstantiate(&s->graph[parity], g, nullptr, nullptr, 0);
   (void)hipGraphDestroy(g);
@@ -181,6 +183,7 @@ int mfc_create(const mfc_config *cfg, mfc_sim **out) {
   if ((rc = s->alloc(&s->tool, 2)) ||
       (rc = s->alloc(&s->partials, (size_t)4 * s->ntiles)) || (rc = s->alloc(&s->force_last, 2)) ||
       (rc = s->alloc(&s->hist, (size_t)2 * HIST_CAP)) || (rc = s->alloc(&s->hist_ctr, 1)) ||
+      (rc = s->alloc(&s->force_pending, 1)) ||
       (rc = s->alloc(&s->acc, (size_t)mfc::ACC_N * N)))
     return bail(rc);
   // zero everything, then upload the initial state (particle_gpu.cu:20-125)
@@ -215,6 +218,8 @@ int mfc_create(const mfc_config *cfg, mfc_sim **out) {
         hipMemsetAsync(b.offsets, 0, sizeof(int) * ((size_t)c.num_cell + 1), s->stream) != hipSuccess ||
         hipMemsetAsync(b.clamped, 0, sizeof(unsigned long long), s->stream) != hipSuccess ||
         hipMemsetAsync(s->hist_ctr, 0, sizeof(int64_t), s->stream) != hipSuccess ||
+        hipMemsetAsync(s->force_pending, 0, sizeof(int), s->stream) != hipSuccess ||
+        hipMemsetAsync(b.bsum, 0, sizeof(int) * (size_t)mfc::scan_blocks(c.num_cell), s->stream) != hipSuccess ||
         hipMemsetAsync(s->force_last, 0, 2 * sizeof(double), s->stream) != hipSuccess ||
         (s->recs.rec && hipMemsetAsync(s->recs.rec, 0, sizeof(double) * mfc::REC * (size_t)N, s->stream) != hipSuccess) ||
         hipMemsetAsync(s->recs.posd, 0, sizeof(double2) * (size_t)N, s->stream) != hipSuccess ||
@@ -268,16 +273,16 @@ int mfc_step(mfc_sim *s, int nsteps) {
   if (!s || nsteps < 0) return fail(MFC_E_INVALID, "bad argument");
   if (s->batched) return fail(MFC_E_INVALID, "instance of a batch: step it with mfc_batch_step");
   int done = 0;
-  if (s->use_graphs && nsteps >= 4) {
-    while (nsteps - done >= 2) {
+  if (s->use_graphs && nsteps >= GRAPH_STEPS) {
+    while (nsteps - done >= GRAPH_STEPS) {
       int par = s->cur;
       if (!s->graph[par]) {
         int rc = build_graph(s, par);
         if (rc) return rc;
       }
       HIP_TRY(hipGraphLaunch(s->graph[par], s->stream));
-      done += 2;
-      s->step += 2;
+      done += GRAPH_STEPS;
+      s->step += GRAPH_STEPS;
     }
   }
   for (; done < nsteps; done++) {
@@ -285,6 +290,7 @@ int mfc_step(mfc_sim *s, int nsteps) {
     s->cur = 1 - s->cur;
     s->step++;
   }
+  finish_steps(s);
   HIP_TRY(hipGetLastError());
   return MFC_OK;
 }

@@ -60,8 +60,7 @@ void batch_release(mfc_batch *b) {
 int build_batch_graph(mfc_batch *b, int parity) {
   hipGraph_t g = nullptr;
   HIP_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-  mfc::launch_batch_step(b->work(parity), b->stream);
-  mfc::launch_batch_step(b->work(1 - parity), b->stream);
+  for (int q = 0; q < GRAPH_STEPS; q++) mfc::launch_batch_step(b->work((parity + q) & 1), b->stream);
   HIP_TRY(hipStreamEndCapture(b->stream, &g));
   hipError_t e = hipGraphInstantiate(&b->graph[parity], g, nullptr, nullptr, 0);
   (void)hipGraphDestroy(g);
@@ -159,23 +158,23 @@ int mfc_batch_set_tuning(mfc_batch *b, const mfc_tuning *t) {
 int mfc_batch_step(mfc_batch *b, int nsteps) {
   if (!b || nsteps < 0) return fail(MFC_E_INVALID, "bad argument");
   int done = 0;
-  if (nsteps >= 4) {
-    while (nsteps - done >= 2) {
+  if (nsteps >= GRAPH_STEPS) {
+    while (nsteps - done >= GRAPH_STEPS) {
       const int par = b->cur;
       if (!b->graph[par]) {
         int rc = build_batch_graph(b, par);
         if (rc) return rc;
       }
       HIP_TRY(hipGraphLaunch(b->graph[par], b->stream));
-      advance(b);
-      advance(b);
-      done += 2;
+      for (int q = 0; q < GRAPH_STEPS; q++) advance(b);
+      done += GRAPH_STEPS;
     }
   }
   for (; done < nsteps; done++) {
     mfc::launch_batch_step(b->work(b->cur), b->stream);
     advance(b);
   }
+  mfc::launch_batch_force(b->work(b->cur), b->stream);  // the last step's tool force
   HIP_TRY(hipGetLastError());
   return MFC_OK;
 }
@@ -197,6 +196,7 @@ int mfc_batch_vjp_forward(mfc_batch *b, int nsteps, int checkpoint_every, int w0
     mfc::launch_batch_step(b->work(b->cur), b->stream);
     advance(b);
   }
+  mfc::launch_batch_force(b->work(b->cur), b->stream);
   HIP_TRY(hipGetLastError());
   for (size_t i = 0; i < b->sims.size(); i++)
     if ((rc = mfc::adj_finish(b->sims[i], Fmean + 2 * i))) return rc;

@@ -57,14 +57,16 @@ struct drecords {
 // 20 bytes each); a tile whose candidates span more sorted indices reads them from memory
 constexpr int STAGE_CAP = 2048;
 
-// tool-force reduction: per-tile partials written by the interaction kernel, reduced by
-// k_force_final, which appends F to the history
+// tool-force reduction: per-tile partials written by the interaction kernel, reduced by the
+// force-final pass (the next step's first kernel, or a kernel of its own after the last step),
+// which appends F to the history
 struct dforce {
   double *partials;      // 4 per tile: sum fc.x, fc.y, ft.x, ft.y
   double *force_last;    // F of the last step
   double *hist;          // force history, 2 per step, cap entries
   int64_t cap;
   int64_t *ctr;          // steps recorded
+  int *pending;          // 1: the partials of a step are not yet reduced
 };
 
 // the particle flags of the state (FIXED, BLANKED, TOOL)

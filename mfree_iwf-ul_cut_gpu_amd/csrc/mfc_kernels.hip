@@ -1,18 +1,20 @@
 // mfc_kernels.hip — gfx950 kernels of one SPH cutting step (FP64 and FP32C).
 //
-// Per step (the reference's ~20 launches of leap_frog::step, leap_frog.cu:94-155, become 8):
-//   binning      hash_count, scan_reduce, scan_down, place, rank: cell ids (grid.cu:34-51) and
-//                a stable counting sort by cell (thrust::sort_by_key +
-//                reorder_data_and_find_cell_start, grid_gpu_green.cu:22-116,177-201)
-//   predict      leap-frog predict (leap_frog.cu:30-66) + EOS (actions_gpu.cu:41-51) + artificial
-//                stress (:53-97) + neighbour record + tool move, reading the start state
-//                through the sort permutation
+// Per step (the reference's ~20 launches of leap_frog::step, leap_frog.cu:94-155, become 5):
+//   binning      hash_count, scan_down, place: cell ids (grid.cu:34-51), per-cell and per-chunk
+//                counts, cell offsets and per-cell buckets of a stable counting sort by cell
+//                (thrust::sort_by_key + reorder_data_and_find_cell_start,
+//                grid_gpu_green.cu:22-116,177-201); hash_count's first block also reduces the
+//                previous step's tool force (output.cu:30-39) from its tile partials
+//   predict      the sort's rank pass, then leap-frog predict (leap_frog.cu:30-66) + EOS
+//                (actions_gpu.cu:41-51) + artificial stress (:53-97) + neighbour record + tool
+//                move, in the start state's storage order
 //   interact     neighbour sums (interactions_gpu.cu:88-345) fused with every per-particle action
 //                that follows them in the step: Jaumann, continuity, momentum, advection, correct,
 //                friction heat, Johnson-Cook return, tool-particle move, contact, boundary
 //                conditions, NaN blanking, the gather into sorted order, and the per-tile partial
 //                sums of the tool force (or split into sums + update)
-//   force_final  tool force of the step from the tile partials (output.cu:30-39)
+// After the last step of a call, force_final reduces the last step's tool force.
 // Each body is a device function of its work index; the kb_* kernels run it over an
 // (instance, tile) work table for any number of instances (a single handle is a batch of one),
 // the k_* kernels over one instance's tiles (used by the reverse mode's replay).
@@ -62,11 +64,13 @@ __device__ inline int block_exclusive_scan(int v, int &total) {
 // (index from blockIdx) and by the batched kernel (index from the (instance, block) work table).
 
 // cell id of every particle (grid.cu:34-51; blanked particles keep their last sorted id,
-// grid.cu:39-41) and its arrival slot inside the cell
+// grid.cu:39-41), its arrival slot inside the cell, and the particle count of every chunk of
+// SCAN_CHUNK cells (bsum, zero before the step)
 __device__ inline void hash_count(const dev_consts &k, int i, const double *__restrict__ px,
                                   const double *__restrict__ py, const int *__restrict__ flags,
                                   const int *__restrict__ key_sorted, int *__restrict__ key, int *__restrict__ slot,
-                                  int *__restrict__ count, unsigned long long *__restrict__ clamped) {
+                                  int *__restrict__ count, int *__restrict__ bsum,
+                                  unsigned long long *__restrict__ clamped) {
   // all 64 lanes of the wave take part (lanes past N carry no cell)
   int h = -1;
   if (i < k.N) {
@@ -93,22 +97,18 @@ __device__ inline void hash_count(const dev_consts &k, int i, const double *__re
   if (head && h >= 0) base = atomicAdd(&count[h], next_head - lane);
   base = __shfl(base, my_head, 64);
   if (h >= 0) slot[i] = base + (lane - my_head);
-}
-
-__device__ inline void scan_reduce(int blk, const int *__restrict__ count, int n, int *__restrict__ bsum) {
-  int base = blk * SCAN_CHUNK + threadIdx.x * SCAN_ITEMS;
-  int s = 0;
-#pragma unroll
-  for (int q = 0; q < SCAN_ITEMS; q++)
-    if (base + q < n) s += count[base + q];
-  int total;
-  block_exclusive_scan(s, total);
-  if (threadIdx.x == 0) bsum[blk] = total;
+  // the same for runs of equal chunk ids (a wave's cells mostly lie in one chunk)
+  const int ch = h >= 0 ? h / SCAN_CHUNK : -1;
+  const bool chead = lane == 0 || __shfl_up(ch, 1, 64) != ch;
+  const unsigned long long cheads = __ballot(chead);
+  const unsigned long long cafter = (lane == 63) ? 0ull : (cheads & ~upto);
+  const int cnext = cafter ? __ffsll((long long)cafter) - 1 : 64;
+  if (chead && ch >= 0) atomicAdd(&bsum[ch], cnext - lane);
 }
 
 // cell offsets from the counts; the counts are cleared for the next step. The offset of chunk
-// blk is the sum of the chunk sums before it, added up here (a few hundred values), so no
-// separate scan of the chunk sums is needed.
+// blk is the sum of the chunk sums (counted by hash_count) before it, added up here (a few
+// hundred values), so no separate scan of the chunk sums is needed.
 __device__ inline void scan_down(int blk, int *__restrict__ count, int n, const int *__restrict__ bsum,
                                  int *__restrict__ offsets) {
   int carry_part = 0;
@@ -137,36 +137,36 @@ __device__ inline void scan_down(int blk, int *__restrict__ count, int n, const 
   }
 }
 
+// bucket of every cell (particles in arrival order); block 0 also clears the chunk sums, which
+// scan_down has consumed, for the next step
 __device__ inline void place(int i, int N, const int *__restrict__ key, const int *__restrict__ slot,
-                             const int *__restrict__ offsets, int *__restrict__ bucket) {
+                             const int *__restrict__ offsets, int *__restrict__ bucket, int *__restrict__ bsum,
+                             int nscan) {
+  if (i < 256)
+    for (int q = i; q < nscan; q += 256) bsum[q] = 0;
   if (i >= N) return;
   bucket[offsets[key[i]] + slot[i]] = i;
 }
 
-// stable position: the cell's offset plus the number of particles of the same cell that
-// precede this one in the current order (thrust::sort_by_key on (hash, 0..N-1))
-__device__ inline void rank(int i, int N, const int *__restrict__ key, const int *__restrict__ bucket,
-                            const int *__restrict__ offsets, int *__restrict__ perm, int *__restrict__ key_sorted) {
-  if (i >= N) return;
-  int h = key[i];
-  int b = offsets[h], e = offsets[h + 1];
+// stable position of particle p: the cell's offset plus the number of particles of the same
+// cell that precede it in the current order (thrust::sort_by_key on (hash, 0..N-1)); records
+// the permutation and the sorted cell ids
+__device__ inline int rank(int p, const int *__restrict__ key, const int *__restrict__ bucket,
+                           const int *__restrict__ offsets, int *__restrict__ perm, int *__restrict__ key_sorted) {
+  const int h = key[p];
+  const int b = offsets[h], e = offsets[h + 1];
   int r = 0;
-  for (int q = b; q < e; q++) r += bucket[q] < i;
-  perm[b + r] = i;
+  for (int q = b; q < e; q++) r += bucket[q] < p;
+  perm[b + r] = p;
   key_sorted[b + r] = h;
+  return b + r;
 }
 
 __global__ __launch_bounds__(256) void k_hash_count(dev_consts k, const double *__restrict__ px,
                                                     const double *__restrict__ py, const int *__restrict__ flags,
-                                                    const int *__restrict__ key_sorted, int *__restrict__ key,
-                                                    int *__restrict__ slot, int *__restrict__ count,
-                                                    unsigned long long *__restrict__ clamped) {
-  hash_count(k, blockIdx.x * blockDim.x + threadIdx.x, px, py, flags, key_sorted, key, slot, count, clamped);
-}
-
-__global__ __launch_bounds__(SCAN_THREADS) void k_scan_reduce(const int *__restrict__ count, int n,
-                                                               int *__restrict__ bsum) {
-  scan_reduce(blockIdx.x, count, n, bsum);
+                                                    dbins b) {
+  hash_count(k, blockIdx.x * blockDim.x + threadIdx.x, px, py, flags, b.key_sorted, b.key, b.slot, b.count, b.bsum,
+             b.clamped);
 }
 
 __global__ __launch_bounds__(SCAN_THREADS) void k_scan_down(int *__restrict__ count, int n,
@@ -174,15 +174,8 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan_down(int *__restrict__ co
   scan_down(blockIdx.x, count, n, bsum, offsets);
 }
 
-__global__ __launch_bounds__(256) void k_place(int N, const int *__restrict__ key, const int *__restrict__ slot,
-                                               const int *__restrict__ offsets, int *__restrict__ bucket) {
-  place(blockIdx.x * blockDim.x + threadIdx.x, N, key, slot, offsets, bucket);
-}
-
-__global__ __launch_bounds__(256) void k_rank(int N, const int *__restrict__ key, const int *__restrict__ bucket,
-                                              const int *__restrict__ offsets, int *__restrict__ perm,
-                                              int *__restrict__ key_sorted) {
-  rank(blockIdx.x * blockDim.x + threadIdx.x, N, key, bucket, offsets, perm, key_sorted);
+__global__ __launch_bounds__(256) void k_place(int N, dbins b, int nscan) {
+  place(blockIdx.x * blockDim.x + threadIdx.x, N, b.key, b.slot, b.offsets, b.bucket, b.bsum, nscan);
 }
 
 // ------------------------------------------------------------------------------- predict
@@ -207,11 +200,13 @@ __device__ inline void artificial_stress_f32(float rho, float p, float Sxx, floa
   Ryy = st2 * q1 + ct2 * q2;
 }
 
-// gather into sorted order, leap-frog init + predict, EOS, artificial stress, neighbour record.
-// F32: the record is 16 floats (64 bytes) plus the FP64 predicted position.
+// the sort's last pass (each particle's sorted position, rank) followed by leap-frog init +
+// predict, EOS, artificial stress and the neighbour record written at the sorted position.
+// Runs in the storage order of the start state (its reads are coalesced). F32: the record is
+// 16 floats (64 bytes) plus the FP64 predicted position.
 template <bool F32, bool GATHER>
-__device__ inline void predict(const dev_consts &k, int t, const int *__restrict__ perm, const dstate &cur,
-                               const dstate &nxt, const dpositional &pos, const drecords &recs,
+__device__ inline void predict(const dev_consts &k, int t, const dbins &bins, const dstate &cur, const dstate &nxt,
+                               const dpositional &pos, const drecords &recs,
                                const mfc_tool_state *__restrict__ tool_in, mfc_tool_state *__restrict__ tool_out) {
   double *__restrict__ rec = recs.rec;
   double2 *__restrict__ posd = recs.posd;
@@ -222,9 +217,9 @@ __device__ inline void predict(const dev_consts &k, int t, const int *__restrict
     tool_move(ts, k.dt);
     *tool_out = ts;
   }
-  const int i = t * TILE + threadIdx.x;
-  if (i >= k.N) return;
-  const int s = perm[i];
+  const int s = t * TILE + threadIdx.x;
+  if (s >= k.N) return;
+  const int i = rank(s, bins.key, bins.bucket, bins.offsets, bins.perm, bins.key_sorted);
   const double x0 = cur.px[s], y0 = cur.py[s], u0 = cur.vx[s], v0 = cur.vy[s];
   const double sxx0 = cur.sxx[s], sxy0 = cur.sxy[s], syy0 = cur.syy[s];
   const double rho0 = cur.rho[s], T0 = cur.T[s];
@@ -308,13 +303,13 @@ __device__ inline void predict(const dev_consts &k, int t, const int *__restrict
 }
 
 template <bool F32, bool GATHER>
-__global__ __launch_bounds__(TILE) void k_predict(dev_consts k, int ntiles, const int *__restrict__ perm, dstate cur,
-                                                  dstate nxt, dpositional pos, drecords r,
+__global__ __launch_bounds__(TILE) void k_predict(dev_consts k, int ntiles, dbins b, dstate cur, dstate nxt,
+                                                  dpositional pos, drecords r,
                                                   const mfc_tool_state *__restrict__ tool_in,
                                                   mfc_tool_state *__restrict__ tool_out) {
   const int t = xcd_tile(blockIdx.x, ntiles);
   if (t < 0) return;
-  predict<F32, GATHER>(k, t, perm, cur, nxt, pos, r, tool_in, tool_out);
+  predict<F32, GATHER>(k, t, b, cur, nxt, pos, r, tool_in, tool_out);
 }
 
 // ------------------------------------------------------------------------------- interact
@@ -442,26 +437,30 @@ __device__ inline void sums_f32(const dev_consts &k, int i, const double2 *__res
     if (rr2 >= fourh2 || rr2 < 1e-8f) return false;
     return !(c.f & MFC_FLAG_BLANKED);
   };
-  // the window in groups of WALK_U candidates, the loads of a group issued before its first
-  // candidate is evaluated; the order of evaluation is the candidates' order
+  // the window in groups of candidates, the loads of a group issued before its first candidate
+  // is evaluated; the order of evaluation is the candidates' order
   if constexpr (!COMPACT) {
-    for (int t0 = 0; t0 < w.total; t0 += WALK_U) {
-      int j[WALK_U];
-      cand c[WALK_U];
+    // direct walk: the support tests of a group come first, then the records of the group's
+    // in-support candidates are loaded together, then their pairs are evaluated in order
+    for (int t0 = 0; t0 < w.total; t0 += WALK_U32) {
+      int j[WALK_U32];
+      cand c[WALK_U32];
+      bool in[WALK_U32];
+      float dx[WALK_U32], dy[WALK_U32], rr2[WALK_U32];
+      rec4 r[WALK_U32];
 #pragma unroll
-      for (int u = 0; u < WALK_U; u++) {
+      for (int u = 0; u < WALK_U32; u++) {
         j[u] = w.at(t0 + u < w.total ? t0 + u : w.total - 1);
         load(j[u], c[u]);
       }
 #pragma unroll
-      for (int u = 0; u < WALK_U; u++) {
-        float dx, dy, rr2;
-        if (t0 + u < w.total && visit(c[u], dx, dy, rr2)) {
-          rec4 r;
-          load_rec(j[u], r);
-          pair(dx, dy, rr2, r);
-        }
-      }
+      for (int u = 0; u < WALK_U32; u++) in[u] = t0 + u < w.total && visit(c[u], dx[u], dy[u], rr2[u]);
+#pragma unroll
+      for (int u = 0; u < WALK_U32; u++)
+        if (in[u]) load_rec(j[u], r[u]);
+#pragma unroll
+      for (int u = 0; u < WALK_U32; u++)
+        if (in[u]) pair(dx[u], dy[u], rr2[u], r[u]);
     }
   } else {
     // walk the window collecting the in-support neighbours in a per-lane list, then evaluate
@@ -726,6 +725,7 @@ __device__ inline void interact(const dev_consts &k, int t, const drecords &r, c
   }
   __syncthreads();  // the force partials reuse the stage
   force_partials(t, fsum, fo.partials, reinterpret_cast<double (*)[TILE]>(st.pos));
+  if (t == 0 && threadIdx.x == 0) *fo.pending = 1;
 }
 
 // split, first half: the neighbour sums of every particle into acc (ACC_N arrays of N doubles)
@@ -759,11 +759,15 @@ __device__ inline void update(const dev_consts &k, int t, const double *__restri
   }
   __shared__ double red[4][TILE];
   force_partials(t, fsum, fo.partials, red);
+  if (t == 0 && threadIdx.x == 0) *fo.pending = 1;
 }
 
-// tool force of the step from the tile partials: lane l adds the tile results l, l + TILE, ...
-// in ascending order, then the lane sums are reduced by halving strides; F = sum(fc) + sum(ft)
+// tool force of the step from the tile partials, when the interaction of a step has left them
+// (pending): lane l adds the tile results l, l + TILE, ... in ascending order, then the lane
+// sums are reduced by halving strides; F = sum(fc) + sum(ft). All threads of the work-group
+// call it.
 __device__ inline void force_final(int ntiles, const dforce &fo) {
+  if (*fo.pending == 0) return;  // the same value in every thread
   __shared__ double red[4][TILE];
   double a0 = 0., a1 = 0., a2 = 0., a3 = 0.;
   for (int q = threadIdx.x; q < ntiles; q += TILE) {
@@ -786,6 +790,7 @@ __device__ inline void force_final(int ntiles, const dforce &fo) {
     const int64_t n = *fo.ctr;
     if (n >= 0 && n < fo.cap) { fo.hist[2 * n] = F0; fo.hist[2 * n + 1] = F1; }
     *fo.ctr = n + 1;
+    *fo.pending = 0;
   }
 }
 
@@ -855,7 +860,7 @@ __device__ inline drecords load_recs(MFC_CONST(drecords) * c) {
 }
 __device__ inline dforce load_force(MFC_CONST(dforce) * c) {
   dforce o;
-  MFC_F(partials); MFC_F(force_last); MFC_F(hist); MFC_F(cap); MFC_F(ctr);
+  MFC_F(partials); MFC_F(force_last); MFC_F(hist); MFC_F(cap); MFC_F(ctr); MFC_F(pending);
   return o;
 }
 __device__ inline MFC_CONST(inst_args) * const_args(const inst_args *A, int q) {
@@ -887,16 +892,9 @@ __global__ __launch_bounds__(256) void kb_hash_count(const inst_args *__restrict
   const int2 it = work[w];
   const inst_args a = load_args(A, it.x);
   hash_count(a.k, it.y * 256 + threadIdx.x, a.cur.px, a.cur.py, a.cur.flags, a.b.key_sorted, a.b.key,
-             a.b.slot, a.b.count, a.b.clamped);
-}
-
-__global__ __launch_bounds__(SCAN_THREADS) void kb_scan_reduce(const inst_args *__restrict__ A,
-                                                                const int2 *__restrict__ work, int nwork) {
-  const int w = xcd_tile(blockIdx.x, nwork);
-  if (w < 0) return;
-  const int2 it = work[w];
-  const inst_args a = load_args(A, it.x);
-  scan_reduce(it.y, a.b.count, a.k.num_cell, a.b.bsum);
+             a.b.slot, a.b.count, a.b.bsum, a.b.clamped);
+  // the tool force of the previous step, if its partials are still pending
+  if (it.y == 0) force_final(a.ntiles, a.f);
 }
 
 __global__ __launch_bounds__(SCAN_THREADS) void kb_scan_down(const inst_args *__restrict__ A,
@@ -914,16 +912,7 @@ __global__ __launch_bounds__(256) void kb_place(const inst_args *__restrict__ A,
   if (w < 0) return;
   const int2 it = work[w];
   const inst_args a = load_args(A, it.x);
-  place(it.y * 256 + threadIdx.x, a.k.N, a.b.key, a.b.slot, a.b.offsets, a.b.bucket);
-}
-
-__global__ __launch_bounds__(256) void kb_rank(const inst_args *__restrict__ A, const int2 *__restrict__ work,
-                                               int nwork) {
-  const int w = xcd_tile(blockIdx.x, nwork);
-  if (w < 0) return;
-  const int2 it = work[w];
-  const inst_args a = load_args(A, it.x);
-  rank(it.y * 256 + threadIdx.x, a.k.N, a.b.key, a.b.bucket, a.b.offsets, a.b.perm, a.b.key_sorted);
+  place(it.y * 256 + threadIdx.x, a.k.N, a.b.key, a.b.slot, a.b.offsets, a.b.bucket, a.b.bsum, a.nscan);
 }
 
 template <bool F32>
@@ -937,8 +926,7 @@ __global__ __launch_bounds__(TILE) void kb_predict(const inst_args *__restrict__
   // scalar registers than there are)
   const inst_args &a = A[it.x];
   MFC_CONST(inst_args) *c = const_args(A, it.x);
-  predict<F32, false>(a.k, it.y, load_bins(&c->b).perm, a.cur, a.nxt, a.pos, load_recs(&c->r), a.tool_in,
-                      a.tool_out);
+  predict<F32, false>(a.k, it.y, load_bins(&c->b), a.cur, a.nxt, a.pos, load_recs(&c->r), a.tool_in, a.tool_out);
 }
 
 template <bool F32, bool COMPACT>
@@ -1012,12 +1000,9 @@ int scan_blocks(int num_cell) { return (num_cell + SCAN_CHUNK - 1) / SCAN_CHUNK;
 void launch_binning(const dev_consts &k, const dstate &cur, dbins &b, hipStream_t s) {
   const int g = (k.N + 255) / 256;
   const int nb = scan_blocks(k.num_cell);
-  hipLaunchKernelGGL(k_hash_count, dim3(g), dim3(256), 0, s, k, cur.px, cur.py, cur.flags, b.key_sorted, b.key,
-                     b.slot, b.count, b.clamped);
-  hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(SCAN_THREADS), 0, s, b.count, k.num_cell, b.bsum);
+  hipLaunchKernelGGL(k_hash_count, dim3(g), dim3(256), 0, s, k, cur.px, cur.py, cur.flags, b);
   hipLaunchKernelGGL(k_scan_down, dim3(nb), dim3(SCAN_THREADS), 0, s, b.count, k.num_cell, b.bsum, b.offsets);
-  hipLaunchKernelGGL(k_place, dim3(g), dim3(256), 0, s, k.N, b.key, b.slot, b.offsets, b.bucket);
-  hipLaunchKernelGGL(k_rank, dim3(g), dim3(256), 0, s, k.N, b.key, b.bucket, b.offsets, b.perm, b.key_sorted);
+  hipLaunchKernelGGL(k_place, dim3(g), dim3(256), 0, s, k.N, b, nb);
 }
 
 static int padded_tiles(int ntiles) { return ((ntiles + 7) / 8) * 8; }
@@ -1028,8 +1013,7 @@ void launch_predict(const dev_consts &k, const dstate &cur, const dstate &nxt, c
   const int nt = (k.N + TILE - 1) / TILE;
   auto kern = r.f32 ? (gather ? k_predict<true, true> : k_predict<true, false>)
                     : (gather ? k_predict<false, true> : k_predict<false, false>);
-  hipLaunchKernelGGL(kern, dim3(padded_tiles(nt)), dim3(TILE), 0, s, k, nt, b.perm, cur, nxt, pos, r, tool_in,
-                     tool_out);
+  hipLaunchKernelGGL(kern, dim3(padded_tiles(nt)), dim3(TILE), 0, s, k, nt, b, cur, nxt, pos, r, tool_in, tool_out);
 }
 
 // the four instantiations of a kernel template <F32, COMPACT>
@@ -1085,10 +1069,8 @@ int scan_chunk_cells() { return SCAN_CHUNK; }
 void launch_batch_binning(const batch_work &w, hipStream_t s) {
   const dim3 gt(padded_tiles(w.ntiles)), gc(padded_tiles(w.nchunks));
   hipLaunchKernelGGL(kb_hash_count, gt, dim3(256), 0, s, w.args, w.tiles, w.ntiles);
-  hipLaunchKernelGGL(kb_scan_reduce, gc, dim3(SCAN_THREADS), 0, s, w.args, w.chunks, w.nchunks);
   hipLaunchKernelGGL(kb_scan_down, gc, dim3(SCAN_THREADS), 0, s, w.args, w.chunks, w.nchunks);
   hipLaunchKernelGGL(kb_place, gt, dim3(256), 0, s, w.args, w.tiles, w.ntiles);
-  hipLaunchKernelGGL(kb_rank, gt, dim3(256), 0, s, w.args, w.tiles, w.ntiles);
 }
 
 void launch_batch_predict(const batch_work &w, hipStream_t s) {
@@ -1116,7 +1098,7 @@ void launch_batch_force(const batch_work &w, hipStream_t s) {
 }
 
 void launch_batch_step(const batch_work &w, hipStream_t s) {
-  launch_batch_binning(w, s);
+  launch_batch_binning(w, s);  // reduces the previous step's tool force first
   launch_batch_predict(w, s);
   if (w.v.split) {
     launch_batch_sums(w, s);
@@ -1124,7 +1106,6 @@ void launch_batch_step(const batch_work &w, hipStream_t s) {
   } else {
     launch_batch_interact_fused(w, s);
   }
-  launch_batch_force(w, s);
 }
 
 }  // namespace mfc

@@ -8,8 +8,10 @@ namespace mfc {
 
 constexpr int REC_DOUBLES = 16;  // doubles per FP64 neighbour record
 
-// candidates of the window walk whose loads are issued together
+// candidates of the window walk whose loads are issued together (FP32C direct walk: WALK_U32,
+// with their whole records)
 constexpr int WALK_U = 4;
+constexpr int WALK_U32 = 2;
 
 // where the window walk reads candidate positions and flags: candidate j at [j - base] of pos
 // and fl (the records in memory with base 0, or a tile's copy in LDS)

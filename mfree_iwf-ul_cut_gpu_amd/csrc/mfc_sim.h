@@ -51,11 +51,12 @@ struct mfc_sim {
   mfc_tool_state *tool = nullptr;  // two states, alternating each step
   double *partials = nullptr, *force_last = nullptr, *hist = nullptr;
   int64_t *hist_ctr = nullptr;
-  mfc::dforce force() const { return {partials, force_last, hist, HIST_CAP, hist_ctr}; }
+  int *force_pending = nullptr;  // the last step's tool force is not yet reduced
+  mfc::dforce force() const { return {partials, force_last, hist, HIST_CAP, hist_ctr, force_pending}; }
   int64_t hist_base = 0;           // steps recorded before the last history reset
   int cur = 0;                     // set holding the current state, and index of the current tool state
   int64_t step = 0;
-  hipGraphExec_t graph[2] = {nullptr, nullptr};  // two steps starting from parity 0 / 1
+  hipGraphExec_t graph[2] = {nullptr, nullptr};  // GRAPH_STEPS steps starting from parity 0 / 1
   bool use_graphs = true;
   mfc::adj_work *adj = nullptr;    // reverse-mode tape of the last mfc_vjp_forward
   bool batched = false;            // instance of a batch: stepped by mfc_batch_step only
@@ -91,6 +92,9 @@ void carve_state(void *base, int N, mfc::dstate &d);
 size_t positional_block_bytes(int N);
 void carve_positional(void *base, int N, mfc::dpositional &p);
 // queue one step: set[c] -> set[1-c]
+// steps per captured hipGraph (even: a graph starts and ends on the same buffer parity)
+constexpr int GRAPH_STEPS = 8;
 void enqueue_step(mfc_sim *s, int c);
+void finish_steps(mfc_sim *s);
 // the batched kernels' view of a handle for start parity c
 mfc::inst_args instance_args(const mfc_sim *s, int c);

@@ -27,14 +27,14 @@ def test_batch_equals_single_runs(prec, split, compact):
     cfgs = pymfc.batch_configs(8, seed=11, precision=prec)
     b = pymfc.Batch(cfgs)
     b.set_tuning(split_interact=split, compact_pairs=compact)
-    b.step(7)   # graph pairs plus one eager step
+    b.step(17)  # two captured graphs of 8 steps plus one eager step
     b.step(3)
     b.sync()
     sizes = [inst.N for inst in b.instances]
     assert max(sizes) > 4 * min(sizes)  # uneven
     for i, c in enumerate(cfgs):
         s = pymfc.Sim(c)
-        s.step(10)
+        s.step(20)
         compare_instance(b[i], s, f"instance {i} ({sizes[i]} particles)")
         s.close()
     b.close()

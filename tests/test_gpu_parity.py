@@ -101,7 +101,7 @@ def test_step_parity(case):
 def test_graph_replay_matches_eager():
     cfg = small_cut(thermal="contact")
     g1 = pymfc.Sim(cfg)
-    g1.step(64)  # graph path (pairs of steps)
+    g1.step(64)  # graph path (captured graphs of 8 steps)
     g2 = pymfc.Sim(cfg)
     for _ in range(64):
         g2.step(1)  # eager path
