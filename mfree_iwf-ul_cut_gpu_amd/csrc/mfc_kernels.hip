@@ -331,9 +331,11 @@ struct kahan {
 };
 
 // FP32 neighbour sums with Kahan compensation (the same pair terms as pair_terms, rearranged
-// for single precision: hardware reciprocals and reciprocal square roots instead of divisions, the particle mass and the
-// factor 2 of Brookshaw taken out of the sums, and e.gradW/r of Brookshaw written as der*fac,
-// which is its exact value). Pair distances are formed in FP64 and then rounded.
+// for single precision: hardware reciprocals and reciprocal square roots instead of divisions,
+// the particle mass and the factor 2 of Brookshaw taken out of the sums, e.gradW/r of Brookshaw
+// written as der*fac, which is its exact value, and the stress and viscosity terms of each
+// acceleration component added before their one compensated sum). Pair distances are formed in
+// FP64 and then rounded.
 template <bool COMPACT>
 __device__ inline void sums_f32(const dev_consts &k, int i, const double2 *__restrict__ posd,
                                 const float *__restrict__ rec32, const int *__restrict__ key_sorted,
@@ -356,7 +358,9 @@ __device__ inline void sums_f32(const dev_consts &k, int i, const double2 *__res
   const float inv_h2 = h1 * h1;
   const float pse_c = (float)(4. / M_PI) * inv_h2 * inv_h2;
   const float massf = (float)k.mass;
-  kahan vd0, vd1, vd2, vd3, sd0, sd1, sd2, sd3, vt0, vt1, xt0, xt1, tl, tp;
+  // the stress and viscosity sums enter the acceleration only as sd0 + sd1 + vt0 and
+  // sd2 + sd3 + vt1 (actions_gpu.cu momentum), so they are accumulated as those two sums
+  kahan vd0, vd1, vd2, vd3, ax, ay, xt0, xt1, tl, tp;
   const window w = make_window(k, offsets, key_sorted[i]);
   // the mechanical and Brookshaw terms of one in-support neighbour
   struct rec4 {
@@ -393,7 +397,7 @@ __device__ inline void sums_f32(const dev_consts &k, int i, const double2 *__res
       const float sx = Axx_i + b1.y + fab * (Rxx_i + b2.x);
       const float sy = Axy_i + b1.z + fab * (Rxy_i + b2.y);
       const float sz = Ayy_i + b1.w + fab * (Ryy_i + b2.z);
-      sd0.add(sx * w_x); sd1.add(sy * w_y); sd2.add(sy * w_x); sd3.add(sz * w_y);
+      float fx = __builtin_fmaf(sx, w_x, sy * w_y), fy = __builtin_fmaf(sy, w_x, sz * w_y);
       const float vijx = -du, vijy = -dv;
       const float vdx = __builtin_fmaf(vijx, dx, vijy * dy);
       const float inv_rhoij = 2.0f * __builtin_amdgcn_rcpf(rhoi + rhoj);
@@ -402,36 +406,34 @@ __device__ inline void sums_f32(const dev_consts &k, int i, const double2 *__res
         const float hij = 0.5f * (hi + hj_);
         const float muij = hij * vdx * __builtin_amdgcn_rcpf(__builtin_fmaf(eta2 * hij, hij, rr2));
         const float piij = (-alpha * cij * muij + beta * muij * muij) * inv_rhoij;
-        vt0.add(-piij * w_x); vt1.add(-piij * w_y);
+        fx = __builtin_fmaf(-piij, w_x, fx);
+        fy = __builtin_fmaf(-piij, w_y, fy);
       }
+      ax.add(fx); ay.add(fy);
       const float xs = -xeps * w * inv_rhoij;
       xt0.add(xs * vijx); xt1.add(xs * vijy);
     }
     if (ta != 0.0f) tl.add(volj * (Ti - b3.x) * gr);
   };
-  // what the walk loads per candidate: FP64 position and flags, and, for PSE, temperature and
-  // density
+  // what the walk loads per candidate ahead of its evaluation: FP64 position and flags (the
+  // temperature and density for PSE are read when the candidate is evaluated)
   struct cand {
     double2 p;
     int f;
-    float T, rho;
   };
   auto load = [&](int j, cand &c) {
     c.p = cs.pos[j - cs.base];
     c.f = cs.fl[j - cs.base];
-    if (run_pse) {
-      c.T = rec32[(size_t)16 * j + 12];
-      c.rho = rec32[(size_t)16 * j + 3];
-    }
   };
   // the PSE term of every window member, and the support test; true: j is a mechanical pair
-  auto visit = [&](const cand &c, float &dx, float &dy, float &rr2) -> bool {
+  auto visit = [&](int j, const cand &c, float &dx, float &dy, float &rr2) -> bool {
     dx = (float)(pi.x - c.p.x);
     dy = (float)(pi.y - c.p.y);
     rr2 = __builtin_fmaf(dx, dx, dy * dy);
     if (run_pse) {
+      const float Tj = rec32[(size_t)16 * j + 12], rhoj = rec32[(size_t)16 * j + 3];
       const float w2 = pse_c * __expf(-rr2 * inv_h2);
-      tp.add((c.T - Tpred) * w2 * (massf * __builtin_amdgcn_rcpf(c.rho)));
+      tp.add((Tj - Tpred) * w2 * (massf * __builtin_amdgcn_rcpf(rhoj)));
     }
     if (!run_mech) return false;
     if (rr2 >= fourh2 || rr2 < 1e-8f) return false;
@@ -454,7 +456,7 @@ __device__ inline void sums_f32(const dev_consts &k, int i, const double2 *__res
         load(j[u], c[u]);
       }
 #pragma unroll
-      for (int u = 0; u < WALK_U32; u++) in[u] = t0 + u < w.total && visit(c[u], dx[u], dy[u], rr2[u]);
+      for (int u = 0; u < WALK_U32; u++) in[u] = t0 + u < w.total && visit(j[u], c[u], dx[u], dy[u], rr2[u]);
 #pragma unroll
       for (int u = 0; u < WALK_U32; u++)
         if (in[u]) load_rec(j[u], r[u]);
@@ -482,7 +484,7 @@ __device__ inline void sums_f32(const dev_consts &k, int i, const double2 *__res
 #pragma unroll
         for (int u = 0; u < WALK_U; u++) {
           float dx, dy, rr2;
-          if (t0 + u < w.total && visit(c[u], dx, dy, rr2)) list[cnt++][threadIdx.x] = j[u];
+          if (t0 + u < w.total && visit(j[u], c[u], dx, dy, rr2)) list[cnt++][threadIdx.x] = j[u];
         }
       }
       {  // evaluate the list
@@ -511,9 +513,9 @@ __device__ inline void sums_f32(const dev_consts &k, int i, const double2 *__res
     }
   }
   acc.vd0 = vd0.value(); acc.vd1 = vd1.value(); acc.vd2 = vd2.value(); acc.vd3 = vd3.value();
-  acc.sd0 = k.mass * sd0.value(); acc.sd1 = k.mass * sd1.value();
-  acc.sd2 = k.mass * sd2.value(); acc.sd3 = k.mass * sd3.value();
-  acc.vt0 = k.mass * vt0.value(); acc.vt1 = k.mass * vt1.value();
+  acc.sd0 = k.mass * ax.value(); acc.sd1 = 0.;
+  acc.sd2 = k.mass * ay.value(); acc.sd3 = 0.;
+  acc.vt0 = 0.; acc.vt1 = 0.;
   acc.xt0 = k.mass * xt0.value(); acc.xt1 = k.mass * xt1.value();
   acc.tl = 2.0 * tl.value();
   tpse = tp.value();
@@ -718,9 +720,13 @@ __device__ inline void interact(const dev_consts &k, int t, const drecords &r, c
     const int fl = src.flags[s] & FLAG_MASK;
     pair_acc acc;
     double tpse;
-    // one call site: the candidate loads are flat loads (LDS or global memory)
-    const cand_src cs = lo >= 0 ? cand_src{st.pos, st.fl, lo} : cand_src{r.posd, r.vfl, 0};
-    particle_sums<F32, COMPACT>(k, i, r.rec, r.posd, r.rec32, key_sorted, offsets, cs, fl, acc, tpse);
+    // two call sites, so that each walk reads one kind of memory (LDS or global)
+    if (lo >= 0)
+      particle_sums<F32, COMPACT>(k, i, r.rec, r.posd, r.rec32, key_sorted, offsets, cand_src{st.pos, st.fl, lo}, fl,
+                                  acc, tpse);
+    else
+      particle_sums<F32, COMPACT>(k, i, r.rec, r.posd, r.rec32, key_sorted, offsets, cand_src{r.posd, r.vfl, 0}, fl,
+                                  acc, tpse);
     update_particle(k, i, s, acc, tpse, src, dst, pos, *tool_now, fsum);
   }
   __syncthreads();  // the force partials reuse the stage
@@ -740,8 +746,12 @@ __device__ inline void sums(const dev_consts &k, int t, const drecords &r, const
   const int fl = src_flags[perm[i]] & FLAG_MASK;
   pair_acc a;
   double tpse;
-  const cand_src cs = lo >= 0 ? cand_src{st.pos, st.fl, lo} : cand_src{r.posd, r.vfl, 0};
-  particle_sums<F32, COMPACT>(k, i, r.rec, r.posd, r.rec32, key_sorted, offsets, cs, fl, a, tpse);
+  if (lo >= 0)
+    particle_sums<F32, COMPACT>(k, i, r.rec, r.posd, r.rec32, key_sorted, offsets, cand_src{st.pos, st.fl, lo}, fl, a,
+                                tpse);
+  else
+    particle_sums<F32, COMPACT>(k, i, r.rec, r.posd, r.rec32, key_sorted, offsets, cand_src{r.posd, r.vfl, 0}, fl, a,
+                                tpse);
   store_acc(acc, k.N, i, a, tpse);
 }
 
