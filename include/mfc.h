@@ -221,16 +221,20 @@ int mfc_vjp_backward(mfc_sim *sim, const double cot[2], double grad[6]);
 int mfc_vjp(mfc_sim *sim, mfc_vjp_req *req, double grad[6]);
 
 /* launch configuration of the step kernels (results do not depend on it: every variant
- * evaluates the same expressions in the same order). split_interact = 0: one fused
+ * evaluates the same expressions in the same order). split_interact = 0 (default): one fused
  * interaction kernel; 1: a neighbour-sum kernel and a per-particle update kernel with the sums
  * stored in between (fewer registers per kernel, 224 B more traffic per particle).
- * compact_pairs = 1 (default): each lane first lists its in-support neighbours, then runs the
- * pair terms over the list; 0: the pair terms run inside the window walk. The benchmark's
- * autotuner picks both per workload. */
+ * compact_pairs = 1: each lane first lists its in-support neighbours, then runs the pair terms
+ * over the list; 0 (default): the pair terms run inside the window walk. stage_limit: the most
+ * window candidates a tile of 256 particles copies into LDS (0 = MFC_STAGE_CAP); a tile whose
+ * candidates span more sorted indices reads them from the records in memory. The benchmark's
+ * autotuner picks split_interact and compact_pairs per workload. */
+#define MFC_STAGE_CAP 2048
 typedef struct mfc_tuning {
   int split_interact;
   int compact_pairs;
-  int reserved[6];
+  int stage_limit;
+  int reserved[5];
 } mfc_tuning;
 int mfc_set_tuning(mfc_sim *sim, const mfc_tuning *tuning);
 int mfc_get_tuning(const mfc_sim *sim, mfc_tuning *tuning);

@@ -68,6 +68,14 @@ void carve_positional(void *base, int N, mfc::dpositional &p) {
 
 void enqueue_step(mfc_sim *s, int c) { mfc::launch_batch_step(s->work(c), s->stream); }
 
+int check_tuning(const mfc_tuning *t) {
+  if ((t->split_interact != 0 && t->split_interact != 1) || (t->compact_pairs != 0 && t->compact_pairs != 1))
+    return fail(MFC_E_INVALID, "split_interact and compact_pairs must be 0 or 1");
+  if (t->stage_limit < 0 || t->stage_limit > MFC_STAGE_CAP)
+    return fail(MFC_E_INVALID, "stage_limit must be in 0.." + std::to_string(MFC_STAGE_CAP));
+  return MFC_OK;
+}
+
 // each step's tool force is reduced by the next step's first kernel; this reduces the last one
 void finish_steps(mfc_sim *s) { mfc::launch_batch_force(s->work(s->cur), s->stream); }
 
@@ -251,10 +259,13 @@ int mfc_create(const mfc_config *cfg, mfc_sim **out) {
 int mfc_set_tuning(mfc_sim *s, const mfc_tuning *t) {
   if (!s || !t) return fail(MFC_E_INVALID, "null argument");
   if (s->batched) return fail(MFC_E_INVALID, "instance of a batch: use mfc_batch_set_tuning");
-  if ((t->split_interact != 0 && t->split_interact != 1) || (t->compact_pairs != 0 && t->compact_pairs != 1))
-    return fail(MFC_E_INVALID, "split_interact and compact_pairs must be 0 or 1");
+  int rc = check_tuning(t);
+  if (rc) return rc;
   HIP_TRY(hipStreamSynchronize(s->stream));
   s->tuning = *t;
+  s->k.stage_limit = t->stage_limit;
+  mfc::inst_args a2[2] = {instance_args(s, 0), instance_args(s, 1)};
+  HIP_TRY(hipMemcpy(s->dargs, a2, sizeof(a2), hipMemcpyHostToDevice));
   for (int p = 0; p < 2; p++)
     if (s->graph[p]) {
       (void)hipGraphExecDestroy(s->graph[p]);

@@ -17,7 +17,7 @@ struct mfc_batch {
   std::vector<mfc_sim *> sims;
   hipStream_t stream = nullptr;
   bool f32 = false;
-  mfc_tuning tuning = {0, 0, {0, 0, 0, 0, 0, 0}};
+  mfc_tuning tuning = {0, 0, 0, {0, 0, 0, 0, 0}};
   mfc::inst_args *args[2] = {nullptr, nullptr};  // descriptors per start parity (device)
   int2 *tiles = nullptr, *chunks = nullptr;
   int ntiles = 0, nchunks = 0;
@@ -146,11 +146,21 @@ int mfc_batch_instance(mfc_batch *b, int i, mfc_sim **sim) {
 
 int mfc_batch_set_tuning(mfc_batch *b, const mfc_tuning *t) {
   if (!b || !t) return fail(MFC_E_INVALID, "null argument");
-  if ((t->split_interact != 0 && t->split_interact != 1) || (t->compact_pairs != 0 && t->compact_pairs != 1))
-    return fail(MFC_E_INVALID, "split_interact and compact_pairs must be 0 or 1");
+  int rc = check_tuning(t);
+  if (rc) return rc;
   HIP_TRY(hipStreamSynchronize(b->stream));
   b->tuning = *t;
-  for (mfc_sim *s : b->sims) s->tuning = *t;
+  const size_t n = b->sims.size();
+  std::vector<mfc::inst_args> a0(n), a1(n);
+  for (size_t i = 0; i < n; i++) {
+    mfc_sim *s = b->sims[i];
+    s->tuning = *t;
+    s->k.stage_limit = t->stage_limit;
+    a0[i] = instance_args(s, 0);
+    a1[i] = instance_args(s, 1);
+  }
+  HIP_TRY(hipMemcpy(b->args[0], a0.data(), sizeof(mfc::inst_args) * n, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(b->args[1], a1.data(), sizeof(mfc::inst_args) * n, hipMemcpyHostToDevice));
   b->drop_graphs();
   return MFC_OK;
 }

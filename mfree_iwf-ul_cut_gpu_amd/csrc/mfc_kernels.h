@@ -55,7 +55,7 @@ struct drecords {
 
 // window candidates of a tile that the interaction kernels stage in LDS (positions and flags,
 // 20 bytes each); a tile whose candidates span more sorted indices reads them from memory
-constexpr int STAGE_CAP = 2048;
+constexpr int STAGE_CAP = MFC_STAGE_CAP;
 
 // tool-force reduction: per-tile partials written by the interaction kernel, reduced by the
 // force-final pass (the next step's first kernel, or a kernel of its own after the last step),

@@ -1,10 +1,10 @@
 // mfc_kernels.hip — gfx950 kernels of one SPH cutting step (FP64 and FP32C).
 //
-// Per step (the reference's ~20 launches of leap_frog::step, leap_frog.cu:94-155, become 5):
-//   binning      hash_count, scan_down, place: cell ids (grid.cu:34-51), per-cell and per-chunk
-//                counts, cell offsets and per-cell buckets of a stable counting sort by cell
-//                (thrust::sort_by_key + reorder_data_and_find_cell_start,
-//                grid_gpu_green.cu:22-116,177-201); hash_count's first block also reduces the
+// Per step (the reference's ~20 launches of leap_frog::step, leap_frog.cu:94-155, become 6):
+//   binning      hash_count, scan_reduce, scan_down, place: cell ids (grid.cu:34-51), per-cell
+//                and per-chunk counts, cell offsets and per-cell buckets of a stable counting
+//                sort by cell (thrust::sort_by_key + reorder_data_and_find_cell_start,
+//                grid_gpu_green.cu:22-116,177-201); scan_reduce's first block also reduces the
 //                previous step's tool force (output.cu:30-39) from its tile partials
 //   predict      the sort's rank pass, then leap-frog predict (leap_frog.cu:30-66) + EOS
 //                (actions_gpu.cu:41-51) + artificial stress (:53-97) + neighbour record + tool
@@ -64,13 +64,11 @@ __device__ inline int block_exclusive_scan(int v, int &total) {
 // (index from blockIdx) and by the batched kernel (index from the (instance, block) work table).
 
 // cell id of every particle (grid.cu:34-51; blanked particles keep their last sorted id,
-// grid.cu:39-41), its arrival slot inside the cell, and the particle count of every chunk of
-// SCAN_CHUNK cells (bsum, zero before the step)
+// grid.cu:39-41) and its arrival slot inside the cell
 __device__ inline void hash_count(const dev_consts &k, int i, const double *__restrict__ px,
                                   const double *__restrict__ py, const int *__restrict__ flags,
                                   const int *__restrict__ key_sorted, int *__restrict__ key, int *__restrict__ slot,
-                                  int *__restrict__ count, int *__restrict__ bsum,
-                                  unsigned long long *__restrict__ clamped) {
+                                  int *__restrict__ count, unsigned long long *__restrict__ clamped) {
   // all 64 lanes of the wave take part (lanes past N carry no cell)
   int h = -1;
   if (i < k.N) {
@@ -97,18 +95,23 @@ __device__ inline void hash_count(const dev_consts &k, int i, const double *__re
   if (head && h >= 0) base = atomicAdd(&count[h], next_head - lane);
   base = __shfl(base, my_head, 64);
   if (h >= 0) slot[i] = base + (lane - my_head);
-  // the same for runs of equal chunk ids (a wave's cells mostly lie in one chunk)
-  const int ch = h >= 0 ? h / SCAN_CHUNK : -1;
-  const bool chead = lane == 0 || __shfl_up(ch, 1, 64) != ch;
-  const unsigned long long cheads = __ballot(chead);
-  const unsigned long long cafter = (lane == 63) ? 0ull : (cheads & ~upto);
-  const int cnext = cafter ? __ffsll((long long)cafter) - 1 : 64;
-  if (chead && ch >= 0) atomicAdd(&bsum[ch], cnext - lane);
+}
+
+// particles per chunk of SCAN_CHUNK cells
+__device__ inline void scan_reduce(int blk, const int *__restrict__ count, int n, int *__restrict__ bsum) {
+  const int base = blk * SCAN_CHUNK + threadIdx.x * SCAN_ITEMS;
+  int s = 0;
+#pragma unroll
+  for (int q = 0; q < SCAN_ITEMS; q++)
+    if (base + q < n) s += count[base + q];
+  int total;
+  block_exclusive_scan(s, total);
+  if (threadIdx.x == 0) bsum[blk] = total;
 }
 
 // cell offsets from the counts; the counts are cleared for the next step. The offset of chunk
-// blk is the sum of the chunk sums (counted by hash_count) before it, added up here (a few
-// hundred values), so no separate scan of the chunk sums is needed.
+// blk is the sum of the chunk sums before it, added up here (a few hundred values), so no
+// separate scan of the chunk sums is needed.
 __device__ inline void scan_down(int blk, int *__restrict__ count, int n, const int *__restrict__ bsum,
                                  int *__restrict__ offsets) {
   int carry_part = 0;
@@ -137,13 +140,9 @@ __device__ inline void scan_down(int blk, int *__restrict__ count, int n, const 
   }
 }
 
-// bucket of every cell (particles in arrival order); block 0 also clears the chunk sums, which
-// scan_down has consumed, for the next step
+// bucket of every cell (particles in arrival order)
 __device__ inline void place(int i, int N, const int *__restrict__ key, const int *__restrict__ slot,
-                             const int *__restrict__ offsets, int *__restrict__ bucket, int *__restrict__ bsum,
-                             int nscan) {
-  if (i < 256)
-    for (int q = i; q < nscan; q += 256) bsum[q] = 0;
+                             const int *__restrict__ offsets, int *__restrict__ bucket) {
   if (i >= N) return;
   bucket[offsets[key[i]] + slot[i]] = i;
 }
@@ -165,8 +164,13 @@ __device__ inline int rank(int p, const int *__restrict__ key, const int *__rest
 __global__ __launch_bounds__(256) void k_hash_count(dev_consts k, const double *__restrict__ px,
                                                     const double *__restrict__ py, const int *__restrict__ flags,
                                                     dbins b) {
-  hash_count(k, blockIdx.x * blockDim.x + threadIdx.x, px, py, flags, b.key_sorted, b.key, b.slot, b.count, b.bsum,
+  hash_count(k, blockIdx.x * blockDim.x + threadIdx.x, px, py, flags, b.key_sorted, b.key, b.slot, b.count,
              b.clamped);
+}
+
+__global__ __launch_bounds__(SCAN_THREADS) void k_scan_reduce(const int *__restrict__ count, int n,
+                                                               int *__restrict__ bsum) {
+  scan_reduce(blockIdx.x, count, n, bsum);
 }
 
 __global__ __launch_bounds__(SCAN_THREADS) void k_scan_down(int *__restrict__ count, int n,
@@ -174,8 +178,8 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan_down(int *__restrict__ co
   scan_down(blockIdx.x, count, n, bsum, offsets);
 }
 
-__global__ __launch_bounds__(256) void k_place(int N, dbins b, int nscan) {
-  place(blockIdx.x * blockDim.x + threadIdx.x, N, b.key, b.slot, b.offsets, b.bucket, b.bsum, nscan);
+__global__ __launch_bounds__(256) void k_place(int N, dbins b) {
+  place(blockIdx.x * blockDim.x + threadIdx.x, N, b.key, b.slot, b.offsets, b.bucket);
 }
 
 // ------------------------------------------------------------------------------- predict
@@ -693,8 +697,9 @@ __device__ inline int stage_tile(const dev_consts &k, int t, const drecords &r, 
                                  const int *__restrict__ offsets, stage_buf &st) {
   const int2 sp = tile_span(k, t, key_sorted, offsets);
   const int n = sp.y - sp.x;
+  const int cap = k.stage_limit > 0 && k.stage_limit < STAGE_CAP ? k.stage_limit : STAGE_CAP;
   int lo = -1;
-  if (n <= STAGE_CAP) {
+  if (n <= cap) {
     for (int q = threadIdx.x; q < n; q += TILE) {
       st.pos[q] = r.posd[sp.x + q];
       st.fl[q] = r.vfl[sp.x + q];
@@ -902,7 +907,16 @@ __global__ __launch_bounds__(256) void kb_hash_count(const inst_args *__restrict
   const int2 it = work[w];
   const inst_args a = load_args(A, it.x);
   hash_count(a.k, it.y * 256 + threadIdx.x, a.cur.px, a.cur.py, a.cur.flags, a.b.key_sorted, a.b.key,
-             a.b.slot, a.b.count, a.b.bsum, a.b.clamped);
+             a.b.slot, a.b.count, a.b.clamped);
+}
+
+__global__ __launch_bounds__(SCAN_THREADS) void kb_scan_reduce(const inst_args *__restrict__ A,
+                                                                const int2 *__restrict__ work, int nwork) {
+  const int w = xcd_tile(blockIdx.x, nwork);
+  if (w < 0) return;
+  const int2 it = work[w];
+  const inst_args a = load_args(A, it.x);
+  scan_reduce(it.y, a.b.count, a.k.num_cell, a.b.bsum);
   // the tool force of the previous step, if its partials are still pending
   if (it.y == 0) force_final(a.ntiles, a.f);
 }
@@ -922,7 +936,7 @@ __global__ __launch_bounds__(256) void kb_place(const inst_args *__restrict__ A,
   if (w < 0) return;
   const int2 it = work[w];
   const inst_args a = load_args(A, it.x);
-  place(it.y * 256 + threadIdx.x, a.k.N, a.b.key, a.b.slot, a.b.offsets, a.b.bucket, a.b.bsum, a.nscan);
+  place(it.y * 256 + threadIdx.x, a.k.N, a.b.key, a.b.slot, a.b.offsets, a.b.bucket);
 }
 
 template <bool F32>
@@ -1011,8 +1025,9 @@ void launch_binning(const dev_consts &k, const dstate &cur, dbins &b, hipStream_
   const int g = (k.N + 255) / 256;
   const int nb = scan_blocks(k.num_cell);
   hipLaunchKernelGGL(k_hash_count, dim3(g), dim3(256), 0, s, k, cur.px, cur.py, cur.flags, b);
+  hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(SCAN_THREADS), 0, s, b.count, k.num_cell, b.bsum);
   hipLaunchKernelGGL(k_scan_down, dim3(nb), dim3(SCAN_THREADS), 0, s, b.count, k.num_cell, b.bsum, b.offsets);
-  hipLaunchKernelGGL(k_place, dim3(g), dim3(256), 0, s, k.N, b, nb);
+  hipLaunchKernelGGL(k_place, dim3(g), dim3(256), 0, s, k.N, b);
 }
 
 static int padded_tiles(int ntiles) { return ((ntiles + 7) / 8) * 8; }
@@ -1079,6 +1094,7 @@ int scan_chunk_cells() { return SCAN_CHUNK; }
 void launch_batch_binning(const batch_work &w, hipStream_t s) {
   const dim3 gt(padded_tiles(w.ntiles)), gc(padded_tiles(w.nchunks));
   hipLaunchKernelGGL(kb_hash_count, gt, dim3(256), 0, s, w.args, w.tiles, w.ntiles);
+  hipLaunchKernelGGL(kb_scan_reduce, gc, dim3(SCAN_THREADS), 0, s, w.args, w.chunks, w.nchunks);
   hipLaunchKernelGGL(kb_scan_down, gc, dim3(SCAN_THREADS), 0, s, w.args, w.chunks, w.nchunks);
   hipLaunchKernelGGL(kb_place, gt, dim3(256), 0, s, w.args, w.tiles, w.ntiles);
 }
@@ -1108,7 +1124,7 @@ void launch_batch_force(const batch_work &w, hipStream_t s) {
 }
 
 void launch_batch_step(const batch_work &w, hipStream_t s) {
-  launch_batch_binning(w, s);  // reduces the previous step's tool force first
+  launch_batch_binning(w, s);  // also reduces the previous step's tool force
   launch_batch_predict(w, s);
   if (w.v.split) {
     launch_batch_sums(w, s);

@@ -178,6 +178,7 @@ void make_dev_consts(const sim_constants &c, dev_consts &k) {
   k.pse_quirk = c.pse_quirk;
   k.promote = c.promote;
   k.secant_tol = c.secant_tol;
+  k.stage_limit = 0;
 }
 
 void tool_advance(mfc_tool_state &t, double dt) {

@@ -61,7 +61,7 @@ struct mfc_sim {
   mfc::adj_work *adj = nullptr;    // reverse-mode tape of the last mfc_vjp_forward
   bool batched = false;            // instance of a batch: stepped by mfc_batch_step only
   double *acc = nullptr;           // neighbour sums between the split interaction kernels
-  mfc_tuning tuning = {0, 0, {0, 0, 0, 0, 0, 0}};
+  mfc_tuning tuning = {0, 0, 0, {0, 0, 0, 0, 0}};
   mfc::variant variant() const { return {tuning.split_interact != 0, tuning.compact_pairs != 0}; }
   // the step runs the batched kernels on a batch of one: descriptors of this handle for start
   // parity 0 and 1, and its (instance, block) work tables
@@ -96,5 +96,6 @@ void carve_positional(void *base, int N, mfc::dpositional &p);
 constexpr int GRAPH_STEPS = 8;
 void enqueue_step(mfc_sim *s, int c);
 void finish_steps(mfc_sim *s);
+int check_tuning(const mfc_tuning *t);
 // the batched kernels' view of a handle for start parity c
 mfc::inst_args instance_args(const mfc_sim *s, int c);

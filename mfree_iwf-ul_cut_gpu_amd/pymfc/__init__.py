@@ -107,9 +107,13 @@ class VjpReq(C.Structure):
                 ("cot", C.c_double * 2), ("Fmean", C.c_double * 2)]
 
 
+STAGE_CAP = 2048  # MFC_STAGE_CAP: window candidates a tile can stage in LDS
+
+
 class Tuning(C.Structure):
     """mfc_tuning (include/mfc.h)."""
-    _fields_ = [("split_interact", C.c_int32), ("compact_pairs", C.c_int32), ("reserved", C.c_int32 * 6)]
+    _fields_ = [("split_interact", C.c_int32), ("compact_pairs", C.c_int32), ("stage_limit", C.c_int32),
+                ("reserved", C.c_int32 * 5)]
 
 
 THETA = ("jc_A", "jc_B", "jc_n", "jc_C", "jc_m", "mu")  # order of the adjoint's parameters
@@ -323,7 +327,7 @@ class Sim:
 
     def set_tuning(self, **kw) -> None:
         """Launch configuration (results are independent of it): split_interact=0|1,
-        compact_pairs=0|1."""
+        compact_pairs=0|1, stage_limit=0..STAGE_CAP (0: the full LDS stage)."""
         t = Tuning()
         _check(lib().mfc_get_tuning(self._h, C.byref(t)), "mfc_get_tuning")
         for k, v in kw.items():
@@ -333,7 +337,7 @@ class Sim:
     def tuning(self) -> dict:
         t = Tuning()
         _check(lib().mfc_get_tuning(self._h, C.byref(t)), "mfc_get_tuning")
-        return {"split_interact": t.split_interact, "compact_pairs": t.compact_pairs}
+        return {"split_interact": t.split_interact, "compact_pairs": t.compact_pairs, "stage_limit": t.stage_limit}
 
     def vjp_forward(self, nsteps: int, checkpoint_every: int, w0: int, w1: int) -> np.ndarray:
         """Run nsteps steps keeping a checkpoint every checkpoint_every steps; returns the mean
@@ -462,7 +466,8 @@ class Batch:
             pass
 
     def set_tuning(self, **kw) -> None:
-        """Launch configuration of every instance: split_interact=0|1, compact_pairs=0|1."""
+        """Launch configuration of every instance: split_interact=0|1, compact_pairs=0|1,
+        stage_limit=0..STAGE_CAP."""
         self._tuning.update({k: int(v) for k, v in kw.items()})
         t = Tuning()
         for k, v in self._tuning.items():

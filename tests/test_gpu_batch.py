@@ -55,6 +55,22 @@ def test_c5_batch_of_32_matches_single_runs():
     b.close()
 
 
+def test_batch_unstaged_equals_single_runs():
+    """Every tile of every instance reading its candidates from memory (stage_limit=1) gives the
+    bits of the single runs with the LDS stage."""
+    cfgs = pymfc.batch_configs(4, seed=5, precision=pymfc.PREC_FP64)
+    b = pymfc.Batch(cfgs)
+    b.set_tuning(stage_limit=1)
+    b.step(12)
+    b.sync()
+    for i, c in enumerate(cfgs):
+        s = pymfc.Sim(c)
+        s.step(12)
+        compare_instance(b[i], s, f"instance {i}")
+        s.close()
+    b.close()
+
+
 def test_batch_reverse_mode_equals_single():
     cfgs = []
     for c in pymfc.batch_configs(3, seed=5):

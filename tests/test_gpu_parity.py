@@ -127,6 +127,34 @@ def test_interaction_variants_match(prec, thermal, split, compact):
     assert equal(g1.force_history(), g2.force_history())
 
 
+@pytest.mark.parametrize("prec", [pymfc.PREC_FP64, pymfc.PREC_FP32C])
+@pytest.mark.parametrize("thermal", ["contact", "pse"])
+@pytest.mark.parametrize("split,compact", [(0, 0), (1, 1)])
+def test_unstaged_tiles_match(prec, thermal, split, compact):
+    """A tile copies its window candidates into LDS when their sorted-index span fits the stage;
+    otherwise it reads them from the records in memory. stage_limit=1 sends every tile down the
+    second path: the results are the bits of the staged run (and so, in FP64, the oracle's)."""
+    cfg = small_cut(thermal=thermal, precision=prec)
+    g1, g2 = pymfc.Sim(cfg), pymfc.Sim(cfg)
+    g1.set_tuning(split_interact=split, compact_pairs=compact)
+    g2.set_tuning(split_interact=split, compact_pairs=compact, stage_limit=1)
+    assert g2.tuning()["stage_limit"] == 1
+    g1.step(30)
+    g2.step(30)
+    for f in ["pos", "vel", "S", "T", "eps_pl", "fc", "ft", "flags"]:
+        assert equal(g1.field(f), g2.field(f)), f
+    assert equal(g1.force_history(), g2.force_history())
+
+
+def test_stage_limit_is_checked():
+    g = pymfc.Sim(small_cut())
+    with pytest.raises(pymfc.MfcError):
+        g.set_tuning(stage_limit=pymfc.STAGE_CAP + 1)
+    with pytest.raises(pymfc.MfcError):
+        g.set_tuning(stage_limit=-1)
+    g.set_tuning(stage_limit=pymfc.STAGE_CAP)
+
+
 def test_nan_blanking_matches_oracle():
     cfg = small_cut(thermal="contact")
     g, o = run_pair(cfg, 10)
