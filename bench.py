@@ -35,10 +35,10 @@ INTERACT_BYTES = {"fp64": 128 + _STATE_RW, "fp32c": 64 + 16 + _STATE_RW}
 # the neighbour-sum kernel of the split variant: own record, flags, cell id and permutation entry
 # read, 14 sums written (8 B each)
 SUMS_BYTES = {"fp64": 128 + 4 + 4 + 4 + 14 * 8, "fp32c": 64 + 16 + 4 + 4 + 4 + 14 * 8}
-# HBM bytes per particle of one launch of the fused interaction kernel, measured with
-# rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE on this workload and scaled by a 512 MiB copy
+# HBM bytes per particle of one launch of the fused interaction kernel (direct walk), measured
+# with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE on this workload and scaled by a 512 MiB copy
 # (profiles/traffic_thermal_contact_150k.md)
-MEASURED_TRAFFIC = {("kb_interact", "fp32c"): 603, ("kb_interact", "fp64"): 703}
+MEASURED_TRAFFIC = {("kb_interact", "fp32c"): 590, ("kb_interact", "fp64"): 699}
 # whole step, compulsory traffic of a fully fused FP64 step (persistent read-only 32 B,
 # mutable 224 B read + written, 12 B of binning keys)
 STEP_BYTES_FP64 = 256 + 224 + 12
@@ -290,7 +290,7 @@ def main() -> int:
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": (MEASURED_TRAFFIC[(kname, head)] * N
                                  if args.workload == "thermal_contact_150k" and (kname, head) in MEASURED_TRAFFIC
-                                 else None),
+                                 and not r["tuning"]["compact_pairs"] else None),
                      "bytes_per_particle": kbytes, "ms_per_launch": r["phases"]["interact"]},
         "tuning": r["tuning"],
         "phases_ms": r["phases"],
