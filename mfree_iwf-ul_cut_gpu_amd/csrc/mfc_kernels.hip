@@ -147,19 +147,6 @@ __device__ inline void place(int i, int N, const int *__restrict__ key, const in
   bucket[offsets[key[i]] + slot[i]] = i;
 }
 
-// stable position of particle p: the cell's offset plus the number of particles of the same
-// cell that precede it in the current order (thrust::sort_by_key on (hash, 0..N-1)); records
-// the permutation and the sorted cell ids
-__device__ inline int rank(int p, const int *__restrict__ key, const int *__restrict__ bucket,
-                           const int *__restrict__ offsets, int *__restrict__ perm, int *__restrict__ key_sorted) {
-  const int h = key[p];
-  const int b = offsets[h], e = offsets[h + 1];
-  int r = 0;
-  for (int q = b; q < e; q++) r += bucket[q] < p;
-  perm[b + r] = p;
-  key_sorted[b + r] = h;
-  return b + r;
-}
 
 __global__ __launch_bounds__(256) void k_hash_count(dev_consts k, const double *__restrict__ px,
                                                     const double *__restrict__ py, const int *__restrict__ flags,
@@ -204,10 +191,12 @@ __device__ inline void artificial_stress_f32(float rho, float p, float Sxx, floa
   Ryy = st2 * q1 + ct2 * q2;
 }
 
-// the sort's last pass (each particle's sorted position, rank) followed by leap-frog init +
-// predict, EOS, artificial stress and the neighbour record written at the sorted position.
-// Runs in the storage order of the start state (its reads are coalesced). F32: the record is
-// 16 floats (64 bytes) plus the FP64 predicted position.
+// the sort's last pass, then leap-frog init + predict, EOS, artificial stress and the neighbour
+// record written at the sorted position. Runs in the storage order of the start state (its
+// reads are coalesced). A particle's sorted position is its cell's offset plus the number of
+// particles of the same cell that precede it in the current order (thrust::sort_by_key on
+// (hash, 0..N-1)), counted over the cell's bucket; perm and the sorted cell ids are written
+// there. F32: the record is 16 floats (64 bytes) plus the FP64 predicted position.
 template <bool F32, bool GATHER>
 __device__ inline void predict(const dev_consts &k, int t, const dbins &bins, const dstate &cur, const dstate &nxt,
                                const dpositional &pos, const drecords &recs,
@@ -223,7 +212,10 @@ __device__ inline void predict(const dev_consts &k, int t, const dbins &bins, co
   }
   const int s = t * TILE + threadIdx.x;
   if (s >= k.N) return;
-  const int i = rank(s, bins.key, bins.bucket, bins.offsets, bins.perm, bins.key_sorted);
+  // the cell range of the particle, then the start state (its loads overlap the cell-range
+  // loads), then the rank in the cell's bucket
+  const int hc = bins.key[s];
+  const int cb = bins.offsets[hc], ce = bins.offsets[hc + 1];
   const double x0 = cur.px[s], y0 = cur.py[s], u0 = cur.vx[s], v0 = cur.vy[s];
   const double sxx0 = cur.sxx[s], sxy0 = cur.sxy[s], syy0 = cur.syy[s];
   const double rho0 = cur.rho[s], T0 = cur.T[s];
@@ -232,6 +224,11 @@ __device__ inline void predict(const dev_consts &k, int t, const dbins &bins, co
   const double Tt = cur.Tt[s];
   const int fl = cur.flags[s] & FLAG_MASK;
   const double h = cur.h[s];
+  int rnk = 0;
+  for (int q = cb; q < ce; q++) rnk += bins.bucket[q] < s;
+  const int i = cb + rnk;
+  bins.perm[i] = s;
+  bins.key_sorted[i] = hc;
   if constexpr (GATHER) {  // the sorted copy of the start state (the reverse mode reads it)
     nxt.px[i] = x0; nxt.py[i] = y0; nxt.vx[i] = u0; nxt.vy[i] = v0;
     nxt.sxx[i] = sxx0; nxt.sxy[i] = sxy0; nxt.syy[i] = syy0; nxt.szz[i] = cur.szz[s];
@@ -945,12 +942,13 @@ __global__ __launch_bounds__(TILE) void kb_predict(const inst_args *__restrict__
   const int w = xcd_tile(blockIdx.x, nwork);
   if (w < 0) return;
   const int2 it = work[w];
-  // records and bins through the constant address space (global loads and stores); the state
-  // pointers through generic loads, where they are used (holding all of them would take more
-  // scalar registers than there are)
+  // records, bins and the start state through the constant address space (global loads and
+  // stores); the other pointers through generic loads, where they are used (holding all of
+  // them would take more scalar registers than there are)
   const inst_args &a = A[it.x];
   MFC_CONST(inst_args) *c = const_args(A, it.x);
-  predict<F32, false>(a.k, it.y, load_bins(&c->b), a.cur, a.nxt, a.pos, load_recs(&c->r), a.tool_in, a.tool_out);
+  predict<F32, false>(a.k, it.y, load_bins(&c->b), load_state(&c->cur), a.nxt, a.pos, load_recs(&c->r), a.tool_in,
+                      a.tool_out);
 }
 
 template <bool F32, bool COMPACT>

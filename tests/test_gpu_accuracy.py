@@ -13,14 +13,20 @@ import pymfc
 pytestmark = pytest.mark.gpu
 
 
-def run_both(workload, steps):
+def run_both(workload, steps, every=500):
+    """Both precisions over the same run; the peak temperature is also sampled every `every`
+    steps."""
     out = {}
     for name, prec in (("fp64", pymfc.PREC_FP64), ("fp32c", pymfc.PREC_FP32C)):
         s = pymfc.Sim(pymfc.workload(workload, prec))
-        s.step(steps)
+        peaks = []
+        for _ in range(steps // every):
+            s.step(every)
+            peaks.append(float(s.field("T").max()))
         s.sync()
         out[name] = {f: s.field(f) for f in ("id", "pos", "T", "eps_pl", "S", "flags")}
         out[name]["F"] = s.force_history()
+        out[name]["peaks"] = np.array(peaks)
         s.close()
     return out["fp64"], out["fp32c"]
 
@@ -32,6 +38,9 @@ def errors(a, b, dx, steps):
     err = {
         "force_rel": (np.abs(Fb - Fa) / np.abs(Fa)).tolist(),
         "Tpeak_K": float(abs(b["T"].max() - a["T"].max())),
+        # peak temperature sampled through the second half of the run, then averaged: the
+        # thermal load of the cut, less dependent on which single particle is hottest at the end
+        "Tpeak_mean_K": float(abs(b["peaks"][len(b["peaks"]) // 2:].mean() - a["peaks"][len(a["peaks"]) // 2:].mean())),
         "T_max_abs_K": float(np.abs(b["T"][ib] - a["T"][ia]).max()),
         "pos_max_dx": float(np.abs(b["pos"][ib] - a["pos"][ia]).max() / dx),
         "eps_pl_max_abs": float(np.abs(b["eps_pl"][ib] - a["eps_pl"][ia]).max()),
@@ -51,19 +60,22 @@ def test_fp32c_full_thermal_contact_cut():
     err = errors(a, b, 2e-4, steps)
     print("fp32c vs fp64, thermal_contact_150k, 20000 steps:", err)
     assert err["force_rel"][0] < 5e-3      # cutting force (mean over the second half) within 0.5 %
-    assert err["Tpeak_K"] < 1.0            # peak temperature within 1 K
+    assert err["Tpeak_K"] < 1.0            # peak temperature at the last step within 1 K
+    assert err["Tpeak_mean_K"] < 1.0       # and through the second half of the run
     assert err["blanked"][1] == err["blanked"][0]
 
 
 def test_fp32c_pse_500k_cut():
     """The 1250 x 400 jittered PSE cut at 1.2 um (500,000 particles), 10,000 steps. The 1 K
-    peak-temperature target is the thermal-contact cut's; here the peak differs by about 1 K
-    (one particle's temperature, after 10,000 steps of PSE sums over all 25 window cells)."""
+    peak-temperature target is the thermal-contact cut's; here the hottest particle at the last
+    step differs by about 1 K (after 10,000 steps of PSE sums over all 25 window cells), so the
+    bound on it is 2 K, and the peak through the second half is held to 1 K."""
     steps = 10000
     a, b = run_both("pse_500k", steps)
     err = errors(a, b, 1.2e-4, steps)
     print("fp32c vs fp64, pse_500k, 10000 steps:", err)
     assert err["force_rel"][0] < 5e-3
-    assert err["Tpeak_K"] < 2.0
+    assert err["Tpeak_mean_K"] < 1.0
+    assert err["Tpeak_K"] < 2.0            # one particle's temperature, see below
     assert err["blanked"][1] == err["blanked"][0]
 
