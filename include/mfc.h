@@ -174,11 +174,11 @@ int mfc_step(mfc_sim *sim, int nsteps);
 int mfc_sync(mfc_sim *sim);
 
 /* run nsteps steps without graphs, timing each phase with HIP events on the handle's
- * stream; ms[phase] receives the mean milliseconds per step of: 0 binning (hash + stable
- * counting sort + cell offsets), 1 gather/predict/EOS/artificial stress, 2 the fused
- * interaction kernel (neighbour sums + per-particle update + contact), or the neighbour-sum
- * kernel when split_interact is set, 3 the update kernel (split_interact only, else ~0),
- * 4 tool-force reduction */
+ * stream; ms[phase] receives the mean milliseconds per step of: 0 binning (cell offsets and
+ * counting sort), 1 gather/predict/EOS/artificial stress, 2 the fused interaction kernel
+ * (neighbour sums + per-particle update + contact + the next step's cell ids and counts), or
+ * the neighbour-sum kernel when split_interact is set, 3 the update kernel (split_interact
+ * only, else ~0; it then computes the next step's cell ids), 4 tool-force reduction */
 #define MFC_PROFILE_PHASES 5
 int mfc_profile(mfc_sim *sim, int nsteps, double *ms);
 

@@ -382,7 +382,7 @@ int mfc_vjp_forward(mfc_sim *s, int nsteps, int checkpoint_every, int w0, int w1
   if (!s || !Fmean) return fail(MFC_E_INVALID, "null argument");
   if (s->batched) return fail(MFC_E_INVALID, "instance of a batch: use mfc_batch_vjp_forward");
   int rc = adj_begin(s, nsteps, checkpoint_every, w0, w1);
-  if (rc) return rc;
+  if (rc || (rc = ensure_keys(s))) return rc;
   for (int t = 0; t < nsteps; t++) {
     if ((rc = adj_checkpoint(s, t))) return rc;
     enqueue_step(s, s->cur);
@@ -406,6 +406,10 @@ int mfc_vjp_backward(mfc_sim *s, const double cot[2], double grad[6]) {
   HIP_TRY(hipMemsetAsync(w->lam_n, 0, sizeof(double) * 2 * n, st));
   HIP_TRY(hipMemsetAsync(w->theta_acc, 0, sizeof(double) * 6, st));
   HIP_TRY(hipMemcpyAsync(w->keys_keep, s->bins.key_sorted, sizeof(int) * n, hipMemcpyDeviceToDevice, st));
+  // the replays bin with the handle's cell table, which holds the next step's counts: clear them
+  // (each replayed binning leaves them cleared), and bin the handle's state again at its next step
+  HIP_TRY(hipMemsetAsync(s->bins.count, 0, sizeof(int) * (size_t)k.num_cell, st));
+  s->keys_fresh = false;
   int cur = 0;  // lam_s[cur] holds the adjoint of the end state of the step being reversed
   const double Fb[2] = {cot[0] / (w->w1 - w->w0), cot[1] / (w->w1 - w->w0)};
   // the step kernels run on the replay buffers; cell table and records are the handle's

@@ -68,6 +68,14 @@ void carve_positional(void *base, int N, mfc::dpositional &p) {
 
 void enqueue_step(mfc_sim *s, int c) { mfc::launch_batch_step(s->work(c), s->stream); }
 
+int ensure_keys(mfc_sim *s) {
+  if (s->keys_fresh) return MFC_OK;
+  HIP_TRY(hipMemsetAsync(s->bins.count, 0, sizeof(int) * (size_t)s->k.num_cell, s->stream));
+  mfc::launch_batch_hash(s->work(s->cur), s->stream);
+  s->keys_fresh = true;
+  return MFC_OK;
+}
+
 int check_tuning(const mfc_tuning *t) {
   if ((t->split_interact != 0 && t->split_interact != 1) || (t->compact_pairs != 0 && t->compact_pairs != 1))
     return fail(MFC_E_INVALID, "split_interact and compact_pairs must be 0 or 1");
@@ -283,14 +291,13 @@ int mfc_get_tuning(const mfc_sim *s, mfc_tuning *t) {
 int mfc_step(mfc_sim *s, int nsteps) {
   if (!s || nsteps < 0) return fail(MFC_E_INVALID, "bad argument");
   if (s->batched) return fail(MFC_E_INVALID, "instance of a batch: step it with mfc_batch_step");
+  int rc = ensure_keys(s);
+  if (rc) return rc;
   int done = 0;
   if (s->use_graphs && nsteps >= GRAPH_STEPS) {
     while (nsteps - done >= GRAPH_STEPS) {
       int par = s->cur;
-      if (!s->graph[par]) {
-        int rc = build_graph(s, par);
-        if (rc) return rc;
-      }
+      if (!s->graph[par] && (rc = build_graph(s, par))) return rc;
       HIP_TRY(hipGraphLaunch(s->graph[par], s->stream));
       done += GRAPH_STEPS;
       s->step += GRAPH_STEPS;
@@ -313,7 +320,7 @@ int mfc_profile(mfc_sim *s, int nsteps, double *ms) {
   hipEvent_t ev[NE];
   for (int q = 0; q < NE; q++) HIP_TRY(hipEventCreate(&ev[q]));
   double acc[MFC_PROFILE_PHASES] = {0., 0., 0., 0., 0.};
-  int rc = MFC_OK;
+  int rc = ensure_keys(s);
   for (int it = 0; it < nsteps && rc == MFC_OK; it++) {
     const int c = s->cur, n = 1 - c;
     const mfc::batch_work w = s->work(c);
@@ -467,6 +474,7 @@ int mfc_set_field(mfc_sim *s, int field, const void *host, size_t bytes) {
   const size_t es = field_is_int(field) ? sizeof(int) : sizeof(double);
   if (bytes != (size_t)N * nc * es) return fail(MFC_E_RANGE, "buffer size does not match the field");
   HIP_TRY(hipStreamSynchronize(s->stream));
+  s->keys_fresh = false;  // positions and flags decide the binning
   if (nc == 1) {
     HIP_TRY(hipMemcpy(ptr[0], host, es * N, hipMemcpyHostToDevice));
     return MFC_OK;

@@ -68,6 +68,19 @@ int build_batch_graph(mfc_batch *b, int parity) {
   return MFC_OK;
 }
 
+// the binning of every instance's current state (computed by the last step's interaction),
+// recomputed for all instances when any of them lacks it
+int batch_keys(mfc_batch *b) {
+  bool fresh = true;
+  for (mfc_sim *s : b->sims) fresh = fresh && s->keys_fresh;
+  if (fresh) return MFC_OK;
+  for (mfc_sim *s : b->sims)
+    HIP_TRY(hipMemsetAsync(s->bins.count, 0, sizeof(int) * (size_t)s->k.num_cell, b->stream));
+  mfc::launch_batch_hash(b->work(b->cur), b->stream);
+  for (mfc_sim *s : b->sims) s->keys_fresh = true;
+  return MFC_OK;
+}
+
 void advance(mfc_batch *b) {
   b->cur = 1 - b->cur;
   b->step++;
@@ -167,14 +180,13 @@ int mfc_batch_set_tuning(mfc_batch *b, const mfc_tuning *t) {
 
 int mfc_batch_step(mfc_batch *b, int nsteps) {
   if (!b || nsteps < 0) return fail(MFC_E_INVALID, "bad argument");
+  int rc = batch_keys(b);
+  if (rc) return rc;
   int done = 0;
   if (nsteps >= GRAPH_STEPS) {
     while (nsteps - done >= GRAPH_STEPS) {
       const int par = b->cur;
-      if (!b->graph[par]) {
-        int rc = build_batch_graph(b, par);
-        if (rc) return rc;
-      }
+      if (!b->graph[par] && (rc = build_batch_graph(b, par))) return rc;
       HIP_TRY(hipGraphLaunch(b->graph[par], b->stream));
       for (int q = 0; q < GRAPH_STEPS; q++) advance(b);
       done += GRAPH_STEPS;
@@ -200,6 +212,7 @@ int mfc_batch_vjp_forward(mfc_batch *b, int nsteps, int checkpoint_every, int w0
   int rc;
   for (mfc_sim *s : b->sims)
     if ((rc = mfc::adj_begin(s, nsteps, checkpoint_every, w0, w1))) return rc;
+  if ((rc = batch_keys(b))) return rc;
   for (int t = 0; t < nsteps; t++) {
     for (mfc_sim *s : b->sims)
       if ((rc = mfc::adj_checkpoint(s, t))) return rc;

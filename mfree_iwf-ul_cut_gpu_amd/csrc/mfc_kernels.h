@@ -109,8 +109,11 @@ struct batch_work {
   variant v;
 };
 
-// one step of every instance of a batch: 9 (fused) or 10 (split) launches in all, and its phases
+// one step of every instance of a batch: 5 (fused) or 6 (split) launches, and its phases. The
+// cell ids, arrival slots and cell counts of a step's start state are computed by the previous
+// step's interaction; launch_batch_hash computes them from the state itself (counts zeroed before)
 void launch_batch_step(const batch_work &w, hipStream_t s);
+void launch_batch_hash(const batch_work &w, hipStream_t s);
 void launch_batch_binning(const batch_work &w, hipStream_t s);
 void launch_batch_predict(const batch_work &w, hipStream_t s);
 void launch_batch_interact_fused(const batch_work &w, hipStream_t s);

@@ -63,26 +63,24 @@ __device__ inline int block_exclusive_scan(int v, int &total) {
 // Each kernel body is a device function of its work index, called by the single-instance kernel
 // (index from blockIdx) and by the batched kernel (index from the (instance, block) work table).
 
-// cell id of every particle (grid.cu:34-51; blanked particles keep their last sorted id,
-// grid.cu:39-41) and its arrival slot inside the cell
-__device__ inline void hash_count(const dev_consts &k, int i, const double *__restrict__ px,
-                                  const double *__restrict__ py, const int *__restrict__ flags,
-                                  const int *__restrict__ key_sorted, int *__restrict__ key, int *__restrict__ slot,
-                                  int *__restrict__ count, unsigned long long *__restrict__ clamped) {
-  // all 64 lanes of the wave take part (lanes past N carry no cell)
-  int h = -1;
-  if (i < k.N) {
-    if (flags[i] & MFC_FLAG_BLANKED) {
-      h = key_sorted[i];
-    } else {
-      bool cl;
-      h = cell_of(k, px[i], py[i], cl);
-      if (cl) atomicAdd(clamped, 1ull);
-    }
-    key[i] = h;
-  }
-  // the storage order is the previous step's cell order, so neighbouring lanes mostly share a
-  // cell: each run of equal cell ids in the wave takes its slots with one atomic
+// cell id of a particle for the binning (grid.cu:34-51; blanked particles keep their last
+// sorted id, grid.cu:39-41); clamped ids are counted
+__device__ inline int particle_cell(const dev_consts &k, double x, double y, int fl, int sorted_id,
+                                    unsigned long long *__restrict__ clamped) {
+  if (fl & MFC_FLAG_BLANKED) return sorted_id;
+  bool cl;
+  const int h = cell_of(k, x, y, cl);
+  if (cl) atomicAdd(clamped, 1ull);
+  return h;
+}
+
+// records cell id h of particle i (storage order) and takes its arrival slot in the cell.
+// All 64 lanes of the wave take part (lanes past N carry h = -1). The storage order is the
+// previous step's cell order, so neighbouring lanes mostly share a cell: each run of equal
+// cell ids in the wave takes its slots with one atomic.
+__device__ inline void count_cell(int i, int h, int *__restrict__ key, int *__restrict__ slot,
+                                  int *__restrict__ count) {
+  if (h >= 0) key[i] = h;
   const int lane = threadIdx.x & 63;
   const int prev = __shfl_up(h, 1, 64);
   const bool head = lane == 0 || prev != h;
@@ -95,6 +93,15 @@ __device__ inline void hash_count(const dev_consts &k, int i, const double *__re
   if (head && h >= 0) base = atomicAdd(&count[h], next_head - lane);
   base = __shfl(base, my_head, 64);
   if (h >= 0) slot[i] = base + (lane - my_head);
+}
+
+// cell id and arrival slot of every particle of the start state
+__device__ inline void hash_count(const dev_consts &k, int i, const double *__restrict__ px,
+                                  const double *__restrict__ py, const int *__restrict__ flags,
+                                  const int *__restrict__ key_sorted, int *__restrict__ key, int *__restrict__ slot,
+                                  int *__restrict__ count, unsigned long long *__restrict__ clamped) {
+  const int h = i < k.N ? particle_cell(k, px[i], py[i], flags[i], key_sorted[i], clamped) : -1;
+  count_cell(i, h, key, slot, count);
 }
 
 // particles per chunk of SCAN_CHUNK cells
@@ -528,7 +535,8 @@ __device__ inline void sums_f32(const dev_consts &k, int i, const double2 *__res
 // particle to position i of dst: the gather into sorted order happens here, once
 __device__ inline void update_particle(const dev_consts &k, int i, int s, const pair_acc &acc, double tpse,
                                        const dstate &src, const dstate &dst, const dpositional &pos,
-                                       const mfc_tool_state &tool_now, double fsum[4]) {
+                                       const mfc_tool_state &tool_now, double fsum[4], double &x_out, double &y_out,
+                                       int &fl_out) {
   const bool pse = k.pse != 0;
   int fl = src.flags[s] & FLAG_MASK;
   const bool blank = fl & MFC_FLAG_BLANKED, tool = fl & MFC_FLAG_TOOL, fixed = fl & MFC_FLAG_FIXED;
@@ -621,6 +629,7 @@ __device__ inline void update_particle(const dev_consts &k, int i, int s, const 
   dst.flags[i] = fl;
   dst.id[i] = src.id[s];
   fsum[0] = fcx; fsum[1] = fcy; fsum[2] = ftx; fsum[3] = fty;
+  x_out = x; y_out = y; fl_out = fl;
 }
 
 // the neighbour sums of particle i as pair_acc + PSE sum
@@ -707,16 +716,28 @@ __device__ inline int stage_tile(const dev_consts &k, int t, const drecords &r, 
   return lo;
 }
 
+// the next step's first binning pass for the updated particle i (its storage position in the
+// next step): cell id and arrival slot, as hash_count would take them from the updated state;
+// next == nullptr: the next step runs hash_count itself
+__device__ inline void next_cell(const dev_consts &k, int i, bool live, double x, double y, int fl,
+                                 const dbins *next) {
+  if (next == nullptr) return;  // the same in every thread
+  const int h = live ? particle_cell(k, x, y, fl, next->key_sorted[i], next->clamped) : -1;
+  count_cell(i, h, next->key, next->slot, next->count);
+}
+
 // fused: sums + update in one pass (registers of both live in one kernel)
 template <bool F32, bool COMPACT>
 __device__ inline void interact(const dev_consts &k, int t, const drecords &r, const int *__restrict__ key_sorted,
                                 const int *__restrict__ offsets, const int *__restrict__ perm, const dstate &src,
                                 const dstate &dst, const dpositional &pos,
-                                const mfc_tool_state *__restrict__ tool_now, const dforce &fo) {
+                                const mfc_tool_state *__restrict__ tool_now, const dforce &fo, const dbins *next) {
   __shared__ stage_buf st;
   const int lo = stage_tile(k, t, r, key_sorted, offsets, st);
   const int i = t * TILE + threadIdx.x;
   double fsum[4] = {0., 0., 0., 0.};
+  double xn = 0., yn = 0.;
+  int fln = 0;
   if (i < k.N) {
     const int s = perm[i];
     const int fl = src.flags[s] & FLAG_MASK;
@@ -729,8 +750,9 @@ __device__ inline void interact(const dev_consts &k, int t, const drecords &r, c
     else
       particle_sums<F32, COMPACT>(k, i, r.rec, r.posd, r.rec32, key_sorted, offsets, cand_src{r.posd, r.vfl, 0}, fl,
                                   acc, tpse);
-    update_particle(k, i, s, acc, tpse, src, dst, pos, *tool_now, fsum);
+    update_particle(k, i, s, acc, tpse, src, dst, pos, *tool_now, fsum, xn, yn, fln);
   }
+  next_cell(k, i, i < k.N, xn, yn, fln, next);
   __syncthreads();  // the force partials reuse the stage
   force_partials(t, fsum, fo.partials, reinterpret_cast<double (*)[TILE]>(st.pos));
   if (t == 0 && threadIdx.x == 0) *fo.pending = 1;
@@ -760,15 +782,19 @@ __device__ inline void sums(const dev_consts &k, int t, const drecords &r, const
 // split, second half: the per-particle update from the stored sums
 __device__ inline void update(const dev_consts &k, int t, const double *__restrict__ acc,
                               const int *__restrict__ perm, const dstate &src, const dstate &dst,
-                              const dpositional &pos, const mfc_tool_state *__restrict__ tool_now, const dforce &fo) {
+                              const dpositional &pos, const mfc_tool_state *__restrict__ tool_now, const dforce &fo,
+                              const dbins *next) {
   const int i = t * TILE + threadIdx.x;
   double fsum[4] = {0., 0., 0., 0.};
+  double xn = 0., yn = 0.;
+  int fln = 0;
   if (i < k.N) {
     pair_acc a;
     double tpse;
     load_acc(acc, k.N, i, a, tpse);
-    update_particle(k, i, perm[i], a, tpse, src, dst, pos, *tool_now, fsum);
+    update_particle(k, i, perm[i], a, tpse, src, dst, pos, *tool_now, fsum, xn, yn, fln);
   }
+  next_cell(k, i, i < k.N, xn, yn, fln, next);
   __shared__ double red[4][TILE];
   force_partials(t, fsum, fo.partials, red);
   if (t == 0 && threadIdx.x == 0) *fo.pending = 1;
@@ -814,7 +840,7 @@ __global__ __launch_bounds__(TILE) SUMS_OCCUPANCY void k_interact(dev_consts k, 
                                                    const mfc_tool_state *__restrict__ tool_now, dforce fo) {
   const int t = xcd_tile(blockIdx.x, ntiles);
   if (t < 0) return;
-  interact<F32, COMPACT>(k, t, r, key_sorted, offsets, perm, src, dst, pos, tool_now, fo);
+  interact<F32, COMPACT>(k, t, r, key_sorted, offsets, perm, src, dst, pos, tool_now, fo, nullptr);
 }
 
 template <bool F32, bool COMPACT>
@@ -833,7 +859,7 @@ __global__ __launch_bounds__(TILE) void k_update(dev_consts k, int ntiles, const
                                                  dforce fo) {
   const int t = xcd_tile(blockIdx.x, ntiles);
   if (t < 0) return;
-  update(k, t, acc, perm, src, dst, pos, tool_now, fo);
+  update(k, t, acc, perm, src, dst, pos, tool_now, fo, nullptr);
 }
 
 __global__ __launch_bounds__(TILE) void k_force_final(int ntiles, dforce fo) { force_final(ntiles, fo); }
@@ -961,7 +987,7 @@ __global__ __launch_bounds__(TILE) SUMS_OCCUPANCY void kb_interact(const inst_ar
   MFC_CONST(inst_args) *c = const_args(A, it.x);
   const dbins b = load_bins(&c->b);
   interact<F32, COMPACT>(a.k, it.y, load_recs(&c->r), b.key_sorted, b.offsets, b.perm, a.cur, a.nxt, a.pos,
-                         a.tool_out, a.f);
+                         a.tool_out, a.f, &b);
 }
 
 template <bool F32, bool COMPACT>
@@ -983,7 +1009,8 @@ __global__ __launch_bounds__(TILE) void kb_update(const inst_args *__restrict__ 
   const int2 it = work[w];
   const inst_args &a = A[it.x];
   MFC_CONST(inst_args) *c = const_args(A, it.x);
-  update(a.k, it.y, c->acc, load_bins(&c->b).perm, a.cur, a.nxt, a.pos, a.tool_out, a.f);
+  const dbins b = load_bins(&c->b);
+  update(a.k, it.y, c->acc, b.perm, a.cur, a.nxt, a.pos, a.tool_out, a.f, &b);
 }
 
 __global__ __launch_bounds__(TILE) void kb_force_final(const inst_args *__restrict__ A) {
@@ -1089,9 +1116,12 @@ void launch_device_math(int fn, const double *x, const double *y, double *out, i
 
 int scan_chunk_cells() { return SCAN_CHUNK; }
 
+void launch_batch_hash(const batch_work &w, hipStream_t s) {
+  hipLaunchKernelGGL(kb_hash_count, dim3(padded_tiles(w.ntiles)), dim3(256), 0, s, w.args, w.tiles, w.ntiles);
+}
+
 void launch_batch_binning(const batch_work &w, hipStream_t s) {
   const dim3 gt(padded_tiles(w.ntiles)), gc(padded_tiles(w.nchunks));
-  hipLaunchKernelGGL(kb_hash_count, gt, dim3(256), 0, s, w.args, w.tiles, w.ntiles);
   hipLaunchKernelGGL(kb_scan_reduce, gc, dim3(SCAN_THREADS), 0, s, w.args, w.chunks, w.nchunks);
   hipLaunchKernelGGL(kb_scan_down, gc, dim3(SCAN_THREADS), 0, s, w.args, w.chunks, w.nchunks);
   hipLaunchKernelGGL(kb_place, gt, dim3(256), 0, s, w.args, w.tiles, w.ntiles);
@@ -1121,6 +1151,8 @@ void launch_batch_force(const batch_work &w, hipStream_t s) {
   hipLaunchKernelGGL(kb_force_final, dim3(w.n), dim3(TILE), 0, s, w.args);
 }
 
+// one step; the cell ids and counts of its start state come from the previous step's interaction
+// (or launch_batch_hash), and its own interaction computes those of the next step
 void launch_batch_step(const batch_work &w, hipStream_t s) {
   launch_batch_binning(w, s);  // also reduces the previous step's tool force
   launch_batch_predict(w, s);

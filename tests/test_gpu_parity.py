@@ -171,6 +171,29 @@ def test_nan_blanking_matches_oracle():
     assert_bitwise(g, o, context="nan blanking")
 
 
+def test_moved_particle_rebins():
+    """A position written between steps is binned by the next step: the cell ids of a step's
+    start state are computed by the previous step's interaction, so writing the state has to
+    bin it again. The highest free particle is lifted by one cell, out of its cell."""
+    cfg = small_cut(thermal="contact")
+    g, o = run_pair(cfg, 10)
+    pos, fl, ids = o.field("pos"), o.field("flags"), o.field("id")
+    free = np.nonzero(fl == 0)[0]
+    top = free[np.argmax(pos[free, 1])]
+    before = o.field("hash")[top]
+    pos[top, 1] += g.info.cell_size
+    g.set_field("pos", pos)
+    o.set_field("pos", pos)
+    g.step(1)
+    o.step(1)
+    after = o.field("hash")[np.nonzero(o.field("id") == ids[top])[0][0]]
+    assert after != before
+    assert_bitwise(g, o, context="moved particle, first step")
+    g.step(19)
+    o.step(19)
+    assert_bitwise(g, o, context="moved particle")
+
+
 def test_adiabatic_6k_full_run():
     """The 150 x 40 FP64 adiabatic cut: 2000 steps, GPU vs oracle."""
     cfg = pymfc.workload("adiabatic_6k")
