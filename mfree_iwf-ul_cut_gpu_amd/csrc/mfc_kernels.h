@@ -39,7 +39,6 @@ struct dpositional {
 
 constexpr int REC = 16;    // doubles per neighbour record
 constexpr int ACC_N = 14;  // neighbour sums per particle
-constexpr int PAIR_LIST = 32;  // per-lane list of in-support neighbours (compacted pair loop)
 
 // neighbour records written by the predict kernel and read by the interaction kernel, in
 // sorted order: posd (FP64 predicted position) and vfl (flags of the record) for the window

@@ -322,9 +322,8 @@ __global__ __launch_bounds__(TILE) void k_predict(dev_consts k, int ntiles, dbin
 
 // ------------------------------------------------------------------------------- interact
 // the sums kernels keep at least 3 waves per SIMD (at most 168 VGPRs): a 150k-particle step
-// has about 3 waves per SIMD in all, so all of them are resident at once. The compacted
-// variant's LDS (candidate stage and pair lists, 72 KB per work-group) allows 2.
-#define SUMS_OCCUPANCY __attribute__((amdgpu_waves_per_eu(COMPACT ? 2 : 3)))
+// has about 3 waves per SIMD in all, so all of them are resident at once
+#define SUMS_OCCUPANCY __attribute__((amdgpu_waves_per_eu(3)))
 
 // compensated (Kahan) accumulation in single precision
 struct kahan {
@@ -473,51 +472,47 @@ __device__ inline void sums_f32(const dev_consts &k, int i, const double2 *__res
         if (in[u]) pair(dx[u], dy[u], rr2[u], r[u]);
     }
   } else {
-    // walk the window collecting the in-support neighbours in a per-lane list, then evaluate
-    // the list: the wave runs the pair body about as often as its busiest lane has pairs,
-    // instead of once per window candidate
-    __shared__ int list[PAIR_LIST][TILE];
-    int cnt = 0, t0 = 0;
-    // collect until the list could overflow or the window is done (each lane on its own),
-    // then evaluate the lists of all lanes together
-    for (;;) {
-      for (; t0 < w.total && cnt <= PAIR_LIST - WALK_U; t0 += WALK_U) {
+    // masked walk: the support tests of up to 64 candidates set the bits of a per-lane mask,
+    // then the candidates of the set bits are evaluated in candidate order, the loads of the
+    // next one issued before the current one is evaluated. The wave runs the pair body about as
+    // often as its busiest lane has pairs instead of once per window candidate, with no list in
+    // memory (the sorted index of a bit is recomputed by window::at).
+    for (int c0 = 0; c0 < w.total; c0 += 64) {
+      const int n = w.total - c0 < 64 ? w.total - c0 : 64;
+      unsigned long long m = 0ull;
+      for (int u0 = 0; u0 < n; u0 += WALK_U) {
         int j[WALK_U];
         cand c[WALK_U];
 #pragma unroll
         for (int u = 0; u < WALK_U; u++) {
-          j[u] = w.at(t0 + u < w.total ? t0 + u : w.total - 1);
+          j[u] = w.at(c0 + (u0 + u < n ? u0 + u : n - 1));
           load(j[u], c[u]);
         }
 #pragma unroll
         for (int u = 0; u < WALK_U; u++) {
           float dx, dy, rr2;
-          if (t0 + u < w.total && visit(j[u], c[u], dx, dy, rr2)) list[cnt++][threadIdx.x] = j[u];
+          if (u0 + u < n && visit(j[u], c[u], dx, dy, rr2)) m |= 1ull << (u0 + u);
         }
       }
-      {  // evaluate the list
-        // the loads of pair q + 1 are issued before pair q is evaluated
-        double2 pn;
-        rec4 rn;
-        if (cnt > 0) {
-          const int jj = list[0][threadIdx.x];
+      double2 pn;
+      rec4 rn;
+      if (m) {
+        const int jj = w.at(c0 + __builtin_ctzll(m));
+        pn = cs.pos[jj - cs.base];
+        load_rec(jj, rn);
+      }
+      while (m) {
+        m &= m - 1ull;
+        const double2 pj = pn;
+        const rec4 r = rn;
+        if (m) {
+          const int jj = w.at(c0 + __builtin_ctzll(m));
           pn = cs.pos[jj - cs.base];
           load_rec(jj, rn);
         }
-        for (int q = 0; q < cnt; q++) {
-          const double2 pj = pn;
-          const rec4 r = rn;
-          if (q + 1 < cnt) {
-            const int jj = list[q + 1][threadIdx.x];
-            pn = cs.pos[jj - cs.base];
-            load_rec(jj, rn);
-          }
-          const float dx = (float)(pi.x - pj.x), dy = (float)(pi.y - pj.y);
-          pair(dx, dy, __builtin_fmaf(dx, dx, dy * dy), r);
-        }
-        cnt = 0;
+        const float dx = (float)(pi.x - pj.x), dy = (float)(pi.y - pj.y);
+        pair(dx, dy, __builtin_fmaf(dx, dx, dy * dy), r);
       }
-      if (t0 >= w.total) break;
     }
   }
   acc.vd0 = vd0.value(); acc.vd1 = vd1.value(); acc.vd2 = vd2.value(); acc.vd3 = vd3.value();

@@ -165,9 +165,10 @@ __device__ inline void pair_terms(const dev_consts &k, pair_acc &a, double xi, d
 
 // FP64 neighbour sums of particle i: the reference's loop over the 5 x 5 cell window
 // (interactions_gpu.cu:219-331), walked as 5 contiguous column ranges of the sorted order.
-// COMPACT: the window is walked first, collecting the neighbours that contribute in a per-lane
-// list (in the same order), and the list is evaluated afterwards, so that a wave runs the pair
-// body about as often as its busiest lane has pairs instead of once per window candidate.
+// COMPACT: the window is walked first, marking the neighbours that contribute in a per-lane
+// bit mask, and the marked ones are evaluated afterwards in the same order, so that a wave runs
+// the pair body about as often as its busiest lane has pairs instead of once per window
+// candidate.
 template <bool COMPACT>
 __device__ inline void sums_f64(const dev_consts &k, int i, const double *__restrict__ rec,
                                 const int *__restrict__ key_sorted, const int *__restrict__ offsets,
@@ -236,32 +237,30 @@ __device__ inline void sums_f64(const dev_consts &k, int i, const double *__rest
         if (t0 + u < w.total && visit(j[u], c[u])) pair(j[u], c[u].p.x, c[u].p.y, c[u].f);
     }
   } else {
-    __shared__ int list[PAIR_LIST][TILE];
-    int cnt = 0, t0 = 0;
-    // collect until the list could overflow or the window is done (each lane on its own),
-    // then evaluate the lists of all lanes together
-    for (;;) {
-      for (; t0 < w.total && cnt <= PAIR_LIST - WALK_U; t0 += WALK_U) {
+    // masked walk: the tests of up to 64 candidates set the bits of a per-lane mask, then the
+    // candidates of the set bits are evaluated in candidate order (the sorted index of a bit is
+    // recomputed by window::at, so no list is kept in memory)
+    for (int c0 = 0; c0 < w.total; c0 += 64) {
+      const int n = w.total - c0 < 64 ? w.total - c0 : 64;
+      unsigned long long m = 0ull;
+      for (int u0 = 0; u0 < n; u0 += WALK_U) {
         int j[WALK_U];
         cand c[WALK_U];
 #pragma unroll
         for (int u = 0; u < WALK_U; u++) {
-          j[u] = w.at(t0 + u < w.total ? t0 + u : w.total - 1);
+          j[u] = w.at(c0 + (u0 + u < n ? u0 + u : n - 1));
           load(j[u], c[u]);
         }
 #pragma unroll
         for (int u = 0; u < WALK_U; u++)
-          if (t0 + u < w.total && visit(j[u], c[u])) list[cnt++][threadIdx.x] = j[u];
+          if (u0 + u < n && visit(j[u], c[u])) m |= 1ull << (u0 + u);
       }
-      {  // evaluate the list
-        for (int q = 0; q < cnt; q++) {
-          const int jj = list[q][threadIdx.x];
-          const double2 p = cs.pos[jj - cs.base];
-          pair(jj, p.x, p.y, cs.fl[jj - cs.base]);
-        }
-        cnt = 0;
+      while (m) {
+        const int jj = w.at(c0 + __builtin_ctzll(m));
+        m &= m - 1ull;
+        const double2 p = cs.pos[jj - cs.base];
+        pair(jj, p.x, p.y, cs.fl[jj - cs.base]);
       }
-      if (t0 >= w.total) break;
     }
   }
 }
