@@ -30,6 +30,19 @@ __device__ inline int xcd_tile(int b, int ntiles) {
   return t < ntiles ? t : -1;
 }
 
+// The interaction kernels' map: the tiles in chunks of XCD_CHUNK consecutive tiles, dealt to
+// the 8 XCDs in turn (XCD x takes chunks x, x + 8, ...). The cost of a tile varies along the
+// sorted order (tool regions, free surfaces, the cut zone); contiguous ranges per XCD leave the
+// XCDs unevenly loaded, while chunks of 8 tiles (2,048 particles) keep each chunk's neighbour
+// data together in one L2 and give every XCD, and every CU, a mix of the domain. The grid must be a multiple of 8 * XCD_CHUNK blocks (chunked_blocks).
+constexpr int XCD_CHUNK = 8;
+__device__ inline int xcd_chunk_tile(int b, int ntiles) {
+  const int x = b & 7, k = b >> 3;
+  const int t = (k / XCD_CHUNK) * (8 * XCD_CHUNK) + x * XCD_CHUNK + k % XCD_CHUNK;
+  return t < ntiles ? t : -1;
+}
+inline int chunked_blocks(int ntiles) { return 8 * XCD_CHUNK * ((ntiles + 8 * XCD_CHUNK - 1) / (8 * XCD_CHUNK)); }
+
 // cell coordinate: C++ truncation of a double, saturated outside the int range
 __device__ inline int64_t cell_coord(double f) {
   if (!(f == f)) return -1;

@@ -833,7 +833,7 @@ __global__ __launch_bounds__(TILE) SUMS_OCCUPANCY void k_interact(dev_consts k, 
                                                    const int *__restrict__ offsets, const int *__restrict__ perm,
                                                    dstate src, dstate dst, dpositional pos,
                                                    const mfc_tool_state *__restrict__ tool_now, dforce fo) {
-  const int t = xcd_tile(blockIdx.x, ntiles);
+  const int t = xcd_chunk_tile(blockIdx.x, ntiles);
   if (t < 0) return;
   interact<F32, COMPACT>(k, t, r, key_sorted, offsets, perm, src, dst, pos, tool_now, fo, nullptr);
 }
@@ -843,7 +843,7 @@ __global__ __launch_bounds__(TILE) SUMS_OCCUPANCY void k_sums(dev_consts k, int 
                                                const int *__restrict__ key_sorted, const int *__restrict__ offsets,
                                                const int *__restrict__ perm, const int *__restrict__ src_flags,
                                                double *__restrict__ acc) {
-  const int t = xcd_tile(blockIdx.x, ntiles);
+  const int t = xcd_chunk_tile(blockIdx.x, ntiles);
   if (t < 0) return;
   sums<F32, COMPACT>(k, t, r, key_sorted, offsets, perm, src_flags, acc);
 }
@@ -915,7 +915,8 @@ __device__ inline inst_args load_args(const inst_args *A, int q) {
 }
 #undef MFC_F
 
-// One launch per phase for all instances of a batch: block b takes work item xcd_tile(b) of the
+// One launch per phase for all instances of a batch: block b takes work item xcd_tile(b) (the
+// interaction kernels: xcd_chunk_tile(b)) of the
 // (instance, block) table, and runs the same device body as the single-instance kernel with
 // that instance's constants and buffers, so every instance computes exactly its own run.
 __global__ __launch_bounds__(256) void kb_hash_count(const inst_args *__restrict__ A, const int2 *__restrict__ work,
@@ -975,7 +976,7 @@ __global__ __launch_bounds__(TILE) void kb_predict(const inst_args *__restrict__
 template <bool F32, bool COMPACT>
 __global__ __launch_bounds__(TILE) SUMS_OCCUPANCY void kb_interact(const inst_args *__restrict__ A, const int2 *__restrict__ work,
                                                     int nwork) {
-  const int w = xcd_tile(blockIdx.x, nwork);
+  const int w = xcd_chunk_tile(blockIdx.x, nwork);
   if (w < 0) return;
   const int2 it = work[w];
   const inst_args &a = A[it.x];
@@ -988,7 +989,7 @@ __global__ __launch_bounds__(TILE) SUMS_OCCUPANCY void kb_interact(const inst_ar
 template <bool F32, bool COMPACT>
 __global__ __launch_bounds__(TILE) SUMS_OCCUPANCY void kb_sums(const inst_args *__restrict__ A, const int2 *__restrict__ work,
                                                 int nwork) {
-  const int w = xcd_tile(blockIdx.x, nwork);
+  const int w = xcd_chunk_tile(blockIdx.x, nwork);
   if (w < 0) return;
   const int2 it = work[w];
   const inst_args &a = A[it.x];
@@ -1068,7 +1069,7 @@ void launch_predict(const dev_consts &k, const dstate &cur, const dstate &nxt, c
 void launch_sums(const dev_consts &k, const int *src_flags, const dbins &b, const drecords &r, double *acc, variant v,
                  hipStream_t s) {
   const int nt = (k.N + TILE - 1) / TILE;
-  hipLaunchKernelGGL(MFC_PICK(k_sums, r.f32, v.compact), dim3(padded_tiles(nt)), dim3(TILE), 0, s, k, nt, r,
+  hipLaunchKernelGGL(MFC_PICK(k_sums, r.f32, v.compact), dim3(chunked_blocks(nt)), dim3(TILE), 0, s, k, nt, r,
                      b.key_sorted, b.offsets, b.perm, src_flags, acc);
 }
 
@@ -1076,7 +1077,7 @@ void launch_interact_fused(const dev_consts &k, const dstate &cur, const dstate 
                            const dpositional &pos, const drecords &r, const mfc_tool_state *tool_now, const dforce &f,
                            variant v, hipStream_t s) {
   const int nt = (k.N + TILE - 1) / TILE;
-  hipLaunchKernelGGL(MFC_PICK(k_interact, r.f32, v.compact), dim3(padded_tiles(nt)), dim3(TILE), 0, s, k, nt, r,
+  hipLaunchKernelGGL(MFC_PICK(k_interact, r.f32, v.compact), dim3(chunked_blocks(nt)), dim3(TILE), 0, s, k, nt, r,
                      b.key_sorted, b.offsets, b.perm, cur, nxt, pos, tool_now, f);
 }
 
@@ -1129,7 +1130,7 @@ void launch_batch_predict(const batch_work &w, hipStream_t s) {
 }
 
 void launch_batch_sums(const batch_work &w, hipStream_t s) {
-  hipLaunchKernelGGL(MFC_PICK(kb_sums, w.f32, w.v.compact), dim3(padded_tiles(w.ntiles)), dim3(TILE), 0, s, w.args,
+  hipLaunchKernelGGL(MFC_PICK(kb_sums, w.f32, w.v.compact), dim3(chunked_blocks(w.ntiles)), dim3(TILE), 0, s, w.args,
                      w.tiles, w.ntiles);
 }
 
@@ -1138,7 +1139,7 @@ void launch_batch_update(const batch_work &w, hipStream_t s) {
 }
 
 void launch_batch_interact_fused(const batch_work &w, hipStream_t s) {
-  hipLaunchKernelGGL(MFC_PICK(kb_interact, w.f32, w.v.compact), dim3(padded_tiles(w.ntiles)), dim3(TILE), 0, s,
+  hipLaunchKernelGGL(MFC_PICK(kb_interact, w.f32, w.v.compact), dim3(chunked_blocks(w.ntiles)), dim3(TILE), 0, s,
                      w.args, w.tiles, w.ntiles);
 }
 
