@@ -71,6 +71,7 @@ void enqueue_step(mfc_sim *s, int c) { mfc::launch_batch_step(s->work(c), s->str
 int ensure_keys(mfc_sim *s) {
   if (s->keys_fresh) return MFC_OK;
   HIP_TRY(hipMemsetAsync(s->bins.count, 0, sizeof(int) * (size_t)s->k.num_cell, s->stream));
+  HIP_TRY(hipMemsetAsync(s->bins.bsum, 0, sizeof(int) * (size_t)mfc::scan_blocks(s->k.num_cell), s->stream));
   mfc::launch_batch_hash(s->work(s->cur), s->stream);
   s->keys_fresh = true;
   return MFC_OK;

@@ -74,8 +74,10 @@ int batch_keys(mfc_batch *b) {
   bool fresh = true;
   for (mfc_sim *s : b->sims) fresh = fresh && s->keys_fresh;
   if (fresh) return MFC_OK;
-  for (mfc_sim *s : b->sims)
+  for (mfc_sim *s : b->sims) {
     HIP_TRY(hipMemsetAsync(s->bins.count, 0, sizeof(int) * (size_t)s->k.num_cell, b->stream));
+    HIP_TRY(hipMemsetAsync(s->bins.bsum, 0, sizeof(int) * (size_t)mfc::scan_blocks(s->k.num_cell), b->stream));
+  }
   mfc::launch_batch_hash(b->work(b->cur), b->stream);
   for (mfc_sim *s : b->sims) s->keys_fresh = true;
   return MFC_OK;

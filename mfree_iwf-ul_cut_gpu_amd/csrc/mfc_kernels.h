@@ -108,9 +108,10 @@ struct batch_work {
   variant v;
 };
 
-// one step of every instance of a batch: 5 (fused) or 6 (split) launches, and its phases. The
-// cell ids, arrival slots and cell counts of a step's start state are computed by the previous
-// step's interaction; launch_batch_hash computes them from the state itself (counts zeroed before)
+// one step of every instance of a batch: 4 (fused) or 5 (split) launches, and its phases. The
+// cell ids, arrival slots, cell counts and chunk sums of a step's start state are computed by
+// the previous step's interaction; launch_batch_hash computes them from the state itself (counts
+// and chunk sums zeroed before)
 void launch_batch_step(const batch_work &w, hipStream_t s);
 void launch_batch_hash(const batch_work &w, hipStream_t s);
 void launch_batch_binning(const batch_work &w, hipStream_t s);

@@ -1,11 +1,14 @@
 // mfc_kernels.hip — gfx950 kernels of one SPH cutting step (FP64 and FP32C).
 //
-// Per step (the reference's ~20 launches of leap_frog::step, leap_frog.cu:94-155, become 6):
-//   binning      hash_count, scan_reduce, scan_down, place: cell ids (grid.cu:34-51), per-cell
-//                and per-chunk counts, cell offsets and per-cell buckets of a stable counting
-//                sort by cell (thrust::sort_by_key + reorder_data_and_find_cell_start,
-//                grid_gpu_green.cu:22-116,177-201); scan_reduce's first block also reduces the
-//                previous step's tool force (output.cu:30-39) from its tile partials
+// Per step (the reference's ~20 launches of leap_frog::step, leap_frog.cu:94-155, become 4):
+//   binning      scan_down, place: cell offsets and per-cell buckets of a stable counting sort
+//                by cell (thrust::sort_by_key + reorder_data_and_find_cell_start,
+//                grid_gpu_green.cu:22-116,177-201) from the cell ids (grid.cu:34-51), per-cell
+//                and per-chunk counts that the previous step's interaction computed for the
+//                updated state (hash_count computes them when the state was written from
+//                outside; the reverse mode's replay runs hash_count and scan_reduce); scan_down's
+//                first block also reduces the previous step's tool force (output.cu:30-39) from
+//                its tile partials
 //   predict      the sort's rank pass, then leap-frog predict (leap_frog.cu:30-66) + EOS
 //                (actions_gpu.cu:41-51) + artificial stress (:53-97) + neighbour record + tool
 //                move, in the start state's storage order
@@ -13,7 +16,8 @@
 //                that follows them in the step: Jaumann, continuity, momentum, advection, correct,
 //                friction heat, Johnson-Cook return, tool-particle move, contact, boundary
 //                conditions, NaN blanking, the gather into sorted order, and the per-tile partial
-//                sums of the tool force (or split into sums + update)
+//                sums of the tool force (or split into sums + update), and the cell ids and
+//                counts of the next step
 // After the last step of a call, force_final reduces the last step's tool force.
 // Each body is a device function of its work index; the kb_* kernels run it over an
 // (instance, tile) work table for any number of instances (a single handle is a batch of one),
@@ -74,12 +78,14 @@ __device__ inline int particle_cell(const dev_consts &k, double x, double y, int
   return h;
 }
 
-// records cell id h of particle i (storage order) and takes its arrival slot in the cell.
-// All 64 lanes of the wave take part (lanes past N carry h = -1). The storage order is the
-// previous step's cell order, so neighbouring lanes mostly share a cell: each run of equal
-// cell ids in the wave takes its slots with one atomic.
+// records cell id h of particle i (storage order) and takes its arrival slot in the cell, and
+// counts the particle in its chunk of SCAN_CHUNK cells (bsum, the chunk sums of the offset
+// scan). All 64 lanes of the wave take part (lanes past N carry h = -1). The storage order is
+// the previous step's cell order, so neighbouring lanes mostly share a cell: each run of equal
+// cell ids in the wave takes its slots with one atomic, each run of equal chunks adds its length
+// with one.
 __device__ inline void count_cell(int i, int h, int *__restrict__ key, int *__restrict__ slot,
-                                  int *__restrict__ count) {
+                                  int *__restrict__ count, int *__restrict__ bsum) {
   if (h >= 0) key[i] = h;
   const int lane = threadIdx.x & 63;
   const int prev = __shfl_up(h, 1, 64);
@@ -93,15 +99,22 @@ __device__ inline void count_cell(int i, int h, int *__restrict__ key, int *__re
   if (head && h >= 0) base = atomicAdd(&count[h], next_head - lane);
   base = __shfl(base, my_head, 64);
   if (h >= 0) slot[i] = base + (lane - my_head);
+  const int c = h >= 0 ? h / SCAN_CHUNK : -1;
+  const int cprev = __shfl_up(c, 1, 64);
+  const unsigned long long cheads = __ballot(lane == 0 || cprev != c);
+  const unsigned long long cafter = (lane == 63) ? 0ull : (cheads & ~upto);
+  const int cnext = cafter ? __ffsll((long long)cafter) - 1 : 64;
+  if ((lane == 0 || cprev != c) && c >= 0) atomicAdd(&bsum[c], cnext - lane);
 }
 
 // cell id and arrival slot of every particle of the start state
 __device__ inline void hash_count(const dev_consts &k, int i, const double *__restrict__ px,
                                   const double *__restrict__ py, const int *__restrict__ flags,
                                   const int *__restrict__ key_sorted, int *__restrict__ key, int *__restrict__ slot,
-                                  int *__restrict__ count, unsigned long long *__restrict__ clamped) {
+                                  int *__restrict__ count, int *__restrict__ bsum,
+                                  unsigned long long *__restrict__ clamped) {
   const int h = i < k.N ? particle_cell(k, px[i], py[i], flags[i], key_sorted[i], clamped) : -1;
-  count_cell(i, h, key, slot, count);
+  count_cell(i, h, key, slot, count, bsum);
 }
 
 // particles per chunk of SCAN_CHUNK cells
@@ -158,7 +171,7 @@ __device__ inline void place(int i, int N, const int *__restrict__ key, const in
 __global__ __launch_bounds__(256) void k_hash_count(dev_consts k, const double *__restrict__ px,
                                                     const double *__restrict__ py, const int *__restrict__ flags,
                                                     dbins b) {
-  hash_count(k, blockIdx.x * blockDim.x + threadIdx.x, px, py, flags, b.key_sorted, b.key, b.slot, b.count,
+  hash_count(k, blockIdx.x * blockDim.x + threadIdx.x, px, py, flags, b.key_sorted, b.key, b.slot, b.count, b.bsum,
              b.clamped);
 }
 
@@ -718,7 +731,7 @@ __device__ inline void next_cell(const dev_consts &k, int i, bool live, double x
                                  const dbins *next) {
   if (next == nullptr) return;  // the same in every thread
   const int h = live ? particle_cell(k, x, y, fl, next->key_sorted[i], next->clamped) : -1;
-  count_cell(i, h, next->key, next->slot, next->count);
+  count_cell(i, h, next->key, next->slot, next->count, next->bsum);
 }
 
 // fused: sums + update in one pass (registers of both live in one kernel)
@@ -926,18 +939,7 @@ __global__ __launch_bounds__(256) void kb_hash_count(const inst_args *__restrict
   const int2 it = work[w];
   const inst_args a = load_args(A, it.x);
   hash_count(a.k, it.y * 256 + threadIdx.x, a.cur.px, a.cur.py, a.cur.flags, a.b.key_sorted, a.b.key,
-             a.b.slot, a.b.count, a.b.clamped);
-}
-
-__global__ __launch_bounds__(SCAN_THREADS) void kb_scan_reduce(const inst_args *__restrict__ A,
-                                                                const int2 *__restrict__ work, int nwork) {
-  const int w = xcd_tile(blockIdx.x, nwork);
-  if (w < 0) return;
-  const int2 it = work[w];
-  const inst_args a = load_args(A, it.x);
-  scan_reduce(it.y, a.b.count, a.k.num_cell, a.b.bsum);
-  // the tool force of the previous step, if its partials are still pending
-  if (it.y == 0) force_final(a.ntiles, a.f);
+             a.b.slot, a.b.count, a.b.bsum, a.b.clamped);
 }
 
 __global__ __launch_bounds__(SCAN_THREADS) void kb_scan_down(const inst_args *__restrict__ A,
@@ -947,6 +949,8 @@ __global__ __launch_bounds__(SCAN_THREADS) void kb_scan_down(const inst_args *__
   const int2 it = work[w];
   const inst_args a = load_args(A, it.x);
   scan_down(it.y, a.b.count, a.k.num_cell, a.b.bsum, a.b.offsets);
+  // the tool force of the previous step, if its partials are still pending
+  if (it.y == 0) force_final(a.ntiles, a.f);
 }
 
 __global__ __launch_bounds__(256) void kb_place(const inst_args *__restrict__ A, const int2 *__restrict__ work,
@@ -956,6 +960,8 @@ __global__ __launch_bounds__(256) void kb_place(const inst_args *__restrict__ A,
   const int2 it = work[w];
   const inst_args a = load_args(A, it.x);
   place(it.y * 256 + threadIdx.x, a.k.N, a.b.key, a.b.slot, a.b.offsets, a.b.bucket);
+  // the chunk sums are consumed (scan_down): cleared for the counts of the next step
+  for (int q = it.y * 256 + threadIdx.x; q < a.nscan; q += a.ntiles * 256) a.b.bsum[q] = 0;
 }
 
 template <bool F32>
@@ -1118,7 +1124,6 @@ void launch_batch_hash(const batch_work &w, hipStream_t s) {
 
 void launch_batch_binning(const batch_work &w, hipStream_t s) {
   const dim3 gt(padded_tiles(w.ntiles)), gc(padded_tiles(w.nchunks));
-  hipLaunchKernelGGL(kb_scan_reduce, gc, dim3(SCAN_THREADS), 0, s, w.args, w.chunks, w.nchunks);
   hipLaunchKernelGGL(kb_scan_down, gc, dim3(SCAN_THREADS), 0, s, w.args, w.chunks, w.nchunks);
   hipLaunchKernelGGL(kb_place, gt, dim3(256), 0, s, w.args, w.tiles, w.ntiles);
 }
