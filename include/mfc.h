@@ -178,7 +178,9 @@ int mfc_sync(mfc_sim *sim);
  * counting sort), 1 gather/predict/EOS/artificial stress, 2 the fused interaction kernel
  * (neighbour sums + per-particle update + contact + the next step's cell ids and counts), or
  * the neighbour-sum kernel when split_interact is set, 3 the update kernel (split_interact
- * only, else ~0; it then computes the next step's cell ids), 4 tool-force reduction */
+ * only, else 0; it then computes the next step's cell ids), 4 tool-force reduction. Phases 2
+ * and 3 are the kernels' own durations (events recorded by their dispatch); the others are
+ * measured between events on the stream and include the gaps between launches. */
 #define MFC_PROFILE_PHASES 5
 int mfc_profile(mfc_sim *sim, int nsteps, double *ms);
 

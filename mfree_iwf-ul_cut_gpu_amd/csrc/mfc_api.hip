@@ -317,9 +317,11 @@ int mfc_step(mfc_sim *s, int nsteps) {
 int mfc_profile(mfc_sim *s, int nsteps, double *ms) {
   if (!s || !ms || nsteps < 1) return fail(MFC_E_INVALID, "bad argument");
   if (s->batched) return fail(MFC_E_INVALID, "instance of a batch: step it with mfc_batch_step");
+  // ev: the phase boundaries on the stream; kt: the interaction kernels' own start and end
   constexpr int NE = MFC_PROFILE_PHASES + 1;
-  hipEvent_t ev[NE];
+  hipEvent_t ev[NE], kt[4];
   for (int q = 0; q < NE; q++) HIP_TRY(hipEventCreate(&ev[q]));
+  for (int q = 0; q < 4; q++) HIP_TRY(hipEventCreate(&kt[q]));
   double acc[MFC_PROFILE_PHASES] = {0., 0., 0., 0., 0.};
   int rc = ensure_keys(s);
   for (int it = 0; it < nsteps && rc == MFC_OK; it++) {
@@ -331,11 +333,11 @@ int mfc_profile(mfc_sim *s, int nsteps, double *ms) {
     mfc::launch_batch_predict(w, s->stream);
     (void)hipEventRecord(ev[2], s->stream);
     if (w.v.split) {
-      mfc::launch_batch_sums(w, s->stream);
+      mfc::launch_batch_sums(w, s->stream, kt[0], kt[1]);
       (void)hipEventRecord(ev[3], s->stream);
-      mfc::launch_batch_update(w, s->stream);
+      mfc::launch_batch_update(w, s->stream, kt[2], kt[3]);
     } else {
-      mfc::launch_batch_interact_fused(w, s->stream);
+      mfc::launch_batch_interact_fused(w, s->stream, kt[0], kt[1]);
       (void)hipEventRecord(ev[3], s->stream);
     }
     (void)hipEventRecord(ev[4], s->stream);
@@ -344,13 +346,19 @@ int mfc_profile(mfc_sim *s, int nsteps, double *ms) {
     s->cur = n;
     s->step++;
     if (hipEventSynchronize(ev[NE - 1]) != hipSuccess) { rc = fail(MFC_E_HIP, "event sync"); break; }
-    for (int q = 0; q < MFC_PROFILE_PHASES; q++) {
-      float t = 0.f;
-      if (hipEventElapsedTime(&t, ev[q], ev[q + 1]) != hipSuccess) { rc = fail(MFC_E_HIP, "event time"); break; }
-      acc[q] += t;
+    float t[MFC_PROFILE_PHASES] = {0.f, 0.f, 0.f, 0.f, 0.f};
+    if (hipEventElapsedTime(&t[0], ev[0], ev[1]) != hipSuccess ||
+        hipEventElapsedTime(&t[1], ev[1], ev[2]) != hipSuccess ||
+        hipEventElapsedTime(&t[2], kt[0], kt[1]) != hipSuccess ||
+        (w.v.split && hipEventElapsedTime(&t[3], kt[2], kt[3]) != hipSuccess) ||
+        hipEventElapsedTime(&t[4], ev[4], ev[5]) != hipSuccess) {
+      rc = fail(MFC_E_HIP, "event time");
+      break;
     }
+    for (int q = 0; q < MFC_PROFILE_PHASES; q++) acc[q] += t[q];
   }
   for (int q = 0; q < NE; q++) (void)hipEventDestroy(ev[q]);
+  for (int q = 0; q < 4; q++) (void)hipEventDestroy(kt[q]);
   if (rc) return rc;
   for (int q = 0; q < MFC_PROFILE_PHASES; q++) ms[q] = acc[q] / nsteps;
   return MFC_OK;

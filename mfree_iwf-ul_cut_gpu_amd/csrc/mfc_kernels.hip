@@ -6,12 +6,12 @@
 //                grid_gpu_green.cu:22-116,177-201) from the cell ids (grid.cu:34-51), per-cell
 //                and per-chunk counts that the previous step's interaction computed for the
 //                updated state (hash_count computes them when the state was written from
-//                outside; the reverse mode's replay runs hash_count and scan_reduce); scan_down's
-//                first block also reduces the previous step's tool force (output.cu:30-39) from
-//                its tile partials
+//                outside; the reverse mode's replay runs hash_count and scan_reduce)
 //   predict      the sort's rank pass, then leap-frog predict (leap_frog.cu:30-66) + EOS
 //                (actions_gpu.cu:41-51) + artificial stress (:53-97) + neighbour record + tool
-//                move, in the start state's storage order
+//                move, in the start state's storage order; the work-group of the first tile
+//                also reduces the previous step's tool force (output.cu:30-39) from its tile
+//                partials
 //   interact     neighbour sums (interactions_gpu.cu:88-345) fused with every per-particle action
 //                that follows them in the step: Jaumann, continuity, momentum, advection, correct,
 //                friction heat, Johnson-Cook return, tool-particle move, contact, boundary
@@ -28,6 +28,8 @@
 // record (two 64-byte lines; FP32C: 64 bytes plus the FP64 position) so that the divergent
 // per-lane neighbour reads are 16-byte vector loads from one cache line. Work-group to tile
 // mapping is XCD-aware.
+#include <hip/hip_ext.h>
+
 #include "mfc_kernels.h"
 #include "mfc_pairs.h"
 
@@ -949,8 +951,6 @@ __global__ __launch_bounds__(SCAN_THREADS) void kb_scan_down(const inst_args *__
   const int2 it = work[w];
   const inst_args a = load_args(A, it.x);
   scan_down(it.y, a.b.count, a.k.num_cell, a.b.bsum, a.b.offsets);
-  // the tool force of the previous step, if its partials are still pending
-  if (it.y == 0) force_final(a.ntiles, a.f);
 }
 
 __global__ __launch_bounds__(256) void kb_place(const inst_args *__restrict__ A, const int2 *__restrict__ work,
@@ -977,6 +977,9 @@ __global__ __launch_bounds__(TILE) void kb_predict(const inst_args *__restrict__
   MFC_CONST(inst_args) *c = const_args(A, it.x);
   predict<F32, false>(a.k, it.y, load_bins(&c->b), load_state(&c->cur), a.nxt, a.pos, load_recs(&c->r), a.tool_in,
                       a.tool_out);
+  // the tool force of the previous step, if its partials are still pending (the first tile's
+  // work-group is dispatched first, so the longer kernel hides the reduction)
+  if (it.y == 0) force_final(a.ntiles, load_force(&c->f));
 }
 
 template <bool F32, bool COMPACT>
@@ -1134,18 +1137,26 @@ void launch_batch_predict(const batch_work &w, hipStream_t s) {
   else hipLaunchKernelGGL(kb_predict<false>, gt, dim3(TILE), 0, s, w.args, w.tiles, w.ntiles);
 }
 
-void launch_batch_sums(const batch_work &w, hipStream_t s) {
-  hipLaunchKernelGGL(MFC_PICK(kb_sums, w.f32, w.v.compact), dim3(chunked_blocks(w.ntiles)), dim3(TILE), 0, s, w.args,
-                     w.tiles, w.ntiles);
+// a batched launch over the work table, with the kernel's own start and end recorded in t0 and
+// t1 when they are given
+template <class K>
+static void launch_timed(K kern, dim3 grid, const batch_work &w, hipStream_t s, hipEvent_t t0, hipEvent_t t1) {
+  if (t0)
+    hipExtLaunchKernelGGL(kern, grid, dim3(TILE), 0, s, t0, t1, 0, w.args, w.tiles, w.ntiles);
+  else
+    hipLaunchKernelGGL(kern, grid, dim3(TILE), 0, s, w.args, w.tiles, w.ntiles);
 }
 
-void launch_batch_update(const batch_work &w, hipStream_t s) {
-  hipLaunchKernelGGL(kb_update, dim3(padded_tiles(w.ntiles)), dim3(TILE), 0, s, w.args, w.tiles, w.ntiles);
+void launch_batch_sums(const batch_work &w, hipStream_t s, hipEvent_t t0, hipEvent_t t1) {
+  launch_timed(MFC_PICK(kb_sums, w.f32, w.v.compact), dim3(chunked_blocks(w.ntiles)), w, s, t0, t1);
 }
 
-void launch_batch_interact_fused(const batch_work &w, hipStream_t s) {
-  hipLaunchKernelGGL(MFC_PICK(kb_interact, w.f32, w.v.compact), dim3(chunked_blocks(w.ntiles)), dim3(TILE), 0, s,
-                     w.args, w.tiles, w.ntiles);
+void launch_batch_update(const batch_work &w, hipStream_t s, hipEvent_t t0, hipEvent_t t1) {
+  launch_timed(kb_update, dim3(padded_tiles(w.ntiles)), w, s, t0, t1);
+}
+
+void launch_batch_interact_fused(const batch_work &w, hipStream_t s, hipEvent_t t0, hipEvent_t t1) {
+  launch_timed(MFC_PICK(kb_interact, w.f32, w.v.compact), dim3(chunked_blocks(w.ntiles)), w, s, t0, t1);
 }
 
 void launch_batch_force(const batch_work &w, hipStream_t s) {
