@@ -29,16 +29,18 @@ HBM_PEAK_GBS = 8000.0
 # compulsory bytes per particle of one launch of the fused interaction kernel: the particle's
 # own neighbour record (neighbour re-reads are cache traffic), 26 state doubles read and written,
 # flags read and written, the positional pressure (read) and contact normal (read, written),
-# the sorted cell id and the permutation entry (read)
-_STATE_RW = 26 * 8 + 4 + 8 + 16 + 4 + 4 + (26 * 8 + 4 + 16)
+# the sorted cell id and the permutation entry (read), and the next step's cell id and arrival
+# slot (written)
+_STATE_RW = 26 * 8 + 4 + 8 + 16 + 4 + 4 + (26 * 8 + 4 + 16) + 8
 INTERACT_BYTES = {"fp64": 128 + _STATE_RW, "fp32c": 64 + 16 + _STATE_RW}
 # the neighbour-sum kernel of the split variant: own record, flags, cell id and permutation entry
 # read, 14 sums written (8 B each)
 SUMS_BYTES = {"fp64": 128 + 4 + 4 + 4 + 14 * 8, "fp32c": 64 + 16 + 4 + 4 + 4 + 14 * 8}
-# HBM bytes per particle of one launch of the fused interaction kernel (direct walk), measured
-# with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE on this workload and scaled by a 512 MiB copy
-# (profiles/traffic_thermal_contact_150k.md)
-MEASURED_TRAFFIC = {("kb_interact", "fp32c"): 590, ("kb_interact", "fp64"): 699}
+# HBM bytes per particle of one launch of the fused interaction kernel, by precision and pair
+# loop (compact_pairs), measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE on this workload
+# and scaled by a 512 MiB copy (profiles/traffic_thermal_contact_150k.md)
+MEASURED_TRAFFIC = {("kb_interact", "fp32c", 0): 661, ("kb_interact", "fp32c", 1): 646,
+                    ("kb_interact", "fp64", 0): 830, ("kb_interact", "fp64", 1): 810}
 # whole step, compulsory traffic of a fully fused FP64 step (persistent read-only 32 B,
 # mutable 224 B read + written, 12 B of binning keys)
 STEP_BYTES_FP64 = 256 + 224 + 12
@@ -288,9 +290,9 @@ def main() -> int:
                    "num_cell": r["info"].num_cell, "precision": head, "thermal": "brookshaw+tool-contact"},
         "roofline": {"bound": "hbm", "kernel": kname, "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": (MEASURED_TRAFFIC[(kname, head)] * N
-                                 if args.workload == "thermal_contact_150k" and (kname, head) in MEASURED_TRAFFIC
-                                 and not r["tuning"]["compact_pairs"] else None),
+                     "traffic": (MEASURED_TRAFFIC[(kname, head, r["tuning"]["compact_pairs"])] * N
+                                 if args.workload == "thermal_contact_150k"
+                                 and (kname, head, r["tuning"]["compact_pairs"]) in MEASURED_TRAFFIC else None),
                      "bytes_per_particle": kbytes, "ms_per_launch": r["phases"]["interact"]},
         "tuning": r["tuning"],
         "phases_ms": r["phases"],
