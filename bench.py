@@ -78,25 +78,12 @@ def cpu_baseline(cfg, target_s: float) -> dict:
                       f"{wall:.1f} s wall on {ncpu} threads ({model})"}
 
 
-VARIANTS = [(0, 0), (0, 1), (1, 0), (1, 1)]  # (split_interact, compact_pairs)
-
-
-def tune(pymfc, sim, steps: int) -> dict:
-    """Autotuner: time each launch variant of the interaction on the live handle (every variant
-    gives the same bits) and keep the fastest."""
-    cand = {}
-    for split, compact in VARIANTS:
-        sim.set_tuning(split_interact=split, compact_pairs=compact)
-        sim.step(4)
-        sim.sync()
-        t0 = time.perf_counter()
-        sim.step(steps)
-        sim.sync()
-        cand[(split, compact)] = (time.perf_counter() - t0) / steps * 1e3
-    best = min(cand, key=cand.get)
-    sim.set_tuning(split_interact=best[0], compact_pairs=best[1])
-    return {"split_interact": best[0], "compact_pairs": best[1],
-            "ms_per_step": {f"split{k[0]}_compact{k[1]}": v for k, v in cand.items()}}
+def tune(pymfc, handle, steps: int) -> dict:
+    """The package's autotuner on the live handle (every launch configuration gives the same
+    bits); off (defaults) when steps is 0."""
+    if steps <= 0:
+        return {"split_interact": 0, "compact_pairs": 0, "stage_limit": 0, "ms_per_step": {}}
+    return pymfc.autotune(handle, steps)
 
 
 def run_path(pymfc, workload: str, precision: int, steps: int, warmup: int, profile_steps: int,
@@ -106,7 +93,7 @@ def run_path(pymfc, workload: str, precision: int, steps: int, warmup: int, prof
     info = sim.info
     N = info.n_particles
     sim.step(warmup)
-    tuning = tune(pymfc, sim, tune_steps) if tune_steps > 0 else {"split_interact": 0, "compact_pairs": 0}
+    tuning = tune(pymfc, sim, tune_steps)
     sim.sync()
     t0 = time.perf_counter()
     sim.step(steps)
@@ -159,17 +146,8 @@ def batch_bench(pymfc, n: int, steps: int, tune_steps: int) -> dict:
         b = pymfc.Batch(cfgs)
         N = sum(inst.N for inst in b.instances)
         b.step(10)
-        cand = {}
-        for split, compact in (VARIANTS if tune_steps > 0 else [(0, 0)]):
-            b.set_tuning(split_interact=split, compact_pairs=compact)
-            b.step(4)
-            b.sync()
-            t0 = time.perf_counter()
-            b.step(max(tune_steps, 2))
-            b.sync()
-            cand[(split, compact)] = (time.perf_counter() - t0) / max(tune_steps, 2)
-        best = min(cand, key=cand.get)
-        b.set_tuning(split_interact=best[0], compact_pairs=best[1])
+        tuning = tune(pymfc, b, max(tune_steps, 2) if tune_steps > 0 else 0)
+        best = {k: tuning[k] for k in ("split_interact", "compact_pairs", "stage_limit")}
         b.step(4)
         b.sync()
         t0 = time.perf_counter()
@@ -181,7 +159,7 @@ def batch_bench(pymfc, n: int, steps: int, tune_steps: int) -> dict:
         seq = 0.0
         for c in cfgs:
             s = pymfc.Sim(c)
-            s.set_tuning(split_interact=best[0], compact_pairs=best[1])
+            s.set_tuning(**best)
             s.step(10)
             s.sync()
             t0 = time.perf_counter()
@@ -192,8 +170,7 @@ def batch_bench(pymfc, n: int, steps: int, tune_steps: int) -> dict:
         out[name] = {"particles": N, "min_instance": min(sizes), "max_instance": max(sizes),
                      "particle_steps_per_s": N * steps / wall, "ms_per_step": wall / steps * 1e3,
                      "sequential_particle_steps_per_s": N * steps / seq, "batch_speedup": seq / wall,
-                     "tuning": {"split_interact": best[0], "compact_pairs": best[1],
-                                "ms_per_step": {f"split{k[0]}_compact{k[1]}": v * 1e3 for k, v in cand.items()}}}
+                     "tuning": tuning}
     return out
 
 
@@ -291,7 +268,7 @@ def main() -> int:
         "roofline": {"bound": "hbm", "kernel": kname, "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": (MEASURED_TRAFFIC[(kname, head, r["tuning"]["compact_pairs"])] * N
-                                 if args.workload == "thermal_contact_150k"
+                                 if args.workload == "thermal_contact_150k" and r["tuning"]["stage_limit"] == 0
                                  and (kname, head, r["tuning"]["compact_pairs"]) in MEASURED_TRAFFIC else None),
                      "bytes_per_particle": kbytes, "ms_per_launch": r["phases"]["interact"]},
         "tuning": r["tuning"],

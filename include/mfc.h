@@ -226,11 +226,12 @@ int mfc_vjp(mfc_sim *sim, mfc_vjp_req *req, double grad[6]);
  * evaluates the same expressions in the same order). split_interact = 0 (default): one fused
  * interaction kernel; 1: a neighbour-sum kernel and a per-particle update kernel with the sums
  * stored in between (fewer registers per kernel, 224 B more traffic per particle).
- * compact_pairs = 1: each lane first lists its in-support neighbours, then runs the pair terms
- * over the list; 0 (default): the pair terms run inside the window walk. stage_limit: the most
- * window candidates a tile of 256 particles copies into LDS (0 = MFC_STAGE_CAP); a tile whose
- * candidates span more sorted indices reads them from the records in memory. The benchmark's
- * autotuner picks split_interact and compact_pairs per workload. */
+ * compact_pairs = 1: each lane first marks its in-support neighbours in a bit mask, then runs
+ * the pair terms of the marked ones in order; 0 (default): the pair terms run inside the window
+ * walk. stage_limit: the most window candidates a tile of 256 particles copies into LDS
+ * (0 = MFC_STAGE_CAP); a tile whose candidates span more sorted indices reads them from the
+ * records in memory. The Python package's autotuner (pymfc.autotune) times the combinations on
+ * a live handle and keeps the fastest. */
 #define MFC_STAGE_CAP 2048
 typedef struct mfc_tuning {
   int split_interact;

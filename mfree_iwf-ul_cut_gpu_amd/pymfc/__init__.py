@@ -15,7 +15,9 @@ Loading fails loudly when lib/libmfc.so is missing: there is no CPU fallback.
 from __future__ import annotations
 
 import ctypes as C
+import itertools
 import os
+import time
 from dataclasses import dataclass
 
 import numpy as np
@@ -108,6 +110,35 @@ class VjpReq(C.Structure):
 
 
 STAGE_CAP = 2048  # MFC_STAGE_CAP: window candidates a tile can stage in LDS
+
+
+# the launch configurations the autotuner tries: fused or split interaction, direct or compacted
+# pair loop, and the LDS stage (0: tiles stage up to STAGE_CAP candidates; 1: no tile stages)
+TUNING_SPACE = {"split_interact": (0, 1), "compact_pairs": (0, 1), "stage_limit": (0, 1)}
+
+
+def autotune(handle, steps: int = 100, space: dict | None = None) -> dict:
+    """Times every combination of the launch configuration space on the live handle (a Sim or a
+    Batch; every configuration gives the same results, the run just advances) over `steps`
+    steps each, and keeps the fastest. Returns the chosen settings and the ms per step of each
+    candidate."""
+    space = TUNING_SPACE if space is None else space
+    keys = list(space)
+    cand = {}
+    for combo in itertools.product(*(space[k] for k in keys)):
+        handle.set_tuning(**dict(zip(keys, combo)))
+        handle.step(4)
+        handle.sync()
+        t0 = time.perf_counter()
+        handle.step(steps)
+        handle.sync()
+        cand[combo] = (time.perf_counter() - t0) / steps * 1e3
+    best = min(cand, key=cand.get)
+    handle.set_tuning(**dict(zip(keys, best)))
+    label = {"split_interact": "split", "compact_pairs": "compact", "stage_limit": "stage"}
+    out = dict(zip(keys, best))
+    out["ms_per_step"] = {"_".join(f"{label.get(k, k)}{v}" for k, v in zip(keys, c)): ms for c, ms in cand.items()}
+    return out
 
 
 class Tuning(C.Structure):

@@ -115,7 +115,7 @@ def test_graph_replay_matches_eager():
 @pytest.mark.parametrize("split,compact", [(1, 0), (0, 1), (1, 1)])
 def test_interaction_variants_match(prec, thermal, split, compact):
     """Every launch variant of the interaction (split sums/update kernels, compacted pair
-    lists) gives the bits of the fused kernel that walks the window directly."""
+    loop) gives the bits of the fused kernel that walks the window directly."""
     cfg = small_cut(thermal=thermal, precision=prec)
     g1, g2 = pymfc.Sim(cfg), pymfc.Sim(cfg)
     g1.set_tuning(split_interact=0, compact_pairs=0)
@@ -153,6 +153,22 @@ def test_stage_limit_is_checked():
     with pytest.raises(pymfc.MfcError):
         g.set_tuning(stage_limit=-1)
     g.set_tuning(stage_limit=pymfc.STAGE_CAP)
+
+
+def test_autotune_keeps_the_run():
+    """The autotuner switches launch configurations while the run advances; every
+    configuration computes the same bits, so the tuned run equals an untuned one."""
+    cfg = small_cut(thermal="contact")
+    a, b = pymfc.Sim(cfg), pymfc.Sim(cfg)
+    t = pymfc.autotune(a, steps=5)
+    assert len(t["ms_per_step"]) == 8
+    assert a.tuning() == {k: t[k] for k in ("split_interact", "compact_pairs", "stage_limit")}
+    b.step(8 * (4 + 5))
+    a.step(10)
+    b.step(10)
+    for f in ("pos", "vel", "S", "T", "eps_pl", "fc", "ft", "flags", "id"):
+        assert np.array_equal(a.field(f), b.field(f)), f
+    assert np.array_equal(a.force_history(), b.force_history())
 
 
 def test_nan_blanking_matches_oracle():
