@@ -136,29 +136,50 @@ __device__ inline void scan_reduce(int blk, const int *__restrict__ count, int n
 // separate scan of the chunk sums is needed.
 __device__ inline void scan_down(int blk, int *__restrict__ count, int n, const int *__restrict__ bsum,
                                  int *__restrict__ offsets) {
+  // the thread's SCAN_ITEMS counts (two 16-byte loads where the chunk is whole), loaded before
+  // the chunk sums so that both loads are in flight together
+  const int base = blk * SCAN_CHUNK + threadIdx.x * SCAN_ITEMS;
+  const bool whole = base + SCAN_ITEMS <= n;
+  int v[SCAN_ITEMS];
+  if (whole) {
+    const int4 a = *reinterpret_cast<const int4 *>(count + base);
+    const int4 b = *reinterpret_cast<const int4 *>(count + base + 4);
+    v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w; v[4] = b.x; v[5] = b.y; v[6] = b.z; v[7] = b.w;
+  } else {
+#pragma unroll
+    for (int q = 0; q < SCAN_ITEMS; q++) v[q] = (base + q < n) ? count[base + q] : 0;
+  }
   int carry_part = 0;
   for (int q = threadIdx.x; q < blk; q += SCAN_THREADS) carry_part += bsum[q];
   int carry;
   block_exclusive_scan(carry_part, carry);
-  int base = blk * SCAN_CHUNK + threadIdx.x * SCAN_ITEMS;
-  int v[SCAN_ITEMS];
   int s = 0;
 #pragma unroll
-  for (int q = 0; q < SCAN_ITEMS; q++) {
-    v[q] = (base + q < n) ? count[base + q] : 0;
-    s += v[q];
-  }
+  for (int q = 0; q < SCAN_ITEMS; q++) s += v[q];
   int total;
   int pre = block_exclusive_scan(s, total) + carry;
+  int o[SCAN_ITEMS];
 #pragma unroll
   for (int q = 0; q < SCAN_ITEMS; q++) {
-    int c = base + q;
-    if (c < n) {
-      offsets[c] = pre;
-      count[c] = 0;
-      if (c == n - 1) offsets[n] = pre + v[q];
-    }
+    o[q] = pre;
     pre += v[q];
+  }
+  if (whole) {
+    *reinterpret_cast<int4 *>(offsets + base) = make_int4(o[0], o[1], o[2], o[3]);
+    *reinterpret_cast<int4 *>(offsets + base + 4) = make_int4(o[4], o[5], o[6], o[7]);
+    *reinterpret_cast<int4 *>(count + base) = make_int4(0, 0, 0, 0);
+    *reinterpret_cast<int4 *>(count + base + 4) = make_int4(0, 0, 0, 0);
+    if (base + SCAN_ITEMS == n) offsets[n] = pre;
+  } else {
+#pragma unroll
+    for (int q = 0; q < SCAN_ITEMS; q++) {
+      const int c = base + q;
+      if (c < n) {
+        offsets[c] = o[q];
+        count[c] = 0;
+        if (c == n - 1) offsets[n] = o[q] + v[q];
+      }
+    }
   }
 }
 
@@ -247,7 +268,13 @@ __device__ inline void predict(const dev_consts &k, int t, const dbins &bins, co
   const int fl = cur.flags[s] & FLAG_MASK;
   const double h = cur.h[s];
   int rnk = 0;
-  for (int q = cb; q < ce; q++) rnk += bins.bucket[q] < s;
+  for (int q = cb; q < ce; q += 4) {  // four bucket entries in flight at a time
+    const int b0 = bins.bucket[q];
+    const int b1 = q + 1 < ce ? bins.bucket[q + 1] : s;
+    const int b2 = q + 2 < ce ? bins.bucket[q + 2] : s;
+    const int b3 = q + 3 < ce ? bins.bucket[q + 3] : s;
+    rnk += (b0 < s) + (b1 < s) + (b2 < s) + (b3 < s);
+  }
   const int i = cb + rnk;
   bins.perm[i] = s;
   bins.key_sorted[i] = hc;
