@@ -106,6 +106,21 @@ def run_path(pymfc, workload: str, precision: int, steps: int, warmup: int, prof
             "tuning": tuning}
 
 
+def other_configs(pymfc, steps: int, tune_steps: int) -> dict:
+    """The other single-GPU configurations of the cut: the 6k adiabatic cut (150 x 40 at 10 um)
+    and the 500k PSE cut (1250 x 400 at 1.2 um, jittered), both precisions, each autotuned."""
+    out = {}
+    for wl in ("adiabatic_6k", "pse_500k"):
+        out[wl] = {}
+        for name, prec in (("fp64", pymfc.PREC_FP64), ("fp32c", pymfc.PREC_FP32C)):
+            r = run_path(pymfc, wl, prec, steps, 20, 10, tune_steps)
+            out[wl][name] = {"n_particles": r["N"], "particle_steps_per_s": r["value"],
+                             "ms_per_step": r["wall"] / steps * 1e3, "tuning": {k: v for k, v in r["tuning"].items()
+                                                                               if k != "ms_per_step"}}
+        out[wl]["fp32c_over_fp64"] = out[wl]["fp32c"]["particle_steps_per_s"] / out[wl]["fp64"]["particle_steps_per_s"]
+    return out
+
+
 def adjoint_ratio(pymfc, workload: str, nsteps: int, K: int) -> dict:
     """Reverse mode on the calibration workload: wall time of the backward pass (replay of the
     checkpoint segments included) over the plain forward of the same steps."""
@@ -216,6 +231,8 @@ def main() -> int:
     ap.add_argument("--batch-instances", type=int, default=32)
     ap.add_argument("--batch-steps", type=int, default=200)
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--other-steps", type=int, default=500, help="timed steps of the other configurations")
+    ap.add_argument("--no-other", action="store_true", help="skip the 6k and 500k configurations")
     ap.add_argument("--probe", action="store_true", help="counter-pass mode: short run + calibration copy")
     ap.add_argument("--split", type=int, default=0, help="split_interact for --probe")
     ap.add_argument("--compact", type=int, default=0, help="compact_pairs for --probe")
@@ -287,6 +304,10 @@ def main() -> int:
         print(f"[bench] adjoint: {adj}", file=sys.stderr, flush=True)
         result["adjoint_over_forward"] = adj["ratio"]
         result["adjoint"] = adj
+    if not args.no_other and args.workload == "thermal_contact_150k":
+        oc = other_configs(pymfc, args.other_steps, args.tune_steps)
+        print(f"[bench] other configurations: {oc}", file=sys.stderr, flush=True)
+        result["other_configs"] = oc
     if not args.no_batch:
         bb = batch_bench(pymfc, args.batch_instances, args.batch_steps, args.tune_steps)
         print(f"[bench] batch: {bb}", file=sys.stderr, flush=True)
