@@ -300,7 +300,12 @@ __device__ inline void predict(const dev_consts &k, int t, const dbins &bins, co
     rho = rho0 + hdt * rhot;
     T = T0 + hdt * Tt;
   }
-  double p = pos.p[i], rxx = pos.rxx[i], rxy = pos.rxy[i], ryy = pos.ryy[i];
+  // the positional values at i are recomputed below for every particle but the tool's, which
+  // keep what the slot holds (the reference's positional arrays are not permuted)
+  double p = 0., rxx = 0., rxy = 0., ryy = 0.;
+  if (fl & MFC_FLAG_TOOL) {
+    p = pos.p[i]; rxx = pos.rxx[i]; rxy = pos.rxy[i]; ryy = pos.ryy[i];
+  }
   if constexpr (F32) {
     float fxx = (float)rxx, fxy = (float)rxy, fyy = (float)ryy;
     if (!(fl & MFC_FLAG_TOOL)) {
