@@ -77,7 +77,7 @@ constexpr int FLAG_NONFINITE = 8;
 // launch variant of the interaction (results are identical for every variant)
 struct variant {
   bool split;    // sums kernel + update kernel instead of one fused kernel
-  bool compact;  // per-lane list of in-support neighbours before the pair loop
+  bool compact;  // per-lane mask of in-support neighbours before the pair loop
 };
 
 // one instance of a batch as the batched kernels see it (an array of these lives in device
