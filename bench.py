@@ -41,8 +41,9 @@ SUMS_BYTES = {"fp64": 128 + 4 + 4 + 4 + 14 * 8, "fp32c": 64 + 16 + 4 + 4 + 4 + 1
 # and scaled by a 512 MiB copy (profiles/traffic_thermal_contact_150k.md)
 MEASURED_TRAFFIC = {("kb_interact", "fp32c", 0): 661, ("kb_interact", "fp32c", 1): 646,
                     ("kb_interact", "fp64", 0): 830, ("kb_interact", "fp64", 1): 810}
-# whole step, compulsory traffic of a fully fused FP64 step (persistent read-only 32 B,
-# mutable 224 B read + written, 12 B of binning keys)
+# whole step, compulsory traffic of a fully fused step (persistent read-only 32 B, mutable 224 B
+# read + written, 12 B of binning keys); the FP32C path keeps the same FP64 state, so the same
+# 492 B apply to it
 STEP_BYTES_FP64 = 256 + 224 + 12
 
 
@@ -289,6 +290,7 @@ def main() -> int:
                                  and (kname, head, r["tuning"]["compact_pairs"]) in MEASURED_TRAFFIC else None),
                      "bytes_per_particle": kbytes, "ms_per_launch": r["phases"]["interact"]},
         "tuning": r["tuning"],
+        "step_hbm_gbs": N * STEP_BYTES_FP64 / (r["wall"] / args.steps) / 1e9,
         "phases_ms": r["phases"],
         "tool_force": r["force"],
     }
