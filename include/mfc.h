@@ -17,6 +17,8 @@
  *   grid_base::get_cells(particles, start, end)        grid.h:280, grid_gpu_green.cu:207-210
  *                                                      -> mfc_get_cells
  *   tool::get_segments / get_fillet (tool.h:75-76)     -> mfc_get_tool
+ *   tool_wear::eval_usui(particles, dt, min, max, avg) tool_wear.h:53, tool_wear.cu:20-60
+ *       + get_accum_wear (tool_wear.cu:62-64)          -> mfc_tool_wear
  *
  * Conventions: every function returns 0 on success and a negative MFC_E_* code on
  * failure (mfc_last_error() gives the message of the calling thread's last failure);
@@ -194,6 +196,17 @@ int mfc_tool_force(mfc_sim *sim, double F[2]);
  * number of recorded steps (at most 2^20 are kept) */
 int mfc_force_history(mfc_sim *sim, double *F, int64_t cap, int64_t *n);
 int mfc_force_history_reset(mfc_sim *sim);
+
+/* tool_wear::eval_usui on the device, over the current state: the Usui wear rate
+ * K exp(alpha / T) p |v_r| of every particle in contact (|fc.x|, |fc.y| or |ft.x| non-zero: the
+ * reference tests fc.y twice and never ft.y), with v_r = v - v*n (v the particle velocity minus
+ * tool_vel, the product component-wise as in the reference's glm code) and p = |fc| / (2 pi h).
+ * out[0] = min, out[1] = max, out[2] = mean over the particles in contact (all 0 when none);
+ * the handle then accumulates dt * mean (get_accum_wear), returned in out[3]. The reference's
+ * setup passes K = 0.078, alpha = 2500 and tool_vel = (0, tool speed) (benchmarks.cpp:508-512).
+ * The mean's sum is a fixed-order tree over tiles of 256 particles, not the reference's
+ * sequential host loop. */
+int mfc_tool_wear(mfc_sim *sim, double K, double alpha, const double tool_vel[2], double dt, double out[4]);
 
 /* ---- reverse mode ----
  * Gradient of the mean tool force over a window of steps with respect to

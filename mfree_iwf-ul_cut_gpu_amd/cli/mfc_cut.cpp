@@ -5,8 +5,9 @@
 //   -v cutting speed [cm/us]   -n rows of the reference lattice   -q feed [cm]
 // plus -s steps (default: the reference's t_final/dt), -p precision (0 FP64, 1 FP32C) and
 // -o output directory (default "results"). Every last_step/135 steps it appends the tool force
-// "Fx Fy" (%.9g) to <out>/force_<fillet>_<feed>, as mfree_gpu.cpp:262-289 does, and prints the
-// progress line. VTK output and the tool-wear log are not part of this driver.
+// "Fx Fy" (%.9g) to <out>/force_<fillet>_<feed> and the Usui wear "min max mean accumulated"
+// (%e) to <out>/wear_<speed>_<feed>, as mfree_gpu.cpp:238-299 does, and prints the progress
+// line. VTK output is not part of this driver.
 #include <sys/stat.h>
 #include <unistd.h>
 
@@ -67,6 +68,19 @@ int main(int argc, char **argv) {
     mfc_destroy(sim);
     return 1;
   }
+  // the wear log with the reference's Usui constants and tool velocity (benchmarks.cpp:384,508-512)
+  std::snprintf(name, sizeof(name), "%s/wear_%f_%f", out.c_str(), speed, feed);
+  FILE *fw = std::fopen(name, "w+");
+  if (!fw) {
+    std::fprintf(stderr, "cannot open %s\n", name);
+    std::fclose(fp);
+    mfc_destroy(sim);
+    return 1;
+  }
+  const double usui_K = 1. / 100 * 7.8, usui_alpha = 2500.;
+  const double vel_tool = speed == 0. ? 83.333328 * 1e-5 : speed;
+  const double wear_tool_vel[2] = {0., vel_tool};
+  const double time_btwn_dumps = freq * info.dt;
   const auto t0 = std::chrono::steady_clock::now();
   int rc = MFC_OK;
   for (long step = 0; step < last_step + 1 && rc == MFC_OK;) {
@@ -75,6 +89,10 @@ int main(int argc, char **argv) {
       if ((rc = mfc_tool_force(sim, F)) != MFC_OK) break;
       std::fprintf(fp, "%.9g %.9g\n", F[0], F[1]);
       std::fflush(fp);
+      double W[4];
+      if ((rc = mfc_tool_wear(sim, usui_K, usui_alpha, wear_tool_vel, time_btwn_dumps, W)) != MFC_OK) break;
+      std::fprintf(fw, "%e %e %e %e\n", W[0], W[1], W[2], W[3]);
+      std::fflush(fw);
       const double s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
       std::printf("%06ld of %06ld: %04.1f percent done, %f s\n", step, last_step, 100.0 * step / last_step, s);
     }
@@ -86,6 +104,7 @@ int main(int argc, char **argv) {
   }
   if (rc == MFC_OK) rc = mfc_sync(sim);
   std::fclose(fp);
+  std::fclose(fw);
   const double s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   if (rc != MFC_OK) std::fprintf(stderr, "step failed: %s\n", mfc_last_error());
   else std::printf("gpu time: (seconds) %f\n", s);

@@ -503,6 +503,29 @@ int mfc_get_cells(mfc_sim *s, int32_t *offsets, size_t count) {
   return copy_out(s, offsets, s->bins.offsets, sizeof(int) * count);
 }
 
+int mfc_tool_wear(mfc_sim *s, double K, double alpha, const double tool_vel[2], double dt, double out[4]) {
+  if (!s || !tool_vel || !out) return fail(MFC_E_INVALID, "null argument");
+  if (!s->wear_part) {
+    int rc = s->alloc(&s->wear_part, 4 * (size_t)s->ntiles + 4);
+    if (rc) return rc;
+  }
+  double *res = s->wear_part + 4 * (size_t)s->ntiles;
+  mfc::launch_tool_wear(s->k, s->set[s->cur], s->pos, K, alpha, tool_vel[0], tool_vel[1], s->wear_part, res,
+                        s->stream);
+  HIP_TRY(hipGetLastError());
+  double r[4];
+  HIP_TRY(hipMemcpyAsync(r, res, sizeof(r), hipMemcpyDeviceToHost, s->stream));
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  const double n = r[3];
+  const double avg = n == 0. ? 0. : r[2] / n;
+  s->wear_accum += dt * avg;
+  out[0] = n == 0. ? 0. : r[0];
+  out[1] = r[1];
+  out[2] = avg;
+  out[3] = s->wear_accum;
+  return MFC_OK;
+}
+
 int mfc_get_tool(mfc_sim *s, mfc_tool_state *tool) {
   if (!s || !tool) return fail(MFC_E_INVALID, "null argument");
   return copy_out(s, tool, s->tool + s->cur, sizeof(mfc_tool_state));

@@ -146,6 +146,10 @@ void launch_interact_fused(const dev_consts &k, const dstate &cur, const dstate 
 void launch_update(const dev_consts &k, const double *acc, const dbins &b, const dstate &cur, const dstate &nxt,
                    const dpositional &pos, const mfc_tool_state *tool_now, const dforce &f, hipStream_t s);
 void launch_force_final(const dev_consts &k, const dforce &f, hipStream_t s);
+// Usui wear rate of state st: out = {min, max, sum, count} over the particles in contact
+// (part: 4 doubles per tile of scratch)
+void launch_tool_wear(const dev_consts &k, const dstate &st, const dpositional &pos, double K, double alpha,
+                      double tvx, double tvy, double *part, double *out, hipStream_t s);
 // neighbour sums only (ACC_N arrays of N doubles); src_flags in the storage order of cur
 void launch_sums(const dev_consts &k, const int *src_flags, const dbins &b, const drecords &r, double *acc, variant v,
                  hipStream_t s);

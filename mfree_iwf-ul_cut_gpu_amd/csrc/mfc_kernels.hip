@@ -30,6 +30,8 @@
 // mapping is XCD-aware.
 #include <hip/hip_ext.h>
 
+#include <cfloat>
+
 #include "mfc_kernels.h"
 #include "mfc_pairs.h"
 
@@ -906,6 +908,71 @@ __global__ __launch_bounds__(TILE) void k_update(dev_consts k, int ntiles, const
 
 __global__ __launch_bounds__(TILE) void k_force_final(int ntiles, dforce fo) { force_final(ntiles, fo); }
 
+// ------------------------------------------------------------------------------- tool wear
+// Usui wear rate (tool_wear.cu:20-60) of the current state: particles count as in contact when
+// |fc.x|, |fc.y| or |ft.x| is non-zero (the reference tests fc.y twice and never ft.y); the
+// relative velocity is v - v*n with the component-wise product of its glm code, the pressure
+// |fc| / (2 pi h), the rate K exp(alpha / T) p |v_r|. Per tile: min (from FLT_MAX, as the
+// reference), max (from 0), sum and count over the tile's particles in contact, then one
+// work-group folds the tiles in a fixed order. Min, max and count do not depend on the order;
+// the sum is the fixed-order tree instead of the reference's sequential loop.
+__device__ inline void wear_tree(double (*red)[TILE]) {
+  for (int st = TILE / 2; st > 0; st >>= 1) {
+    __syncthreads();
+    if (threadIdx.x < st) {
+      red[0][threadIdx.x] = fmin(red[0][threadIdx.x], red[0][threadIdx.x + st]);
+      red[1][threadIdx.x] = fmax(red[1][threadIdx.x], red[1][threadIdx.x + st]);
+      red[2][threadIdx.x] = red[2][threadIdx.x] + red[2][threadIdx.x + st];
+      red[3][threadIdx.x] = red[3][threadIdx.x] + red[3][threadIdx.x + st];
+    }
+  }
+}
+
+__global__ __launch_bounds__(TILE) void k_wear_tiles(int N, int ntiles, dstate st, const double *__restrict__ nx,
+                                                     const double *__restrict__ ny, double K, double alpha,
+                                                     double tvx, double tvy, double *__restrict__ part) {
+  const int t = xcd_tile(blockIdx.x, ntiles);
+  if (t < 0) return;
+  __shared__ double red[4][TILE];
+  const int i = t * TILE + threadIdx.x;
+  double wmin = (double)FLT_MAX, wmax = 0., wsum = 0., cnt = 0.;
+  if (i < N) {
+    const double fcx = st.fcx[i], fcy = st.fcy[i], ftx = st.ftx[i];
+    if (fabs(fcx) > 0. || fabs(fcy) > 0. || fabs(ftx) > 0. || fabs(fcy) > 0.) {
+      const double vx = st.vx[i] - tvx, vy = st.vy[i] - tvy;
+      const double rx = vx - vx * nx[i], ry = vy - vy * ny[i];
+      const double vmag = sqrt(rx * rx + ry * ry);
+      const double fmag = sqrt(fcx * fcx + fcy * fcy);
+      const double A = 2 * st.h[i] * M_PI;
+      const double p = fmag / A;
+      const double w = K * mfcm::exp(alpha / st.T[i]) * p * vmag;
+      wmin = w; wmax = w; wsum = w; cnt = 1.;
+    }
+  }
+  red[0][threadIdx.x] = wmin; red[1][threadIdx.x] = wmax; red[2][threadIdx.x] = wsum; red[3][threadIdx.x] = cnt;
+  wear_tree(red);
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int c = 0; c < 4; c++) part[4 * t + c] = red[c][0];
+  }
+}
+
+__global__ __launch_bounds__(TILE) void k_wear_final(int ntiles, const double *__restrict__ part,
+                                                     double *__restrict__ out) {
+  __shared__ double red[4][TILE];
+  double wmin = (double)FLT_MAX, wmax = 0., wsum = 0., cnt = 0.;
+  for (int q = threadIdx.x; q < ntiles; q += TILE) {
+    wmin = fmin(wmin, part[4 * q]); wmax = fmax(wmax, part[4 * q + 1]);
+    wsum = wsum + part[4 * q + 2]; cnt = cnt + part[4 * q + 3];
+  }
+  red[0][threadIdx.x] = wmin; red[1][threadIdx.x] = wmax; red[2][threadIdx.x] = wsum; red[3][threadIdx.x] = cnt;
+  wear_tree(red);
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int c = 0; c < 4; c++) out[c] = red[c][0];
+  }
+}
+
 // ------------------------------------------------------------------------------- batched
 // Fields of the instance descriptors read through the constant address space: the compiler
 // then takes the pointers loaded from them for global pointers and emits global loads, which
@@ -1127,6 +1194,14 @@ void launch_update(const dev_consts &k, const double *acc, const dbins &b, const
   const int nt = (k.N + TILE - 1) / TILE;
   hipLaunchKernelGGL(k_update, dim3(padded_tiles(nt)), dim3(TILE), 0, s, k, nt, acc, b.perm, cur, nxt, pos,
                      tool_now, f);
+}
+
+void launch_tool_wear(const dev_consts &k, const dstate &st, const dpositional &pos, double K, double alpha,
+                      double tvx, double tvy, double *part, double *out, hipStream_t s) {
+  const int nt = (k.N + TILE - 1) / TILE;
+  hipLaunchKernelGGL(k_wear_tiles, dim3(padded_tiles(nt)), dim3(TILE), 0, s, k.N, nt, st, pos.nx, pos.ny, K, alpha,
+                     tvx, tvy, part);
+  hipLaunchKernelGGL(k_wear_final, dim3(1), dim3(TILE), 0, s, nt, part, out);
 }
 
 void launch_force_final(const dev_consts &k, const dforce &f, hipStream_t s) {

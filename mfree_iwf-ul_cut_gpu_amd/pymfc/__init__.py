@@ -176,6 +176,7 @@ def lib() -> C.CDLL:
         L.mfc_set_field.argtypes = [vp, i32, vp, sz]
         L.mfc_get_cells.argtypes = [vp, C.POINTER(C.c_int32), sz]
         L.mfc_get_tool.argtypes = [vp, C.POINTER(ToolState)]
+        L.mfc_tool_wear.argtypes = [vp, C.c_double, C.c_double, dp, C.c_double, dp]
         L.mfc_device_math.argtypes = [i32, dp, dp, dp, i64]
         L.mfc_scenario.argtypes = [C.POINTER(Config), C.POINTER(Info), dp, dp, dp, dp, C.POINTER(C.c_int32),
                                    C.POINTER(ToolState)]
@@ -421,6 +422,19 @@ class Sim:
         t = ToolState()
         _check(lib().mfc_get_tool(self._h, C.byref(t)), "mfc_get_tool")
         return t.as_arrays()
+
+    def tool_wear(self, dt: float = 0.0, K: float = 0.078, alpha: float = 2500.0, tool_vel=None) -> np.ndarray:
+        """tool_wear::eval_usui over the current state: (min, max, mean) of the Usui wear rate
+        over the particles in contact and the wear accumulated by this handle (sum of dt * mean).
+        tool_vel defaults to (0, tool speed), as the reference's setup passes it."""
+        if tool_vel is None:
+            v = np.asarray(self.tool()["vel"], dtype=np.float64)
+            tool_vel = (0.0, float(np.hypot(v[0], v[1])))
+        tv = np.ascontiguousarray(tool_vel, dtype=np.float64)
+        out = np.zeros(4)
+        _check(lib().mfc_tool_wear(self._h, float(K), float(alpha), tv.ctypes.data_as(C.POINTER(C.c_double)),
+                                   float(dt), out.ctypes.data_as(C.POINTER(C.c_double))), "mfc_tool_wear")
+        return out
 
 
 def device_math(fn: int, x: np.ndarray, y: np.ndarray | None = None) -> np.ndarray:

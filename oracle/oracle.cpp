@@ -1275,6 +1275,37 @@ int oracle_get_cells(const oracle_sim *o, int32_t *off, size_t count) {
   return MFC_OK;
 }
 
+// tool_wear::eval_usui (tool_wear.cu:20-60): the reference's sequential host loop over the
+// particles in storage order; out = {min, max, mean, count} of the wear rate over the particles
+// in contact (min and mean 0 when none). The accumulation dt * mean is the caller's.
+int oracle_tool_wear(const oracle_sim *o, double K, double alpha, const double tool_vel[2], double out[4]) {
+  if (!o || !tool_vel || !out) return MFC_E_INVALID;
+  const sim_t &s = o->s;
+  double umin = FLT_MAX, umax = 0., total = 0.;
+  int n = 0;
+  for (int i = 0; i < s.N; i++) {
+    const bool in_contact = fabs(s.fc[i].x) > 0. || fabs(s.fc[i].y) > 0. || fabs(s.ft[i].x) > 0. ||
+                            fabs(s.fc[i].y) > 0.;  // tool_wear.cu:36, fc.y tested twice
+    if (!in_contact) continue;
+    n++;
+    const double vx = s.vel[i].x - tool_vel[0], vy = s.vel[i].y - tool_vel[1];
+    const double rx = vx - vx * s.n[i].x, ry = vy - vy * s.n[i].y;  // v - v*n, component-wise
+    const double vmag = sqrt(rx * rx + ry * ry);
+    const double fmag = sqrt(s.fc[i].x * s.fc[i].x + s.fc[i].y * s.fc[i].y);
+    const double A = 2 * s.h[i] * M_PI;
+    const double p = fmag / A;
+    const double w = K * mfcm::exp(alpha / s.T[i]) * p * vmag;
+    total += w;
+    umin = fmin(umin, w);
+    umax = fmax(umax, w);
+  }
+  out[0] = n == 0 ? 0. : umin;
+  out[1] = umax;
+  out[2] = n == 0 ? 0. : total / n;
+  out[3] = n;
+  return MFC_OK;
+}
+
 int oracle_get_tool(const oracle_sim *o, mfc_tool_state *t) {
   if (!o || !t) return MFC_E_INVALID;
   const tool_t &T = o->s.tool;

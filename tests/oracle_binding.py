@@ -36,6 +36,7 @@ def lib() -> C.CDLL:
         L.oracle_set_field.argtypes = [vp, i32, vp, sz]
         L.oracle_get_cells.argtypes = [vp, C.POINTER(C.c_int32), sz]
         L.oracle_get_tool.argtypes = [vp, C.POINTER(ToolState)]
+        L.oracle_tool_wear.argtypes = [vp, C.c_double, C.c_double, dp, dp]
         L.oracle_plastic_iterations.argtypes = [vp]
         L.oracle_plastic_iterations.restype = C.c_int64
         L.oracle_out_of_grid.argtypes = [vp]
@@ -115,6 +116,13 @@ class OracleSim:
         F = np.zeros(2)
         _check(lib().oracle_tool_force(self._h, _p(F)), "oracle_tool_force")
         return F
+
+    def tool_wear(self, K=0.078, alpha=2500.0, tool_vel=(0.0, 0.0)):
+        """(min, max, mean, count) of the Usui wear rate over the particles in contact."""
+        out = np.zeros(4)
+        tv = np.array(tool_vel, dtype=np.float64)
+        _check(lib().oracle_tool_wear(self._h, K, alpha, _p(tv), _p(out)), "oracle_tool_wear")
+        return out
 
     def force_history(self):
         n = C.c_int64()

@@ -1,5 +1,5 @@
 """The command-line driver (mfree_iwf-ul_cut_gpu_amd/cli/mfc_cut.cpp): C++ host code over the C
-ABI with the reference main()'s options and force log (mfree_gpu.cpp:146-323)."""
+ABI with the reference main()'s options, force log and wear log (mfree_gpu.cpp:146-323)."""
 from __future__ import annotations
 
 import os
@@ -40,3 +40,16 @@ def test_cli_force_log_matches_the_library(tmp_path):
     F = s.force_history()
     want = np.vstack([[0.0, 0.0], F[755], F[1511]])  # forces after 0, 756 and 1512 steps
     np.testing.assert_allclose(got, want, rtol=1e-8, atol=0)
+    # the wear log (mfree_gpu.cpp:238-299): min, max, mean and accumulated Usui wear at the same
+    # steps, with the reference's constants and tool velocity (0, v) (benchmarks.cpp:508-512)
+    wl = [f for f in os.listdir(tmp_path) if f.startswith("wear_")]
+    assert len(wl) == 1
+    W = np.loadtxt(os.path.join(tmp_path, wl[0]))
+    assert W.shape == (3, 4)
+    s2 = pymfc.Sim(pymfc.config_ref_cut(ny=6))
+    dt = s2.info.dt
+    want_w = []
+    for n in (0, 756, 756):
+        s2.step(n)
+        want_w.append(s2.tool_wear(dt=756 * dt, K=1.0 / 100 * 7.8, alpha=2500.0))
+    np.testing.assert_allclose(W, np.array(want_w), rtol=1e-5, atol=0)  # printed with %e
