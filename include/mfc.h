@@ -19,6 +19,10 @@
  *   tool::get_segments / get_fillet (tool.h:75-76)     -> mfc_get_tool
  *   tool_wear::eval_usui(particles, dt, min, max, avg) tool_wear.h:53, tool_wear.cu:20-60
  *       + get_accum_wear (tool_wear.cu:62-64)          -> mfc_tool_wear
+ *   vtk_writer_write(particles, step)                  vtk_writer.h, vtk_writer.cpp:20-176
+ *                                                      -> mfc_write_vtk
+ *   vtk_writer_write(tool, step)                       vtk_writer.cpp:183-263
+ *                                                      -> mfc_write_tool_vtk
  *
  * Conventions: every function returns 0 on success and a negative MFC_E_* code on
  * failure (mfc_last_error() gives the message of the calling thread's last failure);
@@ -207,6 +211,15 @@ int mfc_force_history_reset(mfc_sim *sim);
  * The mean's sum is a fixed-order tree over tiles of 256 particles, not the reference's
  * sequential host loop. */
 int mfc_tool_wear(mfc_sim *sim, double K, double alpha, const double tool_vel[2], double dt, double out[4]);
+
+/* vtk_writer_write(particles, step): the current state's particles that are not blanked, as
+ * the reference's legacy ASCII VTK file at path (vertex cells; density, Temperature, Fixed,
+ * Tool, EquivAccumPlasticStrain, Velocity, SvM). Unlike the reference, the blanked test reads
+ * the state's flags (the reference never copies them) and cell k is written point k (the
+ * reference writes the particle's slot, which points past the points once any are skipped). */
+int mfc_write_vtk(mfc_sim *sim, const char *path);
+/* vtk_writer_write(tool, step): the tool's body and fillet as triangles (legacy ASCII VTK) */
+int mfc_write_tool_vtk(mfc_sim *sim, const char *path);
 
 /* ---- reverse mode ----
  * Gradient of the mean tool force over a window of steps with respect to

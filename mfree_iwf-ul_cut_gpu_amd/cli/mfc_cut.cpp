@@ -6,8 +6,8 @@
 // plus -s steps (default: the reference's t_final/dt), -p precision (0 FP64, 1 FP32C) and
 // -o output directory (default "results"). Every last_step/135 steps it appends the tool force
 // "Fx Fy" (%.9g) to <out>/force_<fillet>_<feed> and the Usui wear "min max mean accumulated"
-// (%e) to <out>/wear_<speed>_<feed>, as mfree_gpu.cpp:238-299 does, and prints the progress
-// line. VTK output is not part of this driver.
+// (%e) to <out>/wear_<speed>_<feed>, writes <out>/vtk_out_NNNNNN.vtk and vtk_tool_NNNNNN.vtk
+// (-x: no VTK files), as mfree_gpu.cpp:238-304 does, and prints the progress line.
 #include <sys/stat.h>
 #include <unistd.h>
 
@@ -21,10 +21,11 @@
 int main(int argc, char **argv) {
   double rake = 10., clear = 8., fillet = 0.002, speed = 83.333328e-5 / 500. * 70., feed = 0.01;
   int ny = 30, precision = MFC_PREC_FP64;
+  bool vtk = true;
   long steps = -1;
   std::string out = "results";
   int c;
-  while ((c = getopt(argc, argv, "r:c:f:v:n:q:s:p:o:h")) != -1) {
+  while ((c = getopt(argc, argv, "r:c:f:v:n:q:s:p:o:xh")) != -1) {
     switch (c) {
       case 'r': rake = std::atof(optarg); break;
       case 'c': clear = std::atof(optarg); break;
@@ -35,9 +36,10 @@ int main(int argc, char **argv) {
       case 's': steps = std::atol(optarg); break;
       case 'p': precision = std::atoi(optarg); break;
       case 'o': out = optarg; break;
+      case 'x': vtk = false; break;
       default:
         std::fprintf(stderr, "usage: %s [-r rake] [-c clearance] [-f fillet] [-v speed] [-n ny] [-q feed] "
-                             "[-s steps] [-p 0|1] [-o dir]\n", argv[0]);
+                             "[-s steps] [-p 0|1] [-o dir] [-x]\n", argv[0]);
         return c == 'h' ? 0 : 2;
     }
   }
@@ -83,6 +85,7 @@ int main(int argc, char **argv) {
   const double time_btwn_dumps = freq * info.dt;
   const auto t0 = std::chrono::steady_clock::now();
   int rc = MFC_OK;
+  unsigned print_iter = 0;
   for (long step = 0; step < last_step + 1 && rc == MFC_OK;) {
     if (step % freq == 0) {
       double F[2];
@@ -93,6 +96,14 @@ int main(int argc, char **argv) {
       if ((rc = mfc_tool_wear(sim, usui_K, usui_alpha, wear_tool_vel, time_btwn_dumps, W)) != MFC_OK) break;
       std::fprintf(fw, "%e %e %e %e\n", W[0], W[1], W[2], W[3]);
       std::fflush(fw);
+      if (vtk) {
+        char vname[600];
+        std::snprintf(vname, sizeof(vname), "%s/vtk_out_%06u.vtk", out.c_str(), print_iter);
+        if ((rc = mfc_write_vtk(sim, vname)) != MFC_OK) break;
+        std::snprintf(vname, sizeof(vname), "%s/vtk_tool_%06u.vtk", out.c_str(), print_iter);
+        if ((rc = mfc_write_tool_vtk(sim, vname)) != MFC_OK) break;
+      }
+      print_iter++;
       const double s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
       std::printf("%06ld of %06ld: %04.1f percent done, %f s\n", step, last_step, 100.0 * step / last_step, s);
     }

@@ -177,6 +177,8 @@ def lib() -> C.CDLL:
         L.mfc_get_cells.argtypes = [vp, C.POINTER(C.c_int32), sz]
         L.mfc_get_tool.argtypes = [vp, C.POINTER(ToolState)]
         L.mfc_tool_wear.argtypes = [vp, C.c_double, C.c_double, dp, C.c_double, dp]
+        L.mfc_write_vtk.argtypes = [vp, C.c_char_p]
+        L.mfc_write_tool_vtk.argtypes = [vp, C.c_char_p]
         L.mfc_device_math.argtypes = [i32, dp, dp, dp, i64]
         L.mfc_scenario.argtypes = [C.POINTER(Config), C.POINTER(Info), dp, dp, dp, dp, C.POINTER(C.c_int32),
                                    C.POINTER(ToolState)]
@@ -422,6 +424,14 @@ class Sim:
         t = ToolState()
         _check(lib().mfc_get_tool(self._h, C.byref(t)), "mfc_get_tool")
         return t.as_arrays()
+
+    def write_vtk(self, path: str) -> None:
+        """The particles as the reference's legacy VTK file (vtk_writer_write)."""
+        _check(lib().mfc_write_vtk(self._h, os.fsencode(path)), "mfc_write_vtk")
+
+    def write_tool_vtk(self, path: str) -> None:
+        """The tool as the reference's legacy VTK file of triangles."""
+        _check(lib().mfc_write_tool_vtk(self._h, os.fsencode(path)), "mfc_write_tool_vtk")
 
     def tool_wear(self, dt: float = 0.0, K: float = 0.078, alpha: float = 2500.0, tool_vel=None) -> np.ndarray:
         """tool_wear::eval_usui over the current state: (min, max, mean) of the Usui wear rate

@@ -53,3 +53,8 @@ def test_cli_force_log_matches_the_library(tmp_path):
         s2.step(n)
         want_w.append(s2.tool_wear(dt=756 * dt, K=1.0 / 100 * 7.8, alpha=2500.0))
     np.testing.assert_allclose(W, np.array(want_w), rtol=1e-5, atol=0)  # printed with %e
+    # the VTK files of each dump (mfree_gpu.cpp:285-287)
+    for q in range(3):
+        for kind in ("out", "tool"):
+            head = open(os.path.join(tmp_path, f"vtk_{kind}_{q:06d}.vtk")).readline()
+            assert head.startswith("# vtk DataFile Version 2.0")
