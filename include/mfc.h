@@ -54,6 +54,10 @@ extern "C" {
 #define MFC_THERMAL_BROOKSHAW 0
 #define MFC_THERMAL_PSE       1
 
+/* flow stress of the radial return's secant (plasticity.cuh:72, m_s_yield) */
+#define MFC_JC_PLAIN 0  /* Johnson-Cook (plasticity.cuh:30-46), the reference's choice */
+#define MFC_JC_TANH  1  /* tanh-modified Johnson-Cook after Calamaz 2008 (plasticity.cuh:48-70) */
+
 /* arithmetic of the step */
 #define MFC_PREC_FP64   0   /* plain FP64, bit-exact with the oracle */
 #define MFC_PREC_FP32C  1   /* FP32 neighbour sums with Kahan compensation, FP64 islands */
@@ -104,7 +108,12 @@ typedef struct mfc_config {
                              build (sqrtf, fmaxf, atan2f/cosf/sinf) in FP64; 0: as the reference */
   double secant_tol;      /* absolute tolerance of the radial-return secant; 0 selects the
                              reference's 1e-6 (actions_gpu.cu:261) */
-  int32_t reserved[6];
+  int32_t jc_model;       /* MFC_JC_*: the flow stress the secant solves with. As in the
+                             reference, the yield test and the plastic heat use plain JC */
+  int32_t reserved0;
+  double jc_tanh_a, jc_tanh_b, jc_tanh_c, jc_tanh_d;  /* joco_constants tanh_a..d
+                                                         (constants_structs.h:71-74) */
+  int32_t reserved[4];
 } mfc_config;
 
 typedef struct mfc_info {

@@ -21,6 +21,8 @@ struct dev_consts {
   int thermal_bc, fric_heat, thermal_wp, pse, pse_quirk;
   int promote;        // reference float truncations evaluated in FP64
   double secant_tol;  // absolute stop tolerance of the radial-return secant
+  int jc_tanh;        // the secant's flow stress: tanh-modified JC (the yield test keeps plain JC)
+  double tanh_a, tanh_b, tanh_c, tanh_d;
   int stage_limit;    // most window candidates a tile stages in LDS (0: all it has room for)
 };
 

@@ -124,12 +124,28 @@ __device__ inline double sigma_yield(const dev_consts &k, double eps_pl, double 
   return TA * TB * TC;
 }
 
-// residual of the radial return (plasticity.cuh:74-83)
+// tanh-modified Johnson-Cook flow stress after Calamaz 2008 (plasticity.cuh:48-70)
+__device__ inline double sigma_yield_tanh(const dev_consts &k, double eps_pl, double eps_dot_pl, double t) {
+  double theta = (t - k.jc_Tref) / (k.jc_Tmelt - k.jc_Tref);
+  double TA = k.jc_A + k.jc_B * mfcm::pow(eps_pl, k.jc_n) * 1.0 / mfcm::exp(mfcm::pow(eps_pl, k.tanh_a));
+  double TB = 1.0;
+  double ed = eps_dot_pl / k.jc_eps_dot_ref;
+  if (ed > 1.0) TB = 1.0 + k.jc_C * mfcm::log(ed);
+  else TB = mfcm::pow((1.0 + ed), k.jc_C);
+  double TC = 1.0 - mfcm::pow(theta, k.jc_m);
+  double tD = 1.0 - mfcm::pow((t / k.jc_Tmelt), k.tanh_d);
+  double tS = mfcm::pow(t / k.jc_Tmelt, k.tanh_b);
+  double TD = tD + (1.0 - tD) * mfcm::tanh(1.0 / (mfcm::pow(eps_pl + tS, k.tanh_c)));
+  return TA * TB * TC * TD;
+}
+
+// residual of the radial return (plasticity.cuh:74-83); the flow stress is the model of the
+// function pointer m_s_yield (plasticity.cuh:72), which only this residual uses
 __device__ inline double rr_residual(const dev_consts &k, double dl, double norm, double eps0, double t) {
   const double s23 = sqrt(2.0 / 3.0);
   double e = eps0 + s23 * fmax(dl, 0.);
   double ed = s23 * fmax(dl, 0.) / k.dt;
-  double sY = sigma_yield(k, e, ed, t);
+  double sY = k.jc_tanh ? sigma_yield_tanh(k, e, ed, t) : sigma_yield(k, e, ed, t);
   return norm - 2 * k.G * dl - s23 * sY;
 }
 

@@ -178,6 +178,8 @@ void make_dev_consts(const sim_constants &c, dev_consts &k) {
   k.pse_quirk = c.pse_quirk;
   k.promote = c.promote;
   k.secant_tol = c.secant_tol;
+  k.jc_tanh = c.jc_tanh;
+  k.tanh_a = c.tanh_a; k.tanh_b = c.tanh_b; k.tanh_c = c.tanh_c; k.tanh_d = c.tanh_d;
   k.stage_limit = 0;
 }
 
@@ -222,6 +224,7 @@ int build_scenario(const mfc_config &c, scenario &S, std::string &err) {
   if (!(c.hdx > 0) || !(c.rho0 > 0) || !(c.cfl > 0) || !(c.cp > 0)) { err = "hdx, rho0, cfl and cp must be positive"; return MFC_E_INVALID; }
   if (c.thermal_solver != MFC_THERMAL_BROOKSHAW && c.thermal_solver != MFC_THERMAL_PSE) { err = "unknown thermal solver"; return MFC_E_INVALID; }
   if (c.precision != MFC_PREC_FP64 && c.precision != MFC_PREC_FP32C) { err = "unknown precision"; return MFC_E_INVALID; }
+  if (c.jc_model != MFC_JC_PLAIN && c.jc_model != MFC_JC_TANH) { err = "unknown jc_model"; return MFC_E_INVALID; }
   // lattice (benchmarks.cpp:266-273)
   double hi_y = c.hi_y, lo_x = c.lo_x, hi_x, lo_y, dx;
   int nx, ny = c.ny;
@@ -250,6 +253,8 @@ int build_scenario(const mfc_config &c, scenario &S, std::string &err) {
   k.c0 = c.promote_fp64 ? std::sqrt(k.K / k.rho0) : (double)sqrtf((float)(k.K / k.rho0));
   k.promote = c.promote_fp64 != 0;
   k.secant_tol = (c.secant_tol > 0.) ? c.secant_tol : 1e-6;
+  k.jc_tanh = c.jc_model == MFC_JC_TANH;
+  k.tanh_a = c.jc_tanh_a; k.tanh_b = c.jc_tanh_b; k.tanh_c = c.jc_tanh_c; k.tanh_d = c.jc_tanh_d;
   k.jc_A = c.jc_A; k.jc_B = c.jc_B; k.jc_C = c.jc_C; k.jc_n = c.jc_n; k.jc_m = c.jc_m;
   k.jc_Tmelt = c.jc_Tmelt; k.jc_Tref = c.jc_Tref; k.jc_eps_dot_ref = c.jc_eps_dot_ref;
   k.cp = c.cp; k.tq = c.tq; k.eta_heat = c.eta; k.T_init = c.T_init;

@@ -31,6 +31,8 @@ struct sim_constants {
   int thermal_bc, fric_heat, thermal_wp, pse, pse_quirk;
   int promote;        // float truncations of the reference evaluated in FP64
   double secant_tol;  // radial-return secant tolerance
+  int jc_tanh;        // the secant solves with the tanh-modified flow stress
+  double tanh_a, tanh_b, tanh_c, tanh_d;
 };
 
 struct scenario {

@@ -29,6 +29,7 @@ LIB_PATH = os.path.join(PKG_DIR, "lib", "libmfc.so")
 MFC_OK = 0
 THERMAL_BROOKSHAW, THERMAL_PSE = 0, 1
 PREC_FP64, PREC_FP32C = 0, 1
+JC_PLAIN, JC_TANH = 0, 1  # MFC_JC_*: the flow stress of the radial return's secant
 LATTICE_REF, LATTICE_EXPLICIT = 0, 1
 FLAG_FIXED, FLAG_BLANKED, FLAG_TOOL = 1, 2, 4
 
@@ -60,7 +61,9 @@ class Config(C.Structure):
         ("sample_tool", C.c_int32), ("rho_tool", C.c_double), ("cp_tool", C.c_double), ("k_tool", C.c_double),
         ("thermal_solver", C.c_int32), ("precision", C.c_int32), ("cfl", C.c_double),
         ("pse_flag_quirk", C.c_int32), ("promote_fp64", C.c_int32), ("secant_tol", C.c_double),
-        ("reserved", C.c_int32 * 6),
+        ("jc_model", C.c_int32), ("reserved0", C.c_int32),
+        ("jc_tanh_a", C.c_double), ("jc_tanh_b", C.c_double), ("jc_tanh_c", C.c_double), ("jc_tanh_d", C.c_double),
+        ("reserved", C.c_int32 * 4),
     ]
 
     def copy(self) -> "Config":
@@ -69,7 +72,7 @@ class Config(C.Structure):
         return c
 
     def as_dict(self) -> dict:
-        return {n: getattr(self, n) for n, _ in self._fields_ if n != "reserved"}
+        return {n: getattr(self, n) for n, _ in self._fields_ if not n.startswith("reserved")}
 
 
 class Info(C.Structure):

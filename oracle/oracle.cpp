@@ -312,7 +312,11 @@ inline double lapl_pse(V2 pi, V2 pj, double hi) {  // kernels.cuh:72-89
 }
 
 // ---------------------------------------------------------------- plasticity.cuh
-struct joco_t { double A, B, C, n, m, Tmelt, Tref, eps_dot_ref; };
+struct joco_t {
+  double A, B, C, n, m, Tmelt, Tref, eps_dot_ref;
+  bool tanh = false;                    // m_s_yield = sigma_yield_tanh (plasticity.cuh:72)
+  double ta = 0., tb = 0., tc = 0., td = 0.;  // joco_constants tanh_a..d
+};
 
 inline double sigma_yield(const joco_t &jc, double eps_pl, double eps_pl_dot, double t) {
   double theta = (t - jc.Tref) / (jc.Tmelt - jc.Tref);
@@ -325,11 +329,28 @@ inline double sigma_yield(const joco_t &jc, double eps_pl, double eps_pl_dot, do
   return TA * TB * TC;
 }
 
+// sigma_yield_tanh (plasticity.cuh:48-70), the tanh-modified flow stress of Calamaz 2008
+inline double sigma_yield_tanh(const joco_t &jc, double eps_pl, double eps_dot_pl, double t) {
+  double theta = (t - jc.Tref) / (jc.Tmelt - jc.Tref);
+  double TA = jc.A + jc.B * mfcm::pow(eps_pl, jc.n) * 1.0 / mfcm::exp(mfcm::pow(eps_pl, jc.ta));
+  double TB = 1.0;
+  double ed = eps_dot_pl / jc.eps_dot_ref;
+  if (ed > 1.0) TB = 1.0 + jc.C * mfcm::log(ed);
+  else TB = mfcm::pow((1.0 + ed), jc.C);
+  double TC = 1.0 - mfcm::pow(theta, jc.m);
+  double tD = 1.0 - mfcm::pow((t / jc.Tmelt), jc.td);
+  double tS = mfcm::pow(t / jc.Tmelt, jc.tb);
+  double TD = tD + (1.0 - tD) * mfcm::tanh(1.0 / (mfcm::pow(eps_pl + tS, jc.tc)));
+  return TA * TB * TC * TD;
+}
+
+// opt_fun (plasticity.cuh:74-83) through the function pointer m_s_yield (:72); the yield test
+// and the plastic heat of the step call sigma_yield itself (actions_gpu.cu:253,281)
 inline double opt_fun(const joco_t &jc, double dl, double norm_Strial, double eps_pl_init, double t,
                       double delta_t, double mu) {
   double e = eps_pl_init + std::sqrt(2.0 / 3.0) * std::fmax(dl, 0.);
   double ed = std::sqrt(2.0 / 3.0) * std::fmax(dl, 0.) / delta_t;
-  double sY = sigma_yield(jc, e, ed, t);
+  double sY = jc.tanh ? sigma_yield_tanh(jc, e, ed, t) : sigma_yield(jc, e, ed, t);
   return norm_Strial - 2 * mu * dl - std::sqrt(2.0 / 3.0) * sY;
 }
 
@@ -422,6 +443,8 @@ int setup(sim_t &s, const mfc_config &c) {
   s.mass = dx * dx * s.rho0;
   s.jc.A = c.jc_A; s.jc.B = c.jc_B; s.jc.C = c.jc_C; s.jc.m = c.jc_m; s.jc.n = c.jc_n;
   s.jc.Tref = c.jc_Tref; s.jc.Tmelt = c.jc_Tmelt; s.jc.eps_dot_ref = c.jc_eps_dot_ref;
+  s.jc.tanh = c.jc_model == MFC_JC_TANH;
+  s.jc.ta = c.jc_tanh_a; s.jc.tb = c.jc_tanh_b; s.jc.tc = c.jc_tanh_c; s.jc.td = c.jc_tanh_d;
   s.cp = c.cp; s.tq = c.tq; s.alpha_wp = c.k_wp / (s.rho0 * c.cp); s.eta_heat = c.eta; s.T_init = c.T_init;
   s.alpha_tool = c.k_tool / (c.rho_tool * c.cp_tool);
   s.c_alpha = c.av_alpha; s.c_beta = c.av_beta; s.c_eta = c.av_eta; s.xspheps = c.xsph_eps; s.stresseps = c.stress_eps;
@@ -1336,6 +1359,12 @@ static joco_t mkjc(const double jc[8]) {
   return j;
 }
 // jc = {A, B, C, n, m, Tmelt, Tref, eps_dot_ref}
+// jc: A, B, C, n, m, Tmelt, Tref, eps_dot_ref, tanh_a, tanh_b, tanh_c, tanh_d
+double oracle_sigma_yield_tanh(const double jc[12], double eps_pl, double eps_pl_dot, double T) {
+  joco_t j = mkjc(jc);
+  j.ta = jc[8]; j.tb = jc[9]; j.tc = jc[10]; j.td = jc[11];
+  return sigma_yield_tanh(j, eps_pl, eps_pl_dot, T);
+}
 double oracle_sigma_yield(const double jc[8], double eps_pl, double eps_pl_dot, double T) {
   return sigma_yield(mkjc(jc), eps_pl, eps_pl_dot, T);
 }

@@ -50,6 +50,8 @@ def lib() -> C.CDLL:
         L.oracle_lapl_pse.restype = C.c_double
         L.oracle_sigma_yield.argtypes = [dp, C.c_double, C.c_double, C.c_double]
         L.oracle_sigma_yield.restype = C.c_double
+        L.oracle_sigma_yield_tanh.argtypes = [dp, C.c_double, C.c_double, C.c_double]
+        L.oracle_sigma_yield_tanh.restype = C.c_double
         L.oracle_opt_fun.argtypes = [dp] + [C.c_double] * 6
         L.oracle_opt_fun.restype = C.c_double
         L.oracle_solve_secant.argtypes = [dp] + [C.c_double] * 7 + [C.POINTER(C.c_int)]
@@ -204,6 +206,12 @@ def _jc(jc):
 def sigma_yield(jc, eps, epsd, T):
     a = _jc(jc)
     return lib().oracle_sigma_yield(_p(a), eps, epsd, T)
+
+
+def sigma_yield_tanh(jc, tanh, eps, epsd, T):
+    """jc as for sigma_yield, tanh = (a, b, c, d)."""
+    a = np.concatenate([_jc(jc), np.asarray(tanh, dtype=np.float64)])
+    return lib().oracle_sigma_yield_tanh(_p(a), eps, epsd, T)
 
 
 def opt_fun(jc, dl, norm, eps, T, dt, G):

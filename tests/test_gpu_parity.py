@@ -68,9 +68,14 @@ def test_binning_jittered_lattice():
 
 
 @pytest.mark.parametrize("case", ["ref_cut_thermal_tool", "adiabatic_edge", "pse_jitter", "sharp_tool",
-                                  "contact_thermal", "promoted_tight_secant"])
+                                  "contact_thermal", "promoted_tight_secant", "tanh_jc"])
 def test_step_parity(case):
-    if case == "promoted_tight_secant":
+    if case == "tanh_jc":  # the secant solves with the tanh-modified flow stress (Calamaz 2008)
+        cfg = small_cut(thermal="contact", speed_m_min=1000.0)
+        cfg.jc_model = pymfc.JC_TANH
+        cfg.jc_tanh_a, cfg.jc_tanh_b, cfg.jc_tanh_c, cfg.jc_tanh_d = 2.0, 5.0, 1.0, 0.5
+        steps = 200
+    elif case == "promoted_tight_secant":
         cfg = small_cut(thermal="contact")
         cfg.promote_fp64 = 1
         cfg.secant_tol = 1e-13
@@ -96,6 +101,23 @@ def test_step_parity(case):
     assert_bitwise(g, o, context=case)
     assert equal(g.force_history(), o.force_history())
     assert equal(g.tool_force(), o.tool_force())
+
+
+def test_tanh_jc_changes_the_run():
+    """The tanh model is on the path: a cut that yields differs from its plain-JC run, and the
+    reverse mode, which differentiates plain JC, refuses it."""
+    cfg = small_cut(thermal="contact", speed_m_min=1000.0)
+    plain = pymfc.Sim(cfg)
+    cfg.jc_model = pymfc.JC_TANH
+    cfg.jc_tanh_a, cfg.jc_tanh_b, cfg.jc_tanh_c, cfg.jc_tanh_d = 2.0, 5.0, 1.0, 0.5
+    tanh = pymfc.Sim(cfg)
+    plain.step(100)
+    tanh.step(100)
+    assert plain.field("eps_pl").max() > 0
+    assert not np.array_equal(plain.field("S"), tanh.field("S"))
+    cfg.promote_fp64 = 1
+    with pytest.raises(pymfc.MfcError):
+        pymfc.Sim(cfg).vjp_forward(10, 5, 0, 10)
 
 
 def test_graph_replay_matches_eager():
