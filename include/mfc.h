@@ -6,6 +6,9 @@
  *
  *   setup_ref_cut(ny, &grid, rake, clear, chamfer, speed, feed)  benchmarks.h:48,
  *       benchmarks.cpp:255-537                         -> mfc_config_ref_cut + mfc_create
+ *   setup_rings(nbox, &grid) / setup_impact(nbox, &grid)  benchmarks.h:42,45
+ *                                                      -> lattice MFC_LATTICE_RINGS / _IMPACT,
+ *                                                         ny = nbox, + mfc_create
  *   new leap_frog(N, num_cell)                         leap_frog.h:49, leap_frog.cu:157-166
  *                                                      -> part of mfc_create
  *   leap_frog::step(particles, grid)                   leap_frog.h:48, leap_frog.cu:94-155
@@ -66,6 +69,14 @@ extern "C" {
 #define MFC_LATTICE_REF      0  /* setup_ref_cut: lo_y = hi_y - 3 feed, dx = (hi_y-lo_y)/(ny-1),
                                    nx = (hi_x-lo_x)/dx (benchmarks.cpp:266-273) */
 #define MFC_LATTICE_EXPLICIT 1  /* nx x ny particles at spacing dx, top row at hi_y */
+#define MFC_LATTICE_RINGS    2  /* setup_rings (benchmarks.cpp:155-253): two rings, radii 0.03 and
+                                   0.04, centres at -+0.041, on an ny x ny lattice of spacing
+                                   0.08/(ny-1), moving at +-200 towards each other; fixed grid
+                                   (-0.2,-0.08)-(0.2,0.08); no tool; dt from cfl and speed */
+#define MFC_LATTICE_IMPACT   3  /* setup_impact (benchmarks.cpp:23-153): a disk of radius 0.2 at
+                                   x = -0.25 flying at +1e3 into a wall [0, ny/2 dx) x [-1, 1]
+                                   whose top and bottom rows are fixed, dx = 0.4/(ny-1); fixed
+                                   grid (-0.5,-1.2)-(1,1.2) (benchmarks.cpp:136); no tool */
 
 typedef struct mfc_config {
   /* workpiece lattice */
@@ -110,7 +121,8 @@ typedef struct mfc_config {
                              reference's 1e-6 (actions_gpu.cu:261) */
   int32_t jc_model;       /* MFC_JC_*: the flow stress the secant solves with. As in the
                              reference, the yield test and the plastic heat use plain JC */
-  int32_t reserved0;
+  int32_t elastic;        /* 1: no Johnson-Cook return (the reference without
+                             actions_setup_johnson_cook_constants, setup_rings) */
   double jc_tanh_a, jc_tanh_b, jc_tanh_c, jc_tanh_d;  /* joco_constants tanh_a..d
                                                          (constants_structs.h:71-74) */
   int32_t reserved[4];

@@ -338,6 +338,8 @@ int adj_begin(mfc_sim *s, int nsteps, int checkpoint_every, int w0, int w1) {
     return fail(MFC_E_INVALID, "need nsteps >= 1, checkpoint_every >= 1 and 0 <= w0 < w1 <= nsteps");
   if (s->precision != MFC_PREC_FP64 || !s->k.promote)
     return fail(MFC_E_INVALID, "the adjoint needs precision FP64 and promote_fp64 = 1");
+  if (s->sc.tool.num_seg == 0 || !s->k.plastic)
+    return fail(MFC_E_INVALID, "the adjoint differentiates the cutting scenario (tool and plasticity present)");
   if (s->k.jc_tanh) return fail(MFC_E_INVALID, "the adjoint differentiates the plain Johnson-Cook model (jc_model = 0)");
   int64_t base = 0;
   int rc = mfc_force_history(s, nullptr, 0, &base);

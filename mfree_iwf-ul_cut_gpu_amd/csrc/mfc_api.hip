@@ -227,6 +227,8 @@ int mfc_create(const mfc_config *cfg, mfc_sim **out) {
         up(d.rho, S.rho.data(), sizeof(double) * N) || up(d.h, S.h.data(), sizeof(double) * N) ||
         up(d.T, S.T.data(), sizeof(double) * N) || up(d.flags, S.flags.data(), sizeof(int) * N))
       return bail(fail(MFC_E_HIP, "upload"));
+    if (!S.vx.empty() && (up(d.vx, S.vx.data(), sizeof(double) * N) || up(d.vy, S.vy.data(), sizeof(double) * N)))
+      return bail(fail(MFC_E_HIP, "upload"));
     double *pz[6] = {p.nx, p.ny, p.p, p.rxx, p.rxy, p.ryy};
     for (double *a : pz)
       if (up(a, zero.data(), sizeof(double) * N) != hipSuccess) return bail(fail(MFC_E_HIP, "upload"));

@@ -23,6 +23,7 @@ struct dev_consts {
   double secant_tol;  // absolute stop tolerance of the radial-return secant
   int jc_tanh;        // the secant's flow stress: tanh-modified JC (the yield test keeps plain JC)
   double tanh_a, tanh_b, tanh_c, tanh_d;
+  int plastic;        // Johnson-Cook return on
   int stage_limit;    // most window candidates a tile stages in LDS (0: all it has room for)
 };
 

@@ -219,6 +219,7 @@ __device__ inline double fillet_distance(const mfc_tool_state &t, double qx, dou
 // penetration test and closest tool point / normal (tool_gpu.cu:65-139)
 __device__ inline bool tool_contact(const mfc_tool_state &t, double qx, double qy, double &cx, double &cy, double &nx,
                                     double &ny) {
+  if (t.num_seg == 0) return false;  // no tool (the scenarios without one)
   const bool fil = t.num_seg == 5;
   bool in = true;
   for (int i = 0; i < t.num_seg; i++) {

@@ -644,7 +644,7 @@ __device__ inline void update_particle(const dev_consts &k, int i, int s, const 
       T += k.eta_heat * k.dt * fmag * vmag / (k.cp * k.mass);
     }
   }
-  if (!tool) plasticity(k, pos.p[i], rho, sxx, sxy, syy, szz, eps, epsd, T);
+  if (!tool && k.plastic) plasticity(k, pos.p[i], rho, sxx, sxy, syy, szz, eps, epsd, T);
   if (tool) {  // update_tool_particles (tool_gpu.cu:210-221)
     x += k.dt * k.tool_vx;
     y += k.dt * k.tool_vy;

@@ -180,6 +180,7 @@ void make_dev_consts(const sim_constants &c, dev_consts &k) {
   k.secant_tol = c.secant_tol;
   k.jc_tanh = c.jc_tanh;
   k.tanh_a = c.tanh_a; k.tanh_b = c.tanh_b; k.tanh_c = c.tanh_c; k.tanh_d = c.tanh_d;
+  k.plastic = c.plastic;
   k.stage_limit = 0;
 }
 
@@ -218,32 +219,9 @@ void config_ref_cut(mfc_config &c, int ny, double rake, double clear, double cha
   c.cfl = 0.5;
 }
 
-int build_scenario(const mfc_config &c, scenario &S, std::string &err) {
-  sim_constants &k = S.k;
-  std::memset(&k, 0, sizeof(k));
-  if (!(c.hdx > 0) || !(c.rho0 > 0) || !(c.cfl > 0) || !(c.cp > 0)) { err = "hdx, rho0, cfl and cp must be positive"; return MFC_E_INVALID; }
-  if (c.thermal_solver != MFC_THERMAL_BROOKSHAW && c.thermal_solver != MFC_THERMAL_PSE) { err = "unknown thermal solver"; return MFC_E_INVALID; }
-  if (c.precision != MFC_PREC_FP64 && c.precision != MFC_PREC_FP32C) { err = "unknown precision"; return MFC_E_INVALID; }
-  if (c.jc_model != MFC_JC_PLAIN && c.jc_model != MFC_JC_TANH) { err = "unknown jc_model"; return MFC_E_INVALID; }
-  // lattice (benchmarks.cpp:266-273)
-  double hi_y = c.hi_y, lo_x = c.lo_x, hi_x, lo_y, dx;
-  int nx, ny = c.ny;
-  if (c.lattice == MFC_LATTICE_REF) {
-    if (ny < 2) { err = "ny must be >= 2"; return MFC_E_INVALID; }
-    hi_x = c.hi_x;
-    lo_y = hi_y - 3 * c.feed;
-    dx = (hi_y - lo_y) / (ny - 1);
-    nx = (int)((hi_x - lo_x) / dx);
-  } else if (c.lattice == MFC_LATTICE_EXPLICIT) {
-    if (c.nx < 1 || c.ny < 1 || !(c.dx > 0)) { err = "explicit lattice needs nx, ny >= 1 and dx > 0"; return MFC_E_INVALID; }
-    dx = c.dx; nx = c.nx;
-    lo_y = hi_y - (ny - 1) * dx;
-    hi_x = lo_x + nx * dx;
-  } else {
-    err = "unknown lattice rule";
-    return MFC_E_INVALID;
-  }
-  if ((int64_t)nx * ny < 1 || (int64_t)nx * ny > 100000000) { err = "particle count out of range"; return MFC_E_RANGE; }
+// material, Johnson-Cook, thermal and corrector constants of every scenario
+// (benchmarks.cpp:279-332, :31-62, :160-186)
+void set_constants(const mfc_config &c, double dx, sim_constants &k) {
   const double hdx = c.hdx;
   // material constants (benchmarks.cpp:279-284)
   k.rho0 = c.rho0;
@@ -268,6 +246,41 @@ int build_scenario(const mfc_config &c, scenario &S, std::string &err) {
     double fac = 10 * (M_1_PI) / 7.0 * h1 * h1;
     k.wdeltap = fac * (1 - 1.5 * q * q * (1 - 0.5 * q));
   }
+  k.plastic = c.elastic == 0;
+}
+
+int build_bodies(const mfc_config &c, scenario &S, std::string &err);
+
+int build_scenario(const mfc_config &c, scenario &S, std::string &err) {
+  sim_constants &k = S.k;
+  std::memset(&k, 0, sizeof(k));
+  if (!(c.hdx > 0) || !(c.rho0 > 0) || !(c.cfl > 0) || !(c.cp > 0)) { err = "hdx, rho0, cfl and cp must be positive"; return MFC_E_INVALID; }
+  if (c.thermal_solver != MFC_THERMAL_BROOKSHAW && c.thermal_solver != MFC_THERMAL_PSE) { err = "unknown thermal solver"; return MFC_E_INVALID; }
+  if (c.precision != MFC_PREC_FP64 && c.precision != MFC_PREC_FP32C) { err = "unknown precision"; return MFC_E_INVALID; }
+  if (c.jc_model != MFC_JC_PLAIN && c.jc_model != MFC_JC_TANH) { err = "unknown jc_model"; return MFC_E_INVALID; }
+  // lattice (benchmarks.cpp:266-273)
+  double hi_y = c.hi_y, lo_x = c.lo_x, hi_x, lo_y, dx;
+  int nx, ny = c.ny;
+  if (c.lattice == MFC_LATTICE_REF) {
+    if (ny < 2) { err = "ny must be >= 2"; return MFC_E_INVALID; }
+    hi_x = c.hi_x;
+    lo_y = hi_y - 3 * c.feed;
+    dx = (hi_y - lo_y) / (ny - 1);
+    nx = (int)((hi_x - lo_x) / dx);
+  } else if (c.lattice == MFC_LATTICE_EXPLICIT) {
+    if (c.nx < 1 || c.ny < 1 || !(c.dx > 0)) { err = "explicit lattice needs nx, ny >= 1 and dx > 0"; return MFC_E_INVALID; }
+    dx = c.dx; nx = c.nx;
+    lo_y = hi_y - (ny - 1) * dx;
+    hi_x = lo_x + nx * dx;
+  } else if (c.lattice == MFC_LATTICE_RINGS || c.lattice == MFC_LATTICE_IMPACT) {
+    return build_bodies(c, S, err);
+  } else {
+    err = "unknown lattice rule";
+    return MFC_E_INVALID;
+  }
+  if ((int64_t)nx * ny < 1 || (int64_t)nx * ny > 100000000) { err = "particle count out of range"; return MFC_E_RANGE; }
+  const double hdx = c.hdx;
+  set_constants(c, dx, k);
   // workpiece particles, x-major (benchmarks.cpp:345-357) with optional seeded jitter
   const int n_wp = nx * ny;
   S.px.resize(n_wp); S.py.resize(n_wp);
@@ -372,6 +385,82 @@ int build_scenario(const mfc_config &c, scenario &S, std::string &err) {
   // CFL time step (benchmarks.cpp:530-531)
   k.dt = c.cfl * hdx * dx / (std::sqrt(k.K / k.rho0) + std::sqrt(vel_tool));
   k.t_final = 0.1 / vel_tool;
+  return MFC_OK;
+}
+
+// the reference's verification scenarios: two colliding elastic rings (setup_rings,
+// benchmarks.cpp:155-253) and a disk flying into a wall (setup_impact, benchmarks.cpp:23-153).
+// Neither has a tool: the contact test of an empty tool finds no penetration and the tool force
+// stays zero. Both run on a locked cell grid over the box of the reference's fixed-grid
+// constructor (benchmarks.cpp:218 for the rings, :136 for the impact).
+int build_bodies(const mfc_config &c, scenario &S, std::string &err) {
+  sim_constants &k = S.k;
+  const bool rings = c.lattice == MFC_LATTICE_RINGS;
+  const int nbox = c.ny;
+  if (nbox < 2 || nbox > 20000) { err = "ny (lattice points across a body) must be in [2, 20000]"; return MFC_E_INVALID; }
+  if (!(c.speed > 0)) { err = "speed (the velocity of the CFL rule) must be positive"; return MFC_E_INVALID; }
+  const double hdx = c.hdx;
+  const double ro = rings ? 0.04 : 0.2, ri = 0.03;
+  const double dx = 2 * ro / (nbox - 1);
+  set_constants(c, dx, k);
+  S.px.clear(); S.py.clear(); S.vx.clear(); S.vy.clear();
+  auto push = [&](double x, double y, double vx) {
+    S.px.push_back(x); S.py.push_back(y); S.vx.push_back(vx); S.vy.push_back(0.);
+  };
+  if (rings) {  // benchmarks.cpp:189-205, 238-240: interleaved left/right ring points
+    const double spacing = ro + 0.001;
+    for (int i = 0; i < nbox; i++)
+      for (int j = 0; j < nbox; j++) {
+        double px = -ro + i * dx, py = -ro + j * dx;
+        double dist = std::sqrt(px * px + py * py);
+        if (dist < ro && dist >= ri) {
+          push(px - spacing, py, 0.);
+          push(px + spacing, py, 0.);
+        }
+      }
+    for (size_t q = 0; q < S.px.size(); q++) S.vx[q] = (S.px[q] < 0) ? 200 : -200;
+  } else {  // benchmarks.cpp:64-95, 111-118: the flyer, then the wall column by column
+    const double spacing = ro + 0.05;
+    for (int i = 0; i < nbox; i++)
+      for (int j = 0; j < nbox; j++) {
+        double px = -ro + i * dx, py = -ro + j * dx;
+        if (std::sqrt(px * px + py * py) < ro) push(px - spacing, py, 1e3);
+      }
+    for (int i = 0; i < nbox / 2; i++)
+      for (int j = 0; j < 10 * nbox; j++) {
+        if (j * dx - 1. > 1.) continue;
+        push(i * dx, j * dx - 1., 0.);
+      }
+  }
+  const size_t N = S.px.size();
+  if (N < 1 || N > 100000000) { err = "particle count out of range"; return MFC_E_RANGE; }
+  S.rho.assign(N, c.rho0);
+  S.h.assign(N, hdx * dx);
+  S.T.assign(N, c.T_init);
+  S.flags.assign(N, 0);
+  if (!rings)  // top and bottom of the wall (benchmarks.cpp:120-125)
+    for (size_t q = 0; q < N; q++)
+      if (S.py[q] < -1 + dx / 2 || S.py[q] > 1 - dx / 2) S.flags[q] = MFC_FLAG_FIXED;
+  std::memset(&S.tool, 0, sizeof(S.tool));
+  k.contact_alpha = 0.; k.mu = 0.; k.tool_vx = 0.; k.tool_vy = 0.;
+  k.N = (int)N; k.n_tool = 0;
+  const double bx0 = rings ? -5 * ro : -.5, by0 = rings ? -2 * ro : -1.2;
+  const double bx1 = rings ? 5 * ro : 1., by1 = rings ? 2 * ro : 1.2;
+  k.bbmin_x = bx0 - 1e-6; k.bbmin_y = by0 - 1e-6;
+  k.cell = hdx * dx * 2. / 2.;
+  double gnx = std::ceil(((bx1 + 1e-6) - k.bbmin_x) / k.cell);
+  double gny = std::ceil(((by1 + 1e-6) - k.bbmin_y) / k.cell);
+  if (!(gnx >= 1 && gny >= 1 && gnx * gny <= 1073741824.0)) { err = "cell grid too large"; return MFC_E_RANGE; }
+  k.gnx = (int)gnx; k.gny = (int)gny; k.num_cell = k.gnx * k.gny;
+  // no tool: no thermal contact and no friction heat; conduction in the body if k_wp != 0
+  k.thermal_bc = 0;
+  k.fric_heat = 0;
+  k.thermal_wp = k.alpha_wp != 0.;
+  k.pse = c.thermal_solver == MFC_THERMAL_PSE;
+  k.pse_quirk = c.pse_flag_quirk != 0;
+  // CFL step and end time (benchmarks.cpp:210-211, :98-99)
+  k.dt = c.cfl * hdx * dx / (std::sqrt(k.K / k.rho0) + std::sqrt(c.speed));
+  k.t_final = (rings ? 2.5 : 2.0) * ro / c.speed;
   return MFC_OK;
 }
 

@@ -33,11 +33,13 @@ struct sim_constants {
   double secant_tol;  // radial-return secant tolerance
   int jc_tanh;        // the secant solves with the tanh-modified flow stress
   double tanh_a, tanh_b, tanh_c, tanh_d;
+  int plastic;        // Johnson-Cook return on (the reference's m_plastic, actions_gpu.cu:25)
 };
 
 struct scenario {
   sim_constants k;
   std::vector<double> px, py, rho, h, T;
+  std::vector<double> vx, vy;  // initial velocities; empty: at rest
   std::vector<int> flags;
   mfc_tool_state tool;
 };

@@ -30,7 +30,7 @@ MFC_OK = 0
 THERMAL_BROOKSHAW, THERMAL_PSE = 0, 1
 PREC_FP64, PREC_FP32C = 0, 1
 JC_PLAIN, JC_TANH = 0, 1  # MFC_JC_*: the flow stress of the radial return's secant
-LATTICE_REF, LATTICE_EXPLICIT = 0, 1
+LATTICE_REF, LATTICE_EXPLICIT, LATTICE_RINGS, LATTICE_IMPACT = 0, 1, 2, 3
 FLAG_FIXED, FLAG_BLANKED, FLAG_TOOL = 1, 2, 4
 
 FIELDS = {
@@ -61,7 +61,7 @@ class Config(C.Structure):
         ("sample_tool", C.c_int32), ("rho_tool", C.c_double), ("cp_tool", C.c_double), ("k_tool", C.c_double),
         ("thermal_solver", C.c_int32), ("precision", C.c_int32), ("cfl", C.c_double),
         ("pse_flag_quirk", C.c_int32), ("promote_fp64", C.c_int32), ("secant_tol", C.c_double),
-        ("jc_model", C.c_int32), ("reserved0", C.c_int32),
+        ("jc_model", C.c_int32), ("elastic", C.c_int32),
         ("jc_tanh_a", C.c_double), ("jc_tanh_b", C.c_double), ("jc_tanh_c", C.c_double), ("jc_tanh_d", C.c_double),
         ("reserved", C.c_int32 * 4),
     ]
@@ -220,6 +220,40 @@ def config_ref_cut(ny: int = 30, rake: float = 10.0, clear: float = 8.0, chamfer
     c = Config()
     _check(lib().mfc_config_ref_cut(C.byref(c), ny, rake, clear, chamfer, speed, feed), "mfc_config_ref_cut")
     return c
+
+
+def _body_config(c: Config) -> Config:
+    """Corrector constants and solver settings shared by the verification scenarios."""
+    c.hdx = 1.7
+    c.av_alpha, c.av_beta, c.av_eta, c.xsph_eps, c.stress_eps = 1.0, 1.0, 0.1, 0.5, 0.3
+    c.cp = 1.0  # unused: no plastic heat (tq = 0) and no conduction (k_wp = 0)
+    c.thermal_solver = THERMAL_BROOKSHAW
+    return c
+
+
+def config_rings(nbox: int = 80, precision: int = PREC_FP64) -> Config:
+    """setup_rings (benchmarks.cpp:155-253): two rubber-like rings colliding at 2 x 200, elastic,
+    no tool; nbox = 80 is main()'s commented call (mfree_gpu.cpp:209)."""
+    c = Config()
+    c.lattice, c.ny, c.precision = LATTICE_RINGS, nbox, precision
+    c.E, c.nu, c.rho0 = 1e7, 0.4, 1.0
+    c.elastic = 1
+    c.T_init = 0.0
+    c.cfl, c.speed = 0.5, 170.0  # dt = 0.5 h / (c0 + sqrt(vel_ring)) with vel_ring = 170
+    return _body_config(c)
+
+
+def config_impact(nbox: int = 20, precision: int = PREC_FP64) -> Config:
+    """setup_impact (benchmarks.cpp:23-153): a steel 4340 disk (SI units) at 1e3 into a wall,
+    Johnson-Cook plasticity without heat, no tool; nbox = 20 is main()'s commented call."""
+    c = Config()
+    c.lattice, c.ny, c.precision = LATTICE_IMPACT, nbox, precision
+    c.E, c.nu, c.rho0 = 200e9, 0.29, 7830.0
+    c.jc_A, c.jc_B, c.jc_C, c.jc_m, c.jc_n = 792.0e6, 510.0e6, 0.014, 1.03, 0.26
+    c.jc_Tref, c.jc_Tmelt, c.jc_eps_dot_ref = 273.0, 1793.0, 1.0
+    c.T_init = 273.0
+    c.cfl, c.speed = 0.3, 1e3
+    return _body_config(c)
 
 
 # ---------------------------------------------------------------- SI -> bomb units (cm, us, g, Mbar)

@@ -249,6 +249,9 @@ V2 segment_closest_point(const line_t &l, V2 xq) {
 
 bool establish_contact(const tool_t &T, V2 qp, V2 &cp, V2 &n) {
   int num_seg = (int)T.seg.size();
+  // an empty tool (the reference's default tool(), mfree_gpu.cpp:214-216): its contact constants
+  // are zero, so whatever the test returns the forces are zero; here it is no contact
+  if (num_seg == 0) return false;
   bool has_fillet = num_seg == 5;
   bool in = true;
   for (int i = 0; i < num_seg; i++) in = in && dot(sub(T.seg[i].left, qp), T.seg[i].n) < 0.;
@@ -384,6 +387,7 @@ struct sim_t {
   joco_t jc;
   bool thermal_bc = false, fric_heat = false, thermal_wp = false, pse = false;
   bool promote = false;     // reference float truncations evaluated in FP64
+  bool plastic = true;      // m_plastic (actions_gpu.cu:25, set by actions_setup_johnson_cook_constants)
   double secant_tol = 1e-6; // actions_gpu.cu:261
   // grid (grid.cu:139-158)
   int gnx = 0, gny = 0, num_cell = 0;
@@ -418,9 +422,12 @@ uint64_t splitmix64(uint64_t &s) {
 }
 double u01(uint64_t &s) { return (double)(splitmix64(s) >> 11) * (1.0 / 9007199254740992.0); }
 
+int setup_bodies(sim_t &s, const mfc_config &c);
+
 // setup_ref_cut, generalised through mfc_config (benchmarks.cpp:255-537)
 int setup(sim_t &s, const mfc_config &c) {
   s.cfg = c;
+  if (c.lattice == MFC_LATTICE_RINGS || c.lattice == MFC_LATTICE_IMPACT) return setup_bodies(s, c);
   double hi_y = c.hi_y, lo_x = c.lo_x, hi_x, lo_y, dx;
   int nx, ny = c.ny;
   if (c.lattice == MFC_LATTICE_REF) {
@@ -563,6 +570,110 @@ int setup(sim_t &s, const mfc_config &c) {
   // particle_gpu(pos, vel, rho, T, h, fixed, tool_p, N) (particle_gpu.cu:20-125)
   s.N = N; s.n_tool = n_tool;
   s.pos = P; s.vel = vel; s.rho = rho; s.h = h; s.T = T; s.fixed = fixed; s.tool_p = toolp;
+  V2 z2 = v2(0., 0.); V4 z4 = {0., 0., 0., 0.};
+  s.fc.assign(N, z2); s.ft.assign(N, z2); s.n.assign(N, z2); s.pos_t.assign(N, z2); s.vel_t.assign(N, z2);
+  s.p.assign(N, 0.); s.blanked.assign(N, 0.); s.eps_pl.assign(N, 0.); s.eps_pl_dot.assign(N, 0.);
+  s.rho_t.assign(N, 0.); s.T_t.assign(N, 0.);
+  s.S.assign(N, z4); s.R.assign(N, z4); s.S_t.assign(N, z4); s.S_der.assign(N, z4); s.v_der.assign(N, z4);
+  s.idx.resize(N); s.hash.assign(N, 0); s.id.resize(N);
+  for (int i = 0; i < N; i++) { s.idx[i] = i; s.id[i] = i; }
+  s.pos_init.assign(N, z2); s.vel_init.assign(N, z2); s.S_init.assign(N, z4);
+  s.rho_init.assign(N, 0.); s.T_init_arr.assign(N, 0.);
+  s.cell_start.assign(s.num_cell, -1); s.cell_end.assign(s.num_cell, 0);
+  return MFC_OK;
+}
+
+// setup_rings (benchmarks.cpp:155-253) and setup_impact (benchmarks.cpp:23-153): bodies with an
+// initial velocity, no tool, the grid locked to the box of the reference's fixed-grid constructor
+// (set_geometry, grid.cu:139-158, with the boxes of benchmarks.cpp:218 and :136)
+int setup_bodies(sim_t &s, const mfc_config &c) {
+  const bool rings = c.lattice == MFC_LATTICE_RINGS;
+  const int nbox = c.ny;
+  if (nbox < 2 || nbox > 20000) { g_err = "bad ny"; return MFC_E_INVALID; }
+  if (!(c.speed > 0)) { g_err = "speed must be positive"; return MFC_E_INVALID; }
+  const double hdx = c.hdx;
+  double ro, dx;
+  std::vector<V2> P, vel;
+  if (rings) {
+    double ri = 0.03;
+    ro = 0.04;
+    double spacing = ro + 0.001;
+    dx = 2 * ro / (nbox - 1);
+    for (int i = 0; i < nbox; i++)
+      for (int j = 0; j < nbox; j++) {
+        double px = -ro + i * dx, py = -ro + j * dx;
+        double dist = std::sqrt(px * px + py * py);
+        if (dist < ro && dist >= ri) {
+          P.push_back(v2(px - spacing, py));
+          P.push_back(v2(px + spacing, py));
+        }
+      }
+    for (auto &q : P) vel.push_back(v2((q.x < 0) ? 200 : -200, 0.));
+  } else {
+    ro = 0.2;
+    dx = 2 * ro / (nbox - 1);
+    double spacing = ro + 0.05;
+    for (int i = 0; i < nbox; i++)
+      for (int j = 0; j < nbox; j++) {
+        double px = -ro + i * dx, py = -ro + j * dx;
+        double dist = std::sqrt(px * px + py * py);
+        if (dist < ro) { P.push_back(v2(px - spacing, py)); vel.push_back(v2(1e3, 0.)); }
+      }
+    for (int i = 0; i < nbox / 2; i++)
+      for (int j = 0; j < 10 * nbox; j++) {
+        if (j * dx - 1. > 1.) continue;
+        P.push_back(v2(i * dx, j * dx - 1.));
+        vel.push_back(v2(0., 0.));
+      }
+  }
+  const int N = (int)P.size();
+  s.E = c.E; s.nu = c.nu; s.rho0 = c.rho0;
+  s.G = s.E / (2. * (1. + s.nu));
+  s.K = 2.0 * s.G * (1 + s.nu) / (3 * (1 - 2 * s.nu));
+  s.mass = dx * dx * s.rho0;
+  s.jc.A = c.jc_A; s.jc.B = c.jc_B; s.jc.C = c.jc_C; s.jc.m = c.jc_m; s.jc.n = c.jc_n;
+  s.jc.Tref = c.jc_Tref; s.jc.Tmelt = c.jc_Tmelt; s.jc.eps_dot_ref = c.jc_eps_dot_ref;
+  s.jc.tanh = c.jc_model == MFC_JC_TANH;
+  s.jc.ta = c.jc_tanh_a; s.jc.tb = c.jc_tanh_b; s.jc.tc = c.jc_tanh_c; s.jc.td = c.jc_tanh_d;
+  s.cp = c.cp; s.tq = c.tq; s.alpha_wp = c.k_wp / (s.rho0 * c.cp); s.eta_heat = c.eta; s.T_init = c.T_init;
+  s.alpha_tool = 0.;
+  s.c_alpha = c.av_alpha; s.c_beta = c.av_beta; s.c_eta = c.av_eta; s.xspheps = c.xsph_eps; s.stresseps = c.stress_eps;
+  {
+    double h1 = 1. / (hdx * dx);
+    double q = dx * h1;
+    double fac = 10 * (M_1_PI) / 7.0 * h1 * h1;
+    s.wdeltap = fac * (1 - 1.5 * q * q * (1 - 0.5 * q));
+  }
+  s.plastic = c.elastic == 0;
+  std::vector<double> fixed(N, 0.);
+  if (!rings)
+    for (int i = 0; i < N; i++)
+      if (P[i].y < -1 + dx / 2 || P[i].y > 1 - dx / 2) fixed[i] = 1.;
+  s.tool = tool_t();  // empty: no segments, no fillet, at rest
+  s.contact_alpha = 0.;
+  {
+    double bx0 = rings ? -5 * ro : -.5, by0 = rings ? -2 * ro : -1.2;
+    double bx1 = rings ? 5 * ro : 1., by1 = rings ? 2 * ro : 1.2;
+    s.bbmin_x = bx0 - 1e-6; s.bbmin_y = by0 - 1e-6;
+    double mx = bx1 + 1e-6, my = by1 + 1e-6;
+    s.gdx = hdx * dx * 2. / 2.;
+    s.gnx = (int)(unsigned)std::ceil((mx - s.bbmin_x) / s.gdx);
+    s.gny = (int)(unsigned)std::ceil((my - s.bbmin_y) / s.gdx);
+    int64_t nc = (int64_t)s.gnx * s.gny;
+    if (nc <= 0 || nc > (1LL << 30)) { g_err = "grid too large"; return MFC_E_RANGE; }
+    s.num_cell = (int)nc;
+  }
+  s.thermal_bc = false;
+  s.fric_heat = false;
+  s.thermal_wp = s.alpha_wp != 0.;
+  s.pse = c.thermal_solver == MFC_THERMAL_PSE;
+  s.promote = c.promote_fp64 != 0;
+  s.secant_tol = (c.secant_tol > 0.) ? c.secant_tol : 1e-6;
+  s.dt = c.cfl * hdx * dx / (std::sqrt(s.K / s.rho0) + std::sqrt(c.speed));
+  s.t_final = (rings ? 2.5 : 2.0) * ro / c.speed;
+  s.N = N; s.n_tool = 0;
+  s.pos = P; s.vel = vel;
+  s.rho.assign(N, s.rho0); s.h.assign(N, hdx * dx); s.T.assign(N, c.T_init); s.fixed = fixed; s.tool_p.assign(N, 0.);
   V2 z2 = v2(0., 0.); V4 z4 = {0., 0., 0., 0.};
   s.fc.assign(N, z2); s.ft.assign(N, z2); s.n.assign(N, z2); s.pos_t.assign(N, z2); s.vel_t.assign(N, z2);
   s.p.assign(N, 0.); s.blanked.assign(N, 0.); s.eps_pl.assign(N, 0.); s.eps_pl_dot.assign(N, 0.);
@@ -921,6 +1032,7 @@ void plasticity_johnson_cook(sim_t &s, int64_t *iter_total) {  // actions_gpu.cu
 #pragma omp parallel for schedule(dynamic, 256) reduction(+ : itot)
   for (int i = 0; i < s.N; i++) {
     if (s.tool_p[i] != 0.) continue;
+    if (!s.plastic) continue;  // plasticity_johnson_cook returns at once (actions_gpu.cu:400)
     V4 Sv = s.S[i];
     double Sxx = Sv.x, Syy = Sv.z, Szz = Sv.w, Sxy = Sv.y;
     double norm_Strial = tsqrt(Sxx * Sxx + Syy * Syy + Szz * Szz + Sxy * Sxy, s.promote);

@@ -16,6 +16,9 @@ CASES = {
     "adiabatic_6k": lambda: pymfc.workload("adiabatic_6k"),
     "thermal_contact_150k": lambda: pymfc.workload("thermal_contact_150k"),
     "pse_500k": lambda: pymfc.workload("pse_500k"),
+    "rings_80": lambda: pymfc.config_rings(80),
+    "impact_20": lambda: pymfc.config_impact(20),
+    "impact_57": lambda: pymfc.config_impact(57),
 }
 
 
