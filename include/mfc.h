@@ -7,8 +7,8 @@
  *   setup_ref_cut(ny, &grid, rake, clear, chamfer, speed, feed)  benchmarks.h:48,
  *       benchmarks.cpp:255-537                         -> mfc_config_ref_cut + mfc_create
  *   setup_rings(nbox, &grid) / setup_impact(nbox, &grid)  benchmarks.h:42,45
- *                                                      -> lattice MFC_LATTICE_RINGS / _IMPACT,
- *                                                         ny = nbox, + mfc_create
+ *                                                      -> mfc_config_rings / mfc_config_impact
+ *                                                         + mfc_create
  *   new leap_frog(N, num_cell)                         leap_frog.h:49, leap_frog.cu:157-166
  *                                                      -> part of mfc_create
  *   leap_frog::step(particles, grid)                   leap_frog.h:48, leap_frog.cu:94-155
@@ -190,6 +190,16 @@ int mfc_field_components(int field);
  * the reference's 500 m/min (benchmarks.cpp:384). */
 int mfc_config_ref_cut(mfc_config *cfg, int ny, double rake, double clear, double chamfer,
                        double speed, double feed);
+
+/* the constants of setup_rings(nbox, grid) (benchmarks.cpp:155-253, benchmarks.h:42): rubber-like
+ * rings (E 1e7, nu 0.4, rho0 1), hdx 1.7, no plasticity, CFL 0.5 with the rings' 170 as the
+ * velocity of the time-step rule, FP64. The reference's main() names nbox = 80. */
+int mfc_config_rings(mfc_config *cfg, int nbox);
+
+/* the constants of setup_impact(nbox, grid) (benchmarks.cpp:23-153, benchmarks.h:45): steel 4340
+ * in SI units with its Johnson-Cook constants, no heat, hdx 1.7, CFL 0.3 with the flyer's 1e3,
+ * FP64. The reference's main() names nbox = 20. */
+int mfc_config_impact(mfc_config *cfg, int nbox);
 
 /* setup_ref_cut + particle_gpu + grid_gpu_green + leap_frog construction */
 int mfc_create(const mfc_config *cfg, mfc_sim **out);

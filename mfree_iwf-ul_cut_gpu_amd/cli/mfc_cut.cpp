@@ -4,7 +4,9 @@
 //   -r rake angle [deg]   -c clearance angle [deg]   -f edge radius (fillet) [cm]
 //   -v cutting speed [cm/us]   -n rows of the reference lattice   -q feed [cm]
 // plus -s steps (default: the reference's t_final/dt), -p precision (0 FP64, 1 FP32C) and
-// -o output directory (default "results"). Every last_step/135 steps it appends the tool force
+// -o output directory (default "results"), -b rings|impact for the reference's verification
+// setups instead of the cut (main()'s commented lines, mfree_gpu.cpp:209-210; -n is then nbox,
+// default 80 / 20; no tool: no tool VTK files). Every last_step/135 steps it appends the tool force
 // "Fx Fy" (%.9g) to <out>/force_<fillet>_<feed> and the Usui wear "min max mean accumulated"
 // (%e) to <out>/wear_<speed>_<feed>, writes <out>/vtk_out_NNNNNN.vtk and vtk_tool_NNNNNN.vtk
 // (-x: no VTK files), as mfree_gpu.cpp:238-304 does, and prints the progress line.
@@ -20,12 +22,12 @@
 
 int main(int argc, char **argv) {
   double rake = 10., clear = 8., fillet = 0.002, speed = 83.333328e-5 / 500. * 70., feed = 0.01;
-  int ny = 30, precision = MFC_PREC_FP64;
+  int ny = -1, precision = MFC_PREC_FP64;  // ny: 30 for the cut, nbox 80 / 20 for -b
   bool vtk = true;
   long steps = -1;
-  std::string out = "results";
+  std::string out = "results", bench;
   int c;
-  while ((c = getopt(argc, argv, "r:c:f:v:n:q:s:p:o:xh")) != -1) {
+  while ((c = getopt(argc, argv, "r:c:f:v:n:q:s:p:o:b:xh")) != -1) {
     switch (c) {
       case 'r': rake = std::atof(optarg); break;
       case 'c': clear = std::atof(optarg); break;
@@ -37,16 +39,26 @@ int main(int argc, char **argv) {
       case 'p': precision = std::atoi(optarg); break;
       case 'o': out = optarg; break;
       case 'x': vtk = false; break;
+      case 'b': bench = optarg; break;
       default:
         std::fprintf(stderr, "usage: %s [-r rake] [-c clearance] [-f fillet] [-v speed] [-n ny] [-q feed] "
-                             "[-s steps] [-p 0|1] [-o dir] [-x]\n", argv[0]);
+                             "[-s steps] [-p 0|1] [-o dir] [-x] [-b rings|impact]\n", argv[0]);
         return c == 'h' ? 0 : 2;
     }
   }
   std::printf("arguments received %.9g %.9g %.9g %.9g %.9g\n", rake, clear, fillet, speed, feed);
 
   mfc_config cfg;
-  mfc_config_ref_cut(&cfg, ny, rake, clear, fillet, speed, feed);
+  if (bench == "rings") {
+    mfc_config_rings(&cfg, ny > 0 ? ny : 80);
+  } else if (bench == "impact") {
+    mfc_config_impact(&cfg, ny > 0 ? ny : 20);
+  } else if (bench.empty()) {
+    mfc_config_ref_cut(&cfg, ny > 0 ? ny : 30, rake, clear, fillet, speed, feed);
+  } else {
+    std::fprintf(stderr, "unknown setup %s (rings or impact)\n", bench.c_str());
+    return 2;
+  }
   cfg.precision = precision;
   mfc_sim *sim = nullptr;
   if (mfc_create(&cfg, &sim) != MFC_OK) {
@@ -55,6 +67,8 @@ int main(int argc, char **argv) {
   }
   mfc_info info;
   mfc_get_info(sim, &info);
+  mfc_tool_state tool_state;
+  const bool has_tool = mfc_get_tool(sim, &tool_state) == MFC_OK && tool_state.num_seg > 0;
   const long last_step = steps >= 0 ? steps : (long)(info.t_final / info.dt);
   long freq = (long)(info.t_final / info.dt) / 135;
   if (freq < 1) freq = 1;
@@ -101,7 +115,7 @@ int main(int argc, char **argv) {
         std::snprintf(vname, sizeof(vname), "%s/vtk_out_%06u.vtk", out.c_str(), print_iter);
         if ((rc = mfc_write_vtk(sim, vname)) != MFC_OK) break;
         std::snprintf(vname, sizeof(vname), "%s/vtk_tool_%06u.vtk", out.c_str(), print_iter);
-        if ((rc = mfc_write_tool_vtk(sim, vname)) != MFC_OK) break;
+        if (has_tool && (rc = mfc_write_tool_vtk(sim, vname)) != MFC_OK) break;
       }
       print_iter++;
       const double s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();

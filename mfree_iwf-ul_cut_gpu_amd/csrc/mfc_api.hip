@@ -147,6 +147,18 @@ int mfc_config_ref_cut(mfc_config *cfg, int ny, double rake, double clear, doubl
   return MFC_OK;
 }
 
+int mfc_config_rings(mfc_config *cfg, int nbox) {
+  if (!cfg) return fail(MFC_E_INVALID, "null config");
+  mfc::config_rings(*cfg, nbox);
+  return MFC_OK;
+}
+
+int mfc_config_impact(mfc_config *cfg, int nbox) {
+  if (!cfg) return fail(MFC_E_INVALID, "null config");
+  mfc::config_impact(*cfg, nbox);
+  return MFC_OK;
+}
+
 void mfc_destroy(mfc_sim *s) {
   if (!s) return;
   if (s->stream) (void)hipStreamSynchronize(s->stream);

@@ -251,6 +251,36 @@ void set_constants(const mfc_config &c, double dx, sim_constants &k) {
 
 int build_bodies(const mfc_config &c, scenario &S, std::string &err);
 
+// the corrector constants and solver settings of both verification scenarios
+// (benchmarks.cpp:50-62, :178-186)
+static void config_bodies(mfc_config &c, int lattice, int nbox) {
+  std::memset(&c, 0, sizeof(c));
+  c.lattice = lattice;
+  c.ny = nbox;
+  c.hdx = 1.7;
+  c.av_alpha = 1.; c.av_beta = 1.; c.av_eta = 0.1; c.xsph_eps = 0.5; c.stress_eps = 0.3;
+  c.cp = 1.;  // unused: no plastic heat (tq = 0) and no conduction (k_wp = 0)
+  c.thermal_solver = MFC_THERMAL_BROOKSHAW;
+  c.precision = MFC_PREC_FP64;
+}
+
+void config_rings(mfc_config &c, int nbox) {  // benchmarks.cpp:155-253
+  config_bodies(c, MFC_LATTICE_RINGS, nbox);
+  c.E = 1e7; c.nu = 0.4; c.rho0 = 1.;
+  c.elastic = 1;
+  c.T_init = 0.;
+  c.cfl = 0.5; c.speed = 170.;
+}
+
+void config_impact(mfc_config &c, int nbox) {  // benchmarks.cpp:23-153
+  config_bodies(c, MFC_LATTICE_IMPACT, nbox);
+  c.E = 200e9; c.nu = 0.29; c.rho0 = 7830.0;
+  c.jc_A = 792.0e6; c.jc_B = 510.0e6; c.jc_C = 0.014; c.jc_m = 1.03; c.jc_n = 0.26;
+  c.jc_Tref = 273.0; c.jc_Tmelt = 1793.0; c.jc_eps_dot_ref = 1;
+  c.T_init = c.jc_Tref;
+  c.cfl = 0.3; c.speed = 1e3;
+}
+
 int build_scenario(const mfc_config &c, scenario &S, std::string &err) {
   sim_constants &k = S.k;
   std::memset(&k, 0, sizeof(k));

@@ -55,5 +55,7 @@ void tool_advance(mfc_tool_state &t, double dt);
 
 // the literal constants of setup_ref_cut (benchmarks.cpp:255-332,384-421)
 void config_ref_cut(mfc_config &c, int ny, double rake, double clear, double chamfer, double speed, double feed);
+void config_rings(mfc_config &c, int nbox);
+void config_impact(mfc_config &c, int nbox);
 
 }  // namespace mfc

@@ -165,6 +165,8 @@ def lib() -> C.CDLL:
         vp, i32, i64, dp, sz = C.c_void_p, C.c_int, C.c_int64, C.POINTER(C.c_double), C.c_size_t
         L.mfc_last_error.restype = C.c_char_p
         L.mfc_config_ref_cut.argtypes = [C.POINTER(Config), i32, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double]
+        L.mfc_config_rings.argtypes = [C.POINTER(Config), i32]
+        L.mfc_config_impact.argtypes = [C.POINTER(Config), i32]
         L.mfc_create.argtypes = [C.POINTER(Config), C.POINTER(vp)]
         L.mfc_destroy.argtypes = [vp]
         L.mfc_destroy.restype = None
@@ -222,38 +224,22 @@ def config_ref_cut(ny: int = 30, rake: float = 10.0, clear: float = 8.0, chamfer
     return c
 
 
-def _body_config(c: Config) -> Config:
-    """Corrector constants and solver settings shared by the verification scenarios."""
-    c.hdx = 1.7
-    c.av_alpha, c.av_beta, c.av_eta, c.xsph_eps, c.stress_eps = 1.0, 1.0, 0.1, 0.5, 0.3
-    c.cp = 1.0  # unused: no plastic heat (tq = 0) and no conduction (k_wp = 0)
-    c.thermal_solver = THERMAL_BROOKSHAW
-    return c
-
-
 def config_rings(nbox: int = 80, precision: int = PREC_FP64) -> Config:
     """setup_rings (benchmarks.cpp:155-253): two rubber-like rings colliding at 2 x 200, elastic,
     no tool; nbox = 80 is main()'s commented call (mfree_gpu.cpp:209)."""
     c = Config()
-    c.lattice, c.ny, c.precision = LATTICE_RINGS, nbox, precision
-    c.E, c.nu, c.rho0 = 1e7, 0.4, 1.0
-    c.elastic = 1
-    c.T_init = 0.0
-    c.cfl, c.speed = 0.5, 170.0  # dt = 0.5 h / (c0 + sqrt(vel_ring)) with vel_ring = 170
-    return _body_config(c)
+    _check(lib().mfc_config_rings(C.byref(c), nbox), "mfc_config_rings")
+    c.precision = precision
+    return c
 
 
 def config_impact(nbox: int = 20, precision: int = PREC_FP64) -> Config:
     """setup_impact (benchmarks.cpp:23-153): a steel 4340 disk (SI units) at 1e3 into a wall,
     Johnson-Cook plasticity without heat, no tool; nbox = 20 is main()'s commented call."""
     c = Config()
-    c.lattice, c.ny, c.precision = LATTICE_IMPACT, nbox, precision
-    c.E, c.nu, c.rho0 = 200e9, 0.29, 7830.0
-    c.jc_A, c.jc_B, c.jc_C, c.jc_m, c.jc_n = 792.0e6, 510.0e6, 0.014, 1.03, 0.26
-    c.jc_Tref, c.jc_Tmelt, c.jc_eps_dot_ref = 273.0, 1793.0, 1.0
-    c.T_init = 273.0
-    c.cfl, c.speed = 0.3, 1e3
-    return _body_config(c)
+    _check(lib().mfc_config_impact(C.byref(c), nbox), "mfc_config_impact")
+    c.precision = precision
+    return c
 
 
 # ---------------------------------------------------------------- SI -> bomb units (cm, us, g, Mbar)

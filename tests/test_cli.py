@@ -58,3 +58,26 @@ def test_cli_force_log_matches_the_library(tmp_path):
         for kind in ("out", "tool"):
             head = open(os.path.join(tmp_path, f"vtk_{kind}_{q:06d}.vtk")).readline()
             assert head.startswith("# vtk DataFile Version 2.0")
+
+
+def test_cli_rejects_an_unknown_setup():
+    r = subprocess.run([_cli(), "-b", "spheres"], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 2 and "rings or impact" in r.stderr
+
+
+@pytest.mark.gpu
+def test_cli_runs_the_rings_setup(tmp_path):
+    # -b rings -n 20: main()'s setup_rings line with nbox 20; no tool, so no tool VTK files
+    r = subprocess.run([_cli(), "-b", "rings", "-n", "20", "-s", "40", "-o", str(tmp_path)], capture_output=True,
+                       text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    s = pymfc.Sim(pymfc.config_rings(20))
+    assert f"{s.N} particles (0 tool)" in r.stdout
+    out = sorted(os.listdir(tmp_path))
+    assert any(f.startswith("vtk_out_") for f in out) and not any(f.startswith("vtk_tool_") for f in out)
+    F = np.loadtxt(os.path.join(tmp_path, [f for f in out if f.startswith("force_")][0]), ndmin=2)
+    assert not F.any()
+    s.step(40)
+    head = open(os.path.join(tmp_path, "vtk_out_000000.vtk")).read().split("\n")
+    assert head[4] == "DATASET UNSTRUCTURED_GRID" and head[5] == f"POINTS {s.N} float"
+    s.close()
