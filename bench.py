@@ -108,10 +108,11 @@ def run_path(pymfc, workload: str, precision: int, steps: int, warmup: int, prof
 
 
 def other_configs(pymfc, steps: int, tune_steps: int) -> dict:
-    """The other single-GPU configurations of the cut: the 6k adiabatic cut (150 x 40 at 10 um)
-    and the 500k PSE cut (1250 x 400 at 1.2 um, jittered), both precisions, each autotuned."""
+    """The other single-GPU configurations: the 6k adiabatic cut (150 x 40 at 10 um), the 500k
+    PSE cut (1250 x 400 at 1.2 um, jittered) and the colliding rings of setup_rings on a 600 x 600
+    lattice (246k particles, no tool, elastic), both precisions, each autotuned."""
     out = {}
-    for wl in ("adiabatic_6k", "pse_500k"):
+    for wl in ("adiabatic_6k", "pse_500k", "rings_250k"):
         out[wl] = {}
         for name, prec in (("fp64", pymfc.PREC_FP64), ("fp32c", pymfc.PREC_FP32C)):
             r = run_path(pymfc, wl, prec, steps, 20, 10, tune_steps)

@@ -299,6 +299,8 @@ def workload(name: str, precision: int = PREC_FP64) -> Config:
         return cut_config(150, 40, 10.0, thermal="adiabatic", precision=precision)
     if name == "thermal_contact_150k":  # 750 x 200 at 2 um, Brookshaw + tool thermal contact, 20000 steps
         return cut_config(750, 200, 2.0, thermal="contact", precision=precision)
+    if name == "rings_250k":          # setup_rings on a 600 x 600 lattice: 246,488 particles, elastic
+        return config_rings(600, precision)
     if name == "pse_500k":            # 1250 x 400 at 1.2 um, PSE, 120 m/min, jittered, 10000 steps
         return cut_config(1250, 400, 1.2, thermal="pse", speed_m_min=120.0, jitter=0.01, precision=precision)
     # calibration cuts: at 70 m/min the tool advances 0.04 dx in 400 steps and no particle
