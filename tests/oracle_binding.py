@@ -11,7 +11,9 @@ import pymfc
 from pymfc import Config, Info, ToolState, FIELDS, INT_FIELDS
 
 REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-ORACLE_PATH = os.path.join(REPO, "oracle", "liboracle.so")
+# MFC_ORACLE_LIB selects another build of the oracle (the mutation tests load deliberately
+# broken copies through it)
+ORACLE_PATH = os.environ.get("MFC_ORACLE_LIB") or os.path.join(REPO, "oracle", "liboracle.so")
 
 _lib = None
 
