@@ -26,25 +26,30 @@ sys.path.insert(0, os.path.join(REPO, "mfree_iwf-ul_cut_gpu_amd"))
 METRIC = "particle-steps/s at 150k particles (FP32-comp vs FP64); adjoint/forward ratio"
 HBM_PEAK_GBS = 8000.0
 
-# compulsory bytes per particle of one launch of the fused interaction kernel: the particle's
-# own neighbour record (neighbour re-reads are cache traffic), 26 state doubles read and written,
-# flags read and written, the positional pressure (read) and contact normal (read, written),
-# the sorted cell id and the permutation entry (read), and the next step's cell id and arrival
-# slot (written)
-_STATE_RW = 26 * 8 + 4 + 8 + 16 + 4 + 4 + (26 * 8 + 4 + 16) + 8
-INTERACT_BYTES = {"fp64": 128 + _STATE_RW, "fp32c": 64 + 16 + _STATE_RW}
-# the neighbour-sum kernel of the split variant: own record, flags, cell id and permutation entry
-# read, 14 sums written (8 B each)
-SUMS_BYTES = {"fp64": 128 + 4 + 4 + 4 + 14 * 8, "fp32c": 64 + 16 + 4 + 4 + 4 + 14 * 8}
+# Algorithmic (compulsory) bytes per particle-step of a fully fused step, counted in the
+# reference's storage (FP64 fields, flags as three doubles): the read-only set (h 8, flags 24)
+# and the mutable set (pos 16, vel 16, S 32, rho 8, T 8, eps_pl 8, eps_pl_dot 8, pos_t 16,
+# vel_t 16, S_t 32, rho_t 8, T_t 8, ft 16, fc 16, n 16 = 224) read once, the mutable set written
+# once, and 12 B of binning keys (cell id written and read, index): 256 + 224 + 12 = 492 B.
+STEP_BYTES = 256 + 224 + 12
+# The same 492 B split over the step's kernels, each compulsory byte counted in the first kernel
+# that touches it: the binning passes take the 12 B of keys; predict reads the read-only set and
+# the fields of the leap-frog predictor (h, flags, pos, vel, S, rho, T and their rates: 192 B);
+# the interaction kernel reads the rest of the mutable set (eps_pl, eps_pl_dot, fc, ft, n: 64 B)
+# and writes the whole mutable set (224 B). Everything else a kernel moves (neighbour records,
+# the start state read again by the update, cell tables) is traffic above the algorithmic bytes.
+KERNEL_BYTES = {"binning": 12, "predict": 192, "interact": 64 + 224}
+assert sum(KERNEL_BYTES.values()) == STEP_BYTES
+# the split variant: the sums kernel reads nothing compulsory beyond what the interaction kernel
+# is credited with; its 288 B are credited to the pair (sums + update)
 # HBM bytes per particle of one launch of the fused interaction kernel, by precision and pair
-# loop (compact_pairs), measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE on this workload
-# and scaled by a 512 MiB copy (profiles/traffic_thermal_contact_150k.md)
+# loop, measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE passes and scaled by a 512 MiB copy
+# (profiles/traffic_thermal_contact_150k.md, which names the tree it was measured on); reported
+# as `traffic` only for the tuning it was measured with
 MEASURED_TRAFFIC = {("kb_interact", "fp32c", 0): 661, ("kb_interact", "fp32c", 1): 646,
                     ("kb_interact", "fp64", 0): 830, ("kb_interact", "fp64", 1): 810}
-# whole step, compulsory traffic of a fully fused step (persistent read-only 32 B, mutable 224 B
-# read + written, 12 B of binning keys); the FP32C path keeps the same FP64 state, so the same
-# 492 B apply to it
-STEP_BYTES_FP64 = 256 + 224 + 12
+MEASURED_TRAFFIC_SOURCE = ("rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, thermal_contact_150k, fused interaction, "
+                           "profiles/traffic_thermal_contact_150k.md")
 
 
 def cpu_baseline(cfg, target_s: float) -> dict:
@@ -95,6 +100,8 @@ def run_path(pymfc, workload: str, precision: int, steps: int, warmup: int, prof
     N = info.n_particles
     sim.step(warmup)
     tuning = tune(pymfc, sim, tune_steps)
+    if tune_steps <= 0:  # the autotuner leaves its choice with its graphs built
+        sim.step(2 * pymfc.GRAPH_STEPS)
     sim.sync()
     t0 = time.perf_counter()
     sim.step(steps)
@@ -264,10 +271,17 @@ def main() -> int:
     r = res[head]
     N = r["N"]
     split = r["tuning"]["split_interact"]
-    kname = "kb_sums" if split else "kb_interact"
-    kbytes = SUMS_BYTES[head] if split else INTERACT_BYTES[head]
-    t_int = r["phases"]["interact"] * 1e-3
+    # the dominant kernel: the fused interaction, or the sums + update pair of the split variant
+    kname = "kb_sums+kb_update" if split else "kb_interact"
+    kbytes = KERNEL_BYTES["interact"]
+    t_int = (r["phases"]["interact"] + r["phases"]["update"]) * 1e-3
     achieved = N * kbytes / t_int / 1e9
+    tkey = ("kb_interact", head, r["tuning"]["compact_pairs"])
+    traffic = None
+    if args.workload == "thermal_contact_150k" and not split and r["tuning"]["stage_limit"] == 0 \
+            and tkey in MEASURED_TRAFFIC:
+        traffic = MEASURED_TRAFFIC[tkey] * N
+    step_s = r["wall"] / args.steps
     result = {
         "metric": METRIC,
         "value": r["value"],
@@ -285,20 +299,21 @@ def main() -> int:
                    "split_interact": split, "compact_pairs": r["tuning"]["compact_pairs"],
                    "num_cell": r["info"].num_cell, "precision": head, "thermal": "brookshaw+tool-contact"},
         "roofline": {"bound": "hbm", "kernel": kname, "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": (MEASURED_TRAFFIC[(kname, head, r["tuning"]["compact_pairs"])] * N
-                                 if args.workload == "thermal_contact_150k" and r["tuning"]["stage_limit"] == 0
-                                 and (kname, head, r["tuning"]["compact_pairs"]) in MEASURED_TRAFFIC else None),
-                     "bytes_per_particle": kbytes, "ms_per_launch": r["phases"]["interact"]},
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "traffic_source": MEASURED_TRAFFIC_SOURCE if traffic is not None else None,
+                     "bytes_per_particle": kbytes, "ms_per_launch": t_int * 1e3,
+                     "step": {"bytes_per_particle": STEP_BYTES, "achieved": N * STEP_BYTES / step_s / 1e9,
+                              "frac": N * STEP_BYTES / step_s / 1e9 / HBM_PEAK_GBS,
+                              "kernel_bytes_per_particle": KERNEL_BYTES}},
         "tuning": r["tuning"],
-        "step_hbm_gbs": N * STEP_BYTES_FP64 / (r["wall"] / args.steps) / 1e9,
+        "step_hbm_gbs": N * STEP_BYTES / step_s / 1e9,
         "phases_ms": r["phases"],
         "tool_force": r["force"],
     }
     if "fp64" in res:
         result["fp64"] = {"value": res["fp64"]["value"], "ms_per_step": res["fp64"]["wall"] / args.steps * 1e3,
                           "phases_ms": res["fp64"]["phases"], "tuning": res["fp64"]["tuning"],
-                          "step_hbm_gbs": N * STEP_BYTES_FP64 / (res["fp64"]["wall"] / args.steps) / 1e9}
+                          "step_hbm_gbs": N * STEP_BYTES / (res["fp64"]["wall"] / args.steps) / 1e9}
         if head == "fp32c":
             result["fp32c_over_fp64"] = res["fp32c"]["value"] / res["fp64"]["value"]
     result["adjoint_over_forward"] = None

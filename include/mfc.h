@@ -137,6 +137,10 @@ typedef struct mfc_info {
   double mass;            /* particle mass dx^2 rho0 */
   double bbmin_x, bbmin_y, cell_size;
   int64_t step;           /* number of completed steps (global_step) */
+  int64_t out_of_grid;    /* particles binned so far whose position lay outside the grid on
+                             an axis (counted once per binning). Their cell id is the
+                             reference's ix * ny + iy, clamped into the cell table; such a
+                             particle loses neighbours, so a valid run keeps this at 0 */
 } mfc_info;
 
 /* particle fields, returned in the handle's current (cell-sorted) order */

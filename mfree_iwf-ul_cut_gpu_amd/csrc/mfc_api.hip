@@ -116,14 +116,24 @@ int alloc_state(mfc_sim *s, mfc::dstate &d, int N) {
   return MFC_OK;
 }
 
+// captures GRAPH_STEPS steps from buffer parity `parity`. The capture is always ended, and the
+// handle keeps the executable graph only when capture and instantiation both succeeded (a
+// failed instantiation leaves no half-made graph behind for a later launch).
 int build_graph(mfc_sim *s, int parity) {
-  hipGraph_t g = nullptr;
   HIP_TRY(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
   for (int q = 0; q < GRAPH_STEPS; q++) enqueue_step(s, (parity + q) & 1);
-  HIP_TRY(hipStreamEndCapture(s->stream, &g));
-  hipError_t e = hipGraphInstantiate(&s->graph[parity], g, nullptr, nullptr, 0);
+  hipGraph_t g = nullptr;
+  hipError_t e = hipStreamEndCapture(s->stream, &g);
+  if (e != hipSuccess || g == nullptr) {
+    if (g) (void)hipGraphDestroy(g);
+    (void)hipGetLastError();
+    return fail(MFC_E_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+  }
+  hipGraphExec_t x = nullptr;
+  e = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
   (void)hipGraphDestroy(g);
   if (e != hipSuccess) return fail(MFC_E_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+  s->graph[parity] = x;
   return MFC_OK;
 }
 
@@ -424,6 +434,10 @@ int mfc_get_info(const mfc_sim *s, mfc_info *info) {
   info->dt = c.dt; info->t_final = c.t_final; info->mass = c.mass;
   info->bbmin_x = c.bbmin_x; info->bbmin_y = c.bbmin_y; info->cell_size = c.cell;
   info->step = s->step;
+  unsigned long long oog = 0;
+  HIP_TRY(hipMemcpyAsync(&oog, s->bins.clamped, sizeof(oog), hipMemcpyDeviceToHost, s->stream));
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  info->out_of_grid = (int64_t)oog;
   return MFC_OK;
 }
 

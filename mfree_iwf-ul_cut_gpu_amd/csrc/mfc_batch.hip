@@ -57,14 +57,23 @@ void batch_release(mfc_batch *b) {
   delete b;
 }
 
+// as build_graph of a single handle: the capture is always ended, and the batch keeps the
+// executable graph only when capture and instantiation both succeeded
 int build_batch_graph(mfc_batch *b, int parity) {
-  hipGraph_t g = nullptr;
   HIP_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
   for (int q = 0; q < GRAPH_STEPS; q++) mfc::launch_batch_step(b->work((parity + q) & 1), b->stream);
-  HIP_TRY(hipStreamEndCapture(b->stream, &g));
-  hipError_t e = hipGraphInstantiate(&b->graph[parity], g, nullptr, nullptr, 0);
+  hipGraph_t g = nullptr;
+  hipError_t e = hipStreamEndCapture(b->stream, &g);
+  if (e != hipSuccess || g == nullptr) {
+    if (g) (void)hipGraphDestroy(g);
+    (void)hipGetLastError();
+    return fail(MFC_E_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+  }
+  hipGraphExec_t x = nullptr;
+  e = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
   (void)hipGraphDestroy(g);
   if (e != hipSuccess) return fail(MFC_E_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+  b->graph[parity] = x;
   return MFC_OK;
 }
 

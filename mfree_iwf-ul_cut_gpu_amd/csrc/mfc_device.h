@@ -51,14 +51,17 @@ __device__ inline int64_t cell_coord(double f) {
   return (int64_t)(int)f;
 }
 
-// cell id of a position (grid.cu:34-51); out-of-grid ids are clamped and reported
-__device__ inline int cell_of(const dev_consts &k, double x, double y, bool &clamped) {
+// cell id of a position (grid.cu:34-51). A position outside the grid on either axis is
+// reported in `outside` (the reference's id then points into another column or past the cell
+// table); an id outside the table is clamped into it, where the reference would index out of
+// bounds.
+__device__ inline int cell_of(const dev_consts &k, double x, double y, bool &outside) {
   int64_t ix = cell_coord((x - k.bbmin_x) / k.cell);
   int64_t iy = cell_coord((y - k.bbmin_y) / k.cell);
   int64_t h = ix * k.gny + iy;
-  clamped = false;
-  if (h < 0) { h = 0; clamped = true; }
-  if (h >= k.num_cell) { h = k.num_cell - 1; clamped = true; }
+  outside = ix < 0 || ix >= k.gnx || iy < 0 || iy >= k.gny;
+  if (h < 0) h = 0;
+  if (h >= k.num_cell) h = k.num_cell - 1;
   return (int)h;
 }
 

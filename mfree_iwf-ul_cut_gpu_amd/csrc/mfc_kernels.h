@@ -29,7 +29,7 @@ struct dbins {
   int *bucket;       // particle indices grouped by cell, unordered inside a cell
   int *perm;         // sorted position -> current position
   int *bsum;         // scan block sums
-  unsigned long long *clamped;  // particles whose cell id was clamped into the grid
+  unsigned long long *clamped;  // binned particles whose position lay outside the grid on an axis
 };
 
 // positional (not permuted) arrays, as particle_gpu::{n,p,R} (particle_gpu.h:39-47)

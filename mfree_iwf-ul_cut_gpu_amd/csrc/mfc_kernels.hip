@@ -72,7 +72,7 @@ __device__ inline int block_exclusive_scan(int v, int &total) {
 // (index from blockIdx) and by the batched kernel (index from the (instance, block) work table).
 
 // cell id of a particle for the binning (grid.cu:34-51; blanked particles keep their last
-// sorted id, grid.cu:39-41); clamped ids are counted
+// sorted id, grid.cu:39-41); positions outside the grid are counted
 __device__ inline int particle_cell(const dev_consts &k, double x, double y, int fl, int sorted_id,
                                     unsigned long long *__restrict__ clamped) {
   if (fl & MFC_FLAG_BLANKED) return sorted_id;

@@ -237,7 +237,9 @@ void set_constants(const mfc_config &c, double dx, sim_constants &k) {
   k.jc_Tmelt = c.jc_Tmelt; k.jc_Tref = c.jc_Tref; k.jc_eps_dot_ref = c.jc_eps_dot_ref;
   k.cp = c.cp; k.tq = c.tq; k.eta_heat = c.eta; k.T_init = c.T_init;
   k.alpha_wp = c.k_wp / (c.rho0 * c.cp);
-  k.alpha_tool = c.k_tool / (c.rho_tool * c.cp_tool);
+  // the tool's diffusivity; 0 without tool material (the body scenarios zero it), which also
+  // keeps the thermal boundary condition off as the reference leaves it without a tool
+  k.alpha_tool = (c.rho_tool * c.cp_tool > 0.) ? c.k_tool / (c.rho_tool * c.cp_tool) : 0.;
   k.av_alpha = c.av_alpha; k.av_beta = c.av_beta; k.av_eta = c.av_eta;
   k.xsph_eps = c.xsph_eps; k.stress_eps = c.stress_eps;
   {  // W(dx) of the cubic spline (benchmarks.cpp:327-332)

@@ -81,7 +81,7 @@ class Info(C.Structure):
         ("n_particles", C.c_int32), ("n_tool", C.c_int32), ("grid_nx", C.c_int32), ("grid_ny", C.c_int32),
         ("num_cell", C.c_int32), ("precision", C.c_int32), ("dt", C.c_double), ("t_final", C.c_double),
         ("mass", C.c_double), ("bbmin_x", C.c_double), ("bbmin_y", C.c_double), ("cell_size", C.c_double),
-        ("step", C.c_int64),
+        ("step", C.c_int64), ("out_of_grid", C.c_int64),
     ]
 
 
@@ -113,33 +113,48 @@ class VjpReq(C.Structure):
 
 
 STAGE_CAP = 2048  # MFC_STAGE_CAP: window candidates a tile can stage in LDS
+GRAPH_STEPS = 8   # steps per captured hipGraph of a handle (one graph per buffer parity)
 
 
 # the launch configurations the autotuner tries: fused or split interaction, direct or compacted
-# pair loop, and the LDS stage (0: tiles stage up to STAGE_CAP candidates; 1: no tile stages)
-TUNING_SPACE = {"split_interact": (0, 1), "compact_pairs": (0, 1), "stage_limit": (0, 1)}
+# pair loop, and the LDS stage limit (0: tiles stage up to STAGE_CAP candidates; n > 0: tiles
+# whose candidates span more than n sorted indices read them from memory; 1: no tile stages)
+TUNING_SPACE = {"split_interact": (0, 1), "compact_pairs": (0, 1), "stage_limit": (0, 1536, 1)}
 
 
-def autotune(handle, steps: int = 100, space: dict | None = None) -> dict:
+def autotune(handle, steps: int = 100, space: dict | None = None, rounds: int = 2) -> dict:
     """Times every combination of the launch configuration space on the live handle (a Sim or a
     Batch; every configuration gives the same results, the run just advances) over `steps`
-    steps each, and keeps the fastest. Returns the chosen settings and the ms per step of each
-    candidate."""
+    steps each, and keeps the fastest. Each candidate first runs 2 * GRAPH_STEPS untimed steps,
+    so that the hipGraph of the handle's buffer parity (unchanged by an even number of steps) is
+    captured outside the timed window, and the
+    candidates are timed in `rounds` interleaved passes (the run's contact state evolves under
+    them), each keeping its minimum. The handle is left on the chosen settings with its graphs
+    built. Returns the chosen settings, the ms per step of each candidate and the number of
+    steps the handle advanced (steps_run)."""
     space = TUNING_SPACE if space is None else space
     keys = list(space)
-    cand = {}
-    for combo in itertools.product(*(space[k] for k in keys)):
-        handle.set_tuning(**dict(zip(keys, combo)))
-        handle.step(4)
-        handle.sync()
-        t0 = time.perf_counter()
-        handle.step(steps)
-        handle.sync()
-        cand[combo] = (time.perf_counter() - t0) / steps * 1e3
+    combos = list(itertools.product(*(space[k] for k in keys)))
+    cand = {c: float("inf") for c in combos}
+    ran = 0
+    for _ in range(max(1, rounds)):
+        for combo in combos:
+            handle.set_tuning(**dict(zip(keys, combo)))
+            handle.step(2 * GRAPH_STEPS)
+            handle.sync()
+            t0 = time.perf_counter()
+            handle.step(steps)
+            handle.sync()
+            cand[combo] = min(cand[combo], (time.perf_counter() - t0) / steps * 1e3)
+            ran += 2 * GRAPH_STEPS + steps
     best = min(cand, key=cand.get)
     handle.set_tuning(**dict(zip(keys, best)))
+    handle.step(2 * GRAPH_STEPS)
+    handle.sync()
+    ran += 2 * GRAPH_STEPS
     label = {"split_interact": "split", "compact_pairs": "compact", "stage_limit": "stage"}
     out = dict(zip(keys, best))
+    out["steps_run"] = ran
     out["ms_per_step"] = {"_".join(f"{label.get(k, k)}{v}" for k, v in zip(keys, c)): ms for c, ms in cand.items()}
     return out
 
@@ -328,6 +343,7 @@ class SimInfo:
     bbmin_y: float
     cell_size: float
     step: int
+    out_of_grid: int  # particles binned outside the grid so far (0 in a valid run)
 
 
 class Sim:

@@ -453,7 +453,7 @@ int setup(sim_t &s, const mfc_config &c) {
   s.jc.tanh = c.jc_model == MFC_JC_TANH;
   s.jc.ta = c.jc_tanh_a; s.jc.tb = c.jc_tanh_b; s.jc.tc = c.jc_tanh_c; s.jc.td = c.jc_tanh_d;
   s.cp = c.cp; s.tq = c.tq; s.alpha_wp = c.k_wp / (s.rho0 * c.cp); s.eta_heat = c.eta; s.T_init = c.T_init;
-  s.alpha_tool = c.k_tool / (c.rho_tool * c.cp_tool);
+  s.alpha_tool = (c.rho_tool * c.cp_tool > 0.) ? c.k_tool / (c.rho_tool * c.cp_tool) : 0.;
   s.c_alpha = c.av_alpha; s.c_beta = c.av_beta; s.c_eta = c.av_eta; s.xspheps = c.xsph_eps; s.stresseps = c.stress_eps;
   {
     double h1 = 1. / (hdx * dx);
@@ -704,7 +704,10 @@ void assign_hashes(sim_t &s) {  // grid.cu:34-51
     int64_t ix = cell_coord((s.pos[i].x - s.bbmin_x) / s.gdx);
     int64_t iy = cell_coord((s.pos[i].y - s.bbmin_y) / s.gdx);
     int64_t hsh = ix * s.gny + iy;
-    if (hsh < 0 || hsh >= s.num_cell) { hsh = hsh < 0 ? 0 : s.num_cell - 1; oog++; }
+    // outside the grid on an axis: counted (the id itself is the reference's, clamped only
+    // where it would leave the cell table)
+    if (ix < 0 || ix >= s.gnx || iy < 0 || iy >= s.gny) oog++;
+    if (hsh < 0 || hsh >= s.num_cell) hsh = hsh < 0 ? 0 : s.num_cell - 1;
     s.hash[i] = (int)hsh;
   }
   s.out_of_grid += oog;
@@ -1294,6 +1297,7 @@ int oracle_get_info(const oracle_sim *o, mfc_info *info) {
   info->dt = s.dt; info->t_final = s.t_final; info->mass = s.mass;
   info->bbmin_x = s.bbmin_x; info->bbmin_y = s.bbmin_y; info->cell_size = s.gdx;
   info->step = s.step;
+  info->out_of_grid = s.out_of_grid;
   return MFC_OK;
 }
 

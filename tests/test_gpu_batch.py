@@ -40,17 +40,43 @@ def test_batch_equals_single_runs(prec, split, compact):
     b.close()
 
 
-def test_c5_batch_of_32_matches_single_runs():
-    cfgs = pymfc.batch_configs(32)
+@pytest.mark.parametrize("prec", [pymfc.PREC_FP64, pymfc.PREC_FP32C])
+def test_c5_batch_of_32_matches_single_runs(prec):
+    cfgs = pymfc.batch_configs(32, precision=prec)
     b = pymfc.Batch(cfgs)
     b.step(20)
     b.sync()
     n = [inst.N for inst in b.instances]
     print("instances", len(n), "particles", sum(n), "min", min(n), "max", max(n))
-    for i in (0, 7, 15, 31):  # a sample of the 32 single runs
+    for i in range(len(cfgs)):  # every one of the 32 single runs
         s = pymfc.Sim(cfgs[i])
         s.step(20)
         compare_instance(b[i], s, f"instance {i}")
+        assert b[i].info.out_of_grid == 0
+        s.close()
+    b.close()
+
+
+def test_c5_batch_reverse_mode_of_32_equals_single_runs():
+    """The reverse mode of all 32 instances of the seeded batch (promoted FP64, tight secant):
+    mean forces and gradients equal each instance's own single run."""
+    cfgs = []
+    for c in pymfc.batch_configs(32):
+        c.promote_fp64 = 1
+        c.secant_tol = 1e-14
+        cfgs.append(c)
+    b = pymfc.Batch(cfgs)
+    nsteps, K, w0, w1 = 40, 10, 10, 40
+    F = b.vjp_forward(nsteps, K, w0, w1)
+    rng = np.random.default_rng(32)
+    cot = rng.normal(size=(32, 2))
+    G = b.vjp_backward(cot)
+    for i, c in enumerate(cfgs):
+        s = pymfc.Sim(c)
+        Fi = s.vjp_forward(nsteps, K, w0, w1)
+        Gi = s.vjp_backward(cot[i])
+        assert equal(F[i], Fi), i
+        assert equal(G[i], Gi), i
         s.close()
     b.close()
 

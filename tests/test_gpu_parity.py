@@ -67,10 +67,23 @@ def test_binning_jittered_lattice():
     assert equal(g.cells(), o.cells())
 
 
+# particles that have yielded (plastic strain > 0) at the end of each parity case, at least: the
+# faithful plain Johnson-Cook return (float truncations, plastic heating, thermal softening) is
+# exercised by the cuts at 1000 m/min
+MIN_YIELDED = {"plastic_contact_1000": 60, "plastic_adiabatic_1000": 50, "tanh_jc": 40}
+
+
 @pytest.mark.parametrize("case", ["ref_cut_thermal_tool", "adiabatic_edge", "pse_jitter", "sharp_tool",
-                                  "contact_thermal", "promoted_tight_secant", "tanh_jc"])
+                                  "contact_thermal", "promoted_tight_secant", "tanh_jc",
+                                  "plastic_contact_1000", "plastic_adiabatic_1000"])
 def test_step_parity(case):
-    if case == "tanh_jc":  # the secant solves with the tanh-modified flow stress (Calamaz 2008)
+    if case == "plastic_contact_1000":  # faithful plain JC in a thermal-contact cut that yields
+        cfg = small_cut(thermal="contact", speed_m_min=1000.0)
+        steps = 400
+    elif case == "plastic_adiabatic_1000":
+        cfg = small_cut(thermal="adiabatic", speed_m_min=1000.0)
+        steps = 300
+    elif case == "tanh_jc":  # the secant solves with the tanh-modified flow stress (Calamaz 2008)
         cfg = small_cut(thermal="contact", speed_m_min=1000.0)
         cfg.jc_model = pymfc.JC_TANH
         cfg.jc_tanh_a, cfg.jc_tanh_b, cfg.jc_tanh_c, cfg.jc_tanh_d = 2.0, 5.0, 1.0, 0.5
@@ -101,6 +114,10 @@ def test_step_parity(case):
     assert_bitwise(g, o, context=case)
     assert equal(g.force_history(), o.force_history())
     assert equal(g.tool_force(), o.tool_force())
+    assert g.info.out_of_grid == 0 and o.info.out_of_grid == 0
+    yielded = int((o.field("eps_pl") > 0).sum())
+    print(case, "yielded", yielded, "of", o.N)
+    assert yielded >= MIN_YIELDED.get(case, 0)
 
 
 def test_tanh_jc_changes_the_run():
@@ -183,9 +200,10 @@ def test_autotune_keeps_the_run():
     cfg = small_cut(thermal="contact")
     a, b = pymfc.Sim(cfg), pymfc.Sim(cfg)
     t = pymfc.autotune(a, steps=5)
-    assert len(t["ms_per_step"]) == 8
+    assert len(t["ms_per_step"]) == 12
     assert a.tuning() == {k: t[k] for k in ("split_interact", "compact_pairs", "stage_limit")}
-    b.step(8 * (4 + 5))
+    assert t["steps_run"] == 2 * 12 * (16 + 5) + 16
+    b.step(t["steps_run"])
     a.step(10)
     b.step(10)
     for f in ("pos", "vel", "S", "T", "eps_pl", "fc", "ft", "flags", "id"):

@@ -94,3 +94,22 @@ def test_hash_truncates_toward_zero():
     o.step(1)
     k = np.flatnonzero(o.field("id") == ids[0])[0]
     assert o.field("hash")[k] == 2
+
+
+def test_out_of_grid_counted_per_axis():
+    """A particle above the grid gets the reference's id ix * ny + iy, which lands in the next
+    column (inside the cell table) — it is still counted as outside the grid."""
+    import pymfc
+    cfg = pymfc.cut_config(40, 12, 10.0, thermal="contact")
+    o = ob.OracleSim(cfg)
+    o.step(2)
+    assert o.info.out_of_grid == 0
+    info = o.info
+    pos = o.field("pos")
+    k = int(np.flatnonzero(o.field("flags") == 0)[0])
+    pos[k, 1] = info.bbmin_y + (info.grid_ny + 0.5) * info.cell_size
+    ix = int((pos[k, 0] - info.bbmin_x) / info.cell_size)
+    assert 0 <= ix * info.grid_ny + info.grid_ny < info.num_cell  # the id itself stays in the table
+    o.set_field("pos", pos)
+    o.step(1)
+    assert o.info.out_of_grid == 1
