@@ -26,6 +26,8 @@
  *                                                      -> mfc_write_vtk
  *   vtk_writer_write(tool, step)                       vtk_writer.cpp:183-263
  *                                                      -> mfc_write_tool_vtk
+ *   (no counterpart: the reference's only state dump is the text dump_state, debug.cpp:20-64)
+ *                                                      -> mfc_save / mfc_load
  *
  * Conventions: every function returns 0 on success and a negative MFC_E_* code on
  * failure (mfc_last_error() gives the message of the calling thread's last failure);
@@ -324,6 +326,21 @@ int mfc_batch_step(mfc_batch *batch, int nsteps);
 int mfc_batch_sync(mfc_batch *batch);
 int mfc_batch_vjp_forward(mfc_batch *batch, int nsteps, int checkpoint_every, int w0, int w1, double *Fmean);
 int mfc_batch_vjp_backward(mfc_batch *batch, const double *cot, double *grad);
+
+/* ---- checkpoints ----
+ * The reference has no restart format (its only dump is the text dump_state, debug.cpp:20-64).
+ * mfc_save writes the handle's state after its queued steps to a binary file: the creating
+ * configuration, launch tuning, completed steps, the current particle fields in storage order,
+ * the positional arrays (contact normal, pressure, artificial stress), the cell ids of the last
+ * sort, the tool state, the last tool force and the force history, with an FNV-1a checksum of
+ * the payload. mfc_load creates a new handle from such a file; it continues bit for bit as the
+ * saved run would have (the next step recomputes the cell tables and neighbour records).
+ * A file of another format version, a truncated file or a checksum mismatch is refused with
+ * MFC_E_INVALID. */
+int mfc_save(mfc_sim *sim, const char *path);
+int mfc_load(const char *path, mfc_sim **sim);
+/* the configuration a handle was created from (for a loaded handle: the saved run's) */
+int mfc_get_config(const mfc_sim *sim, mfc_config *cfg);
 
 int mfc_get_info(const mfc_sim *sim, mfc_info *info);
 /* copy a field to host memory: bytes must be N * components * sizeof(double)

@@ -186,6 +186,7 @@ int mfc_create(const mfc_config *cfg, mfc_sim **out) {
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(MFC_E_HIP, "no HIP device");
   mfc_sim *s = new mfc_sim();
+  s->cfg = *cfg;
   std::string err;
   int rc = mfc::build_scenario(*cfg, s->sc, err);
   if (rc) { delete s; return fail(rc, err); }

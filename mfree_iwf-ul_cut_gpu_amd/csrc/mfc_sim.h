@@ -38,6 +38,7 @@ int adj_finish(mfc_sim *s, double Fmean[2]);
 }  // namespace mfc
 
 struct mfc_sim {
+  mfc_config cfg;      // the configuration the handle was created from (checkpoints keep it)
   mfc::scenario sc;
   mfc::dev_consts k;
   int ntiles = 0;

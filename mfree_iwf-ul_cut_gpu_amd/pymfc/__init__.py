@@ -216,6 +216,9 @@ def lib() -> C.CDLL:
         L.mfc_batch_sync.argtypes = [vp]
         L.mfc_batch_vjp_forward.argtypes = [vp, i32, i32, i32, i32, dp]
         L.mfc_batch_vjp_backward.argtypes = [vp, dp, dp]
+        L.mfc_save.argtypes = [vp, C.c_char_p]
+        L.mfc_load.argtypes = [C.c_char_p, C.POINTER(vp)]
+        L.mfc_get_config.argtypes = [vp, C.POINTER(Config)]
         L.mfc_set_tuning.argtypes = [vp, C.POINTER(Tuning)]
         L.mfc_get_tuning.argtypes = [vp, C.POINTER(Tuning)]
         _lib = L
@@ -364,6 +367,20 @@ class Sim:
             self.close()
         except Exception:
             pass
+
+    def save(self, path: str) -> None:
+        """Binary checkpoint of the handle (mfc_save)."""
+        _check(lib().mfc_save(self._h, os.fsencode(path)), "mfc_save")
+
+    @classmethod
+    def load(cls, path: str) -> "Sim":
+        """A new handle continuing a checkpoint (mfc_load)."""
+        self = cls.__new__(cls)
+        self._h = C.c_void_p()
+        _check(lib().mfc_load(os.fsencode(path), C.byref(self._h)), "mfc_load")
+        self.cfg = Config()
+        _check(lib().mfc_get_config(self._h, C.byref(self.cfg)), "mfc_get_config")
+        return self
 
     def __enter__(self):
         return self

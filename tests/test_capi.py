@@ -85,3 +85,15 @@ def test_scenario_needs_no_gpu():
     s = pymfc.scenario(pymfc.cut_config(20, 10, 10.0))
     assert s["info"].n_particles == 200 and s["pos"].shape == (200, 2)
     assert np.all(np.isfinite(s["pos"]))
+
+
+def test_load_refuses_a_file_that_is_not_a_checkpoint(tmp_path):
+    """mfc_load checks the header before it touches a device."""
+    p = tmp_path / "not_a_checkpoint.bin"
+    p.write_bytes(b"MFC" + bytes(400))
+    h = C.c_void_p()
+    rc = pymfc.lib().mfc_load(os.fsencode(str(p)), C.byref(h))
+    assert rc == -1 and not h.value
+    assert b"not a checkpoint" in pymfc.lib().mfc_last_error()
+    rc = pymfc.lib().mfc_load(os.fsencode(str(tmp_path / "missing.bin")), C.byref(h))
+    assert rc == -1
