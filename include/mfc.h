@@ -77,8 +77,9 @@ extern "C" {
                                    (-0.2,-0.08)-(0.2,0.08); no tool; dt from cfl and speed */
 #define MFC_LATTICE_IMPACT   3  /* setup_impact (benchmarks.cpp:23-153): a disk of radius 0.2 at
                                    x = -0.25 flying at +1e3 into a wall [0, ny/2 dx) x [-1, 1]
-                                   whose top and bottom rows are fixed, dx = 0.4/(ny-1); fixed
-                                   grid (-0.5,-1.2)-(1,1.2) (benchmarks.cpp:136); no tool */
+                                   whose top and bottom rows are fixed, dx = 0.4/(ny-1); locked
+                                   grid (-0.5,-1.2)-(1,1.2) (benchmarks.cpp:136) unless
+                                   grid_adaptive; no tool */
 
 typedef struct mfc_config {
   /* workpiece lattice */
@@ -127,7 +128,15 @@ typedef struct mfc_config {
                              actions_setup_johnson_cook_constants, setup_rings) */
   double jc_tanh_a, jc_tanh_b, jc_tanh_c, jc_tanh_d;  /* joco_constants tanh_a..d
                                                          (constants_structs.h:71-74) */
-  int32_t reserved[4];
+  int32_t grid_adaptive;  /* 1: the cell grid follows the particles every step: bounding box of
+                             all positions +- 1e-6 and cell size max h, recomputed before the
+                             binning (grid_base::update_geometry, grid.cu:97-136; setup_impact's
+                             active grid, benchmarks.cpp:135); 0: the scenario's locked grid */
+  int32_t max_cell_factor;/* adaptive grid: a cell table of max_cell_factor * N cells (0: 10, the
+                             reference's grid_gpu_green(10*n, n)); a step whose grid would need
+                             as many cells keeps the previous geometry and is counted in
+                             mfc_info.grid_overflow (the reference asserts) */
+  int32_t reserved[2];
 } mfc_config;
 
 typedef struct mfc_info {
@@ -143,6 +152,11 @@ typedef struct mfc_info {
                              an axis (counted once per binning). Their cell id is the
                              reference's ix * ny + iy, clamped into the cell table; such a
                              particle loses neighbours, so a valid run keeps this at 0 */
+  int64_t grid_overflow;  /* adaptive grid: steps whose geometry did not fit the cell table
+                             (0 in a valid run) */
+  int32_t grid_adaptive;  /* the configuration's grid_adaptive; grid_nx, grid_ny, num_cell,
+                             bbmin_* then describe the geometry of the last step */
+  int32_t cell_table;     /* cells of the binning table (num_cell for a locked grid) */
 } mfc_info;
 
 /* particle fields, returned in the handle's current (cell-sorted) order */
@@ -204,7 +218,8 @@ int mfc_config_rings(mfc_config *cfg, int nbox);
 
 /* the constants of setup_impact(nbox, grid) (benchmarks.cpp:23-153, benchmarks.h:45): steel 4340
  * in SI units with its Johnson-Cook constants, no heat, hdx 1.7, CFL 0.3 with the flyer's 1e3,
- * FP64. The reference's main() names nbox = 20. */
+ * FP64, and the adaptive grid of its active line (grid_gpu_green(10*n, n), benchmarks.cpp:135).
+ * The reference's main() names nbox = 20. */
 int mfc_config_impact(mfc_config *cfg, int nbox);
 
 /* setup_ref_cut + particle_gpu + grid_gpu_green + leap_frog construction */
@@ -349,7 +364,8 @@ int mfc_get_field(mfc_sim *sim, int field, void *host, size_t bytes);
 /* overwrite a field from host memory (same layout); used to inject states in tests */
 int mfc_set_field(mfc_sim *sim, int field, const void *host, size_t bytes);
 /* cell ranges of the last binning: offsets[c]..offsets[c+1] is cell c in storage order,
- * an array of num_cell + 1 ints */
+ * an array of num_cell + 1 ints (num_cell of mfc_get_info: for an adaptive grid the last
+ * step's geometry) */
 int mfc_get_cells(mfc_sim *sim, int32_t *offsets, size_t count);
 int mfc_get_tool(mfc_sim *sim, mfc_tool_state *tool);
 

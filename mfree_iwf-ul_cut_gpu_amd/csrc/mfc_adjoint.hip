@@ -341,6 +341,7 @@ int adj_begin(mfc_sim *s, int nsteps, int checkpoint_every, int w0, int w1) {
   if (s->sc.tool.num_seg == 0 || !s->k.plastic)
     return fail(MFC_E_INVALID, "the adjoint differentiates the cutting scenario (tool and plasticity present)");
   if (s->k.jc_tanh) return fail(MFC_E_INVALID, "the adjoint differentiates the plain Johnson-Cook model (jc_model = 0)");
+  if (s->k.adaptive) return fail(MFC_E_INVALID, "the adjoint runs on a locked grid (grid_adaptive = 0)");
   int64_t base = 0;
   int rc = mfc_force_history(s, nullptr, 0, &base);
   if (rc) return rc;
@@ -411,7 +412,7 @@ int mfc_vjp_backward(mfc_sim *s, const double cot[2], double grad[6]) {
   HIP_TRY(hipMemcpyAsync(w->keys_keep, s->bins.key_sorted, sizeof(int) * n, hipMemcpyDeviceToDevice, st));
   // the replays bin with the handle's cell table, which holds the next step's counts: clear them
   // (each replayed binning leaves them cleared), and bin the handle's state again at its next step
-  HIP_TRY(hipMemsetAsync(s->bins.count, 0, sizeof(int) * (size_t)k.num_cell, st));
+  HIP_TRY(hipMemsetAsync(s->bins.count, 0, sizeof(int) * (size_t)k.ncell_table, st));
   s->keys_fresh = false;
   int cur = 0;  // lam_s[cur] holds the adjoint of the end state of the step being reversed
   const double Fb[2] = {cot[0] / (w->w1 - w->w0), cot[1] / (w->w1 - w->w0)};

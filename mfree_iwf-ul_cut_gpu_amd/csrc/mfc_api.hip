@@ -70,8 +70,8 @@ void enqueue_step(mfc_sim *s, int c) { mfc::launch_batch_step(s->work(c), s->str
 
 int ensure_keys(mfc_sim *s) {
   if (s->keys_fresh) return MFC_OK;
-  HIP_TRY(hipMemsetAsync(s->bins.count, 0, sizeof(int) * (size_t)s->k.num_cell, s->stream));
-  HIP_TRY(hipMemsetAsync(s->bins.bsum, 0, sizeof(int) * (size_t)mfc::scan_blocks(s->k.num_cell), s->stream));
+  HIP_TRY(hipMemsetAsync(s->bins.count, 0, sizeof(int) * (size_t)s->k.ncell_table, s->stream));
+  HIP_TRY(hipMemsetAsync(s->bins.bsum, 0, sizeof(int) * (size_t)mfc::scan_blocks(s->k.ncell_table), s->stream));
   mfc::launch_batch_hash(s->work(s->cur), s->stream);
   s->keys_fresh = true;
   return MFC_OK;
@@ -101,9 +101,19 @@ mfc::inst_args instance_args(const mfc_sim *s, int c) {
   a.tool_out = s->tool + (1 - c);
   a.f = s->force();
   a.acc = s->acc;
+  a.geo = s->geo;
   a.ntiles = s->ntiles;
-  a.nscan = mfc::scan_blocks(s->k.num_cell);
+  a.nscan = mfc::scan_blocks(s->k.ncell_table);
   return a;
+}
+
+int current_geometry(const mfc_sim *s, mfc::grid_geo &g) {
+  const mfc::sim_constants &c = s->sc.k;
+  g = {c.bbmin_x, c.bbmin_y, c.cell, c.gnx, c.gny, c.num_cell, 0};
+  if (!s->k.adaptive) return MFC_OK;
+  HIP_TRY(hipMemcpyAsync(&g, s->geo, sizeof(g), hipMemcpyDeviceToHost, s->stream));
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  return MFC_OK;
 }
 
 namespace {
@@ -204,9 +214,10 @@ int mfc_create(const mfc_config *cfg, mfc_sim **out) {
     if ((rc = alloc_state(s, s->set[q], N))) return bail(rc);
   mfc::dbins &b = s->bins;
   if ((rc = s->alloc(&b.key, N)) || (rc = s->alloc(&b.key_sorted, N)) || (rc = s->alloc(&b.slot, N)) ||
-      (rc = s->alloc(&b.count, c.num_cell)) || (rc = s->alloc(&b.offsets, (size_t)c.num_cell + 1)) ||
+      (rc = s->alloc(&b.count, c.ncell_table)) || (rc = s->alloc(&b.offsets, (size_t)c.ncell_table + 1)) ||
       (rc = s->alloc(&b.bucket, N)) || (rc = s->alloc(&b.perm, N)) ||
-      (rc = s->alloc(&b.bsum, mfc::scan_blocks(c.num_cell))) || (rc = s->alloc(&b.clamped, 1)))
+      (rc = s->alloc(&b.bsum, mfc::scan_blocks(c.ncell_table))) || (rc = s->alloc(&b.clamped, 1)) ||
+      (rc = s->alloc(&s->geo, 1)))
     return bail(rc);
   mfc::dpositional &p = s->pos;
   {
@@ -256,12 +267,12 @@ int mfc_create(const mfc_config *cfg, mfc_sim **out) {
     for (double *a : pz)
       if (up(a, zero.data(), sizeof(double) * N) != hipSuccess) return bail(fail(MFC_E_HIP, "upload"));
     if (hipMemsetAsync(b.key_sorted, 0, sizeof(int) * N, s->stream) != hipSuccess ||
-        hipMemsetAsync(b.count, 0, sizeof(int) * c.num_cell, s->stream) != hipSuccess ||
-        hipMemsetAsync(b.offsets, 0, sizeof(int) * ((size_t)c.num_cell + 1), s->stream) != hipSuccess ||
+        hipMemsetAsync(b.count, 0, sizeof(int) * c.ncell_table, s->stream) != hipSuccess ||
+        hipMemsetAsync(b.offsets, 0, sizeof(int) * ((size_t)c.ncell_table + 1), s->stream) != hipSuccess ||
         hipMemsetAsync(b.clamped, 0, sizeof(unsigned long long), s->stream) != hipSuccess ||
         hipMemsetAsync(s->hist_ctr, 0, sizeof(int64_t), s->stream) != hipSuccess ||
         hipMemsetAsync(s->force_pending, 0, sizeof(int), s->stream) != hipSuccess ||
-        hipMemsetAsync(b.bsum, 0, sizeof(int) * (size_t)mfc::scan_blocks(c.num_cell), s->stream) != hipSuccess ||
+        hipMemsetAsync(b.bsum, 0, sizeof(int) * (size_t)mfc::scan_blocks(c.ncell_table), s->stream) != hipSuccess ||
         hipMemsetAsync(s->force_last, 0, 2 * sizeof(double), s->stream) != hipSuccess ||
         (s->recs.rec && hipMemsetAsync(s->recs.rec, 0, sizeof(double) * mfc::REC * (size_t)N, s->stream) != hipSuccess) ||
         hipMemsetAsync(s->recs.posd, 0, sizeof(double2) * (size_t)N, s->stream) != hipSuccess ||
@@ -270,8 +281,10 @@ int mfc_create(const mfc_config *cfg, mfc_sim **out) {
       return bail(fail(MFC_E_HIP, "memset"));
     mfc_tool_state t2[2] = {S.tool, S.tool};
     if (up(s->tool, t2, sizeof(t2)) != hipSuccess) return bail(fail(MFC_E_HIP, "upload tool"));
+    const mfc::grid_geo g0 = {c.bbmin_x, c.bbmin_y, c.cell, c.gnx, c.gny, c.num_cell, 0};
+    if (up(s->geo, &g0, sizeof(g0)) != hipSuccess) return bail(fail(MFC_E_HIP, "upload grid"));
     // descriptors and work tables of the step kernels (a batch of one)
-    const int nscan = mfc::scan_blocks(c.num_cell);
+    const int nscan = mfc::scan_blocks(c.ncell_table);
     std::vector<int2> tiles(s->ntiles), chunks(nscan);
     for (int t = 0; t < s->ntiles; t++) tiles[t] = make_int2(0, t);
     for (int q = 0; q < nscan; q++) chunks[q] = make_int2(0, q);
@@ -435,6 +448,16 @@ int mfc_get_info(const mfc_sim *s, mfc_info *info) {
   info->dt = c.dt; info->t_final = c.t_final; info->mass = c.mass;
   info->bbmin_x = c.bbmin_x; info->bbmin_y = c.bbmin_y; info->cell_size = c.cell;
   info->step = s->step;
+  info->grid_adaptive = s->k.adaptive;
+  info->cell_table = s->k.ncell_table;
+  if (s->k.adaptive) {
+    mfc::grid_geo g;
+    int rc = current_geometry(s, g);
+    if (rc) return rc;
+    info->grid_nx = g.gnx; info->grid_ny = g.gny; info->num_cell = g.num_cell;
+    info->bbmin_x = g.bbmin_x; info->bbmin_y = g.bbmin_y; info->cell_size = g.cell;
+    info->grid_overflow = g.overflow;
+  }
   unsigned long long oog = 0;
   HIP_TRY(hipMemcpyAsync(&oog, s->bins.clamped, sizeof(oog), hipMemcpyDeviceToHost, s->stream));
   HIP_TRY(hipStreamSynchronize(s->stream));
@@ -528,7 +551,10 @@ int mfc_set_field(mfc_sim *s, int field, const void *host, size_t bytes) {
 
 int mfc_get_cells(mfc_sim *s, int32_t *offsets, size_t count) {
   if (!s || !offsets) return fail(MFC_E_INVALID, "null argument");
-  if (count != (size_t)s->k.num_cell + 1) return fail(MFC_E_RANGE, "count must be num_cell + 1");
+  mfc::grid_geo g;
+  int rc = current_geometry(s, g);
+  if (rc) return rc;
+  if (count != (size_t)g.num_cell + 1) return fail(MFC_E_RANGE, "count must be num_cell + 1");
   return copy_out(s, offsets, s->bins.offsets, sizeof(int) * count);
 }
 
@@ -574,6 +600,8 @@ int mfc_scenario(const mfc_config *cfg, mfc_info *info, double *pos, double *rho
   info->precision = cfg->precision;
   info->dt = c.dt; info->t_final = c.t_final; info->mass = c.mass;
   info->bbmin_x = c.bbmin_x; info->bbmin_y = c.bbmin_y; info->cell_size = c.cell;
+  info->grid_adaptive = c.adaptive;
+  info->cell_table = c.ncell_table;
   for (int i = 0; i < c.N; i++) {
     if (pos) { pos[2 * i] = S.px[i]; pos[2 * i + 1] = S.py[i]; }
     if (rho) rho[i] = S.rho[i];

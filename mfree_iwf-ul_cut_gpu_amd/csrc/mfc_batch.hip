@@ -25,9 +25,10 @@ struct mfc_batch {
   int64_t step = 0;
   hipGraphExec_t graph[2] = {nullptr, nullptr};
 
+  bool adaptive = false;  // some instance has an adaptive grid
   mfc::batch_work work(int parity) const {
     return {args[parity], (int)sims.size(), tiles, ntiles, chunks, nchunks, f32,
-            {tuning.split_interact != 0, tuning.compact_pairs != 0}};
+            {tuning.split_interact != 0, tuning.compact_pairs != 0}, adaptive};
   }
   void drop_graphs() {
     for (int p = 0; p < 2; p++)
@@ -84,8 +85,8 @@ int batch_keys(mfc_batch *b) {
   for (mfc_sim *s : b->sims) fresh = fresh && s->keys_fresh;
   if (fresh) return MFC_OK;
   for (mfc_sim *s : b->sims) {
-    HIP_TRY(hipMemsetAsync(s->bins.count, 0, sizeof(int) * (size_t)s->k.num_cell, b->stream));
-    HIP_TRY(hipMemsetAsync(s->bins.bsum, 0, sizeof(int) * (size_t)mfc::scan_blocks(s->k.num_cell), b->stream));
+    HIP_TRY(hipMemsetAsync(s->bins.count, 0, sizeof(int) * (size_t)s->k.ncell_table, b->stream));
+    HIP_TRY(hipMemsetAsync(s->bins.bsum, 0, sizeof(int) * (size_t)mfc::scan_blocks(s->k.ncell_table), b->stream));
   }
   mfc::launch_batch_hash(b->work(b->cur), b->stream);
   for (mfc_sim *s : b->sims) s->keys_fresh = true;
@@ -131,6 +132,7 @@ int mfc_batch_create(const mfc_config *cfgs, int n, mfc_batch **out) {
     s->stream = b->stream;
     s->batched = true;
     s->use_graphs = false;
+    b->adaptive = b->adaptive || s->k.adaptive;
   }
   std::vector<int2> tiles, chunks;
   std::vector<mfc::inst_args> a0(n), a1(n);

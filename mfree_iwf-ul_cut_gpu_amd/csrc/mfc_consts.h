@@ -25,6 +25,18 @@ struct dev_consts {
   double tanh_a, tanh_b, tanh_c, tanh_d;
   int plastic;        // Johnson-Cook return on
   int stage_limit;    // most window candidates a tile stages in LDS (0: all it has room for)
+  // adaptive grid: the geometry fields above (bbmin, cell, gnx, gny, num_cell) are rewritten on
+  // the device at the start of every step; ncell_table is the size of the binning table (the
+  // counting sort runs over it), num_cell for a locked grid
+  int adaptive;
+  int ncell_table;
+};
+
+// the geometry of an adaptive grid for one step, kept on the device for the host to read
+struct grid_geo {
+  double bbmin_x, bbmin_y, cell;
+  int gnx, gny, num_cell;
+  int overflow;  // steps whose geometry did not fit the cell table
 };
 
 }  // namespace mfc

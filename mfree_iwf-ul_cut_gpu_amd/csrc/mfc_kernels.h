@@ -92,6 +92,7 @@ struct inst_args {
   mfc_tool_state *tool_out;
   dforce f;
   double *acc;  // split interaction only
+  grid_geo *geo;  // adaptive grid: the geometry of the current step (written by the device)
   int ntiles;   // particle tiles of TILE
   int nscan;    // scan chunks of the cell table
 };
@@ -106,6 +107,7 @@ struct batch_work {
   int nchunks;
   bool f32;
   variant v;
+  bool adaptive;          // some instance has an adaptive grid (a geometry pass starts the step)
 };
 
 // one step of every instance of a batch: 4 (fused) or 5 (split) launches, and its phases. The

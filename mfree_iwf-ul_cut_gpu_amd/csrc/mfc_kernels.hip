@@ -1024,7 +1024,7 @@ __device__ inline inst_args load_args(const inst_args *A, int q) {
   o.r = load_recs(&c->r);
   MFC_F(tool_in); MFC_F(tool_out);
   o.f = load_force(&c->f);
-  MFC_F(acc); MFC_F(ntiles); MFC_F(nscan);
+  MFC_F(acc); MFC_F(geo); MFC_F(ntiles); MFC_F(nscan);
   return o;
 }
 #undef MFC_F
@@ -1039,8 +1039,77 @@ __global__ __launch_bounds__(256) void kb_hash_count(const inst_args *__restrict
   if (w < 0) return;
   const int2 it = work[w];
   const inst_args a = load_args(A, it.x);
+  if (a.k.adaptive) return;  // its step starts with the geometry pass, which bins the state
   hash_count(a.k, it.y * 256 + threadIdx.x, a.cur.px, a.cur.py, a.cur.flags, a.b.key_sorted, a.b.key,
              a.b.slot, a.b.count, a.b.bsum, a.b.clamped);
+}
+
+// Adaptive grid (grid_base::update_geometry, grid.cu:97-136), one work-group per instance: the
+// box of all positions (blanked particles included, as thrust::min_element / max_element over
+// the whole array) and the largest h give this step's geometry, which the work-group writes
+// into the instance's descriptor for the step's later kernels, then it bins the start state with
+// it (cell ids, arrival slots, cell and chunk counts). Minimum and maximum do not depend on the
+// order they are taken in; NaN positions are passed over. A geometry that needs as many cells as
+// the binning table holds is not taken (the reference asserts): the previous one stays and the
+// step is counted in geo->overflow.
+constexpr int GEO_THREADS = 1024;
+__global__ __launch_bounds__(GEO_THREADS) void kb_geometry(inst_args *__restrict__ A) {
+  // plain loads: this kernel writes the descriptor it reads (not through the constant space)
+  const inst_args a = A[blockIdx.x];
+  if (!a.k.adaptive) return;  // the same in every thread of the work-group
+  __shared__ double red[5][GEO_THREADS / 64];
+  __shared__ double geo_d[3];
+  __shared__ int geo_i[3];
+  const int N = a.k.N;
+  const double inf = __builtin_huge_val();
+  double x0 = inf, x1 = -inf, y0 = inf, y1 = -inf, hm = -inf;
+  for (int i = threadIdx.x; i < N; i += GEO_THREADS) {
+    const double x = a.cur.px[i], y = a.cur.py[i], h = a.cur.h[i];
+    x0 = fmin(x0, x); x1 = fmax(x1, x);
+    y0 = fmin(y0, y); y1 = fmax(y1, y);
+    hm = fmax(hm, h);
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    x0 = fmin(x0, __shfl_xor(x0, o, 64)); x1 = fmax(x1, __shfl_xor(x1, o, 64));
+    y0 = fmin(y0, __shfl_xor(y0, o, 64)); y1 = fmax(y1, __shfl_xor(y1, o, 64));
+    hm = fmax(hm, __shfl_xor(hm, o, 64));
+  }
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  if (lane == 0) { red[0][wv] = x0; red[1][wv] = x1; red[2][wv] = y0; red[3][wv] = y1; red[4][wv] = hm; }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int q = 1; q < GEO_THREADS / 64; q++) {
+      x0 = fmin(x0, red[0][q]); x1 = fmax(x1, red[1][q]);
+      y0 = fmin(y0, red[2][q]); y1 = fmax(y1, red[3][q]);
+      hm = fmax(hm, red[4][q]);
+    }
+    double bx = x0 - 1e-6, by = y0 - 1e-6;
+    const double ex = x1 + 1e-6, ey = y1 + 1e-6;
+    double cell = hm * 2. / 2.;
+    const double fnx = ceil((ex - bx) / cell), fny = ceil((ey - by) / cell);
+    int gnx, gny;
+    if (fnx >= 1. && fny >= 1. && fnx * fny < (double)a.k.ncell_table) {
+      gnx = (int)fnx; gny = (int)fny;
+    } else {  // keep the previous step's geometry
+      bx = a.k.bbmin_x; by = a.k.bbmin_y; cell = a.k.cell; gnx = a.k.gnx; gny = a.k.gny;
+      a.geo->overflow = a.geo->overflow + 1;
+    }
+    geo_d[0] = bx; geo_d[1] = by; geo_d[2] = cell;
+    geo_i[0] = gnx; geo_i[1] = gny; geo_i[2] = gnx * gny;
+    dev_consts *kd = &A[blockIdx.x].k;
+    kd->bbmin_x = bx; kd->bbmin_y = by; kd->cell = cell;
+    kd->gnx = gnx; kd->gny = gny; kd->num_cell = gnx * gny;
+    a.geo->bbmin_x = bx; a.geo->bbmin_y = by; a.geo->cell = cell;
+    a.geo->gnx = gnx; a.geo->gny = gny; a.geo->num_cell = gnx * gny;
+  }
+  __syncthreads();
+  dev_consts k = a.k;
+  k.bbmin_x = geo_d[0]; k.bbmin_y = geo_d[1]; k.cell = geo_d[2];
+  k.gnx = geo_i[0]; k.gny = geo_i[1]; k.num_cell = geo_i[2];
+  for (int base = 0; base < N; base += GEO_THREADS)  // every wave runs every round (count_cell)
+    hash_count(k, base + threadIdx.x, a.cur.px, a.cur.py, a.cur.flags, a.b.key_sorted, a.b.key, a.b.slot,
+               a.b.count, a.b.bsum, a.b.clamped);
 }
 
 __global__ __launch_bounds__(SCAN_THREADS) void kb_scan_down(const inst_args *__restrict__ A,
@@ -1049,7 +1118,7 @@ __global__ __launch_bounds__(SCAN_THREADS) void kb_scan_down(const inst_args *__
   if (w < 0) return;
   const int2 it = work[w];
   const inst_args a = load_args(A, it.x);
-  scan_down(it.y, a.b.count, a.k.num_cell, a.b.bsum, a.b.offsets);
+  scan_down(it.y, a.b.count, a.k.ncell_table, a.b.bsum, a.b.offsets);
 }
 
 __global__ __launch_bounds__(256) void kb_place(const inst_args *__restrict__ A, const int2 *__restrict__ work,
@@ -1090,8 +1159,9 @@ __global__ __launch_bounds__(TILE) SUMS_OCCUPANCY void kb_interact(const inst_ar
   const inst_args &a = A[it.x];
   MFC_CONST(inst_args) *c = const_args(A, it.x);
   const dbins b = load_bins(&c->b);
+  // an adaptive grid bins the next step's start state after its geometry pass instead
   interact<F32, COMPACT>(a.k, it.y, load_recs(&c->r), b.key_sorted, b.offsets, b.perm, a.cur, a.nxt, a.pos,
-                         a.tool_out, a.f, &b);
+                         a.tool_out, a.f, a.k.adaptive ? nullptr : &b);
 }
 
 template <bool F32, bool COMPACT>
@@ -1114,7 +1184,7 @@ __global__ __launch_bounds__(TILE) void kb_update(const inst_args *__restrict__ 
   const inst_args &a = A[it.x];
   MFC_CONST(inst_args) *c = const_args(A, it.x);
   const dbins b = load_bins(&c->b);
-  update(a.k, it.y, c->acc, b.perm, a.cur, a.nxt, a.pos, a.tool_out, a.f, &b);
+  update(a.k, it.y, c->acc, b.perm, a.cur, a.nxt, a.pos, a.tool_out, a.f, a.k.adaptive ? nullptr : &b);
 }
 
 __global__ __launch_bounds__(TILE) void kb_force_final(const inst_args *__restrict__ A) {
@@ -1152,10 +1222,10 @@ int scan_blocks(int num_cell) { return (num_cell + SCAN_CHUNK - 1) / SCAN_CHUNK;
 
 void launch_binning(const dev_consts &k, const dstate &cur, dbins &b, hipStream_t s) {
   const int g = (k.N + 255) / 256;
-  const int nb = scan_blocks(k.num_cell);
+  const int nb = scan_blocks(k.ncell_table);
   hipLaunchKernelGGL(k_hash_count, dim3(g), dim3(256), 0, s, k, cur.px, cur.py, cur.flags, b);
-  hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(SCAN_THREADS), 0, s, b.count, k.num_cell, b.bsum);
-  hipLaunchKernelGGL(k_scan_down, dim3(nb), dim3(SCAN_THREADS), 0, s, b.count, k.num_cell, b.bsum, b.offsets);
+  hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(SCAN_THREADS), 0, s, b.count, k.ncell_table, b.bsum);
+  hipLaunchKernelGGL(k_scan_down, dim3(nb), dim3(SCAN_THREADS), 0, s, b.count, k.ncell_table, b.bsum, b.offsets);
   hipLaunchKernelGGL(k_place, dim3(g), dim3(256), 0, s, k.N, b);
 }
 
@@ -1234,6 +1304,8 @@ void launch_batch_hash(const batch_work &w, hipStream_t s) {
 
 void launch_batch_binning(const batch_work &w, hipStream_t s) {
   const dim3 gt(padded_tiles(w.ntiles)), gc(padded_tiles(w.nchunks));
+  if (w.adaptive)  // the descriptors are device memory of the batch; the geometry pass updates them
+    hipLaunchKernelGGL(kb_geometry, dim3(w.n), dim3(GEO_THREADS), 0, s, const_cast<inst_args *>(w.args));
   hipLaunchKernelGGL(kb_scan_down, gc, dim3(SCAN_THREADS), 0, s, w.args, w.chunks, w.nchunks);
   hipLaunchKernelGGL(kb_place, gt, dim3(256), 0, s, w.args, w.tiles, w.ntiles);
 }

@@ -182,6 +182,45 @@ void make_dev_consts(const sim_constants &c, dev_consts &k) {
   k.tanh_a = c.tanh_a; k.tanh_b = c.tanh_b; k.tanh_c = c.tanh_c; k.tanh_d = c.tanh_d;
   k.plastic = c.plastic;
   k.stage_limit = 0;
+  k.adaptive = c.adaptive;
+  k.ncell_table = c.ncell_table;
+}
+
+void adaptive_geometry(const double *px, const double *py, const double *h, int N, double &bbmin_x, double &bbmin_y,
+                       double &cell, int64_t &gnx, int64_t &gny) {
+  double x0 = px[0], x1 = px[0], y0 = py[0], y1 = py[0], hm = h[0];
+  for (int i = 1; i < N; i++) {
+    x0 = std::fmin(x0, px[i]); x1 = std::fmax(x1, px[i]);
+    y0 = std::fmin(y0, py[i]); y1 = std::fmax(y1, py[i]);
+    hm = std::fmax(hm, h[i]);
+  }
+  bbmin_x = x0 - 1e-6; bbmin_y = y0 - 1e-6;
+  const double bx1 = x1 + 1e-6, by1 = y1 + 1e-6;
+  cell = hm * 2. / 2.;
+  gnx = (int64_t)std::ceil((bx1 - bbmin_x) / cell);
+  gny = (int64_t)std::ceil((by1 - bbmin_y) / cell);
+}
+
+// the adaptive grid of a scenario: the geometry of its initial particles and a binning table of
+// max_cell_factor * N cells (grid_gpu_green(10*n, n) in the reference, benchmarks.cpp:135)
+static int set_adaptive(const mfc_config &c, scenario &S, std::string &err) {
+  sim_constants &k = S.k;
+  k.adaptive = 0;
+  k.ncell_table = k.num_cell;
+  if (!c.grid_adaptive) return MFC_OK;
+  const int64_t factor = c.max_cell_factor > 0 ? c.max_cell_factor : 10;
+  const int64_t table = factor * (int64_t)k.N;
+  if (table > 1073741824LL) { err = "adaptive cell table too large"; return MFC_E_RANGE; }
+  int64_t gnx, gny;
+  adaptive_geometry(S.px.data(), S.py.data(), S.h.data(), k.N, k.bbmin_x, k.bbmin_y, k.cell, gnx, gny);
+  if (!(gnx >= 1 && gny >= 1) || gnx * gny >= table) {
+    err = "the initial adaptive grid needs more cells than max_cell_factor * N";
+    return MFC_E_RANGE;
+  }
+  k.gnx = (int)gnx; k.gny = (int)gny; k.num_cell = (int)(gnx * gny);
+  k.adaptive = 1;
+  k.ncell_table = (int)table;
+  return MFC_OK;
 }
 
 void tool_advance(mfc_tool_state &t, double dt) {
@@ -276,6 +315,8 @@ void config_rings(mfc_config &c, int nbox) {  // benchmarks.cpp:155-253
 
 void config_impact(mfc_config &c, int nbox) {  // benchmarks.cpp:23-153
   config_bodies(c, MFC_LATTICE_IMPACT, nbox);
+  c.grid_adaptive = 1;  // grid_gpu_green(10*n, n), benchmarks.cpp:135
+  c.max_cell_factor = 10;
   c.E = 200e9; c.nu = 0.29; c.rho0 = 7830.0;
   c.jc_A = 792.0e6; c.jc_B = 510.0e6; c.jc_C = 0.014; c.jc_m = 1.03; c.jc_n = 0.26;
   c.jc_Tref = 273.0; c.jc_Tmelt = 1793.0; c.jc_eps_dot_ref = 1;
@@ -305,7 +346,8 @@ int build_scenario(const mfc_config &c, scenario &S, std::string &err) {
     lo_y = hi_y - (ny - 1) * dx;
     hi_x = lo_x + nx * dx;
   } else if (c.lattice == MFC_LATTICE_RINGS || c.lattice == MFC_LATTICE_IMPACT) {
-    return build_bodies(c, S, err);
+    const int rc = build_bodies(c, S, err);
+    return rc ? rc : set_adaptive(c, S, err);
   } else {
     err = "unknown lattice rule";
     return MFC_E_INVALID;
@@ -417,14 +459,15 @@ int build_scenario(const mfc_config &c, scenario &S, std::string &err) {
   // CFL time step (benchmarks.cpp:530-531)
   k.dt = c.cfl * hdx * dx / (std::sqrt(k.K / k.rho0) + std::sqrt(vel_tool));
   k.t_final = 0.1 / vel_tool;
-  return MFC_OK;
+  return set_adaptive(c, S, err);
 }
 
 // the reference's verification scenarios: two colliding elastic rings (setup_rings,
 // benchmarks.cpp:155-253) and a disk flying into a wall (setup_impact, benchmarks.cpp:23-153).
 // Neither has a tool: the contact test of an empty tool finds no penetration and the tool force
-// stays zero. Both run on a locked cell grid over the box of the reference's fixed-grid
-// constructor (benchmarks.cpp:218 for the rings, :136 for the impact).
+// stays zero. The locked cell grid is the box of the reference's fixed-grid constructor
+// (benchmarks.cpp:218 for the rings, :136 for the impact); with grid_adaptive (the impact's
+// active line, :135) the geometry follows the particles instead.
 int build_bodies(const mfc_config &c, scenario &S, std::string &err) {
   sim_constants &k = S.k;
   const bool rings = c.lattice == MFC_LATTICE_RINGS;

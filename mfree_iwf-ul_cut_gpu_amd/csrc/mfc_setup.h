@@ -34,6 +34,8 @@ struct sim_constants {
   int jc_tanh;        // the secant solves with the tanh-modified flow stress
   double tanh_a, tanh_b, tanh_c, tanh_d;
   int plastic;        // Johnson-Cook return on (the reference's m_plastic, actions_gpu.cu:25)
+  int adaptive;       // the grid follows the particles every step (grid.cu:97-136)
+  int ncell_table;    // cells of the binning table: num_cell (locked) or max_cell (adaptive)
 };
 
 struct scenario {
@@ -43,6 +45,12 @@ struct scenario {
   std::vector<int> flags;
   mfc_tool_state tool;
 };
+
+// the geometry of an adaptive grid over positions and smoothing lengths (all N particles,
+// blanked ones included): box of the positions +- 1e-6, cell size max h (kernel width 2),
+// nx = ceil(lx / cell), ny = ceil(ly / cell) (grid_base::update_geometry, grid.cu:97-136)
+void adaptive_geometry(const double *px, const double *py, const double *h, int N, double &bbmin_x, double &bbmin_y,
+                       double &cell, int64_t &gnx, int64_t &gny);
 
 // fills a scenario from a configuration; returns MFC_OK or an MFC_E_* code with err set
 int build_scenario(const mfc_config &cfg, scenario &out, std::string &err);

@@ -74,8 +74,9 @@ struct mfc_sim {
   mfc::inst_args *dargs = nullptr;
   int2 *dtiles = nullptr, *dchunks = nullptr;
   int nchunks = 0;
+  mfc::grid_geo *geo = nullptr;    // adaptive grid: the geometry of the last step (device)
   mfc::batch_work work(int c) const {
-    return {dargs + c, 1, dtiles, ntiles, dchunks, nchunks, recs.f32, variant()};
+    return {dargs + c, 1, dtiles, ntiles, dchunks, nchunks, recs.f32, variant(), k.adaptive != 0};
   }
 
   template <class T>
@@ -107,3 +108,6 @@ void finish_steps(mfc_sim *s);
 int check_tuning(const mfc_tuning *t);
 // the batched kernels' view of a handle for start parity c
 mfc::inst_args instance_args(const mfc_sim *s, int c);
+// the grid of the last step: the creation geometry for a locked grid, the device's record for
+// an adaptive one (synchronises the handle's stream)
+int current_geometry(const mfc_sim *s, mfc::grid_geo &g);

@@ -63,7 +63,8 @@ class Config(C.Structure):
         ("pse_flag_quirk", C.c_int32), ("promote_fp64", C.c_int32), ("secant_tol", C.c_double),
         ("jc_model", C.c_int32), ("elastic", C.c_int32),
         ("jc_tanh_a", C.c_double), ("jc_tanh_b", C.c_double), ("jc_tanh_c", C.c_double), ("jc_tanh_d", C.c_double),
-        ("reserved", C.c_int32 * 4),
+        ("grid_adaptive", C.c_int32), ("max_cell_factor", C.c_int32),
+        ("reserved", C.c_int32 * 2),
     ]
 
     def copy(self) -> "Config":
@@ -81,7 +82,8 @@ class Info(C.Structure):
         ("n_particles", C.c_int32), ("n_tool", C.c_int32), ("grid_nx", C.c_int32), ("grid_ny", C.c_int32),
         ("num_cell", C.c_int32), ("precision", C.c_int32), ("dt", C.c_double), ("t_final", C.c_double),
         ("mass", C.c_double), ("bbmin_x", C.c_double), ("bbmin_y", C.c_double), ("cell_size", C.c_double),
-        ("step", C.c_int64), ("out_of_grid", C.c_int64),
+        ("step", C.c_int64), ("out_of_grid", C.c_int64), ("grid_overflow", C.c_int64),
+        ("grid_adaptive", C.c_int32), ("cell_table", C.c_int32),
     ]
 
 
@@ -251,12 +253,16 @@ def config_rings(nbox: int = 80, precision: int = PREC_FP64) -> Config:
     return c
 
 
-def config_impact(nbox: int = 20, precision: int = PREC_FP64) -> Config:
+def config_impact(nbox: int = 20, precision: int = PREC_FP64, adaptive: bool = True) -> Config:
     """setup_impact (benchmarks.cpp:23-153): a steel 4340 disk (SI units) at 1e3 into a wall,
-    Johnson-Cook plasticity without heat, no tool; nbox = 20 is main()'s commented call."""
+    Johnson-Cook plasticity without heat, no tool; nbox = 20 is main()'s commented call. The
+    grid adapts to the particles every step as the reference's active line does
+    (benchmarks.cpp:135); adaptive=False selects its commented fixed grid (:136)."""
     c = Config()
     _check(lib().mfc_config_impact(C.byref(c), nbox), "mfc_config_impact")
     c.precision = precision
+    if not adaptive:
+        c.grid_adaptive = 0
     return c
 
 
@@ -347,6 +353,9 @@ class SimInfo:
     cell_size: float
     step: int
     out_of_grid: int  # particles binned outside the grid so far (0 in a valid run)
+    grid_overflow: int  # adaptive grid: steps whose geometry did not fit the cell table
+    grid_adaptive: int
+    cell_table: int   # cells of the binning table
 
 
 class Sim:

@@ -389,9 +389,13 @@ struct sim_t {
   bool promote = false;     // reference float truncations evaluated in FP64
   bool plastic = true;      // m_plastic (actions_gpu.cu:25, set by actions_setup_johnson_cook_constants)
   double secant_tol = 1e-6; // actions_gpu.cu:261
-  // grid (grid.cu:139-158)
+  // grid (grid.cu:139-158); an adaptive grid recomputes it every step (grid.cu:97-136) and
+  // sorts over a table of ncell_table = max_cell cells (num_cell for a locked grid)
   int gnx = 0, gny = 0, num_cell = 0;
   double bbmin_x = 0, bbmin_y = 0, gdx = 0;
+  bool adaptive = false;
+  int ncell_table = 0;
+  int64_t grid_overflow = 0;
   // tool
   tool_t tool;
   double contact_alpha = 0.;
@@ -423,6 +427,46 @@ uint64_t splitmix64(uint64_t &s) {
 double u01(uint64_t &s) { return (double)(splitmix64(s) >> 11) * (1.0 / 9007199254740992.0); }
 
 int setup_bodies(sim_t &s, const mfc_config &c);
+
+// grid_base::update_geometry (grid.cu:97-136) with the kernel width 2 of leap_frog::step: the
+// box of all positions (thrust::min_element / max_element over the whole array, blanked
+// particles included) widened by 1e-6, cell size max h. NaN positions are passed over. A
+// geometry that needs at least max_cell cells (the reference asserts) is not taken: the
+// previous one stays and the step is counted.
+void update_geometry(sim_t &s) {
+  double x0 = s.pos[0].x, x1 = x0, y0 = s.pos[0].y, y1 = y0, hm = s.h[0];
+  for (int i = 1; i < s.N; i++) {
+    x0 = std::fmin(x0, s.pos[i].x); x1 = std::fmax(x1, s.pos[i].x);
+    y0 = std::fmin(y0, s.pos[i].y); y1 = std::fmax(y1, s.pos[i].y);
+    hm = std::fmax(hm, s.h[i]);
+  }
+  double bx = x0 - 1e-6, by = y0 - 1e-6;
+  double ex = x1 + 1e-6, ey = y1 + 1e-6;
+  double cell = hm * 2. / 2.;
+  double fnx = std::ceil((ex - bx) / cell), fny = std::ceil((ey - by) / cell);
+  if (!(fnx >= 1. && fny >= 1. && fnx * fny < (double)s.ncell_table)) { s.grid_overflow++; return; }
+  s.bbmin_x = bx; s.bbmin_y = by; s.gdx = cell;
+  s.gnx = (int)fnx; s.gny = (int)fny; s.num_cell = s.gnx * s.gny;
+}
+
+// the cell table of a scenario: num_cell cells for a locked grid; max_cell_factor * N (10 N, as
+// grid_gpu_green(10*n, n) of benchmarks.cpp:135) for an adaptive one, whose first geometry is
+// that of the initial particles
+int adaptive_setup(sim_t &s, const mfc_config &c) {
+  s.adaptive = c.grid_adaptive != 0;
+  s.ncell_table = s.num_cell;
+  if (s.adaptive) {
+    int64_t factor = c.max_cell_factor > 0 ? c.max_cell_factor : 10;
+    int64_t table = factor * (int64_t)s.N;
+    if (table > (1LL << 30)) { g_err = "adaptive cell table too large"; return MFC_E_RANGE; }
+    s.ncell_table = (int)table;
+    update_geometry(s);
+    if (s.grid_overflow) { g_err = "the initial adaptive grid needs more cells than max_cell_factor * N"; return MFC_E_RANGE; }
+  }
+  s.cell_start.assign(s.ncell_table, -1);
+  s.cell_end.assign(s.ncell_table, 0);
+  return MFC_OK;
+}
 
 // setup_ref_cut, generalised through mfc_config (benchmarks.cpp:255-537)
 int setup(sim_t &s, const mfc_config &c) {
@@ -579,8 +623,7 @@ int setup(sim_t &s, const mfc_config &c) {
   for (int i = 0; i < N; i++) { s.idx[i] = i; s.id[i] = i; }
   s.pos_init.assign(N, z2); s.vel_init.assign(N, z2); s.S_init.assign(N, z4);
   s.rho_init.assign(N, 0.); s.T_init_arr.assign(N, 0.);
-  s.cell_start.assign(s.num_cell, -1); s.cell_end.assign(s.num_cell, 0);
-  return MFC_OK;
+  return adaptive_setup(s, c);
 }
 
 // setup_rings (benchmarks.cpp:155-253) and setup_impact (benchmarks.cpp:23-153): bodies with an
@@ -683,8 +726,7 @@ int setup_bodies(sim_t &s, const mfc_config &c) {
   for (int i = 0; i < N; i++) { s.idx[i] = i; s.id[i] = i; }
   s.pos_init.assign(N, z2); s.vel_init.assign(N, z2); s.S_init.assign(N, z4);
   s.rho_init.assign(N, 0.); s.T_init_arr.assign(N, 0.);
-  s.cell_start.assign(s.num_cell, -1); s.cell_end.assign(s.num_cell, 0);
-  return MFC_OK;
+  return adaptive_setup(s, c);
 }
 
 // cell coordinate of a particle: C++ truncation of (x - x0)/dx; values outside the int range
@@ -723,9 +765,9 @@ void permute(std::vector<T> &a, const std::vector<int> &perm) {
 void sort_and_cells(sim_t &s) {  // grid_gpu_green.cu:22-116,177-201 with a stable key sort
   int N = s.N;
   // stable counting sort of (hash, current position)
-  std::vector<int> cnt(s.num_cell + 1, 0);
+  std::vector<int> cnt(s.ncell_table + 1, 0);
   for (int i = 0; i < N; i++) cnt[s.hash[i] + 1]++;
-  for (int c = 0; c < s.num_cell; c++) cnt[c + 1] += cnt[c];
+  for (int c = 0; c < s.ncell_table; c++) cnt[c + 1] += cnt[c];
   std::vector<int> perm(N);
   for (int i = 0; i < N; i++) perm[cnt[s.hash[i]]++] = i;
   s.idx = perm;
@@ -1179,6 +1221,7 @@ void report_force(const sim_t &s, double F[2]) {
 }
 
 void step(sim_t &s, int64_t *plastic_iters) {  // leap_frog.cu:94-155
+  if (s.adaptive) update_geometry(s);
   assign_hashes(s);
   sort_and_cells(s);
   lf_init(s);
@@ -1298,6 +1341,9 @@ int oracle_get_info(const oracle_sim *o, mfc_info *info) {
   info->bbmin_x = s.bbmin_x; info->bbmin_y = s.bbmin_y; info->cell_size = s.gdx;
   info->step = s.step;
   info->out_of_grid = s.out_of_grid;
+  info->grid_overflow = s.grid_overflow;
+  info->grid_adaptive = s.adaptive;
+  info->cell_table = s.ncell_table;
   return MFC_OK;
 }
 
@@ -1404,13 +1450,14 @@ int oracle_get_cells(const oracle_sim *o, int32_t *off, size_t count) {
   if (!o || !off) return MFC_E_INVALID;
   const sim_t &s = o->s;
   if (count != (size_t)s.num_cell + 1) return MFC_E_RANGE;
-  // offsets from start/end of occupied cells; empty cells take the next occupied start
+  // offsets from start/end of occupied cells; empty cells take the next occupied start (cells
+  // of the table past num_cell hold only blanked particles of an adaptive grid, sorted last)
   int next = s.N;
-  off[s.num_cell] = s.N;
-  for (int c = s.num_cell - 1; c >= 0; c--) {
+  for (int c = s.ncell_table - 1; c >= 0; c--) {
     if (s.cell_start[c] != -1) next = s.cell_start[c];
-    off[c] = next;
+    if (c <= s.num_cell) off[c] = next;
   }
+  if (s.ncell_table == s.num_cell) off[s.num_cell] = s.N;
   return MFC_OK;
 }
 

@@ -13,7 +13,8 @@ pytestmark = pytest.mark.gpu
 
 CASES = {
     "rings_30": (lambda p: pymfc.config_rings(30, p), 400),
-    "impact_20": (lambda p: pymfc.config_impact(20, p), 120),
+    "impact_20": (lambda p: pymfc.config_impact(20, p), 120),                   # adaptive grid (active line)
+    "impact_20_locked": (lambda p: pymfc.config_impact(20, p, adaptive=False), 120),  # the commented fixed grid
 }
 
 
@@ -36,7 +37,34 @@ def test_bodies_bitwise_with_oracle(name, split, compact):
     assert_bitwise(g, o, context=name)
     assert equal(g.force_history(), o.force_history())
     assert not np.any(g.tool_force())
+    gi, oi = g.info, o.info
+    for f in ("grid_nx", "grid_ny", "num_cell", "bbmin_x", "bbmin_y", "cell_size", "cell_table", "grid_adaptive",
+              "out_of_grid", "grid_overflow"):
+        assert getattr(gi, f) == getattr(oi, f), f
+    assert gi.out_of_grid == 0 and gi.grid_overflow == 0
+    assert equal(g.cells(), o.cells())
     g.close()
+
+
+def test_adaptive_grid_overflow_matches_oracle():
+    """A particle thrown far out: the geometry pass keeps the previous grid (the cell table of
+    10 N cells cannot hold the new one), counts the overflow and the particle outside the grid,
+    on the device as in the oracle."""
+    cfg = pymfc.config_impact(20)
+    g, o = pymfc.Sim(cfg), ob.OracleSim(cfg)
+    g.step(3)
+    o.step(3)
+    pos = o.field("pos")
+    pos[0, 1] = 50.0
+    g.set_field("pos", pos)
+    o.set_field("pos", pos)
+    g.step(5)
+    o.step(5)
+    assert_bitwise(g, o, context="overflow")
+    gi, oi = g.info, o.info
+    assert gi.grid_overflow == oi.grid_overflow == 5
+    assert gi.out_of_grid == oi.out_of_grid == 5
+    assert (gi.grid_nx, gi.grid_ny, gi.bbmin_x) == (oi.grid_nx, oi.grid_ny, oi.bbmin_x)
 
 
 def test_rings_fp32c_keeps_momentum_and_rebound():
@@ -62,7 +90,9 @@ def test_rings_fp32c_keeps_momentum_and_rebound():
 
 
 def test_bodies_in_a_batch_beside_a_cut():
-    cfgs = [pymfc.config_rings(24), pymfc.config_impact(16), pymfc.cut_config(40, 12, 10.0, thermal="contact")]
+    """A batch mixing locked grids (the rings, the cut) and the impact's adaptive grid."""
+    cfgs = [pymfc.config_rings(24), pymfc.config_impact(16), pymfc.cut_config(40, 12, 10.0, thermal="contact"),
+            pymfc.config_impact(14, adaptive=False)]
     b = pymfc.Batch(cfgs)
     b.step(19)
     b.sync()

@@ -107,3 +107,75 @@ def test_impact_plastic_flow_and_fixed_rows(elastic):
     else:
         assert eps.max() > 0.1 and (eps > 0).sum() > 50
     assert o.out_of_grid() == 0 and np.isfinite(o.field("S")).all()
+
+
+def test_impact_grid_follows_the_particles():
+    """The impact's active grid (grid_gpu_green(10*n, n), benchmarks.cpp:135) is recomputed before
+    every binning from the box of all positions and the largest h (grid.cu:97-136): after each
+    step the geometry the oracle reports is the one of the positions at that step's start, the
+    table holds 10 N cells, and every particle is inside it."""
+    cfg = pymfc.config_impact(20)
+    assert cfg.grid_adaptive == 1 and cfg.max_cell_factor == 10
+    o = ob.OracleSim(cfg)
+    info = o.info
+    assert info.grid_adaptive == 1 and info.cell_table == 10 * o.N
+    bbx = []
+    for _ in range(6):
+        p, h = o.field("pos"), o.field("h")
+        o.step(10)  # the last of these steps bins the positions after nine more steps; recompute below
+        bbx.append(o.info.bbmin_x)
+    # one step from a known state: its geometry is that of the state's positions
+    p, h = o.field("pos"), o.field("h")
+    o.step(1)
+    info = o.info
+    x0, y0 = p[:, 0].min() - 1e-6, p[:, 1].min() - 1e-6
+    cell = h.max() * 2.0 / 2.0
+    nx = math.ceil(((p[:, 0].max() + 1e-6) - x0) / cell)
+    ny = math.ceil(((p[:, 1].max() + 1e-6) - y0) / cell)
+    assert (info.bbmin_x, info.bbmin_y, info.cell_size) == (x0, y0, cell)
+    assert (info.grid_nx, info.grid_ny, info.num_cell) == (nx, ny, nx * ny)
+    assert info.grid_overflow == 0 and o.out_of_grid() == 0
+    assert bbx[-1] > bbx[0]  # the flyer moves right: the box's left edge follows it
+    # the cell ranges cover every particle once, in the order of its cell
+    off = o.cells()
+    assert off[0] == 0 and off[-1] == o.N and np.all(np.diff(off) >= 0)
+    hsh = o.field("hash")
+    for c in np.unique(hsh)[:50]:
+        assert np.all(hsh[off[c]:off[c + 1]] == c)
+
+
+def test_impact_adaptive_and_locked_grids_agree():
+    """The two grids sort the particles into different cells, so the neighbour sums add in a
+    different order; the runs agree to rounding."""
+    runs = []
+    for adaptive in (True, False):
+        o = ob.OracleSim(pymfc.config_impact(20, adaptive=adaptive))
+        o.step(60)
+        runs.append((o.field("id"), o.field("pos"), o.field("eps_pl")))
+    (ia, pa, ea), (ib, pb, eb) = runs
+    sa, sb = np.argsort(ia), np.argsort(ib)
+    assert np.abs(pa[sa] - pb[sb]).max() < 1e-9
+    assert np.abs(ea[sa] - eb[sb]).max() < 1e-9 and ea.max() > 0.05
+
+
+def test_adaptive_grid_overflow_keeps_the_previous_geometry():
+    """A particle thrown far out would need a grid larger than the 10 N cell table: the step
+    keeps the previous geometry, counts the overflow, and counts the particle outside the grid.
+    A first geometry that does not fit the table is refused at setup."""
+    cfg = pymfc.config_impact(20)
+    o = ob.OracleSim(cfg)
+    o.step(3)
+    before = o.info
+    pos = o.field("pos")
+    pos[0, 1] = 50.0
+    o.set_field("pos", pos)
+    o.step(1)
+    after = o.info
+    assert after.grid_overflow == 1 and after.out_of_grid == 1
+    assert (after.bbmin_x, after.bbmin_y, after.grid_nx, after.grid_ny) == \
+        (before.bbmin_x, before.bbmin_y, before.grid_nx, before.grid_ny)
+    cfg.max_cell_factor = 1
+    rings = pymfc.config_rings(30)
+    rings.grid_adaptive, rings.max_cell_factor = 1, 1
+    with pytest.raises(RuntimeError):
+        ob.OracleSim(rings)
