@@ -762,12 +762,13 @@ __device__ inline int stage_tile(const dev_consts &k, int t, const drecords &r, 
 
 // the next step's first binning pass for the updated particle i (its storage position in the
 // next step): cell id and arrival slot, as hash_count would take them from the updated state;
-// next == nullptr: the next step runs hash_count itself
-__device__ inline void next_cell(const dev_consts &k, int i, bool live, double x, double y, int fl,
-                                 const dbins *next) {
-  if (next == nullptr) return;  // the same in every thread
-  const int h = live ? particle_cell(k, x, y, fl, next->key_sorted[i], next->clamped) : -1;
-  count_cell(i, h, next->key, next->slot, next->count, next->bsum);
+// bin == false: the next step bins its start state itself (the bins are passed by value so that
+// they stay in registers)
+__device__ inline void next_cell(const dev_consts &k, int i, bool live, double x, double y, int fl, const dbins next,
+                                 bool bin) {
+  if (!bin) return;  // the same in every thread
+  const int h = live ? particle_cell(k, x, y, fl, next.key_sorted[i], next.clamped) : -1;
+  count_cell(i, h, next.key, next.slot, next.count, next.bsum);
 }
 
 // fused: sums + update in one pass (registers of both live in one kernel)
@@ -775,7 +776,8 @@ template <bool F32, bool COMPACT>
 __device__ inline void interact(const dev_consts &k, int t, const drecords &r, const int *__restrict__ key_sorted,
                                 const int *__restrict__ offsets, const int *__restrict__ perm, const dstate &src,
                                 const dstate &dst, const dpositional &pos,
-                                const mfc_tool_state *__restrict__ tool_now, const dforce &fo, const dbins *next) {
+                                const mfc_tool_state *__restrict__ tool_now, const dforce &fo, const dbins next,
+                                bool bin_next) {
   __shared__ stage_buf st;
   const int lo = stage_tile(k, t, r, key_sorted, offsets, st);
   const int i = t * TILE + threadIdx.x;
@@ -796,7 +798,7 @@ __device__ inline void interact(const dev_consts &k, int t, const drecords &r, c
                                   acc, tpse);
     update_particle(k, i, s, acc, tpse, src, dst, pos, *tool_now, fsum, xn, yn, fln);
   }
-  next_cell(k, i, i < k.N, xn, yn, fln, next);
+  next_cell(k, i, i < k.N, xn, yn, fln, next, bin_next);
   __syncthreads();  // the force partials reuse the stage
   force_partials(t, fsum, fo.partials, reinterpret_cast<double (*)[TILE]>(st.pos));
   if (t == 0 && threadIdx.x == 0) *fo.pending = 1;
@@ -827,7 +829,7 @@ __device__ inline void sums(const dev_consts &k, int t, const drecords &r, const
 __device__ inline void update(const dev_consts &k, int t, const double *__restrict__ acc,
                               const int *__restrict__ perm, const dstate &src, const dstate &dst,
                               const dpositional &pos, const mfc_tool_state *__restrict__ tool_now, const dforce &fo,
-                              const dbins *next) {
+                              const dbins next, bool bin_next) {
   const int i = t * TILE + threadIdx.x;
   double fsum[4] = {0., 0., 0., 0.};
   double xn = 0., yn = 0.;
@@ -838,7 +840,7 @@ __device__ inline void update(const dev_consts &k, int t, const double *__restri
     load_acc(acc, k.N, i, a, tpse);
     update_particle(k, i, perm[i], a, tpse, src, dst, pos, *tool_now, fsum, xn, yn, fln);
   }
-  next_cell(k, i, i < k.N, xn, yn, fln, next);
+  next_cell(k, i, i < k.N, xn, yn, fln, next, bin_next);
   __shared__ double red[4][TILE];
   force_partials(t, fsum, fo.partials, red);
   if (t == 0 && threadIdx.x == 0) *fo.pending = 1;
@@ -884,7 +886,7 @@ __global__ __launch_bounds__(TILE) SUMS_OCCUPANCY void k_interact(dev_consts k, 
                                                    const mfc_tool_state *__restrict__ tool_now, dforce fo) {
   const int t = xcd_chunk_tile(blockIdx.x, ntiles);
   if (t < 0) return;
-  interact<F32, COMPACT>(k, t, r, key_sorted, offsets, perm, src, dst, pos, tool_now, fo, nullptr);
+  interact<F32, COMPACT>(k, t, r, key_sorted, offsets, perm, src, dst, pos, tool_now, fo, dbins{}, false);
 }
 
 template <bool F32, bool COMPACT>
@@ -903,7 +905,7 @@ __global__ __launch_bounds__(TILE) void k_update(dev_consts k, int ntiles, const
                                                  dforce fo) {
   const int t = xcd_tile(blockIdx.x, ntiles);
   if (t < 0) return;
-  update(k, t, acc, perm, src, dst, pos, tool_now, fo, nullptr);
+  update(k, t, acc, perm, src, dst, pos, tool_now, fo, dbins{}, false);
 }
 
 __global__ __launch_bounds__(TILE) void k_force_final(int ntiles, dforce fo) { force_final(ntiles, fo); }
@@ -1161,7 +1163,7 @@ __global__ __launch_bounds__(TILE) SUMS_OCCUPANCY void kb_interact(const inst_ar
   const dbins b = load_bins(&c->b);
   // an adaptive grid bins the next step's start state after its geometry pass instead
   interact<F32, COMPACT>(a.k, it.y, load_recs(&c->r), b.key_sorted, b.offsets, b.perm, a.cur, a.nxt, a.pos,
-                         a.tool_out, a.f, a.k.adaptive ? nullptr : &b);
+                         a.tool_out, a.f, b, !a.k.adaptive);
 }
 
 template <bool F32, bool COMPACT>
@@ -1184,7 +1186,7 @@ __global__ __launch_bounds__(TILE) void kb_update(const inst_args *__restrict__ 
   const inst_args &a = A[it.x];
   MFC_CONST(inst_args) *c = const_args(A, it.x);
   const dbins b = load_bins(&c->b);
-  update(a.k, it.y, c->acc, b.perm, a.cur, a.nxt, a.pos, a.tool_out, a.f, a.k.adaptive ? nullptr : &b);
+  update(a.k, it.y, c->acc, b.perm, a.cur, a.nxt, a.pos, a.tool_out, a.f, b, !a.k.adaptive);
 }
 
 __global__ __launch_bounds__(TILE) void kb_force_final(const inst_args *__restrict__ A) {
