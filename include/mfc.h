@@ -63,6 +63,13 @@ extern "C" {
 #define MFC_JC_PLAIN 0  /* Johnson-Cook (plasticity.cuh:30-46), the reference's choice */
 #define MFC_JC_TANH  1  /* tanh-modified Johnson-Cook after Calamaz 2008 (plasticity.cuh:48-70) */
 
+/* cell grid (grid.h): both sort the particles stably by cell id and give the same cell ranges */
+#define MFC_GRID_GREEN   0  /* grid_gpu_green (grid_gpu_green.cu): ranges from the counting sort */
+#define MFC_GRID_ROTHLIN 1  /* grid_gpu_rothlin (grid_gpu_rothlin.cu:28-250): ranges rebuilt from the
+                               sorted ids by a unique / scatter / scan pass, and, as its sort does
+                               (:183-247), the friction force ft is not carried along with its
+                               particle: contact reads the history at the sorted index */
+
 /* arithmetic of the step */
 #define MFC_PREC_FP64   0   /* plain FP64, bit-exact with the oracle */
 #define MFC_PREC_FP32C  1   /* FP32 neighbour sums with Kahan compensation, FP64 islands */
@@ -136,7 +143,8 @@ typedef struct mfc_config {
                              reference's grid_gpu_green(10*n, n)); a step whose grid would need
                              as many cells keeps the previous geometry and is counted in
                              mfc_info.grid_overflow (the reference asserts) */
-  int32_t reserved[2];
+  int32_t grid_kind;      /* MFC_GRID_*: the reference's two cell grids */
+  int32_t reserved[1];
 } mfc_config;
 
 typedef struct mfc_info {
@@ -243,9 +251,10 @@ int mfc_sync(mfc_sim *sim);
 int mfc_profile(mfc_sim *sim, int nsteps, double *ms);
 
 /* report_force: F = sum_i fc_i + sum_i ft_i after the last step, summed in a fixed
- * order: tiles of 256 particles in storage order, each reduced by halving strides; lane l
- * of 256 adds the tile results l, l + 256, ... in ascending order; the 256 lane sums
- * reduced by halving strides; sum of fc and sum of ft added last */
+ * order: groups of 64 particles in storage order, each reduced by halving strides; lane l
+ * of 256 adds the group results l, l + 256, ... in ascending order (4 groups per 256
+ * particles started); the 256 lane sums reduced by halving strides; sum of fc and sum of ft
+ * added last. The order does not depend on the work-group size of the step kernels. */
 int mfc_tool_force(mfc_sim *sim, double F[2]);
 /* forces of every step since creation or the last reset, as F[step][2];
  * *n receives the number of steps copied (at most cap); with F == NULL, *n receives the

@@ -308,7 +308,7 @@ int setup_work(mfc_sim *s, int nsteps, int K, int w0, int w1) {
     const size_t n = (size_t)N;
     if ((rc = w->alloc(&w->work[0], state_block_bytes(N))) || (rc = w->alloc(&w->work[1], state_block_bytes(N))) ||
         (rc = w->alloc(&w->posw, positional_block_bytes(N))) || (rc = w->alloc(&w->toolw, 2)) ||
-        (rc = w->alloc(&w->partials, 4 * (size_t)s->ntiles)) || (rc = w->alloc(&w->force_last, 2)) ||
+        (rc = w->alloc(&w->partials, 16 * (size_t)s->ntiles)) || (rc = w->alloc(&w->force_last, 2)) ||
         (rc = w->alloc(&w->hist, 2)) || (rc = w->alloc(&w->ctr, 1)) || (rc = w->alloc(&w->pending, 1)) ||
         (rc = w->alloc(&w->keys_keep, n)) ||
         (rc = w->alloc(&w->acc, L_N * n)) || (rc = w->alloc(&w->lam_acc, L_N * n)) ||
@@ -342,6 +342,7 @@ int adj_begin(mfc_sim *s, int nsteps, int checkpoint_every, int w0, int w1) {
     return fail(MFC_E_INVALID, "the adjoint differentiates the cutting scenario (tool and plasticity present)");
   if (s->k.jc_tanh) return fail(MFC_E_INVALID, "the adjoint differentiates the plain Johnson-Cook model (jc_model = 0)");
   if (s->k.adaptive) return fail(MFC_E_INVALID, "the adjoint runs on a locked grid (grid_adaptive = 0)");
+  if (s->k.rothlin) return fail(MFC_E_INVALID, "the adjoint runs on the green grid (grid_kind = 0)");
   int64_t base = 0;
   int rc = mfc_force_history(s, nullptr, 0, &base);
   if (rc) return rc;

@@ -217,7 +217,7 @@ int mfc_create(const mfc_config *cfg, mfc_sim **out) {
       (rc = s->alloc(&b.count, c.ncell_table)) || (rc = s->alloc(&b.offsets, (size_t)c.ncell_table + 1)) ||
       (rc = s->alloc(&b.bucket, N)) || (rc = s->alloc(&b.perm, N)) ||
       (rc = s->alloc(&b.bsum, mfc::scan_blocks(c.ncell_table))) || (rc = s->alloc(&b.clamped, 1)) ||
-      (rc = s->alloc(&s->geo, 1)))
+      (rc = s->alloc(&s->geo, 1)) || (c.rothlin && (rc = s->alloc(&b.offsets_r, (size_t)c.ncell_table + 1))))
     return bail(rc);
   mfc::dpositional &p = s->pos;
   {
@@ -232,7 +232,7 @@ int mfc_create(const mfc_config *cfg, mfc_sim **out) {
     return bail(rc);
   }
   if ((rc = s->alloc(&s->tool, 2)) ||
-      (rc = s->alloc(&s->partials, (size_t)4 * s->ntiles)) || (rc = s->alloc(&s->force_last, 2)) ||
+      (rc = s->alloc(&s->partials, (size_t)16 * s->ntiles)) || (rc = s->alloc(&s->force_last, 2)) ||
       (rc = s->alloc(&s->hist, (size_t)2 * HIST_CAP)) || (rc = s->alloc(&s->hist_ctr, 1)) ||
       (rc = s->alloc(&s->force_pending, 1)) ||
       (rc = s->alloc(&s->acc, (size_t)mfc::ACC_N * N)))
@@ -269,6 +269,7 @@ int mfc_create(const mfc_config *cfg, mfc_sim **out) {
     if (hipMemsetAsync(b.key_sorted, 0, sizeof(int) * N, s->stream) != hipSuccess ||
         hipMemsetAsync(b.count, 0, sizeof(int) * c.ncell_table, s->stream) != hipSuccess ||
         hipMemsetAsync(b.offsets, 0, sizeof(int) * ((size_t)c.ncell_table + 1), s->stream) != hipSuccess ||
+        (b.offsets_r && hipMemsetAsync(b.offsets_r, 0, sizeof(int) * ((size_t)c.ncell_table + 1), s->stream) != hipSuccess) ||
         hipMemsetAsync(b.clamped, 0, sizeof(unsigned long long), s->stream) != hipSuccess ||
         hipMemsetAsync(s->hist_ctr, 0, sizeof(int64_t), s->stream) != hipSuccess ||
         hipMemsetAsync(s->force_pending, 0, sizeof(int), s->stream) != hipSuccess ||
@@ -555,7 +556,7 @@ int mfc_get_cells(mfc_sim *s, int32_t *offsets, size_t count) {
   int rc = current_geometry(s, g);
   if (rc) return rc;
   if (count != (size_t)g.num_cell + 1) return fail(MFC_E_RANGE, "count must be num_cell + 1");
-  return copy_out(s, offsets, s->bins.offsets, sizeof(int) * count);
+  return copy_out(s, offsets, s->k.rothlin ? s->bins.offsets_r : s->bins.offsets, sizeof(int) * count);
 }
 
 int mfc_tool_wear(mfc_sim *s, double K, double alpha, const double tool_vel[2], double dt, double out[4]) {

@@ -26,9 +26,10 @@ struct mfc_batch {
   hipGraphExec_t graph[2] = {nullptr, nullptr};
 
   bool adaptive = false;  // some instance has an adaptive grid
+  bool rothlin = false;   // some instance uses grid_gpu_rothlin's ranges
   mfc::batch_work work(int parity) const {
     return {args[parity], (int)sims.size(), tiles, ntiles, chunks, nchunks, f32,
-            {tuning.split_interact != 0, tuning.compact_pairs != 0}, adaptive};
+            {tuning.split_interact != 0, tuning.compact_pairs != 0}, adaptive, rothlin};
   }
   void drop_graphs() {
     for (int p = 0; p < 2; p++)
@@ -133,6 +134,7 @@ int mfc_batch_create(const mfc_config *cfgs, int n, mfc_batch **out) {
     s->batched = true;
     s->use_graphs = false;
     b->adaptive = b->adaptive || s->k.adaptive;
+    b->rothlin = b->rothlin || s->k.rothlin;
   }
   std::vector<int2> tiles, chunks;
   std::vector<mfc::inst_args> a0(n), a1(n);

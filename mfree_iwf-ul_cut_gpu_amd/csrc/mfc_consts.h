@@ -30,6 +30,8 @@ struct dev_consts {
   // counting sort runs over it), num_cell for a locked grid
   int adaptive;
   int ncell_table;
+  // grid_gpu_rothlin: cell ranges rebuilt from the sorted ids; ft not carried through the sort
+  int rothlin;
 };
 
 // the geometry of an adaptive grid for one step, kept on the device for the host to read

@@ -29,6 +29,7 @@ struct dbins {
   int *bucket;       // particle indices grouped by cell, unordered inside a cell
   int *perm;         // sorted position -> current position
   int *bsum;         // scan block sums
+  int *offsets_r;    // MFC_GRID_ROTHLIN: the ranges rebuilt from the sorted ids (the step reads these)
   unsigned long long *clamped;  // binned particles whose position lay outside the grid on an axis
 };
 
@@ -60,7 +61,7 @@ constexpr int STAGE_CAP = MFC_STAGE_CAP;
 // force-final pass (the next step's first kernel, or a kernel of its own after the last step),
 // which appends F to the history
 struct dforce {
-  double *partials;      // 4 per tile: sum fc.x, fc.y, ft.x, ft.y
+  double *partials;      // 4 per wave of 64 particles: sum fc.x, fc.y, ft.x, ft.y
   double *force_last;    // F of the last step
   double *hist;          // force history, 2 per step, cap entries
   int64_t cap;
@@ -108,6 +109,7 @@ struct batch_work {
   bool f32;
   variant v;
   bool adaptive;          // some instance has an adaptive grid (a geometry pass starts the step)
+  bool rothlin;           // some instance uses grid_gpu_rothlin's cell ranges (a pass after predict)
 };
 
 // one step of every instance of a batch: 4 (fused) or 5 (split) launches, and its phases. The

@@ -594,7 +594,9 @@ __device__ inline void update_particle(const dev_consts &k, int i, int s, const 
   double ptx = src.ptx[s], pty = src.pty[s], vtx = src.vtx[s], vty = src.vty[s];
   double rhot = src.rhot[s], stxx = src.stxx[s], stxy = src.stxy[s], styy = src.styy[s], stzz = src.stzz[s];
   double Tt = src.Tt[s];
-  double fcx = src.fcx[s], fcy = src.fcy[s], ftx = src.ftx[s], fty = src.fty[s];
+  // grid_gpu_rothlin's sort leaves ft in place: the contact reads the history at the sorted index
+  const int sft = k.rothlin ? i : s;
+  double fcx = src.fcx[s], fcy = src.fcy[s], ftx = src.ftx[sft], fty = src.fty[sft];
   double eps = src.eps[s], epsd = src.epsd[s];
   // predicted state (same expressions as k_predict)
   double pu = u0, pv = v0, psxx = sxx0, psxy = sxy0, psyy = syy0, pszz = szz0, prho = rho0;
@@ -697,22 +699,22 @@ __device__ inline void particle_sums(const dev_consts &k, int i, const double *_
   }
 }
 
-// tile partial sums of the tool force by halving strides in red (LDS, 4 x TILE doubles; all
-// threads of the work-group call it)
-__device__ inline void force_partials(int t, const double fsum[4], double *__restrict__ partials,
-                                      double (*red)[TILE]) {
+// partial sums of the tool force per wave of 64 particles (storage order): the halving-stride
+// tree over the wave's lanes (lane k adds lane k + st for st = 32, 16, ..., 1, so lane 0 ends with
+// the tree's root), then lane 0 writes the wave's four sums at partials[4 * wave]. Every lane of
+// the wave calls it (lanes past N carry zeros).
+__device__ inline void force_partials(int wave, const double fsum[4], double *__restrict__ partials) {
+  double v0 = fsum[0], v1 = fsum[1], v2 = fsum[2], v3 = fsum[3];
 #pragma unroll
-  for (int c = 0; c < 4; c++) red[c][threadIdx.x] = fsum[c];
-  for (int s = TILE / 2; s > 0; s >>= 1) {
-    __syncthreads();
-    if (threadIdx.x < s) {
-#pragma unroll
-      for (int c = 0; c < 4; c++) red[c][threadIdx.x] = red[c][threadIdx.x] + red[c][threadIdx.x + s];
-    }
+  for (int st = 32; st > 0; st >>= 1) {
+    v0 = v0 + __shfl_down(v0, st, 64);
+    v1 = v1 + __shfl_down(v1, st, 64);
+    v2 = v2 + __shfl_down(v2, st, 64);
+    v3 = v3 + __shfl_down(v3, st, 64);
   }
-  if (threadIdx.x == 0) {
-#pragma unroll
-    for (int c = 0; c < 4; c++) partials[4 * t + c] = red[c][0];
+  if ((threadIdx.x & 63) == 0) {
+    partials[4 * wave + 0] = v0; partials[4 * wave + 1] = v1;
+    partials[4 * wave + 2] = v2; partials[4 * wave + 3] = v3;
   }
 }
 
@@ -721,7 +723,6 @@ struct stage_buf {
   double2 pos[STAGE_CAP];
   int fl[STAGE_CAP];
 };
-static_assert(sizeof(double2) * STAGE_CAP >= sizeof(double) * 4 * TILE, "force partials reuse the stage");
 
 // [lo, hi): the sorted indices of every window candidate of the particles of tile t. The first
 // and the last cell of a particle's window (column-major cell ids) do not decrease along the
@@ -799,8 +800,7 @@ __device__ inline void interact(const dev_consts &k, int t, const drecords &r, c
     update_particle(k, i, s, acc, tpse, src, dst, pos, *tool_now, fsum, xn, yn, fln);
   }
   next_cell(k, i, i < k.N, xn, yn, fln, next, bin_next);
-  __syncthreads();  // the force partials reuse the stage
-  force_partials(t, fsum, fo.partials, reinterpret_cast<double (*)[TILE]>(st.pos));
+  force_partials(i >> 6, fsum, fo.partials);
   if (t == 0 && threadIdx.x == 0) *fo.pending = 1;
 }
 
@@ -841,20 +841,19 @@ __device__ inline void update(const dev_consts &k, int t, const double *__restri
     update_particle(k, i, perm[i], a, tpse, src, dst, pos, *tool_now, fsum, xn, yn, fln);
   }
   next_cell(k, i, i < k.N, xn, yn, fln, next, bin_next);
-  __shared__ double red[4][TILE];
-  force_partials(t, fsum, fo.partials, red);
+  force_partials(i >> 6, fsum, fo.partials);
   if (t == 0 && threadIdx.x == 0) *fo.pending = 1;
 }
 
-// tool force of the step from the tile partials, when the interaction of a step has left them
-// (pending): lane l adds the tile results l, l + TILE, ... in ascending order, then the lane
+// tool force of the step from the wave partials, when the interaction of a step has left them
+// (pending): lane l adds the results of waves l, l + TILE, ... in ascending order, then the lane
 // sums are reduced by halving strides; F = sum(fc) + sum(ft). All threads of the work-group
 // call it.
 __device__ inline void force_final(int ntiles, const dforce &fo) {
   if (*fo.pending == 0) return;  // the same value in every thread
   __shared__ double red[4][TILE];
   double a0 = 0., a1 = 0., a2 = 0., a3 = 0.;
-  for (int q = threadIdx.x; q < ntiles; q += TILE) {
+  for (int q = threadIdx.x; q < 4 * ntiles; q += TILE) {  // the wave partials, 4 per tile
     const double4 v = reinterpret_cast<const double4 *>(fo.partials)[q];
     a0 = a0 + v.x; a1 = a1 + v.y; a2 = a2 + v.z; a3 = a3 + v.w;
   }
@@ -994,7 +993,7 @@ __device__ inline dstate load_state(MFC_CONST(dstate) * c) {
 __device__ inline dbins load_bins(MFC_CONST(dbins) * c) {
   dbins o;
   MFC_F(key); MFC_F(key_sorted); MFC_F(slot); MFC_F(count); MFC_F(offsets); MFC_F(bucket); MFC_F(perm);
-  MFC_F(bsum); MFC_F(clamped);
+  MFC_F(bsum); MFC_F(offsets_r); MFC_F(clamped);
   return o;
 }
 __device__ inline dpositional load_pos(MFC_CONST(dpositional) * c) {
@@ -1052,8 +1051,8 @@ __global__ __launch_bounds__(256) void kb_hash_count(const inst_args *__restrict
 // into the instance's descriptor for the step's later kernels, then it bins the start state with
 // it (cell ids, arrival slots, cell and chunk counts). Minimum and maximum do not depend on the
 // order they are taken in; NaN positions are passed over. A geometry that needs as many cells as
-// the binning table holds is not taken (the reference asserts): the previous one stays and the
-// step is counted in geo->overflow.
+// the binning table holds is not taken (the reference asserts): the previous step's (kept in
+// geo, which every step writes) stays and the step is counted in geo->overflow.
 constexpr int GEO_THREADS = 1024;
 __global__ __launch_bounds__(GEO_THREADS) void kb_geometry(inst_args *__restrict__ A) {
   // plain loads: this kernel writes the descriptor it reads (not through the constant space)
@@ -1093,8 +1092,9 @@ __global__ __launch_bounds__(GEO_THREADS) void kb_geometry(inst_args *__restrict
     int gnx, gny;
     if (fnx >= 1. && fny >= 1. && fnx * fny < (double)a.k.ncell_table) {
       gnx = (int)fnx; gny = (int)fny;
-    } else {  // keep the previous step's geometry
-      bx = a.k.bbmin_x; by = a.k.bbmin_y; cell = a.k.cell; gnx = a.k.gnx; gny = a.k.gny;
+    } else {  // keep the previous step's geometry (the instance's record: the descriptor of
+              // this buffer parity holds the one of two steps before)
+      bx = a.geo->bbmin_x; by = a.geo->bbmin_y; cell = a.geo->cell; gnx = a.geo->gnx; gny = a.geo->gny;
       a.geo->overflow = a.geo->overflow + 1;
     }
     geo_d[0] = bx; geo_d[1] = by; geo_d[2] = cell;
@@ -1152,6 +1152,32 @@ __global__ __launch_bounds__(TILE) void kb_predict(const inst_args *__restrict__
   if (it.y == 0) force_final(a.ntiles, load_force(&c->f));
 }
 
+// The cell ranges of grid_gpu_rothlin (grid_gpu_rothlin.cu:28-127) rebuilt from the sorted cell
+// ids: the lane at the end of each run of equal ids (where unique_by_key_copy starts the next run)
+// writes the run's end into the offsets of its cell's successor and of every empty cell up to the
+// next run's cell, which is the scatter of the run lengths and their scan over the empty cells
+// between in one pass; the first lane writes 0 up to the first id, the last run fills the table's
+// end with N. Every entry of offsets_r is written by exactly one lane; the same ranges as the
+// counting sort's offsets.
+__global__ __launch_bounds__(256) void kb_rothlin_cells(const inst_args *__restrict__ A, const int2 *__restrict__ work,
+                                                        int nwork) {
+  const int w = xcd_tile(blockIdx.x, nwork);
+  if (w < 0) return;
+  const int2 it = work[w];
+  MFC_CONST(inst_args) *c = const_args(A, it.x);
+  const int rothlin = A[it.x].k.rothlin, N = A[it.x].k.N, nc = A[it.x].k.ncell_table;
+  if (!rothlin) return;
+  const int i = it.y * 256 + threadIdx.x;
+  if (i >= N) return;
+  const dbins b = load_bins(&c->b);
+  const int h = b.key_sorted[i];
+  if (i == 0)
+    for (int q = 0; q <= h; q++) b.offsets_r[q] = 0;
+  const int hn = i + 1 < N ? b.key_sorted[i + 1] : nc;
+  if (hn != h)
+    for (int q = h + 1; q <= hn; q++) b.offsets_r[q] = i + 1;
+}
+
 template <bool F32, bool COMPACT>
 __global__ __launch_bounds__(TILE) SUMS_OCCUPANCY void kb_interact(const inst_args *__restrict__ A, const int2 *__restrict__ work,
                                                     int nwork) {
@@ -1162,8 +1188,8 @@ __global__ __launch_bounds__(TILE) SUMS_OCCUPANCY void kb_interact(const inst_ar
   MFC_CONST(inst_args) *c = const_args(A, it.x);
   const dbins b = load_bins(&c->b);
   // an adaptive grid bins the next step's start state after its geometry pass instead
-  interact<F32, COMPACT>(a.k, it.y, load_recs(&c->r), b.key_sorted, b.offsets, b.perm, a.cur, a.nxt, a.pos,
-                         a.tool_out, a.f, b, !a.k.adaptive);
+  interact<F32, COMPACT>(a.k, it.y, load_recs(&c->r), b.key_sorted, a.k.rothlin ? b.offsets_r : b.offsets, b.perm,
+                         a.cur, a.nxt, a.pos, a.tool_out, a.f, b, !a.k.adaptive);
 }
 
 template <bool F32, bool COMPACT>
@@ -1175,7 +1201,8 @@ __global__ __launch_bounds__(TILE) SUMS_OCCUPANCY void kb_sums(const inst_args *
   const inst_args &a = A[it.x];
   MFC_CONST(inst_args) *c = const_args(A, it.x);
   const dbins b = load_bins(&c->b);
-  sums<F32, COMPACT>(a.k, it.y, load_recs(&c->r), b.key_sorted, b.offsets, b.perm, a.cur.flags, c->acc);
+  sums<F32, COMPACT>(a.k, it.y, load_recs(&c->r), b.key_sorted, a.k.rothlin ? b.offsets_r : b.offsets, b.perm,
+                     a.cur.flags, c->acc);
 }
 
 __global__ __launch_bounds__(TILE) void kb_update(const inst_args *__restrict__ A, const int2 *__restrict__ work,
@@ -1316,6 +1343,7 @@ void launch_batch_predict(const batch_work &w, hipStream_t s) {
   const dim3 gt(padded_tiles(w.ntiles));
   if (w.f32) hipLaunchKernelGGL(kb_predict<true>, gt, dim3(TILE), 0, s, w.args, w.tiles, w.ntiles);
   else hipLaunchKernelGGL(kb_predict<false>, gt, dim3(TILE), 0, s, w.args, w.tiles, w.ntiles);
+  if (w.rothlin) hipLaunchKernelGGL(kb_rothlin_cells, gt, dim3(256), 0, s, w.args, w.tiles, w.ntiles);
 }
 
 // a batched launch over the work table, with the kernel's own start and end recorded in t0 and

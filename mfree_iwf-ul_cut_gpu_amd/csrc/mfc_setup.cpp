@@ -184,6 +184,7 @@ void make_dev_consts(const sim_constants &c, dev_consts &k) {
   k.stage_limit = 0;
   k.adaptive = c.adaptive;
   k.ncell_table = c.ncell_table;
+  k.rothlin = c.rothlin;
 }
 
 void adaptive_geometry(const double *px, const double *py, const double *h, int N, double &bbmin_x, double &bbmin_y,
@@ -205,6 +206,8 @@ void adaptive_geometry(const double *px, const double *py, const double *h, int 
 // max_cell_factor * N cells (grid_gpu_green(10*n, n) in the reference, benchmarks.cpp:135)
 static int set_adaptive(const mfc_config &c, scenario &S, std::string &err) {
   sim_constants &k = S.k;
+  if (c.grid_kind != MFC_GRID_GREEN && c.grid_kind != MFC_GRID_ROTHLIN) { err = "unknown grid_kind"; return MFC_E_INVALID; }
+  k.rothlin = c.grid_kind == MFC_GRID_ROTHLIN;
   k.adaptive = 0;
   k.ncell_table = k.num_cell;
   if (!c.grid_adaptive) return MFC_OK;

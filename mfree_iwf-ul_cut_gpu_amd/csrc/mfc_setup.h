@@ -36,6 +36,7 @@ struct sim_constants {
   int plastic;        // Johnson-Cook return on (the reference's m_plastic, actions_gpu.cu:25)
   int adaptive;       // the grid follows the particles every step (grid.cu:97-136)
   int ncell_table;    // cells of the binning table: num_cell (locked) or max_cell (adaptive)
+  int rothlin;        // grid_gpu_rothlin semantics (MFC_GRID_ROTHLIN)
 };
 
 struct scenario {

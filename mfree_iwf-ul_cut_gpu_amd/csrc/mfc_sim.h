@@ -45,7 +45,7 @@ struct mfc_sim {
   hipStream_t stream = nullptr;
   std::vector<void *> allocs;
   mfc::dstate set[2];
-  mfc::dbins bins;
+  mfc::dbins bins = {};
   mfc::dpositional pos;
   mfc::drecords recs = {false, nullptr, nullptr, nullptr};
   int precision = MFC_PREC_FP64;
@@ -76,7 +76,7 @@ struct mfc_sim {
   int nchunks = 0;
   mfc::grid_geo *geo = nullptr;    // adaptive grid: the geometry of the last step (device)
   mfc::batch_work work(int c) const {
-    return {dargs + c, 1, dtiles, ntiles, dchunks, nchunks, recs.f32, variant(), k.adaptive != 0};
+    return {dargs + c, 1, dtiles, ntiles, dchunks, nchunks, recs.f32, variant(), k.adaptive != 0, k.rothlin != 0};
   }
 
   template <class T>

@@ -31,6 +31,7 @@ THERMAL_BROOKSHAW, THERMAL_PSE = 0, 1
 PREC_FP64, PREC_FP32C = 0, 1
 JC_PLAIN, JC_TANH = 0, 1  # MFC_JC_*: the flow stress of the radial return's secant
 LATTICE_REF, LATTICE_EXPLICIT, LATTICE_RINGS, LATTICE_IMPACT = 0, 1, 2, 3
+GRID_GREEN, GRID_ROTHLIN = 0, 1  # MFC_GRID_*
 FLAG_FIXED, FLAG_BLANKED, FLAG_TOOL = 1, 2, 4
 
 FIELDS = {
@@ -63,8 +64,8 @@ class Config(C.Structure):
         ("pse_flag_quirk", C.c_int32), ("promote_fp64", C.c_int32), ("secant_tol", C.c_double),
         ("jc_model", C.c_int32), ("elastic", C.c_int32),
         ("jc_tanh_a", C.c_double), ("jc_tanh_b", C.c_double), ("jc_tanh_c", C.c_double), ("jc_tanh_d", C.c_double),
-        ("grid_adaptive", C.c_int32), ("max_cell_factor", C.c_int32),
-        ("reserved", C.c_int32 * 2),
+        ("grid_adaptive", C.c_int32), ("max_cell_factor", C.c_int32), ("grid_kind", C.c_int32),
+        ("reserved", C.c_int32 * 1),
     ]
 
     def copy(self) -> "Config":

@@ -396,6 +396,7 @@ struct sim_t {
   bool adaptive = false;
   int ncell_table = 0;
   int64_t grid_overflow = 0;
+  bool rothlin = false;  // grid_gpu_rothlin: ranges by unique/scatter/scan, ft not sorted
   // tool
   tool_t tool;
   double contact_alpha = 0.;
@@ -453,6 +454,8 @@ void update_geometry(sim_t &s) {
 // grid_gpu_green(10*n, n) of benchmarks.cpp:135) for an adaptive one, whose first geometry is
 // that of the initial particles
 int adaptive_setup(sim_t &s, const mfc_config &c) {
+  if (c.grid_kind != MFC_GRID_GREEN && c.grid_kind != MFC_GRID_ROTHLIN) { g_err = "unknown grid_kind"; return MFC_E_INVALID; }
+  s.rothlin = c.grid_kind == MFC_GRID_ROTHLIN;
   s.adaptive = c.grid_adaptive != 0;
   s.ncell_table = s.num_cell;
   if (s.adaptive) {
@@ -762,6 +765,29 @@ void permute(std::vector<T> &a, const std::vector<int> &perm) {
   a.swap(b);
 }
 
+// grid_gpu_rothlin::do_get_cells (grid_gpu_rothlin.cu:28-127) on the sorted ids: the indices
+// where the id changes (unique_by_key_copy) and the ids there (the map); the particles of each
+// occupied cell as the difference of consecutive indices (the stencil, 0 for the last occupied
+// cell); the stencil scattered to its cells and inclusively scanned into offsets[1..]; the
+// offsets from the last occupied cell on set to N, offsets[0] to 0. Cell c is then
+// [offsets[c], offsets[c+1]) and start/end are those (empty cells are empty ranges).
+void rothlin_cells(sim_t &s) {
+  const int N = s.N, ncell = s.ncell_table;
+  std::vector<int> indices, map;
+  for (int i = 0; i < N; i++)
+    if (i == 0 || s.hash[i] != s.hash[i - 1]) { indices.push_back(i); map.push_back(s.hash[i]); }
+  const int occupied = (int)indices.size();
+  std::vector<int> stencil(occupied, 0), scatter(ncell, 0), offsets(ncell + 1, 0);
+  for (int q = 0; q + 1 < occupied; q++) stencil[q] = std::abs(indices[q] - indices[q + 1]);
+  for (int q = 0; q < occupied; q++) scatter[map[q]] = stencil[q];
+  int run = 0;
+  for (int c = 0; c < ncell; c++) { run += scatter[c]; offsets[c + 1] = run; }
+  const int last_hash = s.hash[N - 1];
+  for (int c = last_hash + 1; c <= ncell; c++) offsets[c] = N;
+  offsets[0] = 0;
+  for (int c = 0; c < ncell; c++) { s.cell_start[c] = offsets[c]; s.cell_end[c] = offsets[c + 1]; }
+}
+
 void sort_and_cells(sim_t &s) {  // grid_gpu_green.cu:22-116,177-201 with a stable key sort
   int N = s.N;
   // stable counting sort of (hash, current position)
@@ -772,11 +798,16 @@ void sort_and_cells(sim_t &s) {  // grid_gpu_green.cu:22-116,177-201 with a stab
   for (int i = 0; i < N; i++) perm[cnt[s.hash[i]]++] = i;
   s.idx = perm;
   permute(s.hash, perm);
-  permute(s.pos, perm); permute(s.vel, perm); permute(s.fc, perm); permute(s.ft, perm);
+  permute(s.pos, perm); permute(s.vel, perm); permute(s.fc, perm);
+  if (!s.rothlin) permute(s.ft, perm);  // grid_gpu_rothlin's sort leaves ft in place (:183-247)
   permute(s.h, perm); permute(s.rho, perm); permute(s.S, perm); permute(s.eps_pl, perm);
   permute(s.eps_pl_dot, perm); permute(s.T, perm); permute(s.fixed, perm); permute(s.blanked, perm);
   permute(s.tool_p, perm); permute(s.pos_t, perm); permute(s.vel_t, perm); permute(s.rho_t, perm);
   permute(s.S_t, perm); permute(s.T_t, perm); permute(s.id, perm);
+  if (s.rothlin) {
+    rothlin_cells(s);
+    return;
+  }
   std::fill(s.cell_start.begin(), s.cell_start.end(), -1);
   for (int i = 0; i < N; i++) {
     int hsh = s.hash[i];
@@ -1198,15 +1229,21 @@ void invalidate(sim_t &s) {  // actions_gpu.cu:317-349
 // with thrust::reduce, whose order is unspecified): tiles of 256 particles in storage order, each reduced by halving
 // strides; then lane l of 256 adds the tile results l, l + 256, l + 512, ... in ascending order
 // starting from 0, and the 256 lane sums are reduced by halving strides; sum(fc) + sum(ft)
+// the build's fixed summation order of report_force (thrust's reduce order is unspecified):
+// groups of 64 particles in storage order, each reduced by halving strides; lane l of 256 adds
+// the group results l, l + 256, ... in ascending order (groups counted up to 4 per 256 particles
+// started, the missing ones adding zeros); the 256 lane sums reduced by halving strides
 double tree_sum(const std::vector<V2> &a, bool y) {
   int N = (int)a.size();
-  double v[256], lane[256];
+  double v[64], lane[256];
   for (int k = 0; k < 256; k++) lane[k] = 0.;
-  for (int t0 = 0, t = 0; t0 < N; t0 += 256, t++) {
-    for (int k = 0; k < 256; k++) v[k] = (t0 + k < N) ? (y ? a[t0 + k].y : a[t0 + k].x) : 0.;
-    for (int st = 128; st > 0; st >>= 1)
+  const int ngroups = 4 * ((N + 255) / 256);
+  for (int g = 0; g < ngroups; g++) {
+    const int g0 = 64 * g;
+    for (int k = 0; k < 64; k++) v[k] = (g0 + k < N) ? (y ? a[g0 + k].y : a[g0 + k].x) : 0.;
+    for (int st = 32; st > 0; st >>= 1)
       for (int k = 0; k < st; k++) v[k] = v[k] + v[k + st];
-    lane[t % 256] = lane[t % 256] + v[0];
+    lane[g % 256] = lane[g % 256] + v[0];
   }
   for (int st = 128; st > 0; st >>= 1)
     for (int k = 0; k < st; k++) lane[k] = lane[k] + lane[k + st];

@@ -113,3 +113,34 @@ def test_out_of_grid_counted_per_axis():
     o.set_field("pos", pos)
     o.step(1)
     assert o.info.out_of_grid == 1
+
+
+def test_rothlin_ranges_and_friction_history():
+    """grid_gpu_rothlin (grid_gpu_rothlin.cu:28-250): the unique / scatter / scan ranges equal the
+    green grid's; without friction (the rings) the runs are identical, in a cut the friction force
+    left in place by its sort changes the run."""
+    import pymfc
+    for make, differs in ((lambda: pymfc.config_rings(24), False),
+                          (lambda: pymfc.cut_config(40, 12, 10.0, thermal="contact", speed_m_min=1000.0), True)):
+        runs = []
+        for kind in (pymfc.GRID_GREEN, pymfc.GRID_ROTHLIN):
+            cfg = make()
+            cfg.grid_kind = kind
+            o = ob.OracleSim(cfg)
+            o.step(60)
+            runs.append(o)
+        g, r = runs
+        if not differs:
+            for f in ("pos", "vel", "S", "ft", "hash", "id"):
+                assert np.array_equal(g.field(f), r.field(f)), f
+            assert np.array_equal(g.cells(), r.cells())
+        else:
+            assert not np.array_equal(g.field("ft"), r.field("ft"))
+        # the rothlin ranges of its own state are the counting-sort ranges of that state
+        h = r.field("hash")
+        off = r.cells()
+        occupied = np.zeros(r.info.num_cell, bool)
+        occupied[h] = True
+        assert np.all((np.diff(off) > 0) == occupied)
+        starts = np.flatnonzero(np.r_[True, h[1:] != h[:-1]])
+        assert np.array_equal(off[h[starts]], starts)
