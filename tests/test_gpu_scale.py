@@ -85,10 +85,11 @@ def test_calibration_gradient_at_full_size(workload):
     the few particles of the contact zone switching between elastic and plastic. On the 6k cut
     (10 um spacing) the three difference estimates disagree by 10-5000 % on every entry: they do not
     resolve a derivative there, and the reverse mode is checked against the host restatement of
-    the reverse chain instead (built from the same blocks, pairs pushed instead of pulled, itself
-    matched to central differences on cuts where those converge), to 1e-8 of its largest entry. On
-    the 40k cut (4 um) the estimates agree to 1e-4 on most entries, and there the reverse mode
-    matches them to 1e-4; the other entries must lie within twice the estimates' spread."""
+    the reverse chain instead (built from the same blocks, pairs pushed instead of pulled and the
+    sums taken in another order, itself matched to central differences on cuts where those
+    converge), to 1e-7 of its largest entry. On the 40k cut (4 um) the estimates agree to 1e-3 on
+    the entries of n, C, m and mu, and there the reverse mode matches them to 1e-4; the A and B
+    entries must lie within twice the estimates' spread."""
     cfg = calibration_config(workload)
     nsteps, K, w0, w1 = 400, 50, 0, 400
     F, J, sim = gpu_jacobian(cfg, nsteps, K, w0, w1)
@@ -100,7 +101,7 @@ def test_calibration_gradient_at_full_size(workload):
     print(workload, "N", sim.N, "F", F, "\nvjp", J, "\nfd ", fd, "\nfd spread (relative)", spread,
           "\nvjp - fd (relative)", err / np.maximum(np.abs(fd), 1e-300))
     if workload == "adjoint_40k":
-        determinate = spread < 1e-4
+        determinate = spread < 1e-3
         assert determinate.sum() >= 6
         assert (err[determinate] <= 1e-4 * np.abs(fd[determinate])).all()
         assert (err <= 1e-4 * np.abs(fd) + 2.0 * unc).all()
@@ -109,7 +110,7 @@ def test_calibration_gradient_at_full_size(workload):
         for c, cot in enumerate(([1.0, 0.0], [0.0, 1.0])):
             Fh, gh, _, _ = host_chain(cfg, nsteps, w0, w1, cot)
             assert np.abs(F - Fh).max() <= 1e-12 * np.abs(F).max()
-            assert (np.abs(J[c] - gh) <= 1e-8 * np.abs(gh).max()).all(), (J[c], gh)
+            assert (np.abs(J[c] - gh) <= 1e-7 * np.abs(gh).max()).all(), (J[c], gh)
     # the loss gradient
     th0 = theta_of(cfg)
     rng = np.random.default_rng(0x355)
