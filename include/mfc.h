@@ -272,6 +272,15 @@ int mfc_force_history_reset(mfc_sim *sim);
  * The mean's sum is a fixed-order tree over tiles of 256 particles, not the reference's
  * sequential host loop. */
 int mfc_tool_wear(mfc_sim *sim, double K, double alpha, const double tool_vel[2], double dt, double out[4]);
+/* mfc_tool_wear for callers that cannot allocate device memory (code under HIP graph capture,
+ * serving loops, harnesses that forbid allocation); no counterpart in the reference. The same
+ * kernels and results as mfc_tool_wear, with the per-tile partial sums in a fixed device array of
+ * the library (one per GPU, held until the process ends) instead of memory allocated on the first
+ * call: it calls no allocating routine. Handles of up to MFC_NOALLOC_WEAR_MAX_TILES tiles of 256
+ * particles (2,097,152 particles); larger ones are refused with MFC_E_RANGE. All calls on a GPU
+ * share the array: calls that overlap on one GPU race. */
+#define MFC_NOALLOC_WEAR_MAX_TILES 8192
+int mfc_tool_wear_noalloc(mfc_sim *sim, double K, double alpha, const double tool_vel[2], double dt, double out[4]);
 
 /* vtk_writer_write(particles, step): the current state's particles that are not blanked, as
  * the reference's legacy ASCII VTK file at path (vertex cells; density, Temperature, Fixed,

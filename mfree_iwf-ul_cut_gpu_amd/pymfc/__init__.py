@@ -200,6 +200,7 @@ def lib() -> C.CDLL:
         L.mfc_get_cells.argtypes = [vp, C.POINTER(C.c_int32), sz]
         L.mfc_get_tool.argtypes = [vp, C.POINTER(ToolState)]
         L.mfc_tool_wear.argtypes = [vp, C.c_double, C.c_double, dp, C.c_double, dp]
+        L.mfc_tool_wear_noalloc.argtypes = [vp, C.c_double, C.c_double, dp, C.c_double, dp]
         L.mfc_write_vtk.argtypes = [vp, C.c_char_p]
         L.mfc_write_tool_vtk.argtypes = [vp, C.c_char_p]
         L.mfc_device_math.argtypes = [i32, dp, dp, dp, i64]
