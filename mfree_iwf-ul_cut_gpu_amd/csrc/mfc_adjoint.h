@@ -299,6 +299,22 @@ MFC_HD jc_grad sigma_yield_grad(const dev_consts &k, double e, double ed, double
   return g;
 }
 
+// the temperature factor 1 - theta^m of sigma_yield_grad, and its value sigma_Y alone with that
+// factor given: the same expressions, for the secant's residuals, which evaluate the flow stress
+// at one temperature many times
+MFC_HD double jc_temperature_factor(const dev_consts &k, double t) {
+  const double th = (t - k.jc_Tref) / (k.jc_Tmelt - k.jc_Tref);
+  return 1.0 - mfcm::pow(th, k.jc_m);
+}
+MFC_HD double sigma_yield_value(const dev_consts &k, double e, double ed, double TC) {
+  const double TA = k.jc_A + k.jc_B * mfcm::pow(e, k.jc_n);
+  const double es = ed / k.jc_eps_dot_ref;
+  double TB;
+  if (es > 1.0) TB = 1.0 + k.jc_C * mfcm::log(es);
+  else TB = mfcm::pow(1.0 + es, k.jc_C);
+  return TA * TB * TC;
+}
+
 // ------------------------------------------------------------------ contact
 // penalty contact and Coulomb friction of one particle (tool_gpu.cu:87-207, promoted) with its
 // VJP: forward outputs fc, ft, n; adjoints b_fc, b_ft, b_n in; adjoints of the position, the
@@ -510,7 +526,8 @@ MFC_HD void tail_vjp(const dev_consts &k, const mfc_tool_state &tool, const tail
     const double p = in.p;
     const double cxx = SXX - p, cyy = SYY - p, czz = SZZ - p, cxy = SXY;
     const double svm = sqrt((cxx * cxx + cyy * cyy + czz * czz) - cxx * cyy - cxx * czz - cyy * czz + 3.0 * cxy * cxy);
-    const double sY0 = sigma_yield_grad(k, e0, s[TS_EPSD], Tf).s;
+    const double TCf = jc_temperature_factor(k, Tf);
+    const double sY0 = sigma_yield_value(k, e0, s[TS_EPSD], TCf);
     if (svm < sY0) {
       ED2 = 0.;
     } else {
@@ -518,9 +535,9 @@ MFC_HD void tail_vjp(const dev_consts &k, const mfc_tool_state &tool, const tail
       double x = fmax(s[TS_EPSD] * dt * s23, 1e-8), xo;
       int it = 0;
       while (true) {
-        const double g0 = norm - 2 * G * x - s23 * sigma_yield_grad(k, e0 + s23 * fmax(x, 0.), s23 * fmax(x, 0.) / dt, Tf).s;
+        const double g0 = norm - 2 * G * x - s23 * sigma_yield_value(k, e0 + s23 * fmax(x, 0.), s23 * fmax(x, 0.) / dt, TCf);
         const double x1 = x * 1.01;
-        const double g1 = norm - 2 * G * x1 - s23 * sigma_yield_grad(k, e0 + s23 * fmax(x1, 0.), s23 * fmax(x1, 0.) / dt, Tf).s;
+        const double g1 = norm - 2 * G * x1 - s23 * sigma_yield_value(k, e0 + s23 * fmax(x1, 0.), s23 * fmax(x1, 0.) / dt, TCf);
         const double slope = (g0 - g1) / (x - x1);
         xo = x;
         x = x - g0 / slope;

@@ -115,27 +115,33 @@ __device__ inline void jaumann(double G, double vx_x, double vx_y, double vy_x, 
   tzz = twoG * (0. - s13 * 1.) + a22 + b22;
 }
 
-// Johnson-Cook flow stress (plasticity.cuh:30-46)
-__device__ inline double sigma_yield(const dev_consts &k, double eps_pl, double eps_pl_dot, double t) {
+// The temperature factor 1 - theta^m of the Johnson-Cook flow stress (plasticity.cuh:44). A
+// particle's return evaluates the flow stress at one temperature many times (the yield test, two
+// residuals per secant iteration, the plastic heating), so the factor is evaluated once per
+// particle; it is the same expression of the same values, so the results are unchanged.
+__device__ inline double jc_temperature_factor(const dev_consts &k, double t) {
   double theta = (t - k.jc_Tref) / (k.jc_Tmelt - k.jc_Tref);
+  return 1.0 - mfcm::pow(theta, k.jc_m);
+}
+
+// Johnson-Cook flow stress (plasticity.cuh:30-46), TC = jc_temperature_factor(k, t)
+__device__ inline double sigma_yield(const dev_consts &k, double eps_pl, double eps_pl_dot, double TC) {
   double TA = k.jc_A + k.jc_B * mfcm::pow(eps_pl, k.jc_n);
   double TB = 1.0;
   double ed = eps_pl_dot / k.jc_eps_dot_ref;
   if (ed > 1.0) TB = 1.0 + k.jc_C * mfcm::log(ed);
   else TB = mfcm::pow((1.0 + ed), k.jc_C);
-  double TC = 1.0 - mfcm::pow(theta, k.jc_m);
   return TA * TB * TC;
 }
 
-// tanh-modified Johnson-Cook flow stress after Calamaz 2008 (plasticity.cuh:48-70)
-__device__ inline double sigma_yield_tanh(const dev_consts &k, double eps_pl, double eps_dot_pl, double t) {
-  double theta = (t - k.jc_Tref) / (k.jc_Tmelt - k.jc_Tref);
+// tanh-modified Johnson-Cook flow stress after Calamaz 2008 (plasticity.cuh:48-70),
+// TC = jc_temperature_factor(k, t)
+__device__ inline double sigma_yield_tanh(const dev_consts &k, double eps_pl, double eps_dot_pl, double t, double TC) {
   double TA = k.jc_A + k.jc_B * mfcm::pow(eps_pl, k.jc_n) * 1.0 / mfcm::exp(mfcm::pow(eps_pl, k.tanh_a));
   double TB = 1.0;
   double ed = eps_dot_pl / k.jc_eps_dot_ref;
   if (ed > 1.0) TB = 1.0 + k.jc_C * mfcm::log(ed);
   else TB = mfcm::pow((1.0 + ed), k.jc_C);
-  double TC = 1.0 - mfcm::pow(theta, k.jc_m);
   double tD = 1.0 - mfcm::pow((t / k.jc_Tmelt), k.tanh_d);
   double tS = mfcm::pow(t / k.jc_Tmelt, k.tanh_b);
   double TD = tD + (1.0 - tD) * mfcm::tanh(1.0 / (mfcm::pow(eps_pl + tS, k.tanh_c)));
@@ -144,22 +150,22 @@ __device__ inline double sigma_yield_tanh(const dev_consts &k, double eps_pl, do
 
 // residual of the radial return (plasticity.cuh:74-83); the flow stress is the model of the
 // function pointer m_s_yield (plasticity.cuh:72), which only this residual uses
-__device__ inline double rr_residual(const dev_consts &k, double dl, double norm, double eps0, double t) {
+__device__ inline double rr_residual(const dev_consts &k, double dl, double norm, double eps0, double t, double TC) {
   const double s23 = sqrt(2.0 / 3.0);
   double e = eps0 + s23 * fmax(dl, 0.);
   double ed = s23 * fmax(dl, 0.) / k.dt;
-  double sY = k.jc_tanh ? sigma_yield_tanh(k, e, ed, t) : sigma_yield(k, e, ed, t);
+  double sY = k.jc_tanh ? sigma_yield_tanh(k, e, ed, t, TC) : sigma_yield(k, e, ed, TC);
   return norm - 2 * k.G * dl - s23 * sY;
 }
 
 // secant iteration of the radial return (plasticity.cuh:85-119)
-__device__ inline double rr_secant(const dev_consts &k, double init, double norm, double eps0, double t) {
+__device__ inline double rr_secant(const dev_consts &k, double init, double norm, double eps0, double t, double TC) {
   double dl = init, dl_old;
   int iter = 0;
   while (true) {
-    double g = rr_residual(k, dl, norm, eps0, t);
+    double g = rr_residual(k, dl, norm, eps0, t, TC);
     double dl1 = dl * 1.01;
-    double g1 = rr_residual(k, dl1, norm, eps0, t);
+    double g1 = rr_residual(k, dl1, norm, eps0, t, TC);
     double slope = (g - g1) / (dl - dl1);
     dl_old = dl;
     dl = dl - g / slope;
@@ -176,10 +182,11 @@ __device__ inline void plasticity(const dev_consts &k, double p, double rho, dou
   double norm = tsqrt(k, Sxx * Sxx + Syy * Syy + Szz * Szz + Sxy * Sxy);
   double cxx = Sxx - p, cyy = Syy - p, czz = Szz - p, cxy = Sxy;
   double svm = tsqrt(k, (cxx * cxx + cyy * cyy + czz * czz) - cxx * cyy - cxx * czz - cyy * czz + 3.0 * cxy * cxy);
-  double sY = sigma_yield(k, eps_pl, eps_pl_dot, T);
+  const double TC = jc_temperature_factor(k, T);
+  double sY = sigma_yield(k, eps_pl, eps_pl_dot, TC);
   if (svm < sY) { eps_pl_dot = 0.; return; }
   const double s23 = sqrt(2.0 / 3.0);
-  double dl = rr_secant(k, fmax(eps_pl_dot * k.dt * s23, 1e-8), norm, eps_pl, T);
+  double dl = rr_secant(k, fmax(eps_pl_dot * k.dt * s23, 1e-8), norm, eps_pl, T, TC);
   double inc;
   if (k.promote) inc = s23 * fmax(dl, 0.);
   else inc = (double)(sqrtf((float)(2.0 / 3.0)) * fmaxf((float)dl, 0.f));
@@ -195,7 +202,7 @@ __device__ inline void plasticity(const dev_consts &k, double p, double rho, dou
   eps_pl = e_new; eps_pl_dot = ed_new;
   if (k.tq != 0.) {
     double deps = e_new - e_old;
-    double sY2 = sigma_yield(k, e_new, ed_new, T);
+    double sY2 = sigma_yield(k, e_new, ed_new, TC);  // T is still the temperature of the return
     T += k.tq / (k.cp * rho) * deps * sY2;
   }
 }
