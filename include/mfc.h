@@ -326,14 +326,18 @@ int mfc_vjp(mfc_sim *sim, mfc_vjp_req *req, double grad[6]);
  * the pair terms of the marked ones in order; 0 (default): the pair terms run inside the window
  * walk. stage_limit: the most window candidates a tile of 256 particles copies into LDS
  * (0 = MFC_STAGE_CAP); a tile whose candidates span more sorted indices reads them from the
- * records in memory. The Python package's autotuner (pymfc.autotune) times the combinations on
- * a live handle and keeps the fastest. */
+ * records in memory. xcd_chunk: the interaction kernels deal their tiles to the 8 XCDs in chunks
+ * of this many consecutive tiles (1..MFC_XCD_CHUNK_MAX; 0 = 8); longer chunks fetch fewer
+ * neighbour columns into two L2s, shorter ones balance the XCDs better. The Python package's
+ * autotuner (pymfc.autotune) times the combinations on a live handle and keeps the fastest. */
 #define MFC_STAGE_CAP 2048
+#define MFC_XCD_CHUNK_MAX 64
 typedef struct mfc_tuning {
   int split_interact;
   int compact_pairs;
   int stage_limit;
-  int reserved[5];
+  int xcd_chunk;
+  int reserved[4];
 } mfc_tuning;
 int mfc_set_tuning(mfc_sim *sim, const mfc_tuning *tuning);
 int mfc_get_tuning(const mfc_sim *sim, mfc_tuning *tuning);

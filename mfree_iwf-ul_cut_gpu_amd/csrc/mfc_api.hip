@@ -77,11 +77,15 @@ int ensure_keys(mfc_sim *s) {
   return MFC_OK;
 }
 
+static_assert(mfc::XCD_CHUNK_MAX == MFC_XCD_CHUNK_MAX, "XCD chunk bound of the header and the kernels");
+
 int check_tuning(const mfc_tuning *t) {
   if ((t->split_interact != 0 && t->split_interact != 1) || (t->compact_pairs != 0 && t->compact_pairs != 1))
     return fail(MFC_E_INVALID, "split_interact and compact_pairs must be 0 or 1");
   if (t->stage_limit < 0 || t->stage_limit > MFC_STAGE_CAP)
     return fail(MFC_E_INVALID, "stage_limit must be in 0.." + std::to_string(MFC_STAGE_CAP));
+  if (t->xcd_chunk < 0 || t->xcd_chunk > mfc::XCD_CHUNK_MAX)
+    return fail(MFC_E_INVALID, "xcd_chunk must be in 0.." + std::to_string(mfc::XCD_CHUNK_MAX));
   return MFC_OK;
 }
 
@@ -104,6 +108,7 @@ mfc::inst_args instance_args(const mfc_sim *s, int c) {
   a.geo = s->geo;
   a.ntiles = s->ntiles;
   a.nscan = mfc::scan_blocks(s->k.ncell_table);
+  a.bin_next = 1;
   return a;
 }
 

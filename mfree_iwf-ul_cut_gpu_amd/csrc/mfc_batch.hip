@@ -17,7 +17,7 @@ struct mfc_batch {
   std::vector<mfc_sim *> sims;
   hipStream_t stream = nullptr;
   bool f32 = false;
-  mfc_tuning tuning = {0, 0, 0, {0, 0, 0, 0, 0}};
+  mfc_tuning tuning = {0, 0, 0, 0, {0, 0, 0, 0}};
   mfc::inst_args *args[2] = {nullptr, nullptr};  // descriptors per start parity (device)
   int2 *tiles = nullptr, *chunks = nullptr;
   int ntiles = 0, nchunks = 0;
@@ -29,7 +29,8 @@ struct mfc_batch {
   bool rothlin = false;   // some instance uses grid_gpu_rothlin's ranges
   mfc::batch_work work(int parity) const {
     return {args[parity], (int)sims.size(), tiles, ntiles, chunks, nchunks, f32,
-            {tuning.split_interact != 0, tuning.compact_pairs != 0}, adaptive, rothlin};
+            {tuning.split_interact != 0, tuning.compact_pairs != 0, tuning.xcd_chunk ? tuning.xcd_chunk : mfc::XCD_CHUNK},
+            adaptive, rothlin};
   }
   void drop_graphs() {
     for (int p = 0; p < 2; p++)

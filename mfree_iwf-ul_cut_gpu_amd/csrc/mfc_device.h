@@ -30,18 +30,24 @@ __device__ inline int xcd_tile(int b, int ntiles) {
   return t < ntiles ? t : -1;
 }
 
-// The interaction kernels' map: the tiles in chunks of XCD_CHUNK consecutive tiles, dealt to
+// The interaction kernels' map: the tiles in chunks of `chunk` consecutive tiles, dealt to
 // the 8 XCDs in turn (XCD x takes chunks x, x + 8, ...). The cost of a tile varies along the
 // sorted order (tool regions, free surfaces, the cut zone); contiguous ranges per XCD leave the
-// XCDs unevenly loaded, while chunks of 8 tiles (2,048 particles) keep each chunk's neighbour
-// data together in one L2 and give every XCD, and every CU, a mix of the domain. The grid must be a multiple of 8 * XCD_CHUNK blocks (chunked_blocks).
+// XCDs unevenly loaded, while chunks keep each chunk's neighbour data together in one L2 and give
+// every XCD, and every CU, a mix of the domain. A chunk's window reaches two cell columns past
+// its ends, whose particles are fetched into two L2s: longer chunks re-read less, shorter ones
+// balance better (a launch setting, XCD_CHUNK by default). The grid must be a multiple of
+// 8 * chunk blocks (chunked_blocks).
 constexpr int XCD_CHUNK = 8;
-__device__ inline int xcd_chunk_tile(int b, int ntiles) {
+constexpr int XCD_CHUNK_MAX = 64;  // MFC_XCD_CHUNK_MAX
+__device__ inline int xcd_chunk_tile(int b, int ntiles, int chunk = XCD_CHUNK) {
   const int x = b & 7, k = b >> 3;
-  const int t = (k / XCD_CHUNK) * (8 * XCD_CHUNK) + x * XCD_CHUNK + k % XCD_CHUNK;
+  const int t = (k / chunk) * (8 * chunk) + x * chunk + k % chunk;
   return t < ntiles ? t : -1;
 }
-inline int chunked_blocks(int ntiles) { return 8 * XCD_CHUNK * ((ntiles + 8 * XCD_CHUNK - 1) / (8 * XCD_CHUNK)); }
+inline int chunked_blocks(int ntiles, int chunk = XCD_CHUNK) {
+  return 8 * chunk * ((ntiles + 8 * chunk - 1) / (8 * chunk));
+}
 
 // cell coordinate: C++ truncation of a double, saturated outside the int range
 __device__ inline int64_t cell_coord(double f) {

@@ -79,6 +79,7 @@ constexpr int FLAG_NONFINITE = 8;
 struct variant {
   bool split;    // sums kernel + update kernel instead of one fused kernel
   bool compact;  // per-lane mask of in-support neighbours before the pair loop
+  int chunk;     // tiles per XCD chunk of the interaction kernels' tile map (1..XCD_CHUNK_MAX)
 };
 
 // one instance of a batch as the batched kernels see it (an array of these lives in device
@@ -96,6 +97,8 @@ struct inst_args {
   grid_geo *geo;  // adaptive grid: the geometry of the current step (written by the device)
   int ntiles;   // particle tiles of TILE
   int nscan;    // scan chunks of the cell table
+  int bin_next;  // 1: the interaction bins the next step's start state (0: the reverse mode's
+                 // replay, which bins every restored state itself)
 };
 
 // the work of one batched step: instance descriptors and (instance, block) work tables

@@ -1180,8 +1180,8 @@ __global__ __launch_bounds__(256) void kb_rothlin_cells(const inst_args *__restr
 
 template <bool F32, bool COMPACT>
 __global__ __launch_bounds__(TILE) SUMS_OCCUPANCY void kb_interact(const inst_args *__restrict__ A, const int2 *__restrict__ work,
-                                                    int nwork) {
-  const int w = xcd_chunk_tile(blockIdx.x, nwork);
+                                                    int nwork, int chunk) {
+  const int w = xcd_chunk_tile(blockIdx.x, nwork, chunk);
   if (w < 0) return;
   const int2 it = work[w];
   const inst_args &a = A[it.x];
@@ -1189,13 +1189,13 @@ __global__ __launch_bounds__(TILE) SUMS_OCCUPANCY void kb_interact(const inst_ar
   const dbins b = load_bins(&c->b);
   // an adaptive grid bins the next step's start state after its geometry pass instead
   interact<F32, COMPACT>(a.k, it.y, load_recs(&c->r), b.key_sorted, a.k.rothlin ? b.offsets_r : b.offsets, b.perm,
-                         a.cur, a.nxt, a.pos, a.tool_out, a.f, b, !a.k.adaptive);
+                         a.cur, a.nxt, a.pos, a.tool_out, a.f, b, a.bin_next && !a.k.adaptive);
 }
 
 template <bool F32, bool COMPACT>
 __global__ __launch_bounds__(TILE) SUMS_OCCUPANCY void kb_sums(const inst_args *__restrict__ A, const int2 *__restrict__ work,
-                                                int nwork) {
-  const int w = xcd_chunk_tile(blockIdx.x, nwork);
+                                                int nwork, int chunk) {
+  const int w = xcd_chunk_tile(blockIdx.x, nwork, chunk);
   if (w < 0) return;
   const int2 it = work[w];
   const inst_args &a = A[it.x];
@@ -1213,7 +1213,7 @@ __global__ __launch_bounds__(TILE) void kb_update(const inst_args *__restrict__ 
   const inst_args &a = A[it.x];
   MFC_CONST(inst_args) *c = const_args(A, it.x);
   const dbins b = load_bins(&c->b);
-  update(a.k, it.y, c->acc, b.perm, a.cur, a.nxt, a.pos, a.tool_out, a.f, b, !a.k.adaptive);
+  update(a.k, it.y, c->acc, b.perm, a.cur, a.nxt, a.pos, a.tool_out, a.f, b, a.bin_next && !a.k.adaptive);
 }
 
 __global__ __launch_bounds__(TILE) void kb_force_final(const inst_args *__restrict__ A) {
@@ -1356,8 +1356,18 @@ static void launch_timed(K kern, dim3 grid, const batch_work &w, hipStream_t s, 
     hipLaunchKernelGGL(kern, grid, dim3(TILE), 0, s, w.args, w.tiles, w.ntiles);
 }
 
+// the same for the interaction kernels, whose tile map takes the XCD chunk length
+template <class K>
+static void launch_chunked(K kern, const batch_work &w, hipStream_t s, hipEvent_t t0, hipEvent_t t1) {
+  const dim3 grid(chunked_blocks(w.ntiles, w.v.chunk));
+  if (t0)
+    hipExtLaunchKernelGGL(kern, grid, dim3(TILE), 0, s, t0, t1, 0, w.args, w.tiles, w.ntiles, w.v.chunk);
+  else
+    hipLaunchKernelGGL(kern, grid, dim3(TILE), 0, s, w.args, w.tiles, w.ntiles, w.v.chunk);
+}
+
 void launch_batch_sums(const batch_work &w, hipStream_t s, hipEvent_t t0, hipEvent_t t1) {
-  launch_timed(MFC_PICK(kb_sums, w.f32, w.v.compact), dim3(chunked_blocks(w.ntiles)), w, s, t0, t1);
+  launch_chunked(MFC_PICK(kb_sums, w.f32, w.v.compact), w, s, t0, t1);
 }
 
 void launch_batch_update(const batch_work &w, hipStream_t s, hipEvent_t t0, hipEvent_t t1) {
@@ -1365,7 +1375,7 @@ void launch_batch_update(const batch_work &w, hipStream_t s, hipEvent_t t0, hipE
 }
 
 void launch_batch_interact_fused(const batch_work &w, hipStream_t s, hipEvent_t t0, hipEvent_t t1) {
-  launch_timed(MFC_PICK(kb_interact, w.f32, w.v.compact), dim3(chunked_blocks(w.ntiles)), w, s, t0, t1);
+  launch_chunked(MFC_PICK(kb_interact, w.f32, w.v.compact), w, s, t0, t1);
 }
 
 void launch_batch_force(const batch_work &w, hipStream_t s) {

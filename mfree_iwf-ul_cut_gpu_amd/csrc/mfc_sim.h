@@ -67,8 +67,10 @@ struct mfc_sim {
   double *acc = nullptr;           // neighbour sums between the split interaction kernels
   double *wear_part = nullptr;     // tool wear: 4 doubles per tile, then the 4 results
   double wear_accum = 0.;          // accumulated wear, sum of dt * mean rate (tool_wear.cu:59)
-  mfc_tuning tuning = {0, 0, 0, {0, 0, 0, 0, 0}};
-  mfc::variant variant() const { return {tuning.split_interact != 0, tuning.compact_pairs != 0}; }
+  mfc_tuning tuning = {0, 0, 0, 0, {0, 0, 0, 0}};
+  mfc::variant variant() const {
+    return {tuning.split_interact != 0, tuning.compact_pairs != 0, tuning.xcd_chunk ? tuning.xcd_chunk : mfc::XCD_CHUNK};
+  }
   // the step runs the batched kernels on a batch of one: descriptors of this handle for start
   // parity 0 and 1, and its (instance, block) work tables
   mfc::inst_args *dargs = nullptr;

@@ -116,13 +116,14 @@ class VjpReq(C.Structure):
 
 
 STAGE_CAP = 2048  # MFC_STAGE_CAP: window candidates a tile can stage in LDS
+XCD_CHUNK_MAX = 64  # MFC_XCD_CHUNK_MAX: longest XCD chunk of the interaction kernels' tile map
 GRAPH_STEPS = 8   # steps per captured hipGraph of a handle (one graph per buffer parity)
 
 
 # the launch configurations the autotuner tries: fused or split interaction, direct or compacted
 # pair loop, and the LDS stage limit (0: tiles stage up to STAGE_CAP candidates; n > 0: tiles
 # whose candidates span more than n sorted indices read them from memory; 1: no tile stages)
-TUNING_SPACE = {"split_interact": (0, 1), "compact_pairs": (0, 1), "stage_limit": (0, 1536, 1)}
+TUNING_SPACE = {"split_interact": (0, 1), "compact_pairs": (0, 1), "stage_limit": (0, 1536, 1), "xcd_chunk": (8, 32)}
 
 
 def autotune(handle, steps: int = 100, space: dict | None = None, rounds: int = 2) -> dict:
@@ -155,7 +156,7 @@ def autotune(handle, steps: int = 100, space: dict | None = None, rounds: int = 
     handle.step(2 * GRAPH_STEPS)
     handle.sync()
     ran += 2 * GRAPH_STEPS
-    label = {"split_interact": "split", "compact_pairs": "compact", "stage_limit": "stage"}
+    label = {"split_interact": "split", "compact_pairs": "compact", "stage_limit": "stage", "xcd_chunk": "chunk"}
     out = dict(zip(keys, best))
     out["steps_run"] = ran
     out["ms_per_step"] = {"_".join(f"{label.get(k, k)}{v}" for k, v in zip(keys, c)): ms for c, ms in cand.items()}
@@ -165,7 +166,7 @@ def autotune(handle, steps: int = 100, space: dict | None = None, rounds: int = 
 class Tuning(C.Structure):
     """mfc_tuning (include/mfc.h)."""
     _fields_ = [("split_interact", C.c_int32), ("compact_pairs", C.c_int32), ("stage_limit", C.c_int32),
-                ("reserved", C.c_int32 * 5)]
+                ("xcd_chunk", C.c_int32), ("reserved", C.c_int32 * 4)]
 
 
 THETA = ("jc_A", "jc_B", "jc_n", "jc_C", "jc_m", "mu")  # order of the adjoint's parameters
@@ -440,7 +441,8 @@ class Sim:
     def tuning(self) -> dict:
         t = Tuning()
         _check(lib().mfc_get_tuning(self._h, C.byref(t)), "mfc_get_tuning")
-        return {"split_interact": t.split_interact, "compact_pairs": t.compact_pairs, "stage_limit": t.stage_limit}
+        return {"split_interact": t.split_interact, "compact_pairs": t.compact_pairs, "stage_limit": t.stage_limit,
+                "xcd_chunk": t.xcd_chunk}
 
     def vjp_forward(self, nsteps: int, checkpoint_every: int, w0: int, w1: int) -> np.ndarray:
         """Run nsteps steps keeping a checkpoint every checkpoint_every steps; returns the mean

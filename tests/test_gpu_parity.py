@@ -185,6 +185,35 @@ def test_unstaged_tiles_match(prec, thermal, split, compact):
     assert equal(g1.force_history(), g2.force_history())
 
 
+@pytest.mark.parametrize("prec", [pymfc.PREC_FP64, pymfc.PREC_FP32C])
+@pytest.mark.parametrize("split", [0, 1])
+def test_xcd_chunk_lengths_match(prec, split):
+    """The interaction kernels deal their tiles to the XCDs in chunks of xcd_chunk tiles; the
+    map decides only where a tile runs, so every chunk length gives the same bits (a cut of 55
+    tiles: chunks of 1, 3, 32 and 64 tiles, the last one holding every tile)."""
+    cfg = pymfc.cut_config(200, 60, 10.0, thermal="contact", speed_m_min=1000.0, precision=prec)
+    runs = []
+    for chunk in (0, 1, 3, 32, 64):
+        g = pymfc.Sim(cfg)
+        g.set_tuning(split_interact=split, xcd_chunk=chunk)
+        assert g.tuning()["xcd_chunk"] == chunk
+        g.step(30)
+        runs.append(g)
+    assert 48 * 256 < runs[0].N <= 64 * 256
+    for g in runs[1:]:
+        for f in ["pos", "vel", "S", "T", "eps_pl", "fc", "ft", "flags"]:
+            assert equal(runs[0].field(f), g.field(f)), f
+        assert equal(runs[0].force_history(), g.force_history())
+
+
+def test_xcd_chunk_is_checked():
+    g = pymfc.Sim(small_cut())
+    for bad in (-1, pymfc.XCD_CHUNK_MAX + 1):
+        with pytest.raises(pymfc.MfcError):
+            g.set_tuning(xcd_chunk=bad)
+    g.set_tuning(xcd_chunk=pymfc.XCD_CHUNK_MAX)
+
+
 def test_stage_limit_is_checked():
     g = pymfc.Sim(small_cut())
     with pytest.raises(pymfc.MfcError):
@@ -200,9 +229,9 @@ def test_autotune_keeps_the_run():
     cfg = small_cut(thermal="contact")
     a, b = pymfc.Sim(cfg), pymfc.Sim(cfg)
     t = pymfc.autotune(a, steps=5)
-    assert len(t["ms_per_step"]) == 12
-    assert a.tuning() == {k: t[k] for k in ("split_interact", "compact_pairs", "stage_limit")}
-    assert t["steps_run"] == 2 * 12 * (16 + 5) + 16
+    assert len(t["ms_per_step"]) == 24
+    assert {k: a.tuning()[k] for k in pymfc.TUNING_SPACE} == {k: t[k] for k in pymfc.TUNING_SPACE}
+    assert t["steps_run"] == 2 * 24 * (16 + 5) + 16
     b.step(t["steps_run"])
     a.step(10)
     b.step(10)
