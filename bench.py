@@ -20,6 +20,8 @@ import os
 import sys
 import time
 
+import numpy as np
+
 REPO = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(REPO, "mfree_iwf-ul_cut_gpu_amd"))
 
@@ -85,23 +87,34 @@ def cpu_baseline(cfg, target_s: float) -> dict:
 
 
 def tune(pymfc, handle, steps: int) -> dict:
-    """The package's autotuner on the live handle (every launch configuration gives the same
-    bits); off (defaults) when steps is 0."""
+    """The package's autotuner on a live handle (every launch configuration gives the same bits);
+    off (defaults) when steps is 0."""
     if steps <= 0:
-        return {"split_interact": 0, "compact_pairs": 0, "stage_limit": 0, "ms_per_step": {}}
+        return {k: v[0] for k, v in pymfc.TUNING_SPACE.items()} | {"ms_per_step": {}}
     return pymfc.autotune(handle, steps)
 
 
 def run_path(pymfc, workload: str, precision: int, steps: int, warmup: int, profile_steps: int,
              tune_steps: int) -> dict:
+    """The timed steps are steps warmup + 16 .. warmup + 16 + steps of the run whatever the tuning:
+    the autotuner runs on a handle of its own (after the same warm-up), and the measured handle
+    takes its choice, runs warmup steps and 2 * GRAPH_STEPS more (its hipGraphs are captured
+    there), then the timed steps. The cost of a step changes along the cut as the chip forms."""
     cfg = pymfc.workload(workload, precision)
+    tuning = {"ms_per_step": {}}
+    if tune_steps > 0:
+        t = pymfc.Sim(cfg)
+        t.step(warmup)
+        tuning = tune(pymfc, t, tune_steps)
+        t.close()
+    else:
+        tuning = tune(pymfc, None, 0)
     sim = pymfc.Sim(cfg)
+    sim.set_tuning(**{k: tuning[k] for k in pymfc.TUNING_SPACE})
     info = sim.info
     N = info.n_particles
     sim.step(warmup)
-    tuning = tune(pymfc, sim, tune_steps)
-    if tune_steps <= 0:  # the autotuner leaves its choice with its graphs built
-        sim.step(2 * pymfc.GRAPH_STEPS)
+    sim.step(2 * pymfc.GRAPH_STEPS)
     sim.sync()
     t0 = time.perf_counter()
     sim.step(steps)
@@ -163,16 +176,24 @@ def adjoint_ratio(pymfc, workload: str, nsteps: int, K: int) -> dict:
 
 def batch_bench(pymfc, n: int, steps: int, tune_steps: int) -> dict:
     """C5: n independent uneven cuts in one batch (one launch per phase for all instances),
-    against the same instances run one after another as single handles."""
+    against the same instances run one after another as single handles, over the same steps of
+    the run (10 + 2 * GRAPH_STEPS warm-up steps, the autotuner on a batch of its own)."""
     out = {"instances": n}
+    warm = 10 + 2 * pymfc.GRAPH_STEPS
     for name, prec in (("fp64", pymfc.PREC_FP64), ("fp32c", pymfc.PREC_FP32C)):
         cfgs = pymfc.batch_configs(n, precision=prec)
+        if tune_steps > 0:
+            tb = pymfc.Batch(cfgs)
+            tb.step(10)
+            tuning = tune(pymfc, tb, max(tune_steps, 2))
+            tb.close()
+        else:
+            tuning = tune(pymfc, None, 0)
+        best = {k: tuning[k] for k in pymfc.TUNING_SPACE}
         b = pymfc.Batch(cfgs)
+        b.set_tuning(**best)
         N = sum(inst.N for inst in b.instances)
-        b.step(10)
-        tuning = tune(pymfc, b, max(tune_steps, 2) if tune_steps > 0 else 0)
-        best = {k: tuning[k] for k in ("split_interact", "compact_pairs", "stage_limit")}
-        b.step(4)
+        b.step(warm)
         b.sync()
         t0 = time.perf_counter()
         b.step(steps)
@@ -184,7 +205,7 @@ def batch_bench(pymfc, n: int, steps: int, tune_steps: int) -> dict:
         for c in cfgs:
             s = pymfc.Sim(c)
             s.set_tuning(**best)
-            s.step(10)
+            s.step(warm)
             s.sync()
             t0 = time.perf_counter()
             s.step(steps)
@@ -195,7 +216,47 @@ def batch_bench(pymfc, n: int, steps: int, tune_steps: int) -> dict:
                      "particle_steps_per_s": N * steps / wall, "ms_per_step": wall / steps * 1e3,
                      "sequential_particle_steps_per_s": N * steps / seq, "batch_speedup": seq / wall,
                      "tuning": tuning}
+    out["adjoint"] = batch_adjoint_bench(pymfc, n)
     return out
+
+
+def batch_adjoint_bench(pymfc, n: int, nsteps: int = 100, K: int = 25) -> dict:
+    """C5's reverse half: the taped forward and the backward pass of the n instances as one batch
+    (one launch per phase over all instances' tiles), against the same instances one after
+    another; promoted FP64 with a tight secant, a window over all nsteps steps."""
+    cfgs = []
+    for c in pymfc.batch_configs(n):
+        c.promote_fp64 = 1
+        c.secant_tol = 1e-14
+        cfgs.append(c)
+    cot = np.tile([1.0, 0.0], (n, 1))
+    b = pymfc.Batch(cfgs)
+    N = sum(inst.N for inst in b.instances)
+    b.step(4)
+    b.sync()
+    t0 = time.perf_counter()
+    b.vjp_forward(nsteps, K, 0, nsteps)
+    t_tape = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    G = b.vjp_backward(cot)
+    t_bwd = time.perf_counter() - t0
+    b.close()
+    seq_tape = seq_bwd = 0.0
+    for c in cfgs:
+        s = pymfc.Sim(c)
+        s.step(4)
+        s.sync()
+        t0 = time.perf_counter()
+        s.vjp_forward(nsteps, K, 0, nsteps)
+        seq_tape += time.perf_counter() - t0
+        t0 = time.perf_counter()
+        s.vjp_backward(cot[0])
+        seq_bwd += time.perf_counter() - t0
+        s.close()
+    return {"particles": N, "steps": nsteps, "checkpoint_every": K, "tape_ms_per_step": t_tape / nsteps * 1e3,
+            "backward_ms_per_step": t_bwd / nsteps * 1e3, "sequential_tape_ms_per_step": seq_tape / nsteps * 1e3,
+            "sequential_backward_ms_per_step": seq_bwd / nsteps * 1e3, "backward_speedup": seq_bwd / t_bwd,
+            "grad_Fx_instance0": G[0].tolist()}
 
 
 COPY_PROBE_BYTES = 512 << 20

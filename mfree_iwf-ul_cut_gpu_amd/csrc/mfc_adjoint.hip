@@ -7,20 +7,25 @@
 // checkpoint segments from last to first; each segment is replayed forward with the step
 // kernels to recover the start state of each of its steps, and the steps are then reversed
 // one by one:
-//   F1  binning          launch_binning (cell ids, stable counting sort)
-//   F2  predict          launch_predict with the sorted copy of the start state (gather,
-//                        predictor, EOS, artificial stress, records)
-//   F3  launch_sums      neighbour sums of every particle (the sums kernel of the step)
-//   B1  k_adj_tail       VJP of everything after the sums, per particle (tail_vjp), and the
-//                        per-tile theta partials of the step (k_adj_theta adds them up)
-//   B2  k_adj_pull       VJP of the neighbour sums with respect to the records: each particle
+//   F1  binning          cell ids and the stable counting sort of the restored state
+//   F2  predict          with the sorted copy of the start state (gather, predictor, EOS,
+//                        artificial stress, records)
+//   F3  sums             neighbour sums of every particle (the sums kernel of the step)
+//   B1  kb_adj_tail      VJP of everything after the sums, per particle (tail_vjp), and the
+//                        per-tile theta partials of the step (kb_adj_theta adds them up)
+//   B2  kb_adj_pull      VJP of the neighbour sums with respect to the records: each particle
 //                        pulls the contributions of the pairs it takes part in over the same
 //                        symmetric 5 x 5 cell window, in a fixed order (no atomics); two passes,
 //                        pairs of the neighbours' sums, then of its own sums
-//   B3  k_adj_record     VJP of the record and predictor, then the scatter back to the
+//   B3  kb_adj_record    VJP of the record and predictor, then the scatter back to the
 //                        storage order of the previous step (inverse of the sort permutation)
 // The positional contact normal (read by the friction heat of the next step at the same sorted
 // index) carries its adjoint from step to step without permutation, like its forward value.
+//
+// A group of instances that ran the same taped forward (one handle, or the instances of a
+// batch) is reversed together: every phase, the checkpoint copies included, is one launch over
+// the group's (instance, tile) work table, with per-instance descriptors in device memory, as
+// the batched forward step runs. Each instance computes exactly what it computes alone.
 //
 // Derivatives follow the smooth map of promote_fp64 = 1 with a tight secant tolerance; the
 // handle must use FP64 records.
@@ -39,45 +44,105 @@ namespace {
 
 using namespace adj;
 
+}  // namespace
+
+// a checkpoint of one instance: state block, positional block, cell ids, tool
+struct snapshot {
+  char *state;            // state block (state_block_bytes)
+  char *pos;              // positional block
+  int *keys;              // cell ids of the previous step (kept by blanked particles)
+  mfc_tool_state *tool;
+};
+
+namespace {
+
+// one instance of the backward pass as its kernels see it (device memory, one per instance)
+struct adj_args {
+  dev_consts k;
+  int ntiles;
+  const double *st;                 // sorted start state of the reversed step (TS_* order, stride N)
+  const int *flags;                 // its flags
+  dpositional pos;                  // replay positional block
+  const double *rec;                // FP64 neighbour records
+  const int *key_sorted, *offsets, *perm;
+  const mfc_tool_state *tool_now;   // the tool after the reversed step's move
+  double *acc, *lam_acc, *lam_s[2], *lam_n, *g_s, *g_rec, *theta_part, *theta_acc;
+  double Fb[2];                     // cotangent over the window length
+  // checkpoint copies: the live replay buffers and the instance's snapshots
+  size_t state_words, pos_words;    // 8-byte words of a state block and of a positional block
+  uint64_t *live_state[2];
+  uint64_t *live_pos;
+  int *live_keys;                   // the handle's sorted cell ids
+  mfc_tool_state *live_tool;        // 2 replay tool states
+  const snapshot *ckpt, *seg;       // checkpoints of the tape, and the start states of a segment
+};
+
+// ------------------------------------------------------------------ checkpoint copies
+// dir 0: a snapshot (ckpt[idx] or seg[idx]) into replay set 0, the positional block, the cell
+// ids and replay tool 0; dir 1: replay set wa, the positional block, the cell ids and replay
+// tool wa into seg[idx]. Grid (copy_blocks, instances).
+__global__ __launch_bounds__(256) void kb_adj_copy(const adj_args *__restrict__ D, int dir, int from_ckpt, int idx,
+                                                   int wa) {
+  const adj_args &d = D[blockIdx.y];
+  const snapshot sn = from_ckpt ? d.ckpt[idx] : d.seg[idx];
+  const int lw = dir ? wa : 0;
+  uint64_t *ls = d.live_state[lw];
+  const uint64_t *src_s = dir ? ls : (const uint64_t *)sn.state;
+  uint64_t *dst_s = dir ? (uint64_t *)sn.state : ls;
+  const uint64_t *src_p = dir ? d.live_pos : (const uint64_t *)sn.pos;
+  uint64_t *dst_p = dir ? (uint64_t *)sn.pos : d.live_pos;
+  const int *src_k = dir ? d.live_keys : sn.keys;
+  int *dst_k = dir ? sn.keys : d.live_keys;
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  const size_t i0 = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  for (size_t q = i0; q < d.state_words; q += stride) dst_s[q] = src_s[q];
+  for (size_t q = i0; q < d.pos_words; q += stride) dst_p[q] = src_p[q];
+  for (size_t q = i0; q < (size_t)d.k.N; q += stride) dst_k[q] = src_k[q];
+  if (i0 == 0) {
+    if (dir) *sn.tool = d.live_tool[lw];
+    else d.live_tool[0] = *sn.tool;
+  }
+}
+
 // ------------------------------------------------------------------ B1: per-particle tail
-// st: sorted start state (DSTATE_DOUBLES arrays of stride N, TS_* order); lam_s: adjoint of the
-// end state of the step (same layout); lam_n: adjoint of the positional normal written by this
-// step (in) and of the one read by it (out)
-__global__ __launch_bounds__(TILE) void k_adj_tail(dev_consts k, int ntiles, const double *__restrict__ st,
-                                                   const int *__restrict__ flags, dpositional pos,
-                                                   const double *__restrict__ acc, const double *__restrict__ lam_s,
-                                                   double *__restrict__ lam_n, double Fbx, double Fby,
-                                                   const mfc_tool_state *__restrict__ tool_now,
-                                                   double *__restrict__ g_s, double *__restrict__ lam_acc,
-                                                   double *__restrict__ theta_part) {
-  const int t = xcd_tile(blockIdx.x, ntiles);
-  if (t < 0) return;
+// st: sorted start state; lam_s[cur]: adjoint of the end state of the step (same layout);
+// lam_n: adjoint of the positional normal written by this step (in) and of the one read by it
+// (out); in_win: the step's force is in the loss window
+__global__ __launch_bounds__(TILE) void kb_adj_tail(const adj_args *__restrict__ D, const int2 *__restrict__ work,
+                                                    int nwork, int cur, int in_win) {
+  const int w = xcd_tile(blockIdx.x, nwork);
+  if (w < 0) return;
+  const int2 it = work[w];
+  const adj_args &d = D[it.x];
+  const dev_consts &k = d.k;
+  const int t = it.y;
   const int i = t * TILE + threadIdx.x;
   const size_t N = k.N;
   double th[6] = {0., 0., 0., 0., 0., 0.};
   if (i < k.N) {
     tail_in in;
 #pragma unroll
-    for (int q = 0; q < TS_N; q++) in.s[q] = st[q * N + i];
-    in.p = pos.p[i];
-    in.nx = pos.nx[i];
-    in.ny = pos.ny[i];
-    in.flags = flags[i];
+    for (int q = 0; q < TS_N; q++) in.s[q] = d.st[q * N + i];
+    in.p = d.pos.p[i];
+    in.nx = d.pos.nx[i];
+    in.ny = d.pos.ny[i];
+    in.flags = d.flags[i];
     in.pse_flag_zero = k.pse_quirk && i >= k.N / 2;
 #pragma unroll
-    for (int q = 0; q < L_N; q++) in.acc[q] = acc[q * N + i];
+    for (int q = 0; q < L_N; q++) in.acc[q] = d.acc[q * N + i];
+    const double *lam_s = d.lam_s[cur];
     double ob[TS_N];
 #pragma unroll
     for (int q = 0; q < TS_N; q++) ob[q] = lam_s[q * N + i];
-    const double nbx = lam_n[i], nby = lam_n[N + i];
+    const double nbx = d.lam_n[i], nby = d.lam_n[N + i];
     tail_grad g;
-    tail_vjp(k, *tool_now, in, ob, nbx, nby, Fbx, Fby, g, nullptr);
+    tail_vjp(k, *d.tool_now, in, ob, nbx, nby, in_win ? d.Fb[0] : 0., in_win ? d.Fb[1] : 0., g, nullptr);
 #pragma unroll
-    for (int q = 0; q < TS_N; q++) g_s[q * N + i] = g.s[q];
+    for (int q = 0; q < TS_N; q++) d.g_s[q * N + i] = g.s[q];
 #pragma unroll
-    for (int q = 0; q < L_N; q++) lam_acc[q * N + i] = g.acc[q];
-    lam_n[i] = g.nx;
-    lam_n[N + i] = g.ny;
+    for (int q = 0; q < L_N; q++) d.lam_acc[q * N + i] = g.acc[q];
+    d.lam_n[i] = g.nx;
+    d.lam_n[N + i] = g.ny;
 #pragma unroll
     for (int q = 0; q < 6; q++) th[q] = g.theta[q];
   }
@@ -94,19 +159,19 @@ __global__ __launch_bounds__(TILE) void k_adj_tail(dev_consts k, int ntiles, con
   }
   if (threadIdx.x == 0) {
 #pragma unroll
-    for (int q = 0; q < 6; q++) theta_part[6 * t + q] = red[q][0];
+    for (int q = 0; q < 6; q++) d.theta_part[6 * t + q] = red[q][0];
   }
 }
 
-// adds the step's theta partials to the running totals: lane l sums the tiles l, l + TILE, ...
-// in ascending order, lane sums reduced by halving strides
-__global__ __launch_bounds__(TILE) void k_adj_theta(int ntiles, const double *__restrict__ theta_part,
-                                                    double *__restrict__ theta_acc) {
+// adds the step's theta partials of instance blockIdx.x to its running totals: lane l sums the
+// tiles l, l + TILE, ... in ascending order, lane sums reduced by halving strides
+__global__ __launch_bounds__(TILE) void kb_adj_theta(const adj_args *__restrict__ D) {
+  const adj_args &d = D[blockIdx.x];
   __shared__ double red[6][TILE];
   double a[6] = {0., 0., 0., 0., 0., 0.};
-  for (int u = threadIdx.x; u < ntiles; u += TILE)
+  for (int u = threadIdx.x; u < d.ntiles; u += TILE)
 #pragma unroll
-    for (int q = 0; q < 6; q++) a[q] = a[q] + theta_part[6 * u + q];
+    for (int q = 0; q < 6; q++) a[q] = a[q] + d.theta_part[6 * u + q];
 #pragma unroll
   for (int q = 0; q < 6; q++) red[q][threadIdx.x] = a[q];
   for (int s = TILE / 2; s > 0; s >>= 1) {
@@ -116,7 +181,7 @@ __global__ __launch_bounds__(TILE) void k_adj_theta(int ntiles, const double *__
       for (int q = 0; q < 6; q++) red[q][threadIdx.x] = red[q][threadIdx.x] + red[q][threadIdx.x + s];
     }
   }
-  if (threadIdx.x < 6) theta_acc[threadIdx.x] = theta_acc[threadIdx.x] + red[threadIdx.x][0];
+  if (threadIdx.x < 6) d.theta_acc[threadIdx.x] = d.theta_acc[threadIdx.x] + red[threadIdx.x][0];
 }
 
 // ------------------------------------------------------------------ B2: pull over pairs
@@ -124,15 +189,21 @@ __global__ __launch_bounds__(TILE) void k_adj_theta(int ntiles, const double *__
 // adjoint of a's sums; g_rec[b] is written. OWN = true: the pairs (b, a) of b's own sums, with
 // b's adjoint; added to g_rec[b]. Two passes keep fewer records live per lane than one.
 template <bool OWN>
-__global__ __launch_bounds__(TILE) void k_adj_pull(dev_consts k, int ntiles, const double *__restrict__ rec,
-                                                   const int *__restrict__ key_sorted,
-                                                   const int *__restrict__ offsets, const int *__restrict__ flags,
-                                                   const double *__restrict__ lam_acc, double *__restrict__ g_rec) {
-  const int t = xcd_tile(blockIdx.x, ntiles);
-  if (t < 0) return;
-  const int b = t * TILE + threadIdx.x;
+__global__ __launch_bounds__(TILE) void kb_adj_pull(const adj_args *__restrict__ D, const int2 *__restrict__ work,
+                                                    int nwork) {
+  const int w = xcd_tile(blockIdx.x, nwork);
+  if (w < 0) return;
+  const int2 it = work[w];
+  const adj_args &d = D[it.x];
+  const dev_consts &k = d.k;
+  const int b = it.y * TILE + threadIdx.x;
   if (b >= k.N) return;
   const size_t N = k.N;
+  const double *__restrict__ rec = d.rec;
+  const int *__restrict__ flags = d.flags;
+  const int *__restrict__ offsets = d.offsets;
+  const double *__restrict__ lam_acc = d.lam_acc;
+  double *__restrict__ g_rec = d.g_rec;
   const bool pse = k.pse != 0;
   double rb[REC_DOUBLES], gb[REC_DOUBLES], lb[L_N], dummy[REC_DOUBLES];
 #pragma unroll
@@ -149,7 +220,7 @@ __global__ __launch_bounds__(TILE) void k_adj_pull(dev_consts k, int ntiles, con
   const double ta_b = pse ? 0. : (tool_b ? k.alpha_tool : k.alpha_wp);
   const bool mech_b = !blank_b && (!pse || !tool_b);
   const bool pse_b = !blank_b && pse && k.thermal_wp;
-  const int hsh = key_sorted[b];
+  const int hsh = d.key_sorted[b];
   const int gi = hsh / k.gny, gj = hsh - gi * k.gny;
   const int li = gi - 2 < 0 ? 0 : gi - 2;
   const int lj = gj - 2 < 0 ? 0 : gj - 2;
@@ -195,23 +266,24 @@ __global__ __launch_bounds__(TILE) void k_adj_pull(dev_consts k, int ntiles, con
 }
 
 // ------------------------------------------------------------------ B3: record VJP + unsort
-__global__ __launch_bounds__(TILE) void k_adj_record(dev_consts k, int ntiles, const double *__restrict__ st,
-                                                     const int *__restrict__ flags, const double *__restrict__ p,
-                                                     const double *__restrict__ g_rec,
-                                                     const double *__restrict__ g_s, const int *__restrict__ perm,
-                                                     double *__restrict__ lam_prev) {
-  const int t = xcd_tile(blockIdx.x, ntiles);
-  if (t < 0) return;
-  const int a = t * TILE + threadIdx.x;
+__global__ __launch_bounds__(TILE) void kb_adj_record(const adj_args *__restrict__ D, const int2 *__restrict__ work,
+                                                      int nwork, int cur) {
+  const int w = xcd_tile(blockIdx.x, nwork);
+  if (w < 0) return;
+  const int2 it = work[w];
+  const adj_args &d = D[it.x];
+  const dev_consts &k = d.k;
+  const int a = it.y * TILE + threadIdx.x;
   if (a >= k.N) return;
   const size_t N = k.N;
   double s[TS_N], gs[TS_N], br[REC_DOUBLES];
 #pragma unroll
-  for (int q = 0; q < TS_N; q++) { s[q] = st[q * N + a]; gs[q] = g_s[q * N + a]; }
+  for (int q = 0; q < TS_N; q++) { s[q] = d.st[q * N + a]; gs[q] = d.g_s[q * N + a]; }
 #pragma unroll
-  for (int q = 0; q < REC_DOUBLES; q++) br[q] = g_rec[q * N + a];
-  record_vjp(k, s, flags[a], p[a], br, gs);
-  const int dst = perm[a];
+  for (int q = 0; q < REC_DOUBLES; q++) br[q] = d.g_rec[q * N + a];
+  record_vjp(k, s, d.flags[a], d.pos.p[a], br, gs);
+  const int dst = d.perm[a];
+  double *lam_prev = d.lam_s[1 - cur];
 #pragma unroll
   for (int q = 0; q < TS_N; q++) lam_prev[q * N + dst] = gs[q];
 }
@@ -221,13 +293,6 @@ int padded(int ntiles) { return ((ntiles + 7) / 8) * 8; }
 }  // namespace
 
 // ------------------------------------------------------------------ workspace
-struct snapshot {
-  char *state;            // state block (state_block_bytes)
-  char *pos;              // positional block
-  int *keys;              // cell ids of the previous step (kept by blanked particles)
-  mfc_tool_state *tool;
-};
-
 struct adj_work {
   int N = 0, nsteps = 0, K = 1, w0 = 0, w1 = 0;
   bool valid = false;
@@ -377,6 +442,170 @@ int adj_finish(mfc_sim *s, double Fmean[2]) {
   return MFC_OK;
 }
 
+int adj_backward(mfc_sim *const *sims, int n, const batch_work &fw, hipStream_t st, const double *cot, double *grad) {
+  if (n < 1) return fail(MFC_E_INVALID, "empty group");
+  const adj_work *w0 = sims[0]->adj;
+  for (int i = 0; i < n; i++) {
+    const adj_work *w = sims[i]->adj;
+    if (!w || !w->valid) return fail(MFC_E_INVALID, "no recorded run: call mfc_vjp_forward first");
+    if (w->nsteps != w0->nsteps || w->K != w0->K || w->w0 != w0->w0 || w->w1 != w0->w1)
+      return fail(MFC_E_INVALID, "the instances recorded different runs");
+  }
+  const int nsteps = w0->nsteps, K = w0->K;
+  const int nck = (nsteps + K - 1) / K;
+  // descriptors: replay steps of start parity 0 and 1 (inst_args, the step kernels' view, with
+  // the replay buffers as state and no binning of the next step), the backward kernels' view
+  // (adj_args), and each instance's checkpoints and segment snapshots
+  std::vector<inst_args> rep(2 * (size_t)n);
+  std::vector<adj_args> da(n);
+  std::vector<snapshot> snaps((size_t)n * (nck + K));
+  for (int i = 0; i < n; i++) {
+    mfc_sim *s = sims[i];
+    adj_work *w = s->adj;
+    const int N = s->k.N;
+    dstate A, B;
+    dpositional P;
+    carve_state(w->work[0], N, A);
+    carve_state(w->work[1], N, B);
+    carve_positional(w->posw, N, P);
+    for (int p = 0; p < 2; p++) {
+      inst_args &a = rep[(size_t)p * n + i];
+      std::memset(&a, 0, sizeof(a));
+      a.k = s->k;
+      a.cur = p ? B : A;
+      a.nxt = p ? A : B;
+      a.b = s->bins;
+      a.pos = P;
+      a.r = s->recs;
+      a.tool_in = w->toolw + p;
+      a.tool_out = w->toolw + 1 - p;
+      a.f = {w->partials, w->force_last, w->hist, 0, w->ctr, w->pending};  // no history, counted
+      a.acc = w->acc;
+      a.ntiles = s->ntiles;
+      a.nscan = scan_blocks(s->k.ncell_table);
+      a.bin_next = 0;
+    }
+    adj_args &d = da[i];
+    std::memset(&d, 0, sizeof(d));
+    d.k = s->k;
+    d.ntiles = s->ntiles;
+    d.st = B.px;  // the sorted start state, TS_* order with stride N
+    d.flags = B.flags;
+    d.pos = P;
+    d.rec = s->recs.rec;
+    d.key_sorted = s->bins.key_sorted;
+    d.offsets = s->bins.offsets;
+    d.perm = s->bins.perm;
+    d.tool_now = w->toolw + 1;
+    d.acc = w->acc;
+    d.lam_acc = w->lam_acc;
+    d.lam_s[0] = w->lam_s[0];
+    d.lam_s[1] = w->lam_s[1];
+    d.lam_n = w->lam_n;
+    d.g_s = w->g_s;
+    d.g_rec = w->g_rec;
+    d.theta_part = w->theta_part;
+    d.theta_acc = w->theta_acc;
+    d.Fb[0] = cot[2 * i] / (w->w1 - w->w0);
+    d.Fb[1] = cot[2 * i + 1] / (w->w1 - w->w0);
+    d.state_words = state_block_bytes(N) / sizeof(uint64_t);
+    d.pos_words = positional_block_bytes(N) / sizeof(uint64_t);
+    d.live_state[0] = (uint64_t *)w->work[0];
+    d.live_state[1] = (uint64_t *)w->work[1];
+    d.live_pos = (uint64_t *)w->posw;
+    d.live_keys = s->bins.key_sorted;
+    d.live_tool = w->toolw;
+    for (int c = 0; c < nck; c++) snaps[(size_t)i * (nck + K) + c] = w->ckpt[c];
+    for (int c = 0; c < K; c++) snaps[(size_t)i * (nck + K) + nck + c] = w->seg[c];
+  }
+  auto up = [](size_t b) { return (b + 255) & ~(size_t)255; };
+  const size_t b_rep = up(sizeof(inst_args) * rep.size()), b_da = up(sizeof(adj_args) * da.size());
+  const size_t b_sn = up(sizeof(snapshot) * snaps.size());
+  char *dmem = nullptr;
+  HIP_TRY(hipMalloc(&dmem, b_rep + b_da + b_sn));
+  inst_args *d_rep = (inst_args *)dmem;
+  adj_args *d_da = (adj_args *)(dmem + b_rep);
+  snapshot *d_sn = (snapshot *)(dmem + b_rep + b_da);
+  for (int i = 0; i < n; i++) {
+    da[i].ckpt = d_sn + (size_t)i * (nck + K);
+    da[i].seg = da[i].ckpt + nck;
+  }
+  // copy grid: about 8 words of the largest state block per thread
+  int maxN = 0;
+  for (int i = 0; i < n; i++) maxN = sims[i]->k.N > maxN ? sims[i]->k.N : maxN;
+  const size_t want = (state_block_bytes(maxN) / sizeof(uint64_t) + 2047) / 2048;
+  const int copy_blocks = want < 16 ? 16 : (want > 1024 ? 1024 : (int)want);
+  int rc = MFC_OK;
+  auto run = [&]() -> int {
+    HIP_TRY(hipMemcpyAsync(d_rep, rep.data(), sizeof(inst_args) * rep.size(), hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(d_da, da.data(), sizeof(adj_args) * da.size(), hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(d_sn, snaps.data(), sizeof(snapshot) * snaps.size(), hipMemcpyHostToDevice, st));
+    for (int i = 0; i < n; i++) {
+      mfc_sim *s = sims[i];
+      adj_work *w = s->adj;
+      const size_t N = (size_t)s->k.N;
+      HIP_TRY(hipMemsetAsync(w->lam_s[0], 0, sizeof(double) * TS_N * N, st));
+      HIP_TRY(hipMemsetAsync(w->lam_n, 0, sizeof(double) * 2 * N, st));
+      HIP_TRY(hipMemsetAsync(w->theta_acc, 0, sizeof(double) * 6, st));
+      HIP_TRY(hipMemcpyAsync(w->keys_keep, s->bins.key_sorted, sizeof(int) * N, hipMemcpyDeviceToDevice, st));
+      // every binning of the pass counts into a cleared table and leaves it cleared; the
+      // handle's table holds the next step's counts: clear it, and bin the handle's state again
+      // at its next step
+      HIP_TRY(hipMemsetAsync(s->bins.count, 0, sizeof(int) * (size_t)s->k.ncell_table, st));
+      HIP_TRY(hipMemsetAsync(s->bins.bsum, 0, sizeof(int) * (size_t)scan_blocks(s->k.ncell_table), st));
+      s->keys_fresh = false;
+    }
+    const batch_work rw[2] = {{d_rep, n, fw.tiles, fw.ntiles, fw.chunks, fw.nchunks, false, fw.v, false, false},
+                              {d_rep + n, n, fw.tiles, fw.ntiles, fw.chunks, fw.nchunks, false, fw.v, false, false}};
+    const dim3 gcopy(copy_blocks, n), gt(padded(fw.ntiles));
+    int cur = 0;  // lam_s[cur] holds the adjoint of the end state of the step being reversed
+    const int nseg = nck;
+    for (int sg = nseg - 1; sg >= 0; sg--) {
+      const int t0 = sg * K, t1 = (t0 + K < nsteps) ? t0 + K : nsteps;
+      // replay the segment, keeping the start of each of its steps
+      int wa = 0;
+      hipLaunchKernelGGL(kb_adj_copy, gcopy, dim3(256), 0, st, d_da, 0, 1, sg, 0);
+      for (int t = t0; t < t1; t++) {
+        hipLaunchKernelGGL(kb_adj_copy, gcopy, dim3(256), 0, st, d_da, 1, 0, t - t0, wa);
+        if (t + 1 == t1) break;
+        launch_batch_hash(rw[wa], st);
+        launch_batch_step(rw[wa], st);
+        wa = 1 - wa;
+      }
+      // reverse the steps of the segment
+      for (int t = t1 - 1; t >= t0; t--) {
+        hipLaunchKernelGGL(kb_adj_copy, gcopy, dim3(256), 0, st, d_da, 0, 0, t - t0, 0);
+        launch_batch_hash(rw[0], st);
+        launch_batch_binning(rw[0], st);
+        launch_batch_predict_gather(rw[0], st);
+        launch_batch_sums(rw[0], st);
+        const int in_win = t >= w0->w0 && t < w0->w1;
+        hipLaunchKernelGGL(kb_adj_tail, gt, dim3(TILE), 0, st, d_da, fw.tiles, fw.ntiles, cur, in_win);
+        hipLaunchKernelGGL(kb_adj_theta, dim3(n), dim3(TILE), 0, st, d_da);
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(kb_adj_pull<false>), gt, dim3(TILE), 0, st, d_da, fw.tiles, fw.ntiles);
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(kb_adj_pull<true>), gt, dim3(TILE), 0, st, d_da, fw.tiles, fw.ntiles);
+        hipLaunchKernelGGL(kb_adj_record, gt, dim3(TILE), 0, st, d_da, fw.tiles, fw.ntiles, cur);
+        cur = 1 - cur;
+      }
+    }
+    HIP_TRY(hipGetLastError());
+    for (int i = 0; i < n; i++) {
+      mfc_sim *s = sims[i];
+      adj_work *w = s->adj;
+      // the replay overwrote the handle's cell ids, which blanked particles keep into the next step
+      HIP_TRY(hipMemcpyAsync(s->bins.key_sorted, w->keys_keep, sizeof(int) * (size_t)s->k.N, hipMemcpyDeviceToDevice,
+                             st));
+      HIP_TRY(hipMemcpyAsync(grad + 6 * i, w->theta_acc, sizeof(double) * 6, hipMemcpyDeviceToHost, st));
+    }
+    HIP_TRY(hipStreamSynchronize(st));
+    return MFC_OK;
+  };
+  rc = run();
+  if (rc) (void)hipStreamSynchronize(st);  // nothing queued may still read the descriptors
+  (void)hipFree(dmem);
+  return rc;
+}
+
 }  // namespace mfc
 
 using namespace mfc;
@@ -401,76 +630,8 @@ int mfc_vjp_forward(mfc_sim *s, int nsteps, int checkpoint_every, int w0, int w1
 
 int mfc_vjp_backward(mfc_sim *s, const double cot[2], double grad[6]) {
   if (!s || !cot || !grad) return fail(MFC_E_INVALID, "null argument");
-  adj_work *w = s->adj;
-  if (!w || !w->valid) return fail(MFC_E_INVALID, "no recorded run: call mfc_vjp_forward first");
-  const int N = s->k.N, nt = s->ntiles, K = w->K;
-  const dev_consts &k = s->k;
-  hipStream_t st = s->stream;
-  const size_t n = (size_t)N;
-  HIP_TRY(hipMemsetAsync(w->lam_s[0], 0, sizeof(double) * TS_N * n, st));
-  HIP_TRY(hipMemsetAsync(w->lam_n, 0, sizeof(double) * 2 * n, st));
-  HIP_TRY(hipMemsetAsync(w->theta_acc, 0, sizeof(double) * 6, st));
-  HIP_TRY(hipMemcpyAsync(w->keys_keep, s->bins.key_sorted, sizeof(int) * n, hipMemcpyDeviceToDevice, st));
-  // the replays bin with the handle's cell table, which holds the next step's counts: clear them
-  // (each replayed binning leaves them cleared), and bin the handle's state again at its next step
-  HIP_TRY(hipMemsetAsync(s->bins.count, 0, sizeof(int) * (size_t)k.ncell_table, st));
-  s->keys_fresh = false;
-  int cur = 0;  // lam_s[cur] holds the adjoint of the end state of the step being reversed
-  const double Fb[2] = {cot[0] / (w->w1 - w->w0), cot[1] / (w->w1 - w->w0)};
-  // the step kernels run on the replay buffers; cell table and records are the handle's
-  // (recomputed every step, so the handle's next step is unaffected)
-  dstate A, B;
-  dpositional P;
-  carve_positional(w->posw, N, P);
-  dforce fo = {w->partials, w->force_last, w->hist, 0, w->ctr, w->pending};
-  const int nseg = (w->nsteps + K - 1) / K;
-  int rc;
-  for (int sg = nseg - 1; sg >= 0; sg--) {
-    const int t0 = sg * K, t1 = (t0 + K < w->nsteps) ? t0 + K : w->nsteps;
-    // replay the segment, keeping the start of each of its steps
-    int wa = 0;
-    if ((rc = restore(st, w->ckpt[sg], w->work[wa], w->posw, s->bins.key_sorted, w->toolw, N))) return rc;
-    for (int t = t0; t < t1; t++) {
-      if ((rc = save(st, w->seg[t - t0], w->work[wa], w->posw, s->bins.key_sorted, w->toolw, N))) return rc;
-      if (t + 1 == t1) break;
-      carve_state(w->work[wa], N, A);
-      carve_state(w->work[1 - wa], N, B);
-      launch_binning(k, A, s->bins, st);
-      launch_predict(k, A, B, s->bins, P, s->recs, w->toolw, w->toolw + 1, false, st);
-      launch_interact(k, A, B, s->bins, P, s->recs, w->toolw + 1, fo, s->acc, s->variant(), st);
-      HIP_TRY(hipMemcpyAsync(w->toolw, w->toolw + 1, sizeof(mfc_tool_state), hipMemcpyDeviceToDevice, st));
-      wa = 1 - wa;
-    }
-    // reverse the steps of the segment
-    for (int t = t1 - 1; t >= t0; t--) {
-      if ((rc = restore(st, w->seg[t - t0], w->work[0], w->posw, s->bins.key_sorted, w->toolw, N))) return rc;
-      carve_state(w->work[0], N, A);
-      carve_state(w->work[1], N, B);
-      launch_binning(k, A, s->bins, st);
-      launch_predict(k, A, B, s->bins, P, s->recs, w->toolw, w->toolw + 1, true, st);
-      const double *rec = s->recs.rec;
-      const double *sb = B.px;  // sorted start state, TS_* order with stride N
-      const bool in_win = t >= w->w0 && t < w->w1;
-      launch_sums(k, A.flags, s->bins, s->recs, w->acc, s->variant(), st);
-      hipLaunchKernelGGL(k_adj_tail, dim3(padded(nt)), dim3(TILE), 0, st, k, nt, sb, B.flags, P, w->acc,
-                         w->lam_s[cur], w->lam_n, in_win ? Fb[0] : 0., in_win ? Fb[1] : 0., w->toolw + 1, w->g_s,
-                         w->lam_acc, w->theta_part);
-      hipLaunchKernelGGL(k_adj_theta, dim3(1), dim3(TILE), 0, st, nt, w->theta_part, w->theta_acc);
-      hipLaunchKernelGGL(k_adj_pull<false>, dim3(padded(nt)), dim3(TILE), 0, st, k, nt, rec, s->bins.key_sorted,
-                         s->bins.offsets, B.flags, w->lam_acc, w->g_rec);
-      hipLaunchKernelGGL(k_adj_pull<true>, dim3(padded(nt)), dim3(TILE), 0, st, k, nt, rec, s->bins.key_sorted,
-                         s->bins.offsets, B.flags, w->lam_acc, w->g_rec);
-      hipLaunchKernelGGL(k_adj_record, dim3(padded(nt)), dim3(TILE), 0, st, k, nt, sb, B.flags, P.p, w->g_rec,
-                         w->g_s, s->bins.perm, w->lam_s[1 - cur]);
-      cur = 1 - cur;
-    }
-  }
-  HIP_TRY(hipGetLastError());
-  // the replay overwrote the handle's cell ids, which blanked particles keep into the next step
-  HIP_TRY(hipMemcpyAsync(s->bins.key_sorted, w->keys_keep, sizeof(int) * n, hipMemcpyDeviceToDevice, st));
-  HIP_TRY(hipMemcpyAsync(grad, w->theta_acc, sizeof(double) * 6, hipMemcpyDeviceToHost, st));
-  HIP_TRY(hipStreamSynchronize(st));
-  return MFC_OK;
+  if (s->batched) return fail(MFC_E_INVALID, "instance of a batch: use mfc_batch_vjp_backward");
+  return adj_backward(&s, 1, s->work(s->cur), s->stream, cot, grad);
 }
 
 int mfc_vjp(mfc_sim *s, mfc_vjp_req *req, double grad[6]) {

@@ -244,11 +244,8 @@ int mfc_batch_vjp_forward(mfc_batch *b, int nsteps, int checkpoint_every, int w0
 
 int mfc_batch_vjp_backward(mfc_batch *b, const double *cot, double *grad) {
   if (!b || !cot || !grad) return fail(MFC_E_INVALID, "null argument");
-  for (size_t i = 0; i < b->sims.size(); i++) {
-    int rc = mfc_vjp_backward(b->sims[i], cot + 2 * i, grad + 6 * i);
-    if (rc) return rc;
-  }
-  return MFC_OK;
+  if (b->f32) return fail(MFC_E_INVALID, "the adjoint needs precision FP64");
+  return mfc::adj_backward(b->sims.data(), (int)b->sims.size(), b->work(b->cur), b->stream, cot, grad);
 }
 
 }  // extern "C"

@@ -123,6 +123,9 @@ void launch_batch_step(const batch_work &w, hipStream_t s);
 void launch_batch_hash(const batch_work &w, hipStream_t s);
 void launch_batch_binning(const batch_work &w, hipStream_t s);
 void launch_batch_predict(const batch_work &w, hipStream_t s);
+// predict that also writes the sorted copy of the start state into nxt (FP64 records; the
+// reverse mode reads it)
+void launch_batch_predict_gather(const batch_work &w, hipStream_t s);
 // with events t0 and t1 the launch records the kernel's own start and end in them
 // (hipExtLaunchKernelGGL), which times the kernel without the gap before it
 void launch_batch_interact_fused(const batch_work &w, hipStream_t s, hipEvent_t t0 = nullptr,

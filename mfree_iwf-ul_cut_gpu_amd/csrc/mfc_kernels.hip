@@ -1134,7 +1134,7 @@ __global__ __launch_bounds__(256) void kb_place(const inst_args *__restrict__ A,
   for (int q = it.y * 256 + threadIdx.x; q < a.nscan; q += a.ntiles * 256) a.b.bsum[q] = 0;
 }
 
-template <bool F32>
+template <bool F32, bool GATHER>
 __global__ __launch_bounds__(TILE) void kb_predict(const inst_args *__restrict__ A, const int2 *__restrict__ work,
                                                    int nwork) {
   const int w = xcd_tile(blockIdx.x, nwork);
@@ -1145,8 +1145,8 @@ __global__ __launch_bounds__(TILE) void kb_predict(const inst_args *__restrict__
   // them would take more scalar registers than there are)
   const inst_args &a = A[it.x];
   MFC_CONST(inst_args) *c = const_args(A, it.x);
-  predict<F32, false>(a.k, it.y, load_bins(&c->b), load_state(&c->cur), a.nxt, a.pos, load_recs(&c->r), a.tool_in,
-                      a.tool_out);
+  predict<F32, GATHER>(a.k, it.y, load_bins(&c->b), load_state(&c->cur), a.nxt, a.pos, load_recs(&c->r), a.tool_in,
+                       a.tool_out);
   // the tool force of the previous step, if its partials are still pending (the first tile's
   // work-group is dispatched first, so the longer kernel hides the reduction)
   if (it.y == 0) force_final(a.ntiles, load_force(&c->f));
@@ -1341,9 +1341,14 @@ void launch_batch_binning(const batch_work &w, hipStream_t s) {
 
 void launch_batch_predict(const batch_work &w, hipStream_t s) {
   const dim3 gt(padded_tiles(w.ntiles));
-  if (w.f32) hipLaunchKernelGGL(kb_predict<true>, gt, dim3(TILE), 0, s, w.args, w.tiles, w.ntiles);
-  else hipLaunchKernelGGL(kb_predict<false>, gt, dim3(TILE), 0, s, w.args, w.tiles, w.ntiles);
+  if (w.f32) hipLaunchKernelGGL(HIP_KERNEL_NAME(kb_predict<true, false>), gt, dim3(TILE), 0, s, w.args, w.tiles, w.ntiles);
+  else hipLaunchKernelGGL(HIP_KERNEL_NAME(kb_predict<false, false>), gt, dim3(TILE), 0, s, w.args, w.tiles, w.ntiles);
   if (w.rothlin) hipLaunchKernelGGL(kb_rothlin_cells, gt, dim3(256), 0, s, w.args, w.tiles, w.ntiles);
+}
+
+void launch_batch_predict_gather(const batch_work &w, hipStream_t s) {
+  hipLaunchKernelGGL(HIP_KERNEL_NAME(kb_predict<false, true>), dim3(padded_tiles(w.ntiles)), dim3(TILE), 0, s, w.args,
+                     w.tiles, w.ntiles);
 }
 
 // a batched launch over the work table, with the kernel's own start and end recorded in t0 and

@@ -35,6 +35,11 @@ void adj_free(adj_work *w);
 int adj_begin(mfc_sim *s, int nsteps, int checkpoint_every, int w0, int w1);
 int adj_checkpoint(mfc_sim *s, int t);
 int adj_finish(mfc_sim *s, double Fmean[2]);
+// the backward pass of a group of instances that recorded the same run (steps, checkpoint
+// interval, window): one handle, or the instances of a batch, reversed together with one launch
+// per phase over the group's work tables (fw: tile and chunk tables, launch variant);
+// cot: 2 per instance, grad: 6 per instance
+int adj_backward(mfc_sim *const *sims, int n, const batch_work &fw, hipStream_t st, const double *cot, double *grad);
 }  // namespace mfc
 
 struct mfc_sim {
