@@ -718,7 +718,7 @@ MFC_HD void tail_vjp(const dev_consts &k, const mfc_tool_state &tool, const tail
 }
 
 // ------------------------------------------------------------------ record
-// the neighbour record of one particle from its sorted start-of-step state (k_predict, promoted);
+// the neighbour record of one particle from its sorted start-of-step state (predict, promoted);
 // p: in/out positional pressure (recomputed unless tool particle), R3: in/out artificial stress
 MFC_HD void record_fwd(const dev_consts &k, const double *s, int flags, double &p, double *R3, double *rec) {
   const bool blank = flags & MFC_FLAG_BLANKED, tool_p = flags & MFC_FLAG_TOOL;

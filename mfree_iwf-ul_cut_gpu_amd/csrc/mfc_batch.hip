@@ -3,7 +3,7 @@
 // A batch owns one stream and n instance handles created by mfc_create (each with its own
 // buffers and constants). One batched step is one launch per phase for all instances
 // (launch_batch_step): every block takes an (instance, tile) or (instance, scan chunk) item from
-// a work table and runs the single-instance device body on that instance's descriptor, so each
+// a work table and runs the per-tile device body on that instance's descriptor, so each
 // instance computes exactly the bits of its own single run while small instances share the
 // chip. Two consecutive batched steps are captured in a hipGraph per step parity.
 #include <hip/hip_runtime.h>

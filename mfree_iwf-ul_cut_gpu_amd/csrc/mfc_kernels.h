@@ -135,35 +135,10 @@ void launch_batch_update(const batch_work &w, hipStream_t s, hipEvent_t t0 = nul
 void launch_batch_force(const batch_work &w, hipStream_t s);
 int scan_chunk_cells();
 
-// step kernels. The start state cur is in the storage order of the previous step; predict reads
-// it through the sort permutation and writes the neighbour records (and, with gather, the
-// sorted copy of cur into nxt); the interaction reads cur through the permutation and writes the
-// updated state to nxt in sorted order.
-void launch_binning(const dev_consts &k, const dstate &cur, dbins &b, hipStream_t s);
-void launch_predict(const dev_consts &k, const dstate &cur, const dstate &nxt, const dbins &b, const dpositional &pos,
-                    const drecords &r, const mfc_tool_state *tool_in, mfc_tool_state *tool_out, bool gather,
-                    hipStream_t s);
-// neighbour sums + per-particle update + contact, then the tool-force reduction; v.split: the
-// sums go through acc (ACC_N arrays of N doubles) between a sums kernel and an update kernel
-// (fewer registers per kernel), else one fused kernel
-void launch_interact(const dev_consts &k, const dstate &cur, const dstate &nxt, const dbins &b,
-                     const dpositional &pos, const drecords &r, const mfc_tool_state *tool_now, const dforce &f,
-                     double *acc, variant v, hipStream_t s);
-// the pieces of launch_interact, for per-kernel timing
-void launch_interact_fused(const dev_consts &k, const dstate &cur, const dstate &nxt, const dbins &b,
-                           const dpositional &pos, const drecords &r, const mfc_tool_state *tool_now, const dforce &f,
-                           variant v, hipStream_t s);
-void launch_update(const dev_consts &k, const double *acc, const dbins &b, const dstate &cur, const dstate &nxt,
-                   const dpositional &pos, const mfc_tool_state *tool_now, const dforce &f, hipStream_t s);
-void launch_force_final(const dev_consts &k, const dforce &f, hipStream_t s);
 // Usui wear rate of state st: out = {min, max, sum, count} over the particles in contact
 // (part: 4 doubles per tile of scratch)
 void launch_tool_wear(const dev_consts &k, const dstate &st, const dpositional &pos, double K, double alpha,
                       double tvx, double tvy, double *part, double *out, hipStream_t s);
-// neighbour sums only (ACC_N arrays of N doubles); src_flags in the storage order of cur
-void launch_sums(const dev_consts &k, const int *src_flags, const dbins &b, const drecords &r, double *acc, variant v,
-                 hipStream_t s);
-
 int scan_blocks(int num_cell);
 void launch_device_math(int fn, const double *x, const double *y, double *out, int64_t n, hipStream_t s);
 

@@ -6,7 +6,7 @@
 //                grid_gpu_green.cu:22-116,177-201) from the cell ids (grid.cu:34-51), per-cell
 //                and per-chunk counts that the previous step's interaction computed for the
 //                updated state (hash_count computes them when the state was written from
-//                outside; the reverse mode's replay runs hash_count and scan_reduce)
+//                outside, and for every state the reverse mode's replay restores)
 //   predict      the sort's rank pass, then leap-frog predict (leap_frog.cu:30-66) + EOS
 //                (actions_gpu.cu:41-51) + artificial stress (:53-97) + neighbour record + tool
 //                move, in the start state's storage order; the work-group of the first tile
@@ -15,13 +15,12 @@
 //   interact     neighbour sums (interactions_gpu.cu:88-345) fused with every per-particle action
 //                that follows them in the step: Jaumann, continuity, momentum, advection, correct,
 //                friction heat, Johnson-Cook return, tool-particle move, contact, boundary
-//                conditions, NaN blanking, the gather into sorted order, and the per-tile partial
+//                conditions, NaN blanking, the gather into sorted order, and the per-wave partial
 //                sums of the tool force (or split into sums + update), and the cell ids and
 //                counts of the next step
 // After the last step of a call, force_final reduces the last step's tool force.
 // Each body is a device function of its work index; the kb_* kernels run it over an
-// (instance, tile) work table for any number of instances (a single handle is a batch of one),
-// the k_* kernels over one instance's tiles (used by the reverse mode's replay).
+// (instance, tile) work table for any number of instances (a single handle is a batch of one).
 //
 // Memory layout: the persistent particle state is a structure of arrays in cell-sorted order,
 // double-buffered across the step. Neighbour data are packed per particle into one 128-byte
@@ -68,8 +67,8 @@ __device__ inline int block_exclusive_scan(int v, int &total) {
 }
 
 // ------------------------------------------------------------------------------- binning
-// Each kernel body is a device function of its work index, called by the single-instance kernel
-// (index from blockIdx) and by the batched kernel (index from the (instance, block) work table).
+// Each kernel body is a device function of its work index, called by the batched kernel with
+// the index of an (instance, block) item of its work table.
 
 // cell id of a particle for the binning (grid.cu:34-51; blanked particles keep their last
 // sorted id, grid.cu:39-41); positions outside the grid are counted
@@ -119,18 +118,6 @@ __device__ inline void hash_count(const dev_consts &k, int i, const double *__re
                                   unsigned long long *__restrict__ clamped) {
   const int h = i < k.N ? particle_cell(k, px[i], py[i], flags[i], key_sorted[i], clamped) : -1;
   count_cell(i, h, key, slot, count, bsum);
-}
-
-// particles per chunk of SCAN_CHUNK cells
-__device__ inline void scan_reduce(int blk, const int *__restrict__ count, int n, int *__restrict__ bsum) {
-  const int base = blk * SCAN_CHUNK + threadIdx.x * SCAN_ITEMS;
-  int s = 0;
-#pragma unroll
-  for (int q = 0; q < SCAN_ITEMS; q++)
-    if (base + q < n) s += count[base + q];
-  int total;
-  block_exclusive_scan(s, total);
-  if (threadIdx.x == 0) bsum[blk] = total;
 }
 
 // cell offsets from the counts; the counts are cleared for the next step. The offset of chunk
@@ -192,27 +179,6 @@ __device__ inline void place(int i, int N, const int *__restrict__ key, const in
   bucket[offsets[key[i]] + slot[i]] = i;
 }
 
-
-__global__ __launch_bounds__(256) void k_hash_count(dev_consts k, const double *__restrict__ px,
-                                                    const double *__restrict__ py, const int *__restrict__ flags,
-                                                    dbins b) {
-  hash_count(k, blockIdx.x * blockDim.x + threadIdx.x, px, py, flags, b.key_sorted, b.key, b.slot, b.count, b.bsum,
-             b.clamped);
-}
-
-__global__ __launch_bounds__(SCAN_THREADS) void k_scan_reduce(const int *__restrict__ count, int n,
-                                                               int *__restrict__ bsum) {
-  scan_reduce(blockIdx.x, count, n, bsum);
-}
-
-__global__ __launch_bounds__(SCAN_THREADS) void k_scan_down(int *__restrict__ count, int n,
-                                                             const int *__restrict__ bsum, int *__restrict__ offsets) {
-  scan_down(blockIdx.x, count, n, bsum, offsets);
-}
-
-__global__ __launch_bounds__(256) void k_place(int N, dbins b) {
-  place(blockIdx.x * blockDim.x + threadIdx.x, N, b.key, b.slot, b.offsets, b.bucket);
-}
 
 // ------------------------------------------------------------------------------- predict
 // artificial stress in single precision without trigonometry: with d = sqrt((sxx-syy)^2 + 4 sxy^2),
@@ -357,16 +323,6 @@ __device__ inline void predict(const dev_consts &k, int t, const dbins &bins, co
   for (int q = 0; q < REC / 2; q++) dst[q] = make_double2(r[2 * q], r[2 * q + 1]);
   posd[i] = make_double2(x, y);
   vfl[i] = rfl;
-}
-
-template <bool F32, bool GATHER>
-__global__ __launch_bounds__(TILE) void k_predict(dev_consts k, int ntiles, dbins b, dstate cur, dstate nxt,
-                                                  dpositional pos, drecords r,
-                                                  const mfc_tool_state *__restrict__ tool_in,
-                                                  mfc_tool_state *__restrict__ tool_out) {
-  const int t = xcd_tile(blockIdx.x, ntiles);
-  if (t < 0) return;
-  predict<F32, GATHER>(k, t, b, cur, nxt, pos, r, tool_in, tool_out);
 }
 
 // ------------------------------------------------------------------------------- interact
@@ -598,7 +554,7 @@ __device__ inline void update_particle(const dev_consts &k, int i, int s, const 
   const int sft = k.rothlin ? i : s;
   double fcx = src.fcx[s], fcy = src.fcy[s], ftx = src.ftx[sft], fty = src.fty[sft];
   double eps = src.eps[s], epsd = src.epsd[s];
-  // predicted state (same expressions as k_predict)
+  // predicted state (same expressions as predict)
   double pu = u0, pv = v0, psxx = sxx0, psxy = sxy0, psyy = syy0, pszz = szz0, prho = rho0;
   if (!blank) {
     const double hdt = 0.5 * k.dt;
@@ -877,38 +833,6 @@ __device__ inline void force_final(int ntiles, const dforce &fo) {
   }
 }
 
-template <bool F32, bool COMPACT>
-__global__ __launch_bounds__(TILE) SUMS_OCCUPANCY void k_interact(dev_consts k, int ntiles, drecords r,
-                                                   const int *__restrict__ key_sorted,
-                                                   const int *__restrict__ offsets, const int *__restrict__ perm,
-                                                   dstate src, dstate dst, dpositional pos,
-                                                   const mfc_tool_state *__restrict__ tool_now, dforce fo) {
-  const int t = xcd_chunk_tile(blockIdx.x, ntiles);
-  if (t < 0) return;
-  interact<F32, COMPACT>(k, t, r, key_sorted, offsets, perm, src, dst, pos, tool_now, fo, dbins{}, false);
-}
-
-template <bool F32, bool COMPACT>
-__global__ __launch_bounds__(TILE) SUMS_OCCUPANCY void k_sums(dev_consts k, int ntiles, drecords r,
-                                               const int *__restrict__ key_sorted, const int *__restrict__ offsets,
-                                               const int *__restrict__ perm, const int *__restrict__ src_flags,
-                                               double *__restrict__ acc) {
-  const int t = xcd_chunk_tile(blockIdx.x, ntiles);
-  if (t < 0) return;
-  sums<F32, COMPACT>(k, t, r, key_sorted, offsets, perm, src_flags, acc);
-}
-
-__global__ __launch_bounds__(TILE) void k_update(dev_consts k, int ntiles, const double *__restrict__ acc,
-                                                 const int *__restrict__ perm, dstate src, dstate dst,
-                                                 dpositional pos, const mfc_tool_state *__restrict__ tool_now,
-                                                 dforce fo) {
-  const int t = xcd_tile(blockIdx.x, ntiles);
-  if (t < 0) return;
-  update(k, t, acc, perm, src, dst, pos, tool_now, fo, dbins{}, false);
-}
-
-__global__ __launch_bounds__(TILE) void k_force_final(int ntiles, dforce fo) { force_final(ntiles, fo); }
-
 // ------------------------------------------------------------------------------- tool wear
 // Usui wear rate (tool_wear.cu:20-60) of the current state: particles count as in contact when
 // |fc.x|, |fc.y| or |ft.x| is non-zero (the reference tests fc.y twice and never ft.y); the
@@ -1031,9 +955,9 @@ __device__ inline inst_args load_args(const inst_args *A, int q) {
 #undef MFC_F
 
 // One launch per phase for all instances of a batch: block b takes work item xcd_tile(b) (the
-// interaction kernels: xcd_chunk_tile(b)) of the
-// (instance, block) table, and runs the same device body as the single-instance kernel with
-// that instance's constants and buffers, so every instance computes exactly its own run.
+// interaction kernels: xcd_chunk_tile(b)) of the (instance, block) table, and runs the device
+// body with that instance's constants and buffers, so every instance computes exactly its own
+// run.
 __global__ __launch_bounds__(256) void kb_hash_count(const inst_args *__restrict__ A, const int2 *__restrict__ work,
                                                      int nwork) {
   const int w = xcd_tile(blockIdx.x, nwork);
@@ -1249,51 +1173,11 @@ __global__ void k_device_math(int fn, const double *__restrict__ x, const double
 
 int scan_blocks(int num_cell) { return (num_cell + SCAN_CHUNK - 1) / SCAN_CHUNK; }
 
-void launch_binning(const dev_consts &k, const dstate &cur, dbins &b, hipStream_t s) {
-  const int g = (k.N + 255) / 256;
-  const int nb = scan_blocks(k.ncell_table);
-  hipLaunchKernelGGL(k_hash_count, dim3(g), dim3(256), 0, s, k, cur.px, cur.py, cur.flags, b);
-  hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(SCAN_THREADS), 0, s, b.count, k.ncell_table, b.bsum);
-  hipLaunchKernelGGL(k_scan_down, dim3(nb), dim3(SCAN_THREADS), 0, s, b.count, k.ncell_table, b.bsum, b.offsets);
-  hipLaunchKernelGGL(k_place, dim3(g), dim3(256), 0, s, k.N, b);
-}
-
 static int padded_tiles(int ntiles) { return ((ntiles + 7) / 8) * 8; }
-
-void launch_predict(const dev_consts &k, const dstate &cur, const dstate &nxt, const dbins &b, const dpositional &pos,
-                    const drecords &r, const mfc_tool_state *tool_in, mfc_tool_state *tool_out, bool gather,
-                    hipStream_t s) {
-  const int nt = (k.N + TILE - 1) / TILE;
-  auto kern = r.f32 ? (gather ? k_predict<true, true> : k_predict<true, false>)
-                    : (gather ? k_predict<false, true> : k_predict<false, false>);
-  hipLaunchKernelGGL(kern, dim3(padded_tiles(nt)), dim3(TILE), 0, s, k, nt, b, cur, nxt, pos, r, tool_in, tool_out);
-}
 
 // the four instantiations of a kernel template <F32, COMPACT>
 #define MFC_PICK(kern, f32, compact) \
   ((f32) ? ((compact) ? kern<true, true> : kern<true, false>) : ((compact) ? kern<false, true> : kern<false, false>))
-
-void launch_sums(const dev_consts &k, const int *src_flags, const dbins &b, const drecords &r, double *acc, variant v,
-                 hipStream_t s) {
-  const int nt = (k.N + TILE - 1) / TILE;
-  hipLaunchKernelGGL(MFC_PICK(k_sums, r.f32, v.compact), dim3(chunked_blocks(nt)), dim3(TILE), 0, s, k, nt, r,
-                     b.key_sorted, b.offsets, b.perm, src_flags, acc);
-}
-
-void launch_interact_fused(const dev_consts &k, const dstate &cur, const dstate &nxt, const dbins &b,
-                           const dpositional &pos, const drecords &r, const mfc_tool_state *tool_now, const dforce &f,
-                           variant v, hipStream_t s) {
-  const int nt = (k.N + TILE - 1) / TILE;
-  hipLaunchKernelGGL(MFC_PICK(k_interact, r.f32, v.compact), dim3(chunked_blocks(nt)), dim3(TILE), 0, s, k, nt, r,
-                     b.key_sorted, b.offsets, b.perm, cur, nxt, pos, tool_now, f);
-}
-
-void launch_update(const dev_consts &k, const double *acc, const dbins &b, const dstate &cur, const dstate &nxt,
-                   const dpositional &pos, const mfc_tool_state *tool_now, const dforce &f, hipStream_t s) {
-  const int nt = (k.N + TILE - 1) / TILE;
-  hipLaunchKernelGGL(k_update, dim3(padded_tiles(nt)), dim3(TILE), 0, s, k, nt, acc, b.perm, cur, nxt, pos,
-                     tool_now, f);
-}
 
 void launch_tool_wear(const dev_consts &k, const dstate &st, const dpositional &pos, double K, double alpha,
                       double tvx, double tvy, double *part, double *out, hipStream_t s) {
@@ -1301,23 +1185,6 @@ void launch_tool_wear(const dev_consts &k, const dstate &st, const dpositional &
   hipLaunchKernelGGL(k_wear_tiles, dim3(padded_tiles(nt)), dim3(TILE), 0, s, k.N, nt, st, pos.nx, pos.ny, K, alpha,
                      tvx, tvy, part);
   hipLaunchKernelGGL(k_wear_final, dim3(1), dim3(TILE), 0, s, nt, part, out);
-}
-
-void launch_force_final(const dev_consts &k, const dforce &f, hipStream_t s) {
-  const int nt = (k.N + TILE - 1) / TILE;
-  hipLaunchKernelGGL(k_force_final, dim3(1), dim3(TILE), 0, s, nt, f);
-}
-
-void launch_interact(const dev_consts &k, const dstate &cur, const dstate &nxt, const dbins &b,
-                     const dpositional &pos, const drecords &r, const mfc_tool_state *tool_now, const dforce &f,
-                     double *acc, variant v, hipStream_t s) {
-  if (v.split) {
-    launch_sums(k, cur.flags, b, r, acc, v, s);
-    launch_update(k, acc, b, cur, nxt, pos, tool_now, f, s);
-  } else {
-    launch_interact_fused(k, cur, nxt, b, pos, r, tool_now, f, v, s);
-  }
-  launch_force_final(k, f, s);
 }
 
 void launch_device_math(int fn, const double *x, const double *y, double *out, int64_t n, hipStream_t s) {
