@@ -48,13 +48,16 @@ KERNEL_BYTES = {"binning": 12, "predict": 192, "interact": 64 + 224}
 assert sum(KERNEL_BYTES.values()) == STEP_BYTES
 # the split variant: the sums kernel reads nothing compulsory beyond what the interaction kernel
 # is credited with; its 288 B are credited to the pair (sums + update)
-# HBM bytes per particle of one launch of the fused interaction kernel, by precision and pair
-# loop, measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE passes and scaled by a 512 MiB copy
-# (profiles/traffic_thermal_contact_150k.md, which names the tree it was measured on); reported
-# as `traffic` only for the tuning it was measured with
-MEASURED_TRAFFIC = {("kb_interact", "fp32c", 0): 661, ("kb_interact", "fp32c", 1): 646,
-                    ("kb_interact", "fp64", 0): 830, ("kb_interact", "fp64", 1): 810}
-MEASURED_TRAFFIC_SOURCE = ("rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, thermal_contact_150k, fused interaction, "
+# HBM bytes per particle of one launch of the fused interaction kernel on thermal_contact_150k,
+# by precision, pair loop and XCD chunk length (full LDS stage), measured with rocprofv3
+# --pmc FETCH_SIZE / WRITE_SIZE passes and scaled by a 512 MiB copy (the commands and tables are
+# in profiles/traffic_thermal_contact_150k.md); a table of measurements, not measured in this
+# run, and reported as `traffic` only for the tuning it was measured with
+MEASURED_TRAFFIC = {("kb_interact", "fp32c", 0, 8): 662, ("kb_interact", "fp32c", 0, 32): 612,
+                    ("kb_interact", "fp32c", 1, 8): 648, ("kb_interact", "fp32c", 1, 32): 600,
+                    ("kb_interact", "fp64", 0, 8): 831, ("kb_interact", "fp64", 0, 32): 741,
+                    ("kb_interact", "fp64", 1, 8): 812, ("kb_interact", "fp64", 1, 32): 694}
+MEASURED_TRAFFIC_SOURCE = ("table: rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, thermal_contact_150k, fused interaction, "
                            "profiles/traffic_thermal_contact_150k.md")
 
 
@@ -90,11 +93,14 @@ def cpu_baseline(cfg, target_s: float) -> dict:
                       f"{wall:.1f} s wall on {ncpu} threads ({model})"}
 
 
+FIXED_TUNING = {}  # --tuning k=v,...: the settings used when the autotuner is off
+
+
 def tune(pymfc, handle, steps: int) -> dict:
     """The package's autotuner on a live handle (every launch configuration gives the same bits);
-    off (defaults) when steps is 0."""
+    off when steps is 0: the defaults, or the settings given with --tuning."""
     if steps <= 0:
-        return {k: v[0] for k, v in pymfc.TUNING_SPACE.items()} | {"ms_per_step": {}}
+        return {k: v[0] for k, v in pymfc.TUNING_SPACE.items()} | FIXED_TUNING | {"ms_per_step": {}}
     return pymfc.autotune(handle, steps)
 
 
@@ -272,7 +278,7 @@ def counter_probe(pymfc, workload: str, precision: int, steps: int, split: int, 
     readings calibrate the bytes per counted unit."""
     import torch
     sim = pymfc.Sim(pymfc.workload(workload, precision))
-    sim.set_tuning(split_interact=split, compact_pairs=compact)
+    sim.set_tuning(**(FIXED_TUNING | {"split_interact": split, "compact_pairs": compact}))
     for _ in range(steps):
         sim.step(1)
     sim.sync()
@@ -298,6 +304,7 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--only", choices=["fp64", "fp32c"], default=None)
     ap.add_argument("--tune-steps", type=int, default=100, help="steps per candidate of the autotuner (0: off)")
+    ap.add_argument("--tuning", default="", help="with --tune-steps 0: launch settings k=v,... (e.g. compact_pairs=1)")
     ap.add_argument("--adjoint-workload", default="adjoint_40k")
     ap.add_argument("--adjoint-steps", type=int, default=400)
     ap.add_argument("--adjoint-k", type=int, default=50)
@@ -318,6 +325,12 @@ def main() -> int:
 
     import pymfc
 
+    for kv in filter(None, args.tuning.split(",")):
+        k, v = kv.split("=")
+        if k not in pymfc.TUNING_SPACE:
+            print(f"bench.py: unknown launch setting {k} (one of {list(pymfc.TUNING_SPACE)})", file=sys.stderr)
+            return 2
+        FIXED_TUNING[k] = int(v)
     if args.probe:
         counter_probe(pymfc, args.workload, pymfc.PREC_FP32C if args.only == "fp32c" else pymfc.PREC_FP64,
                       args.steps, args.split, args.compact)
@@ -343,7 +356,7 @@ def main() -> int:
     # kernel's roofline
     t_int = (r["phases"]["interact"] + r["phases"]["update"]) * 1e-3 if r["phases"] else None
     achieved = N * kbytes / t_int / 1e9 if t_int else None
-    tkey = ("kb_interact", head, r["tuning"]["compact_pairs"])
+    tkey = ("kb_interact", head, r["tuning"]["compact_pairs"], r["tuning"].get("xcd_chunk", 8))
     traffic = None
     if args.workload == "thermal_contact_150k" and not split and r["tuning"]["stage_limit"] == 0 \
             and tkey in MEASURED_TRAFFIC:
