@@ -155,28 +155,54 @@ __device__ inline double sigma_yield_tanh(const dev_consts &k, double eps_pl, do
 }
 
 // residual of the radial return (plasticity.cuh:74-83); the flow stress is the model of the
-// function pointer m_s_yield (plasticity.cuh:72), which only this residual uses
-__device__ inline double rr_residual(const dev_consts &k, double dl, double norm, double eps0, double t, double TC) {
+// function pointer m_s_yield (plasticity.cuh:72), which only this residual uses. The same
+// expressions as sigma_yield / sigma_yield_tanh, with the constants passed in registers: the
+// secant below calls it twice per iteration, and as an outlined function it keeps the step
+// kernels' register budget (inlined, two interleaved residuals would cost them occupancy), while
+// taking its constants from the caller instead of reloading them from the instance descriptor in
+// memory at every call, which made a wave with a long return the last of the interaction launch
+__device__ __attribute__((noinline)) double rr_residual_jc(double dl, double norm, double eps0, double TC, double A,
+                                                          double B, double n, double C, double edr, double dt,
+                                                          double G) {
+  const double s23 = sqrt(2.0 / 3.0);
+  double e = eps0 + s23 * fmax(dl, 0.);
+  double ed = s23 * fmax(dl, 0.) / dt;
+  double TA = A + B * mfcm::pow(e, n);
+  double TB = 1.0;
+  double es = ed / edr;
+  if (es > 1.0) TB = 1.0 + C * mfcm::log(es);
+  else TB = mfcm::pow((1.0 + es), C);
+  double sY = TA * TB * TC;
+  return norm - 2 * G * dl - s23 * sY;
+}
+
+__device__ __attribute__((noinline)) double rr_residual_tanh(const dev_consts &k, double dl, double norm, double eps0,
+                                                            double t, double TC) {
   const double s23 = sqrt(2.0 / 3.0);
   double e = eps0 + s23 * fmax(dl, 0.);
   double ed = s23 * fmax(dl, 0.) / k.dt;
-  double sY = k.jc_tanh ? sigma_yield_tanh(k, e, ed, t, TC) : sigma_yield(k, e, ed, TC);
+  double sY = sigma_yield_tanh(k, e, ed, t, TC);
   return norm - 2 * k.G * dl - s23 * sY;
 }
 
 // secant iteration of the radial return (plasticity.cuh:85-119)
 __device__ inline double rr_secant(const dev_consts &k, double init, double norm, double eps0, double t, double TC) {
+  const double A = k.jc_A, B = k.jc_B, n = k.jc_n, C = k.jc_C, edr = k.jc_eps_dot_ref, dt = k.dt, G = k.G;
+  const double tol = k.secant_tol;
+  const bool tanh_model = k.jc_tanh != 0;
   double dl = init, dl_old;
   int iter = 0;
   while (true) {
-    double g = rr_residual(k, dl, norm, eps0, t, TC);
+    double g = tanh_model ? rr_residual_tanh(k, dl, norm, eps0, t, TC)
+                          : rr_residual_jc(dl, norm, eps0, TC, A, B, n, C, edr, dt, G);
     double dl1 = dl * 1.01;
-    double g1 = rr_residual(k, dl1, norm, eps0, t, TC);
+    double g1 = tanh_model ? rr_residual_tanh(k, dl1, norm, eps0, t, TC)
+                           : rr_residual_jc(dl1, norm, eps0, TC, A, B, n, C, edr, dt, G);
     double slope = (g - g1) / (dl - dl1);
     dl_old = dl;
     dl = dl - g / slope;
     if (dl < 0.) dl = -0.1 * dl;
-    if (fabs(dl - dl_old) < k.secant_tol) return dl;
+    if (fabs(dl - dl_old) < tol) return dl;
     if (iter > 100) return dl;
     iter++;
   }
@@ -192,7 +218,8 @@ __device__ inline void plasticity(const dev_consts &k, double p, double rho, dou
   double sY = sigma_yield(k, eps_pl, eps_pl_dot, TC);
   if (svm < sY) { eps_pl_dot = 0.; return; }
   const double s23 = sqrt(2.0 / 3.0);
-  double dl = rr_secant(k, fmax(eps_pl_dot * k.dt * s23, 1e-8), norm, eps_pl, T, TC);
+  const double init = fmax(eps_pl_dot * k.dt * s23, 1e-8);
+  double dl = rr_secant(k, init, norm, eps_pl, T, TC);
   double inc;
   if (k.promote) inc = s23 * fmax(dl, 0.);
   else inc = (double)(sqrtf((float)(2.0 / 3.0)) * fmaxf((float)dl, 0.f));
