@@ -79,13 +79,19 @@ struct adj_args {
   const snapshot *ckpt, *seg;       // checkpoints of the tape, and the start states of a segment
 };
 
+// an instance's descriptor read through the constant address space: the pointers loaded from it
+// are then known to point to global memory, and the kernels use global instead of flat loads
+// (a flat load also holds up every wait for an LDS or scalar load)
+#define MFC_ADJ_ARGS(d, q) const __attribute__((address_space(4))) adj_args &d = \
+    *(const __attribute__((address_space(4))) adj_args *)(D + (q))
+
 // ------------------------------------------------------------------ checkpoint copies
 // dir 0: a snapshot (ckpt[idx] or seg[idx]) into replay set 0, the positional block, the cell
 // ids and replay tool 0; dir 1: replay set wa, the positional block, the cell ids and replay
 // tool wa into seg[idx]. Grid (copy_blocks, instances).
 __global__ __launch_bounds__(256) void kb_adj_copy(const adj_args *__restrict__ D, int dir, int from_ckpt, int idx,
                                                    int wa) {
-  const adj_args &d = D[blockIdx.y];
+  MFC_ADJ_ARGS(d, blockIdx.y);
   const snapshot sn = from_ckpt ? d.ckpt[idx] : d.seg[idx];
   const int lw = dir ? wa : 0;
   uint64_t *ls = d.live_state[lw];
@@ -115,8 +121,8 @@ __global__ __launch_bounds__(TILE) void kb_adj_tail(const adj_args *__restrict__
   const int w = xcd_tile(blockIdx.x, nwork);
   if (w < 0) return;
   const int2 it = work[w];
-  const adj_args &d = D[it.x];
-  const dev_consts &k = d.k;
+  MFC_ADJ_ARGS(d, it.x);
+  const dev_consts &k = D[it.x].k;
   const int t = it.y;
   const int i = t * TILE + threadIdx.x;
   const size_t N = k.N;
@@ -168,7 +174,7 @@ __global__ __launch_bounds__(TILE) void kb_adj_tail(const adj_args *__restrict__
 // adds the step's theta partials of instance blockIdx.x to its running totals: lane l sums the
 // tiles l, l + TILE, ... in ascending order, lane sums reduced by halving strides
 __global__ __launch_bounds__(TILE) void kb_adj_theta(const adj_args *__restrict__ D) {
-  const adj_args &d = D[blockIdx.x];
+  MFC_ADJ_ARGS(d, blockIdx.x);
   __shared__ double red[6][TILE];
   double a[6] = {0., 0., 0., 0., 0., 0.};
   for (int u = threadIdx.x; u < d.ntiles; u += TILE)
@@ -208,8 +214,8 @@ __global__ __launch_bounds__(TILE) void kb_adj_pull(const adj_args *__restrict__
   const int u = xcd_tile(blockIdx.x, nwork * PULL_LANES);
   if (u < 0) return;
   const int2 it = work[u / PULL_LANES];
-  const adj_args &d = D[it.x];
-  const dev_consts &k = d.k;
+  MFC_ADJ_ARGS(d, it.x);
+  const dev_consts &k = D[it.x].k;
   const int r = threadIdx.x % PULL_LANES;
   const int b = it.y * TILE + (u % PULL_LANES) * (TILE / PULL_LANES) + threadIdx.x / PULL_LANES;
   const bool live = b < k.N;  // the lanes of a particle stay together for the shuffles
@@ -288,8 +294,8 @@ __global__ __launch_bounds__(TILE) void kb_adj_record(const adj_args *__restrict
   const int w = xcd_tile(blockIdx.x, nwork);
   if (w < 0) return;
   const int2 it = work[w];
-  const adj_args &d = D[it.x];
-  const dev_consts &k = d.k;
+  MFC_ADJ_ARGS(d, it.x);
+  const dev_consts &k = D[it.x].k;
   const int a = it.y * TILE + threadIdx.x;
   if (a >= k.N) return;
   const size_t N = k.N;
