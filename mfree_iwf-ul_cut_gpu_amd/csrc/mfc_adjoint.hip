@@ -10,7 +10,8 @@
 //   F1  binning          cell ids and the stable counting sort of the restored state
 //   F2  predict          with the sorted copy of the start state (gather, predictor, EOS,
 //                        artificial stress, records)
-//   F3  sums             neighbour sums of every particle (the sums kernel of the step)
+//   F3  sums             neighbour sums of every particle, kept from the segment's replay (the
+//                        replayed interaction stores them, one slot per step of the segment)
 //   B1  kb_adj_tail      VJP of everything after the sums, per particle (tail_vjp), and the
 //                        per-tile theta partials of the step (kb_adj_theta adds them up)
 //   B2  kb_adj_pull      VJP of the neighbour sums with respect to the records: each particle
@@ -68,7 +69,9 @@ struct adj_args {
   const double *rec;                // FP64 neighbour records
   const int *key_sorted, *offsets, *perm;
   const mfc_tool_state *tool_now;   // the tool after the reversed step's move
-  double *acc, *lam_acc, *lam_s[2], *lam_n, *g_s, *g_rec, *theta_part, *theta_acc;
+  double *acc_seg;                  // the replay's neighbour sums, one slot of ACC_N x N per step
+  size_t acc_stride;                // doubles per slot
+  double *lam_acc, *lam_s[2], *lam_n, *g_s, *g_rec, *theta_part, *theta_acc;
   double Fb[2];                     // cotangent over the window length
   // checkpoint copies: the live replay buffers and the instance's snapshots
   size_t state_words, pos_words;    // 8-byte words of a state block and of a positional block
@@ -112,12 +115,24 @@ __global__ __launch_bounds__(256) void kb_adj_copy(const adj_args *__restrict__ 
   }
 }
 
+// points the replay descriptors of both parities at the neighbour-sum slot of a step (one
+// thread per instance; the next launches read the descriptors after this one completes)
+__global__ __launch_bounds__(64) void kb_adj_acc_slot(inst_args *__restrict__ rep, const adj_args *__restrict__ D,
+                                                      int n, int slot) {
+  const int q = blockIdx.x * 64 + threadIdx.x;
+  if (q >= n) return;
+  double *a = D[q].acc_seg + (size_t)slot * D[q].acc_stride;
+  rep[q].acc = a; rep[q].acc_out = a;
+  rep[n + q].acc = a; rep[n + q].acc_out = a;
+}
+
 // ------------------------------------------------------------------ B1: per-particle tail
-// st: sorted start state; lam_s[cur]: adjoint of the end state of the step (same layout);
-// lam_n: adjoint of the positional normal written by this step (in) and of the one read by it
-// (out); in_win: the step's force is in the loss window
+// st: sorted start state; acc: the step's neighbour sums (slot of the segment's replay);
+// lam_s[cur]: adjoint of the end state of the step (same layout); lam_n: adjoint of the positional
+// normal written by this step (in) and of the one read by it (out); in_win: the step's force is
+// in the loss window
 __global__ __launch_bounds__(TILE) void kb_adj_tail(const adj_args *__restrict__ D, const int2 *__restrict__ work,
-                                                    int nwork, int cur, int in_win) {
+                                                    int nwork, int cur, int in_win, int slot) {
   const int w = xcd_tile(blockIdx.x, nwork);
   if (w < 0) return;
   const int2 it = work[w];
@@ -126,6 +141,7 @@ __global__ __launch_bounds__(TILE) void kb_adj_tail(const adj_args *__restrict__
   const int t = it.y;
   const int i = t * TILE + threadIdx.x;
   const size_t N = k.N;
+  const double *__restrict__ acc = d.acc_seg + (size_t)slot * d.acc_stride;
   double th[6] = {0., 0., 0., 0., 0., 0.};
   if (i < k.N) {
     tail_in in;
@@ -137,7 +153,7 @@ __global__ __launch_bounds__(TILE) void kb_adj_tail(const adj_args *__restrict__
     in.flags = d.flags[i];
     in.pse_flag_zero = k.pse_quirk && i >= k.N / 2;
 #pragma unroll
-    for (int q = 0; q < L_N; q++) in.acc[q] = d.acc[q * N + i];
+    for (int q = 0; q < L_N; q++) in.acc[q] = acc[q * N + i];
     const double *lam_s = d.lam_s[cur];
     double ob[TS_N];
 #pragma unroll
@@ -329,7 +345,7 @@ struct adj_work {
   int64_t *ctr = nullptr;
   int *pending = nullptr;
   int *keys_keep = nullptr;  // the handle's cell ids, kept across the replay of the backward pass
-  double *acc = nullptr, *lam_acc = nullptr, *lam_s[2] = {nullptr, nullptr}, *lam_n = nullptr, *g_s = nullptr,
+  double *acc_seg = nullptr, *lam_acc = nullptr, *lam_s[2] = {nullptr, nullptr}, *lam_n = nullptr, *g_s = nullptr,
          *g_rec = nullptr, *theta_part = nullptr, *theta_acc = nullptr;
 
   template <class T>
@@ -398,7 +414,7 @@ int setup_work(mfc_sim *s, int nsteps, int K, int w0, int w1) {
         (rc = w->alloc(&w->partials, 16 * (size_t)s->ntiles)) || (rc = w->alloc(&w->force_last, 2)) ||
         (rc = w->alloc(&w->hist, 2)) || (rc = w->alloc(&w->ctr, 1)) || (rc = w->alloc(&w->pending, 1)) ||
         (rc = w->alloc(&w->keys_keep, n)) ||
-        (rc = w->alloc(&w->acc, L_N * n)) || (rc = w->alloc(&w->lam_acc, L_N * n)) ||
+        (rc = w->alloc(&w->acc_seg, (size_t)K * L_N * n)) || (rc = w->alloc(&w->lam_acc, L_N * n)) ||
         (rc = w->alloc(&w->lam_s[0], TS_N * n)) || (rc = w->alloc(&w->lam_s[1], TS_N * n)) ||
         (rc = w->alloc(&w->lam_n, 2 * n)) || (rc = w->alloc(&w->g_s, TS_N * n)) ||
         (rc = w->alloc(&w->g_rec, REC_DOUBLES * n)) || (rc = w->alloc(&w->theta_part, 6 * (size_t)s->ntiles)) ||
@@ -502,7 +518,7 @@ int adj_backward(mfc_sim *const *sims, int n, const batch_work &fw, hipStream_t 
       a.tool_in = w->toolw + p;
       a.tool_out = w->toolw + 1 - p;
       a.f = {w->partials, w->force_last, w->hist, 0, w->ctr, w->pending};  // no history, counted
-      a.acc = w->acc;
+      a.acc = a.acc_out = w->acc_seg;  // pointed at each step's slot by kb_adj_acc_slot
       a.ntiles = s->ntiles;
       a.nscan = scan_blocks(s->k.ncell_table);
       a.bin_next = 0;
@@ -519,7 +535,8 @@ int adj_backward(mfc_sim *const *sims, int n, const batch_work &fw, hipStream_t 
     d.offsets = s->bins.offsets;
     d.perm = s->bins.perm;
     d.tool_now = w->toolw + 1;
-    d.acc = w->acc;
+    d.acc_seg = w->acc_seg;
+    d.acc_stride = (size_t)L_N * N;
     d.lam_acc = w->lam_acc;
     d.lam_s[0] = w->lam_s[0];
     d.lam_s[1] = w->lam_s[1];
@@ -580,6 +597,7 @@ int adj_backward(mfc_sim *const *sims, int n, const batch_work &fw, hipStream_t 
     const batch_work rw[2] = {{d_rep, n, fw.tiles, fw.ntiles, fw.chunks, fw.nchunks, false, fw.v, false, false},
                               {d_rep + n, n, fw.tiles, fw.ntiles, fw.chunks, fw.nchunks, false, fw.v, false, false}};
     const dim3 gcopy(copy_blocks, n), gt(padded(fw.ntiles)), gpull(padded(fw.ntiles * PULL_LANES));
+    const dim3 gslot((n + 63) / 64);
     int cur = 0;  // lam_s[cur] holds the adjoint of the end state of the step being reversed
     const int nseg = nck;
     for (int sg = nseg - 1; sg >= 0; sg--) {
@@ -589,9 +607,9 @@ int adj_backward(mfc_sim *const *sims, int n, const batch_work &fw, hipStream_t 
       hipLaunchKernelGGL(kb_adj_copy, gcopy, dim3(256), 0, st, d_da, 0, 1, sg, 0);
       for (int t = t0; t < t1; t++) {
         hipLaunchKernelGGL(kb_adj_copy, gcopy, dim3(256), 0, st, d_da, 1, 0, t - t0, wa);
-        if (t + 1 == t1) break;
+        hipLaunchKernelGGL(kb_adj_acc_slot, gslot, dim3(64), 0, st, d_rep, d_da, n, t - t0);
         launch_batch_hash(rw[wa], st);
-        launch_batch_step(rw[wa], st);
+        launch_batch_step(rw[wa], st);  // the interaction keeps its sums in the step's slot
         wa = 1 - wa;
       }
       // reverse the steps of the segment
@@ -600,9 +618,8 @@ int adj_backward(mfc_sim *const *sims, int n, const batch_work &fw, hipStream_t 
         launch_batch_hash(rw[0], st);
         launch_batch_binning(rw[0], st);
         launch_batch_predict_gather(rw[0], st);
-        launch_batch_sums(rw[0], st);
         const int in_win = t >= w0->w0 && t < w0->w1;
-        hipLaunchKernelGGL(kb_adj_tail, gt, dim3(TILE), 0, st, d_da, fw.tiles, fw.ntiles, cur, in_win);
+        hipLaunchKernelGGL(kb_adj_tail, gt, dim3(TILE), 0, st, d_da, fw.tiles, fw.ntiles, cur, in_win, t - t0);
         hipLaunchKernelGGL(kb_adj_theta, dim3(n), dim3(TILE), 0, st, d_da);
         hipLaunchKernelGGL(HIP_KERNEL_NAME(kb_adj_pull<false>), gpull, dim3(TILE), 0, st, d_da, fw.tiles, fw.ntiles);
         hipLaunchKernelGGL(HIP_KERNEL_NAME(kb_adj_pull<true>), gpull, dim3(TILE), 0, st, d_da, fw.tiles, fw.ntiles);

@@ -185,41 +185,33 @@ __device__ __attribute__((noinline)) double rr_residual_tanh(const dev_consts &k
   return norm - 2 * k.G * dl - s23 * sY;
 }
 
-// secant iteration of the radial return (plasticity.cuh:85-119)
-__device__ inline double rr_secant(const dev_consts &k, double init, double norm, double eps0, double t, double TC) {
-  const double A = k.jc_A, B = k.jc_B, n = k.jc_n, C = k.jc_C, edr = k.jc_eps_dot_ref, dt = k.dt, G = k.G;
-  const double tol = k.secant_tol;
-  const bool tanh_model = k.jc_tanh != 0;
-  double dl = init, dl_old;
-  int iter = 0;
-  while (true) {
-    double g = tanh_model ? rr_residual_tanh(k, dl, norm, eps0, t, TC)
-                          : rr_residual_jc(dl, norm, eps0, TC, A, B, n, C, edr, dt, G);
-    double dl1 = dl * 1.01;
-    double g1 = tanh_model ? rr_residual_tanh(k, dl1, norm, eps0, t, TC)
-                           : rr_residual_jc(dl1, norm, eps0, TC, A, B, n, C, edr, dt, G);
-    double slope = (g - g1) / (dl - dl1);
-    dl_old = dl;
-    dl = dl - g / slope;
-    if (dl < 0.) dl = -0.1 * dl;
-    if (fabs(dl - dl_old) < tol) return dl;
-    if (iter > 100) return dl;
-    iter++;
-  }
-}
+// Johnson-Cook plasticity by radial return on one particle (actions_gpu.cu:226-285), in three
+// parts: the yield test (plasticity_test), the secant for dl (rr_secant_wave, for all the
+// returning particles of a wave at once), and the return itself (plasticity_finish).
+struct rr_input {
+  double norm, TC, init;  // |S| (S_xy counted once), the temperature factor, the secant's start
+};
 
-// Johnson-Cook plasticity by radial return on one particle (actions_gpu.cu:226-285)
-__device__ inline void plasticity(const dev_consts &k, double p, double rho, double &Sxx, double &Sxy, double &Syy,
-                                  double &Szz, double &eps_pl, double &eps_pl_dot, double &T) {
-  double norm = tsqrt(k, Sxx * Sxx + Syy * Syy + Szz * Szz + Sxy * Sxy);
+// the yield test; false (and eps_pl_dot = 0) for an elastic step
+__device__ inline bool plasticity_test(const dev_consts &k, double p, double Sxx, double Sxy, double Syy, double Szz,
+                                       double eps_pl, double &eps_pl_dot, double T, rr_input &in) {
+  in.norm = tsqrt(k, Sxx * Sxx + Syy * Syy + Szz * Szz + Sxy * Sxy);
   double cxx = Sxx - p, cyy = Syy - p, czz = Szz - p, cxy = Sxy;
   double svm = tsqrt(k, (cxx * cxx + cyy * cyy + czz * czz) - cxx * cyy - cxx * czz - cyy * czz + 3.0 * cxy * cxy);
-  const double TC = jc_temperature_factor(k, T);
-  double sY = sigma_yield(k, eps_pl, eps_pl_dot, TC);
-  if (svm < sY) { eps_pl_dot = 0.; return; }
+  in.TC = jc_temperature_factor(k, T);
+  double sY = sigma_yield(k, eps_pl, eps_pl_dot, in.TC);
+  if (svm < sY) { eps_pl_dot = 0.; return false; }
   const double s23 = sqrt(2.0 / 3.0);
-  const double init = fmax(eps_pl_dot * k.dt * s23, 1e-8);
-  double dl = rr_secant(k, init, norm, eps_pl, T, TC);
+  in.init = fmax(eps_pl_dot * k.dt * s23, 1e-8);
+  return true;
+}
+
+// the return with the secant's dl: stress, plastic strain and rate, plastic heating
+__device__ inline void plasticity_finish(const dev_consts &k, double rho, double dl, const rr_input &in, double &Sxx,
+                                         double &Sxy, double &Syy, double &Szz, double &eps_pl, double &eps_pl_dot,
+                                         double &T) {
+  const double s23 = sqrt(2.0 / 3.0);
+  const double norm = in.norm;
   double inc;
   if (k.promote) inc = s23 * fmax(dl, 0.);
   else inc = (double)(sqrtf((float)(2.0 / 3.0)) * fmaxf((float)dl, 0.f));
@@ -235,9 +227,72 @@ __device__ inline void plasticity(const dev_consts &k, double p, double rho, dou
   eps_pl = e_new; eps_pl_dot = ed_new;
   if (k.tq != 0.) {
     double deps = e_new - e_old;
-    double sY2 = sigma_yield(k, e_new, ed_new, TC);  // T is still the temperature of the return
+    double sY2 = sigma_yield(k, e_new, ed_new, in.TC);  // T is still the temperature of the return
     T += k.tq / (k.cp * rho) * deps * sY2;
   }
+}
+
+// position of the n-th set bit (from 0) of m, which has more than n set bits
+__device__ inline int nth_set_bit(unsigned long long m, int n) {
+  int b = 0;
+#pragma unroll
+  for (int s = 32; s > 0; s >>= 1)
+    if (__popcll(m & ((1ull << (b + s)) - 1)) <= n) b += s;
+  return b;
+}
+
+// The secant iteration of the radial return (plasticity.cuh:85-119) for every returning particle
+// of a wave, two lanes per particle: the lanes 2r and 2r + 1 take the r-th returning particle of
+// the pass (up to 32 per pass) and evaluate the two residuals of an iteration at once, g(dl) and
+// g(1.01 dl), exchanging them by a shuffle; both then take the same secant update (dl - g/slope,
+// negative iterates folded to -0.1 dl, stop when the step is below the tolerance or after 101
+// iterations), so each particle gets the iterates of a one-lane secant, bit for bit, in half the
+// dependent chain. Called by all 64 lanes of the wave together; need marks the lanes whose
+// particle returns, with their inputs. Returns dl on those lanes.
+__device__ inline double rr_secant_wave(const dev_consts &k, bool need, const rr_input &in, double eps0, double t) {
+  const int lane = (int)(threadIdx.x & 63);
+  unsigned long long pending = __ballot(need);
+  double out = 0.;
+  const double A = k.jc_A, B = k.jc_B, n = k.jc_n, C = k.jc_C, edr = k.jc_eps_dot_ref, dt = k.dt, G = k.G;
+  const double tol = k.secant_tol;
+  const bool tanh_model = k.jc_tanh != 0;
+  while (pending) {  // the same in every lane
+    const int cnt = __popcll(pending);
+    const int take = cnt < 32 ? cnt : 32;
+    const unsigned long long pass = take == cnt ? pending : pending & ((1ull << nth_set_bit(pending, take)) - 1);
+    const int slot = lane >> 1;
+    const bool active = slot < take;
+    const int owner = active ? nth_set_bit(pass, slot) : lane;
+    const double norm = __shfl(in.norm, owner, 64), TC = __shfl(in.TC, owner, 64);
+    const double e0 = __shfl(eps0, owner, 64), tt = __shfl(t, owner, 64);
+    double dl = __shfl(in.init, owner, 64);
+    if (active) {
+      const bool second = lane & 1;
+      int iter = 0;
+      while (true) {
+        const double dl1 = dl * 1.01;
+        const double x = second ? dl1 : dl;
+        const double gx = tanh_model ? rr_residual_tanh(k, x, norm, e0, tt, TC)
+                                     : rr_residual_jc(x, norm, e0, TC, A, B, n, C, edr, dt, G);
+        const double gy = __shfl_xor(gx, 1, 64);
+        const double g = second ? gy : gx, g1 = second ? gx : gy;
+        const double slope = (g - g1) / (dl - dl1);
+        const double dl_old = dl;
+        dl = dl - g / slope;
+        if (dl < 0.) dl = -0.1 * dl;
+        if (fabs(dl - dl_old) < tol) break;
+        if (iter > 100) break;
+        iter++;
+      }
+    }
+    // the owner of slot r reads the result of lane 2r
+    const bool mine = (pass >> lane) & 1ull;
+    const int src = mine ? 2 * __popcll(pass & ((1ull << lane) - 1)) : lane;
+    const double got = __shfl(dl, src, 64);
+    if (mine) out = got;
+    pending &= ~pass;
+  }
+  return out;
 }
 
 // ---------------------------------------------------------------- tool contact (tool_gpu.cu)

@@ -529,14 +529,25 @@ __device__ inline void sums_f32(const dev_consts &k, int i, const double2 *__res
   tpse = tp.value();
 }
 
+// a particle's Johnson-Cook return, left for the whole wave's secant after update_particle: the
+// yield test's inputs and the state the return starts from (the stress, plastic strain and rate
+// and temperature that update_particle stored are replaced once the return is done)
+struct deferred_return {
+  bool need;   // the particle returns this step
+  bool fixed;  // the thermal boundary condition resets its temperature after the return
+  rr_input in;
+  double rho, sxx, sxy, syy, szz, eps, epsd, T;
+};
+
 // everything after the neighbour sums for particle i (sorted index), in the reference's order;
-// returns the particle's fc and ft for the tool force
+// returns the particle's fc and ft for the tool force. The secant of a Johnson-Cook return runs
+// for the whole wave after this call (finish_return); the other results do not depend on it
 // reads particle s of the unsorted start state src (s = perm[i]) and writes the updated
 // particle to position i of dst: the gather into sorted order happens here, once
 __device__ inline void update_particle(const dev_consts &k, int i, int s, const pair_acc &acc, double tpse,
                                        const dstate &src, const dstate &dst, const dpositional &pos,
                                        const mfc_tool_state &tool_now, double fsum[4], double &x_out, double &y_out,
-                                       int &fl_out) {
+                                       int &fl_out, deferred_return &dr) {
   const bool pse = k.pse != 0;
   int fl = src.flags[s] & FLAG_MASK;
   const bool blank = fl & MFC_FLAG_BLANKED, tool = fl & MFC_FLAG_TOOL, fixed = fl & MFC_FLAG_FIXED;
@@ -602,7 +613,14 @@ __device__ inline void update_particle(const dev_consts &k, int i, int s, const 
       T += k.eta_heat * k.dt * fmag * vmag / (k.cp * k.mass);
     }
   }
-  if (!tool && k.plastic) plasticity(k, pos.p[i], rho, sxx, sxy, syy, szz, eps, epsd, T);
+  if (!tool && k.plastic) {
+    dr.need = plasticity_test(k, pos.p[i], sxx, sxy, syy, szz, eps, epsd, T, dr.in);
+    if (dr.need) {
+      dr.fixed = fixed;
+      dr.rho = rho; dr.sxx = sxx; dr.sxy = sxy; dr.syy = syy; dr.szz = szz;
+      dr.eps = eps; dr.epsd = epsd; dr.T = T;
+    }
+  }
   if (tool) {  // update_tool_particles (tool_gpu.cu:210-221)
     x += k.dt * k.tool_vx;
     y += k.dt * k.tool_vy;
@@ -632,6 +650,22 @@ __device__ inline void update_particle(const dev_consts &k, int i, int s, const 
   dst.id[i] = src.id[s];
   fsum[0] = fcx; fsum[1] = fcy; fsum[2] = ftx; fsum[3] = fty;
   x_out = x; y_out = y; fl_out = fl;
+}
+
+// the return of a particle that needed one, with the secant's dl: the stress, plastic strain and
+// rate and temperature at sorted index i (the thermal boundary condition after the return, as in
+// update_particle)
+__device__ inline void finish_return(const dev_consts &k, int i, double dl, deferred_return &dr, const dstate &dst) {
+  plasticity_finish(k, dr.rho, dl, dr.in, dr.sxx, dr.sxy, dr.syy, dr.szz, dr.eps, dr.epsd, dr.T);
+  if (k.thermal_bc && dr.fixed) dr.T = k.T_init;  // actions_gpu.cu:287-294
+  dst.sxx[i] = dr.sxx; dst.sxy[i] = dr.sxy; dst.syy[i] = dr.syy; dst.szz[i] = dr.szz;
+  dst.eps[i] = dr.eps; dst.epsd[i] = dr.epsd; dst.T[i] = dr.T;
+}
+
+// the deferred returns of the wave (all 64 lanes together)
+__device__ inline void wave_returns(const dev_consts &k, int i, deferred_return &dr, const dstate &dst) {
+  const double dl = rr_secant_wave(k, dr.need, dr.in, dr.eps, dr.T);
+  if (dr.need) finish_return(k, i, dl, dr, dst);
 }
 
 // the neighbour sums of particle i as pair_acc + PSE sum
@@ -734,13 +768,15 @@ __device__ inline void interact(const dev_consts &k, int t, const drecords &r, c
                                 const int *__restrict__ offsets, const int *__restrict__ perm, const dstate &src,
                                 const dstate &dst, const dpositional &pos,
                                 const mfc_tool_state *__restrict__ tool_now, const dforce &fo, const dbins next,
-                                bool bin_next) {
+                                bool bin_next, double *__restrict__ acc_out) {
   __shared__ stage_buf st;
   const int lo = stage_tile(k, t, r, key_sorted, offsets, st);
   const int i = t * TILE + threadIdx.x;
   double fsum[4] = {0., 0., 0., 0.};
   double xn = 0., yn = 0.;
   int fln = 0;
+  deferred_return dr;
+  dr.need = false;
   if (i < k.N) {
     const int s = perm[i];
     const int fl = src.flags[s] & FLAG_MASK;
@@ -753,8 +789,10 @@ __device__ inline void interact(const dev_consts &k, int t, const drecords &r, c
     else
       particle_sums<F32, COMPACT>(k, i, r.rec, r.posd, r.rec32, key_sorted, offsets, cand_src{r.posd, r.vfl, 0}, fl,
                                   acc, tpse);
-    update_particle(k, i, s, acc, tpse, src, dst, pos, *tool_now, fsum, xn, yn, fln);
+    if (acc_out) store_acc(acc_out, k.N, i, acc, tpse);  // the same in every thread
+    update_particle(k, i, s, acc, tpse, src, dst, pos, *tool_now, fsum, xn, yn, fln, dr);
   }
+  wave_returns(k, i, dr, dst);
   next_cell(k, i, i < k.N, xn, yn, fln, next, bin_next);
   force_partials(i >> 6, fsum, fo.partials);
   if (t == 0 && threadIdx.x == 0) *fo.pending = 1;
@@ -790,12 +828,15 @@ __device__ inline void update(const dev_consts &k, int t, const double *__restri
   double fsum[4] = {0., 0., 0., 0.};
   double xn = 0., yn = 0.;
   int fln = 0;
+  deferred_return dr;
+  dr.need = false;
   if (i < k.N) {
     pair_acc a;
     double tpse;
     load_acc(acc, k.N, i, a, tpse);
-    update_particle(k, i, perm[i], a, tpse, src, dst, pos, *tool_now, fsum, xn, yn, fln);
+    update_particle(k, i, perm[i], a, tpse, src, dst, pos, *tool_now, fsum, xn, yn, fln, dr);
   }
+  wave_returns(k, i, dr, dst);
   next_cell(k, i, i < k.N, xn, yn, fln, next, bin_next);
   force_partials(i >> 6, fsum, fo.partials);
   if (t == 0 && threadIdx.x == 0) *fo.pending = 1;
@@ -1113,7 +1154,7 @@ __global__ __launch_bounds__(TILE) SUMS_OCCUPANCY void kb_interact(const inst_ar
   const dbins b = load_bins(&c->b);
   // an adaptive grid bins the next step's start state after its geometry pass instead
   interact<F32, COMPACT>(a.k, it.y, load_recs(&c->r), b.key_sorted, a.k.rothlin ? b.offsets_r : b.offsets, b.perm,
-                         a.cur, a.nxt, a.pos, a.tool_out, a.f, b, a.bin_next && !a.k.adaptive);
+                         a.cur, a.nxt, a.pos, a.tool_out, a.f, b, a.bin_next && !a.k.adaptive, a.acc_out);
 }
 
 template <bool F32, bool COMPACT>
