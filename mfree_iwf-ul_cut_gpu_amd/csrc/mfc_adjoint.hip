@@ -16,8 +16,8 @@
 //                        per-tile theta partials of the step (kb_adj_theta adds them up)
 //   B2  kb_adj_pull      VJP of the neighbour sums with respect to the records: each particle
 //                        pulls the contributions of the pairs it takes part in over the same
-//                        symmetric 5 x 5 cell window, in a fixed order (no atomics); two passes,
-//                        pairs of the neighbours' sums, then of its own sums
+//                        symmetric 5 x 5 cell window, in a fixed order (no atomics), the pairs
+//                        of the neighbours' sums and of its own sums in one walk
 //   B3  kb_adj_record    VJP of the record and predictor, then the scatter back to the
 //                        storage order of the previous step (inverse of the sort permutation)
 // The positional contact normal (read by the friction heat of the next step at the same sorted
@@ -209,22 +209,23 @@ __global__ __launch_bounds__(TILE) void kb_adj_theta(const adj_args *__restrict_
 }
 
 // ------------------------------------------------------------------ B2: pull over pairs
-// OWN = false: the pairs (a, b) of the sums of a window member a that include b, with the
-// adjoint of a's sums; g_rec[b] is written. OWN = true: the pairs (b, a) of b's own sums, with
-// b's adjoint; added to g_rec[b]. Two passes keep fewer records live per lane than one.
+// For each window member a of particle b, both kinds of pair that link them: the pairs (a, b) of
+// a's sums that include b, with the adjoint of a's sums, and the pairs (b, a) of b's own sums,
+// with b's adjoint; a's record is loaded once for both, and g_rec[b] is written. (Two passes,
+// one per kind, kept fewer records live per lane at the same 2 waves per SIMD and walked every
+// window twice: 71 + 62 us per step against 113 us for the one pass on the 37.5k cut.)
 // PULL_LANES consecutive lanes share a particle: lane r takes the window candidates r,
 // r + PULL_LANES, ... (numbered column by column, as the forward walk numbers them), and the
 // lanes' partial sums are added by a butterfly of shuffles, which gives every lane the same bits
 // ((g0 + g1) + (g2 + g3)). The pass is latency-bound (a long dependent chain of record loads and
 // pair adjoints per particle, 2 waves per SIMD at its register count), so more lanes per
 // particle shorten the chain. The count is fixed, so that an instance's gradient has the same
-// bits alone and in a batch. Measured backward pass, ms per step with 1 / 2 / 4 lanes: 6k
-// particles 0.87 / 0.76 / 0.71, 37.5k 0.60 / 0.55 / 0.52, the 32-instance batch of 577k
-// 2.22 / 2.20 / 2.33. A block takes the PULL_LANES-th part of a tile: grid
+// bits alone and in a batch. Measured backward pass (two pull passes then), ms per step with
+// 1 / 2 / 4 lanes: 6k particles 0.87 / 0.76 / 0.71, 37.5k 0.60 / 0.55 / 0.52, the 32-instance
+// batch of 577k 2.22 / 2.20 / 2.33. A block takes the PULL_LANES-th part of a tile: grid
 // padded(ntiles * PULL_LANES).
 constexpr int PULL_LANES = 4;
 
-template <bool OWN>
 __global__ __launch_bounds__(TILE) void kb_adj_pull(const adj_args *__restrict__ D, const int2 *__restrict__ work,
                                                     int nwork) {
   const int u = xcd_tile(blockIdx.x, nwork * PULL_LANES);
@@ -246,12 +247,10 @@ __global__ __launch_bounds__(TILE) void kb_adj_pull(const adj_args *__restrict__
 #pragma unroll
   for (int q = 0; q < REC_DOUBLES; q++) {
     rb[q] = rec[REC_DOUBLES * (size_t)bb + q];
-    gb[q] = (OWN && live && r == 0) ? g_rec[q * N + bb] : 0.;
+    gb[q] = 0.;
   }
-  if constexpr (OWN) {
 #pragma unroll
-    for (int q = 0; q < L_N; q++) lb[q] = lam_acc[q * N + bb];
-  }
+  for (int q = 0; q < L_N; q++) lb[q] = lam_acc[q * N + bb];
   const int fb = flags[bb];
   const bool blank_b = fb & MFC_FLAG_BLANKED, tool_b = fb & MFC_FLAG_TOOL;
   const double ta_b = pse ? 0. : (tool_b ? k.alpha_tool : k.alpha_wp);
@@ -269,35 +268,28 @@ __global__ __launch_bounds__(TILE) void kb_adj_pull(const adj_args *__restrict__
     const double *pa = rec + REC_DOUBLES * (size_t)a;
     const double dx = pa[R_X] - rb[R_X], dy = pa[R_Y] - rb[R_Y];
     const double rr2 = dx * dx + dy * dy;
-    if constexpr (!OWN) {
-      const double ha = pa[R_H];
-      const bool mech = mech_a && !blank_b && rr2 < 4 * ha * ha && rr2 >= 1e-8;  // a's sums include b
-      if (!mech && !pse_a) continue;
-      double ra[REC_DOUBLES], la[L_N];
+    const double ha = pa[R_H], hb = rb[R_H];
+    const bool mech1 = mech_a && !blank_b && rr2 < 4 * ha * ha && rr2 >= 1e-8;  // a's sums include b
+    const bool mech2 = mech_b && !blank_a && rr2 < 4 * hb * hb && rr2 >= 1e-8;  // b's sums include a
+    if (!mech1 && !pse_a && !mech2 && !pse_b) continue;
+    double ra[REC_DOUBLES];
 #pragma unroll
-      for (int q = 0; q < REC_DOUBLES; q++) ra[q] = pa[q];
+    for (int q = 0; q < REC_DOUBLES; q++) ra[q] = pa[q];
+    if (mech1 || pse_a) {
+      double la[L_N];
 #pragma unroll
       for (int q = 0; q < L_N; q++) la[q] = lam_acc[q * N + a];
       const double ta_a = pse ? 0. : (tool_a ? k.alpha_tool : k.alpha_wp);
-      if (mech) pair_vjp(k, ra, rb, tool_a, tool_b, ta_a, la, false, true, dummy, gb);
+      if (mech1) pair_vjp(k, ra, rb, tool_a, tool_b, ta_a, la, false, true, dummy, gb);
       if (pse_a) pse_vjp(k, ra, rb, la[L_TPSE], false, true, dummy, gb);
-    } else {
-      const double hb = rb[R_H];
-      const bool mech = mech_b && !blank_a && rr2 < 4 * hb * hb && rr2 >= 1e-8;  // b's sums include a
-      if (!mech && !pse_b) continue;
-      double ra[REC_DOUBLES];
-#pragma unroll
-      for (int q = 0; q < REC_DOUBLES; q++) ra[q] = pa[q];
-      if (mech) pair_vjp(k, rb, ra, tool_b, tool_a, ta_b, lb, true, false, gb, dummy);
-      if (pse_b) pse_vjp(k, rb, ra, lb[L_TPSE], true, false, gb, dummy);
     }
+    if (mech2) pair_vjp(k, rb, ra, tool_b, tool_a, ta_b, lb, true, false, gb, dummy);
+    if (pse_b) pse_vjp(k, rb, ra, lb[L_TPSE], true, false, gb, dummy);
   }
-  if constexpr (PULL_LANES > 1) {
 #pragma unroll
-    for (int st = 1; st < PULL_LANES; st <<= 1)
+  for (int st = 1; st < PULL_LANES; st <<= 1)
 #pragma unroll
-      for (int q = 0; q < REC_DOUBLES; q++) gb[q] = gb[q] + __shfl_xor(gb[q], st, 64);
-  }
+    for (int q = 0; q < REC_DOUBLES; q++) gb[q] = gb[q] + __shfl_xor(gb[q], st, 64);
   if (live && r == 0) {
 #pragma unroll
     for (int q = 0; q < REC_DOUBLES; q++) g_rec[q * N + b] = gb[q];
@@ -621,8 +613,7 @@ int adj_backward(mfc_sim *const *sims, int n, const batch_work &fw, hipStream_t 
         const int in_win = t >= w0->w0 && t < w0->w1;
         hipLaunchKernelGGL(kb_adj_tail, gt, dim3(TILE), 0, st, d_da, fw.tiles, fw.ntiles, cur, in_win, t - t0);
         hipLaunchKernelGGL(kb_adj_theta, dim3(n), dim3(TILE), 0, st, d_da);
-        hipLaunchKernelGGL(HIP_KERNEL_NAME(kb_adj_pull<false>), gpull, dim3(TILE), 0, st, d_da, fw.tiles, fw.ntiles);
-        hipLaunchKernelGGL(HIP_KERNEL_NAME(kb_adj_pull<true>), gpull, dim3(TILE), 0, st, d_da, fw.tiles, fw.ntiles);
+        hipLaunchKernelGGL(kb_adj_pull, gpull, dim3(TILE), 0, st, d_da, fw.tiles, fw.ntiles);
         hipLaunchKernelGGL(kb_adj_record, gt, dim3(TILE), 0, st, d_da, fw.tiles, fw.ntiles, cur);
         cur = 1 - cur;
       }
