@@ -579,10 +579,9 @@ int adj_backward(mfc_sim *const *sims, int n, const batch_work &fw, hipStream_t 
       HIP_TRY(hipMemsetAsync(w->lam_n, 0, sizeof(double) * 2 * N, st));
       HIP_TRY(hipMemsetAsync(w->theta_acc, 0, sizeof(double) * 6, st));
       HIP_TRY(hipMemcpyAsync(w->keys_keep, s->bins.key_sorted, sizeof(int) * N, hipMemcpyDeviceToDevice, st));
-      // every binning of the pass counts into a cleared table and leaves it cleared; the
-      // handle's table holds the next step's counts: clear it, and bin the handle's state again
-      // at its next step
-      HIP_TRY(hipMemsetAsync(s->bins.count, 0, sizeof(int) * (size_t)s->k.ncell_table, st));
+      // every binning of the pass counts into cleared chunk sums and leaves them cleared; the
+      // handle's chunk sums hold the next step's counts: clear them, and bin the handle's state
+      // again at its next step
       HIP_TRY(hipMemsetAsync(s->bins.bsum, 0, sizeof(int) * (size_t)scan_blocks(s->k.ncell_table), st));
       s->keys_fresh = false;
     }

@@ -70,7 +70,6 @@ void enqueue_step(mfc_sim *s, int c) { mfc::launch_batch_step(s->work(c), s->str
 
 int ensure_keys(mfc_sim *s) {
   if (s->keys_fresh) return MFC_OK;
-  HIP_TRY(hipMemsetAsync(s->bins.count, 0, sizeof(int) * (size_t)s->k.ncell_table, s->stream));
   HIP_TRY(hipMemsetAsync(s->bins.bsum, 0, sizeof(int) * (size_t)mfc::scan_blocks(s->k.ncell_table), s->stream));
   mfc::launch_batch_hash(s->work(s->cur), s->stream);
   s->keys_fresh = true;
@@ -218,10 +217,12 @@ int mfc_create(const mfc_config *cfg, mfc_sim **out) {
   for (int q = 0; q < 2; q++)
     if ((rc = alloc_state(s, s->set[q], N))) return bail(rc);
   mfc::dbins &b = s->bins;
-  if ((rc = s->alloc(&b.key, N)) || (rc = s->alloc(&b.key_sorted, N)) || (rc = s->alloc(&b.slot, N)) ||
-      (rc = s->alloc(&b.count, c.ncell_table)) || (rc = s->alloc(&b.offsets, (size_t)c.ncell_table + 1)) ||
-      (rc = s->alloc(&b.bucket, N)) || (rc = s->alloc(&b.perm, N)) ||
+  b.ccap = mfc::chunk_list_cap(N);
+  if ((rc = s->alloc(&b.key, N)) || (rc = s->alloc(&b.key_sorted, N)) ||
+      (rc = s->alloc(&b.offsets, (size_t)c.ncell_table + 1)) ||
+      (rc = s->alloc(&b.bucket, N)) || (rc = s->alloc(&b.perm, N)) || (rc = s->alloc(&b.spos, N)) ||
       (rc = s->alloc(&b.bsum, mfc::scan_blocks(c.ncell_table))) || (rc = s->alloc(&b.clamped, 1)) ||
+      (rc = s->alloc(&b.clist, (size_t)mfc::scan_blocks(c.ncell_table) * mfc::chunk_list_cap(N))) ||
       (rc = s->alloc(&s->geo, 1)) || (c.rothlin && (rc = s->alloc(&b.offsets_r, (size_t)c.ncell_table + 1))))
     return bail(rc);
   mfc::dpositional &p = s->pos;
@@ -272,7 +273,6 @@ int mfc_create(const mfc_config *cfg, mfc_sim **out) {
     for (double *a : pz)
       if (up(a, zero.data(), sizeof(double) * N) != hipSuccess) return bail(fail(MFC_E_HIP, "upload"));
     if (hipMemsetAsync(b.key_sorted, 0, sizeof(int) * N, s->stream) != hipSuccess ||
-        hipMemsetAsync(b.count, 0, sizeof(int) * c.ncell_table, s->stream) != hipSuccess ||
         hipMemsetAsync(b.offsets, 0, sizeof(int) * ((size_t)c.ncell_table + 1), s->stream) != hipSuccess ||
         (b.offsets_r && hipMemsetAsync(b.offsets_r, 0, sizeof(int) * ((size_t)c.ncell_table + 1), s->stream) != hipSuccess) ||
         hipMemsetAsync(b.clamped, 0, sizeof(unsigned long long), s->stream) != hipSuccess ||

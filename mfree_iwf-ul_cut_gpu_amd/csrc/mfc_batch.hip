@@ -87,7 +87,6 @@ int batch_keys(mfc_batch *b) {
   for (mfc_sim *s : b->sims) fresh = fresh && s->keys_fresh;
   if (fresh) return MFC_OK;
   for (mfc_sim *s : b->sims) {
-    HIP_TRY(hipMemsetAsync(s->bins.count, 0, sizeof(int) * (size_t)s->k.ncell_table, b->stream));
     HIP_TRY(hipMemsetAsync(s->bins.bsum, 0, sizeof(int) * (size_t)mfc::scan_blocks(s->k.ncell_table), b->stream));
   }
   mfc::launch_batch_hash(b->work(b->cur), b->stream);

@@ -23,12 +23,15 @@ constexpr int DSTATE_DOUBLES = 27;
 struct dbins {
   int *key;          // cell id per particle in the current (unsorted) order
   int *key_sorted;   // cell id per particle in sorted order (= reference's sorted hash)
-  int *slot;         // arrival slot inside the cell (order-free, fixed by the rank kernel)
-  int *count;        // particles per cell, zero between steps
   int *offsets;      // num_cell + 1 exclusive prefix sums: cell c = [offsets[c], offsets[c+1])
   int *bucket;       // particle indices grouped by cell, unordered inside a cell
   int *perm;         // sorted position -> current position
-  int *bsum;         // scan block sums
+  int *spos;         // current position -> sorted position
+  int *bsum;         // particles counted in each scan chunk of cells, zero between steps
+  int2 *clist;       // per scan chunk, (current position, cell id) of the particles counted in
+                     // its cells, in arrival order (ccap entries per chunk; a chunk that counts
+                     // more finds them by a pass over all particles)
+  int ccap;
   int *offsets_r;    // MFC_GRID_ROTHLIN: the ranges rebuilt from the sorted ids (the step reads these)
   unsigned long long *clamped;  // binned particles whose position lay outside the grid on an axis
 };
@@ -117,10 +120,10 @@ struct batch_work {
   bool rothlin;           // some instance uses grid_gpu_rothlin's cell ranges (a pass after predict)
 };
 
-// one step of every instance of a batch: 4 (fused) or 5 (split) launches, and its phases. The
-// cell ids, arrival slots, cell counts and chunk sums of a step's start state are computed by
-// the previous step's interaction; launch_batch_hash computes them from the state itself (counts
-// and chunk sums zeroed before)
+// one step of every instance of a batch: 3 (fused) or 4 (split) launches, and its phases. The
+// cell ids, chunk lists and chunk sums of a step's start state are computed by the previous
+// step's interaction; launch_batch_hash computes them from the state itself (chunk sums zeroed
+// before)
 void launch_batch_step(const batch_work &w, hipStream_t s);
 void launch_batch_hash(const batch_work &w, hipStream_t s);
 void launch_batch_binning(const batch_work &w, hipStream_t s);
@@ -136,6 +139,9 @@ void launch_batch_sums(const batch_work &w, hipStream_t s, hipEvent_t t0 = nullp
 void launch_batch_update(const batch_work &w, hipStream_t s, hipEvent_t t0 = nullptr, hipEvent_t t1 = nullptr);
 void launch_batch_force(const batch_work &w, hipStream_t s);
 int scan_chunk_cells();
+// entries of a scan chunk's particle list: 8 particles per cell of the chunk (a cell holds
+// about 2.25 at the lattice spacing), at most N
+int chunk_list_cap(int N);
 
 // Usui wear rate of state st: out = {min, max, sum, count} over the particles in contact
 // (part: 4 doubles per tile of scratch)

@@ -1,13 +1,15 @@
 // mfc_kernels.hip — gfx950 kernels of one SPH cutting step (FP64 and FP32C).
 //
-// Per step (the reference's ~20 launches of leap_frog::step, leap_frog.cu:94-155, become 4):
-//   binning      scan_down, place: cell offsets and per-cell buckets of a stable counting sort
-//                by cell (thrust::sort_by_key + reorder_data_and_find_cell_start,
+// Per step (the reference's ~20 launches of leap_frog::step, leap_frog.cu:94-155, become 3):
+//   sort         one work-group per chunk of cells: the cells' offsets, buckets and the rank of
+//                every particle in its cell, a stable counting sort by cell
+//                (thrust::sort_by_key + reorder_data_and_find_cell_start,
 //                grid_gpu_green.cu:22-116,177-201) from the cell ids (grid.cu:34-51), per-cell
-//                and per-chunk counts that the previous step's interaction computed for the
-//                updated state (hash_count computes them when the state was written from
-//                outside, and for every state the reverse mode's replay restores)
-//   predict      the sort's rank pass, then leap-frog predict (leap_frog.cu:30-66) + EOS
+//                and per-chunk counts and the chunks' particle lists that the previous step's
+//                interaction computed for the updated state (hash_count computes them when the
+//                state was written from outside, and for every state the reverse mode's replay
+//                restores)
+//   predict      leap-frog predict (leap_frog.cu:30-66) + EOS
 //                (actions_gpu.cu:41-51) + artificial stress (:53-97) + neighbour record + tool
 //                move, in the start state's storage order; the work-group of the first tile
 //                also reduces the previous step's tool force (output.cu:30-39) from its tile
@@ -39,8 +41,9 @@ namespace mfc {
 namespace {
 
 constexpr int SCAN_THREADS = 256;
-constexpr int SCAN_ITEMS = 8;
-constexpr int SCAN_CHUNK = SCAN_THREADS * SCAN_ITEMS;
+constexpr int SCAN_CHUNK = SCAN_THREADS;  // cells per chunk of the sort: one per thread
+constexpr int SORT_BATCH = 4;             // particles per thread the sort holds in registers
+constexpr int CARRY_BATCH = 8;            // chunk sums per thread the sort loads at once
 
 
 // exclusive scan of one int per thread over a 256-thread block; total receives the block sum
@@ -81,102 +84,144 @@ __device__ inline int particle_cell(const dev_consts &k, double x, double y, int
   return h;
 }
 
-// records cell id h of particle i (storage order) and takes its arrival slot in the cell, and
-// counts the particle in its chunk of SCAN_CHUNK cells (bsum, the chunk sums of the offset
-// scan). All 64 lanes of the wave take part (lanes past N carry h = -1). The storage order is
-// the previous step's cell order, so neighbouring lanes mostly share a cell: each run of equal
-// cell ids in the wave takes its slots with one atomic, each run of equal chunks adds its length
-// with one.
-__device__ inline void count_cell(int i, int h, int *__restrict__ key, int *__restrict__ slot,
-                                  int *__restrict__ count, int *__restrict__ bsum) {
+// records cell id h of particle i (storage order) and appends (i, h) to the particle list of
+// its chunk of SCAN_CHUNK cells (bsum counts the chunk's particles; the list keeps the first ccap
+// of them). All 64 lanes of the wave take part (lanes past N carry h = -1). The storage order is
+// the previous step's cell order, so neighbouring lanes mostly share a chunk: each run of equal
+// chunks in the wave takes its list places with one atomic.
+__device__ inline void count_cell(int i, int h, int *__restrict__ key, int *__restrict__ bsum,
+                                  int2 *__restrict__ clist, int ccap) {
   if (h >= 0) key[i] = h;
   const int lane = threadIdx.x & 63;
-  const int prev = __shfl_up(h, 1, 64);
-  const bool head = lane == 0 || prev != h;
-  const unsigned long long heads = __ballot(head);
   const unsigned long long upto = (lane == 63) ? ~0ull : ((2ull << lane) - 1ull);
-  const int my_head = 63 - __clzll(heads & upto);
-  const unsigned long long after = (lane == 63) ? 0ull : (heads & ~upto);
-  const int next_head = after ? __ffsll((long long)after) - 1 : 64;
-  int base = 0;
-  if (head && h >= 0) base = atomicAdd(&count[h], next_head - lane);
-  base = __shfl(base, my_head, 64);
-  if (h >= 0) slot[i] = base + (lane - my_head);
   const int c = h >= 0 ? h / SCAN_CHUNK : -1;
   const int cprev = __shfl_up(c, 1, 64);
-  const unsigned long long cheads = __ballot(lane == 0 || cprev != c);
+  const bool chead = lane == 0 || cprev != c;
+  const unsigned long long cheads = __ballot(chead);
+  const int my_chead = 63 - __clzll(cheads & upto);
   const unsigned long long cafter = (lane == 63) ? 0ull : (cheads & ~upto);
   const int cnext = cafter ? __ffsll((long long)cafter) - 1 : 64;
-  if ((lane == 0 || cprev != c) && c >= 0) atomicAdd(&bsum[c], cnext - lane);
+  int cbase = 0;
+  if (chead && c >= 0) cbase = atomicAdd(&bsum[c], cnext - lane);
+  cbase = __shfl(cbase, my_chead, 64);
+  const int place = cbase + (lane - my_chead);
+  if (c >= 0 && place < ccap) clist[(size_t)c * ccap + place] = make_int2(i, h);
 }
 
-// cell id and arrival slot of every particle of the start state
+// cell id of every particle of the start state, counted in its chunk
 __device__ inline void hash_count(const dev_consts &k, int i, const double *__restrict__ px,
                                   const double *__restrict__ py, const int *__restrict__ flags,
-                                  const int *__restrict__ key_sorted, int *__restrict__ key, int *__restrict__ slot,
-                                  int *__restrict__ count, int *__restrict__ bsum,
-                                  unsigned long long *__restrict__ clamped) {
-  const int h = i < k.N ? particle_cell(k, px[i], py[i], flags[i], key_sorted[i], clamped) : -1;
-  count_cell(i, h, key, slot, count, bsum);
+                                  const dbins &b) {
+  const int h = i < k.N ? particle_cell(k, px[i], py[i], flags[i], b.key_sorted[i], b.clamped) : -1;
+  count_cell(i, h, b.key, b.bsum, b.clist, b.ccap);
 }
 
-// cell offsets from the counts; the counts are cleared for the next step. The offset of chunk
-// blk is the sum of the chunk sums before it, added up here (a few hundred values), so no
-// separate scan of the chunk sums is needed.
-__device__ inline void scan_down(int blk, int *__restrict__ count, int n, const int *__restrict__ bsum,
-                                 int *__restrict__ offsets) {
-  // the thread's SCAN_ITEMS counts (two 16-byte loads where the chunk is whole), loaded before
-  // the chunk sums so that both loads are in flight together
-  const int base = blk * SCAN_CHUNK + threadIdx.x * SCAN_ITEMS;
-  const bool whole = base + SCAN_ITEMS <= n;
-  int v[SCAN_ITEMS];
-  if (whole) {
-    const int4 a = *reinterpret_cast<const int4 *>(count + base);
-    const int4 b = *reinterpret_cast<const int4 *>(count + base + 4);
-    v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w; v[4] = b.x; v[5] = b.y; v[6] = b.z; v[7] = b.w;
-  } else {
+// The counting sort of one chunk of SCAN_CHUNK cells (one per thread), by one work-group, from
+// the chunk's particle list:
+// 1. the particles counted per cell (LDS atomics, which also give each its arrival slot);
+// 2. the cells' offsets (the chunk's offset is the sum of the chunk sums before it, added up
+//    here, so no separate scan of the chunk sums);
+// 3. every particle into its cell's bucket at its arrival slot;
+// 4. each one ranked in its cell by current position, the order thrust::sort_by_key gives on
+//    (cell id, 0..N-1): its sorted position (spos), perm and the sorted cell id there.
+// The chunks' cells, bucket ranges and particles are disjoint, so the work-groups share nothing
+// but the chunk sums they read. A chunk of at most SCAN_THREADS * SORT_BATCH particles (a dense
+// chunk holds about 2.25 per cell at the lattice spacing) keeps them in registers and their
+// buckets in LDS, and its list, chunk sum and preceding chunk sums are loaded at once: the
+// only global loads of the work-group. A larger chunk walks its list (or, when the list
+// overflowed, every particle's cell id) three times and uses the global bucket.
+__device__ inline void sort_chunk(int blk, int N, int n, const dbins &b) {
+  __shared__ int off_l[SCAN_CHUNK], cnt_l[SCAN_CHUNK], fill_l[SCAN_CHUNK];
+  __shared__ int bk_l[SCAN_THREADS * SORT_BATCH];
+  const int c0 = blk * SCAN_CHUNK;
+  const int cell = c0 + threadIdx.x;
+  const int2 *__restrict__ list = b.clist + (size_t)blk * b.ccap;
+  const int fast_cap = b.ccap < SCAN_THREADS * SORT_BATCH ? b.ccap : SCAN_THREADS * SORT_BATCH;
+  // the list's first entries, loaded before the chunk sum says how many are valid
+  int2 e[SORT_BATCH];
 #pragma unroll
-    for (int q = 0; q < SCAN_ITEMS; q++) v[q] = (base + q < n) ? count[base + q] : 0;
+  for (int u = 0; u < SORT_BATCH; u++) {
+    const int p = u * SCAN_THREADS + threadIdx.x;
+    e[u] = p < fast_cap ? list[p] : make_int2(-1, -1);
   }
+  const int members = b.bsum[blk];
+  // the chunk sums before this chunk, CARRY_BATCH per thread in flight at once
   int carry_part = 0;
-  for (int q = threadIdx.x; q < blk; q += SCAN_THREADS) carry_part += bsum[q];
-  int carry;
-  block_exclusive_scan(carry_part, carry);
-  int s = 0;
+  for (int q0 = 0; q0 < blk; q0 += SCAN_THREADS * CARRY_BATCH) {
+    int t[CARRY_BATCH];
 #pragma unroll
-  for (int q = 0; q < SCAN_ITEMS; q++) s += v[q];
-  int total;
-  int pre = block_exclusive_scan(s, total) + carry;
-  int o[SCAN_ITEMS];
+    for (int u = 0; u < CARRY_BATCH; u++) {
+      const int q = q0 + u * SCAN_THREADS + threadIdx.x;
+      t[u] = q < blk ? b.bsum[q] : 0;
+    }
 #pragma unroll
-  for (int q = 0; q < SCAN_ITEMS; q++) {
-    o[q] = pre;
-    pre += v[q];
+    for (int u = 0; u < CARRY_BATCH; u++) carry_part += t[u];
   }
-  if (whole) {
-    *reinterpret_cast<int4 *>(offsets + base) = make_int4(o[0], o[1], o[2], o[3]);
-    *reinterpret_cast<int4 *>(offsets + base + 4) = make_int4(o[4], o[5], o[6], o[7]);
-    *reinterpret_cast<int4 *>(count + base) = make_int4(0, 0, 0, 0);
-    *reinterpret_cast<int4 *>(count + base + 4) = make_int4(0, 0, 0, 0);
-    if (base + SCAN_ITEMS == n) offsets[n] = pre;
-  } else {
+  cnt_l[threadIdx.x] = 0;
+  fill_l[threadIdx.x] = 0;
+  const bool fast = members <= fast_cap;  // the same in every thread
+  const bool listed = members <= b.ccap;
+  const int lim = listed ? members : N;
+  __syncthreads();
+  int ls[SORT_BATCH];
+  if (fast) {
 #pragma unroll
-    for (int q = 0; q < SCAN_ITEMS; q++) {
-      const int c = base + q;
-      if (c < n) {
-        offsets[c] = o[q];
-        count[c] = 0;
-        if (c == n - 1) offsets[n] = o[q] + v[q];
-      }
+    for (int u = 0; u < SORT_BATCH; u++) {
+      const int p = u * SCAN_THREADS + threadIdx.x;
+      if (p >= members) e[u].x = -1;
+      ls[u] = e[u].x >= 0 ? atomicAdd(&cnt_l[e[u].y - c0], 1) : 0;
+    }
+  } else {
+    for (int p = threadIdx.x; p < lim; p += SCAN_THREADS) {
+      const int h = (listed ? list[p].y : b.key[p]) - c0;
+      if (h >= 0 && h < SCAN_CHUNK) atomicAdd(&cnt_l[h], 1);
     }
   }
-}
-
-// bucket of every cell (particles in arrival order)
-__device__ inline void place(int i, int N, const int *__restrict__ key, const int *__restrict__ slot,
-                             const int *__restrict__ offsets, int *__restrict__ bucket) {
-  if (i >= N) return;
-  bucket[offsets[key[i]] + slot[i]] = i;
+  int carry;
+  block_exclusive_scan(carry_part, carry);  // its barriers also complete the counts
+  const int v = cnt_l[threadIdx.x];
+  int total;
+  const int o = block_exclusive_scan(v, total) + carry;
+  off_l[threadIdx.x] = o;
+  if (cell < n) {
+    b.offsets[cell] = o;
+    if (cell == n - 1) b.offsets[n] = o + v;
+  }
+  __syncthreads();  // off_l
+  const int base = off_l[0];
+  // the sorted position of particle i (cell h of the chunk): its rank among the entries of the
+  // cell's bucket (bk, indexed from offset bbase)
+  auto rank = [&](int i, int h, const int *bk, int bbase) {
+    const int cb = off_l[h], ce = cb + cnt_l[h];
+    int rnk = 0;
+    for (int q = cb; q < ce; q++) rnk += bk[q - bbase] < i;
+    const int j = cb + rnk;
+    b.spos[i] = j;
+    b.perm[j] = i;
+    b.key_sorted[j] = h + c0;
+  };
+  if (fast) {
+#pragma unroll
+    for (int u = 0; u < SORT_BATCH; u++)
+      if (e[u].x >= 0) bk_l[off_l[e[u].y - c0] - base + ls[u]] = e[u].x;
+    __syncthreads();  // the chunk's buckets
+#pragma unroll
+    for (int u = 0; u < SORT_BATCH; u++)
+      if (e[u].x >= 0) rank(e[u].x, e[u].y - c0, bk_l, base);
+  } else {
+    int *__restrict__ bucket = b.bucket;
+    for (int p = threadIdx.x; p < lim; p += SCAN_THREADS) {
+      const int2 q = listed ? list[p] : make_int2(p, b.key[p]);
+      const int h = q.y - c0;
+      if (h >= 0 && h < SCAN_CHUNK) bucket[off_l[h] + atomicAdd(&fill_l[h], 1)] = q.x;
+    }
+    __syncthreads();  // the chunk's buckets
+    for (int p = threadIdx.x; p < lim; p += SCAN_THREADS) {
+      const int2 q = listed ? list[p] : make_int2(p, b.key[p]);
+      const int h = q.y - c0;
+      if (h >= 0 && h < SCAN_CHUNK) rank(q.x, h, bucket, 0);
+    }
+  }
 }
 
 
@@ -202,12 +247,10 @@ __device__ inline void artificial_stress_f32(float rho, float p, float Sxx, floa
   Ryy = st2 * q1 + ct2 * q2;
 }
 
-// the sort's last pass, then leap-frog init + predict, EOS, artificial stress and the neighbour
-// record written at the sorted position. Runs in the storage order of the start state (its
-// reads are coalesced). A particle's sorted position is its cell's offset plus the number of
-// particles of the same cell that precede it in the current order (thrust::sort_by_key on
-// (hash, 0..N-1)), counted over the cell's bucket; perm and the sorted cell ids are written
-// there. F32: the record is 16 floats (64 bytes) plus the FP64 predicted position.
+// leap-frog init + predict, EOS, artificial stress and the neighbour record written at the
+// particle's sorted position (from the sort). Runs in the storage order of the start state (its
+// reads are coalesced). F32: the record is 16 floats (64 bytes) plus the FP64 predicted
+// position.
 template <bool F32, bool GATHER>
 __device__ inline void predict(const dev_consts &k, int t, const dbins &bins, const dstate &cur, const dstate &nxt,
                                const dpositional &pos, const drecords &recs,
@@ -223,10 +266,7 @@ __device__ inline void predict(const dev_consts &k, int t, const dbins &bins, co
   }
   const int s = t * TILE + threadIdx.x;
   if (s >= k.N) return;
-  // the cell range of the particle, then the start state (its loads overlap the cell-range
-  // loads), then the rank in the cell's bucket
-  const int hc = bins.key[s];
-  const int cb = bins.offsets[hc], ce = bins.offsets[hc + 1];
+  const int i = bins.spos[s];
   const double x0 = cur.px[s], y0 = cur.py[s], u0 = cur.vx[s], v0 = cur.vy[s];
   const double sxx0 = cur.sxx[s], sxy0 = cur.sxy[s], syy0 = cur.syy[s];
   const double rho0 = cur.rho[s], T0 = cur.T[s];
@@ -235,17 +275,6 @@ __device__ inline void predict(const dev_consts &k, int t, const dbins &bins, co
   const double Tt = cur.Tt[s];
   const int fl = cur.flags[s] & FLAG_MASK;
   const double h = cur.h[s];
-  int rnk = 0;
-  for (int q = cb; q < ce; q += 4) {  // four bucket entries in flight at a time
-    const int b0 = bins.bucket[q];
-    const int b1 = q + 1 < ce ? bins.bucket[q + 1] : s;
-    const int b2 = q + 2 < ce ? bins.bucket[q + 2] : s;
-    const int b3 = q + 3 < ce ? bins.bucket[q + 3] : s;
-    rnk += (b0 < s) + (b1 < s) + (b2 < s) + (b3 < s);
-  }
-  const int i = cb + rnk;
-  bins.perm[i] = s;
-  bins.key_sorted[i] = hc;
   if constexpr (GATHER) {  // the sorted copy of the start state (the reverse mode reads it)
     nxt.px[i] = x0; nxt.py[i] = y0; nxt.vx[i] = u0; nxt.vy[i] = v0;
     nxt.sxx[i] = sxx0; nxt.sxy[i] = sxy0; nxt.syy[i] = syy0; nxt.szz[i] = cur.szz[s];
@@ -759,7 +788,7 @@ __device__ inline void next_cell(const dev_consts &k, int i, bool live, double x
                                  bool bin) {
   if (!bin) return;  // the same in every thread
   const int h = live ? particle_cell(k, x, y, fl, next.key_sorted[i], next.clamped) : -1;
-  count_cell(i, h, next.key, next.slot, next.count, next.bsum);
+  count_cell(i, h, next.key, next.bsum, next.clist, next.ccap);
 }
 
 // fused: sums + update in one pass (registers of both live in one kernel)
@@ -957,8 +986,8 @@ __device__ inline dstate load_state(MFC_CONST(dstate) * c) {
 }
 __device__ inline dbins load_bins(MFC_CONST(dbins) * c) {
   dbins o;
-  MFC_F(key); MFC_F(key_sorted); MFC_F(slot); MFC_F(count); MFC_F(offsets); MFC_F(bucket); MFC_F(perm);
-  MFC_F(bsum); MFC_F(offsets_r); MFC_F(clamped);
+  MFC_F(key); MFC_F(key_sorted); MFC_F(offsets); MFC_F(bucket); MFC_F(perm);
+  MFC_F(spos); MFC_F(bsum); MFC_F(clist); MFC_F(ccap); MFC_F(offsets_r); MFC_F(clamped);
   return o;
 }
 __device__ inline dpositional load_pos(MFC_CONST(dpositional) * c) {
@@ -1006,8 +1035,7 @@ __global__ __launch_bounds__(256) void kb_hash_count(const inst_args *__restrict
   const int2 it = work[w];
   const inst_args a = load_args(A, it.x);
   if (a.k.adaptive) return;  // its step starts with the geometry pass, which bins the state
-  hash_count(a.k, it.y * 256 + threadIdx.x, a.cur.px, a.cur.py, a.cur.flags, a.b.key_sorted, a.b.key,
-             a.b.slot, a.b.count, a.b.bsum, a.b.clamped);
+  hash_count(a.k, it.y * 256 + threadIdx.x, a.cur.px, a.cur.py, a.cur.flags, a.b);
 }
 
 // Adaptive grid (grid_base::update_geometry, grid.cu:97-136), one work-group per instance: the
@@ -1075,28 +1103,16 @@ __global__ __launch_bounds__(GEO_THREADS) void kb_geometry(inst_args *__restrict
   k.bbmin_x = geo_d[0]; k.bbmin_y = geo_d[1]; k.cell = geo_d[2];
   k.gnx = geo_i[0]; k.gny = geo_i[1]; k.num_cell = geo_i[2];
   for (int base = 0; base < N; base += GEO_THREADS)  // every wave runs every round (count_cell)
-    hash_count(k, base + threadIdx.x, a.cur.px, a.cur.py, a.cur.flags, a.b.key_sorted, a.b.key, a.b.slot,
-               a.b.count, a.b.bsum, a.b.clamped);
+    hash_count(k, base + threadIdx.x, a.cur.px, a.cur.py, a.cur.flags, a.b);
 }
 
-__global__ __launch_bounds__(SCAN_THREADS) void kb_scan_down(const inst_args *__restrict__ A,
-                                                              const int2 *__restrict__ work, int nwork) {
+__global__ __launch_bounds__(SCAN_THREADS) void kb_sort(const inst_args *__restrict__ A, const int2 *__restrict__ work,
+                                                         int nwork) {
   const int w = xcd_tile(blockIdx.x, nwork);
   if (w < 0) return;
   const int2 it = work[w];
-  const inst_args a = load_args(A, it.x);
-  scan_down(it.y, a.b.count, a.k.ncell_table, a.b.bsum, a.b.offsets);
-}
-
-__global__ __launch_bounds__(256) void kb_place(const inst_args *__restrict__ A, const int2 *__restrict__ work,
-                                                int nwork) {
-  const int w = xcd_tile(blockIdx.x, nwork);
-  if (w < 0) return;
-  const int2 it = work[w];
-  const inst_args a = load_args(A, it.x);
-  place(it.y * 256 + threadIdx.x, a.k.N, a.b.key, a.b.slot, a.b.offsets, a.b.bucket);
-  // the chunk sums are consumed (scan_down): cleared for the counts of the next step
-  for (int q = it.y * 256 + threadIdx.x; q < a.nscan; q += a.ntiles * 256) a.b.bsum[q] = 0;
+  MFC_CONST(inst_args) *c = const_args(A, it.x);
+  sort_chunk(it.y, A[it.x].k.N, A[it.x].k.ncell_table, load_bins(&c->b));
 }
 
 template <bool F32, bool GATHER>
@@ -1110,8 +1126,10 @@ __global__ __launch_bounds__(TILE) void kb_predict(const inst_args *__restrict__
   // them would take more scalar registers than there are)
   const inst_args &a = A[it.x];
   MFC_CONST(inst_args) *c = const_args(A, it.x);
-  predict<F32, GATHER>(a.k, it.y, load_bins(&c->b), load_state(&c->cur), a.nxt, a.pos, load_recs(&c->r), a.tool_in,
-                       a.tool_out);
+  const dbins b = load_bins(&c->b);
+  predict<F32, GATHER>(a.k, it.y, b, load_state(&c->cur), a.nxt, a.pos, load_recs(&c->r), a.tool_in, a.tool_out);
+  // the chunk sums are consumed (the sort): cleared for the counts of the next step
+  for (int q = it.y * TILE + threadIdx.x; q < a.nscan; q += a.ntiles * TILE) b.bsum[q] = 0;
   // the tool force of the previous step, if its partials are still pending (the first tile's
   // work-group is dispatched first, so the longer kernel hides the reduction)
   if (it.y == 0) force_final(a.ntiles, load_force(&c->f));
@@ -1234,17 +1252,17 @@ void launch_device_math(int fn, const double *x, const double *y, double *out, i
 }
 
 int scan_chunk_cells() { return SCAN_CHUNK; }
+int chunk_list_cap(int N) { return N < 8 * SCAN_CHUNK ? N : 8 * SCAN_CHUNK; }
 
 void launch_batch_hash(const batch_work &w, hipStream_t s) {
   hipLaunchKernelGGL(kb_hash_count, dim3(padded_tiles(w.ntiles)), dim3(256), 0, s, w.args, w.tiles, w.ntiles);
 }
 
 void launch_batch_binning(const batch_work &w, hipStream_t s) {
-  const dim3 gt(padded_tiles(w.ntiles)), gc(padded_tiles(w.nchunks));
+  const dim3 gc(padded_tiles(w.nchunks));
   if (w.adaptive)  // the descriptors are device memory of the batch; the geometry pass updates them
     hipLaunchKernelGGL(kb_geometry, dim3(w.n), dim3(GEO_THREADS), 0, s, const_cast<inst_args *>(w.args));
-  hipLaunchKernelGGL(kb_scan_down, gc, dim3(SCAN_THREADS), 0, s, w.args, w.chunks, w.nchunks);
-  hipLaunchKernelGGL(kb_place, gt, dim3(256), 0, s, w.args, w.tiles, w.ntiles);
+  hipLaunchKernelGGL(kb_sort, gc, dim3(SCAN_THREADS), 0, s, w.args, w.chunks, w.nchunks);
 }
 
 void launch_batch_predict(const batch_work &w, hipStream_t s) {

@@ -66,7 +66,7 @@ struct mfc_sim {
   bool use_graphs = true;
   mfc::adj_work *adj = nullptr;    // reverse-mode tape of the last mfc_vjp_forward
   bool batched = false;            // instance of a batch: stepped by mfc_batch_step only
-  // bins.key/slot/count hold the binning of the current state (computed by the last step's
+  // bins.key/bsum/clist hold the binning of the current state (computed by the last step's
   // interaction); false after creation and after anything else wrote the state or the bins
   bool keys_fresh = false;
   double *acc = nullptr;           // neighbour sums between the split interaction kernels
@@ -109,7 +109,7 @@ void carve_positional(void *base, int N, mfc::dpositional &p);
 // steps per captured hipGraph (even: a graph starts and ends on the same buffer parity)
 constexpr int GRAPH_STEPS = 8;
 void enqueue_step(mfc_sim *s, int c);
-// bins.key/slot/count of the current state, recomputed when keys_fresh is false
+// bins.key/bsum/clist of the current state, recomputed when keys_fresh is false
 int ensure_keys(mfc_sim *s);
 void finish_steps(mfc_sim *s);
 int check_tuning(const mfc_tuning *t);
