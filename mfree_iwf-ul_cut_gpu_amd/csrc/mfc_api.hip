@@ -221,7 +221,7 @@ int mfc_create(const mfc_config *cfg, mfc_sim **out) {
   if ((rc = s->alloc(&b.key, N)) || (rc = s->alloc(&b.key_sorted, N)) ||
       (rc = s->alloc(&b.offsets, (size_t)c.ncell_table + 1)) ||
       (rc = s->alloc(&b.bucket, N)) || (rc = s->alloc(&b.perm, N)) || (rc = s->alloc(&b.spos, N)) ||
-      (rc = s->alloc(&b.bsum, mfc::scan_blocks(c.ncell_table))) || (rc = s->alloc(&b.clamped, 1)) ||
+      (rc = s->alloc(&b.bsum, (mfc::scan_blocks(c.ncell_table) + 3) & ~3)) || (rc = s->alloc(&b.clamped, 1)) ||
       (rc = s->alloc(&b.clist, (size_t)mfc::scan_blocks(c.ncell_table) * mfc::chunk_list_cap(N))) ||
       (rc = s->alloc(&s->geo, 1)) || (c.rothlin && (rc = s->alloc(&b.offsets_r, (size_t)c.ncell_table + 1))))
     return bail(rc);

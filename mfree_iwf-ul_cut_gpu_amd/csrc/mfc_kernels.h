@@ -139,7 +139,7 @@ void launch_batch_sums(const batch_work &w, hipStream_t s, hipEvent_t t0 = nullp
 void launch_batch_update(const batch_work &w, hipStream_t s, hipEvent_t t0 = nullptr, hipEvent_t t1 = nullptr);
 void launch_batch_force(const batch_work &w, hipStream_t s);
 int scan_chunk_cells();
-// entries of a scan chunk's particle list: 8 particles per cell of the chunk (a cell holds
+// entries of a scan chunk's particle list: 4 particles per cell of the chunk (a cell holds
 // about 2.25 at the lattice spacing), at most N
 int chunk_list_cap(int N);
 

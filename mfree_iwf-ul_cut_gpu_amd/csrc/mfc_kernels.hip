@@ -41,9 +41,11 @@ namespace mfc {
 namespace {
 
 constexpr int SCAN_THREADS = 256;
-constexpr int SCAN_CHUNK = SCAN_THREADS;  // cells per chunk of the sort: one per thread
-constexpr int SORT_BATCH = 4;             // particles per thread the sort holds in registers
-constexpr int CARRY_BATCH = 8;            // chunk sums per thread the sort loads at once
+constexpr int SCAN_ITEMS = 4;                         // cells per thread of the sort
+constexpr int SCAN_CHUNK = SCAN_THREADS * SCAN_ITEMS;  // cells per chunk of the sort
+constexpr int SORT_BATCH = 12;            // particles per thread the sort holds in registers
+constexpr int CARRY_BATCH = 4;            // 16-byte loads of chunk sums per thread the sort
+                                          // has in flight at once
 
 
 // exclusive scan of one int per thread over a 256-thread block; total receives the block sum
@@ -116,8 +118,8 @@ __device__ inline void hash_count(const dev_consts &k, int i, const double *__re
   count_cell(i, h, b.key, b.bsum, b.clist, b.ccap);
 }
 
-// The counting sort of one chunk of SCAN_CHUNK cells (one per thread), by one work-group, from
-// the chunk's particle list:
+// The counting sort of one chunk of SCAN_CHUNK cells (SCAN_ITEMS per thread), by one work-group,
+// from the chunk's particle list:
 // 1. the particles counted per cell (LDS atomics, which also give each its arrival slot);
 // 2. the cells' offsets (the chunk's offset is the sum of the chunk sums before it, added up
 //    here, so no separate scan of the chunk sums);
@@ -129,12 +131,13 @@ __device__ inline void hash_count(const dev_consts &k, int i, const double *__re
 // chunk holds about 2.25 per cell at the lattice spacing) keeps them in registers and their
 // buckets in LDS, and its list, chunk sum and preceding chunk sums are loaded at once: the
 // only global loads of the work-group. A larger chunk walks its list (or, when the list
-// overflowed, every particle's cell id) three times and uses the global bucket.
+// overflowed, every particle's cell id) three times and uses the global bucket. Several cells
+// per thread keep the work-groups of a large table resident at once (a 500k-particle cut has
+// 1.18 million cells).
 __device__ inline void sort_chunk(int blk, int N, int n, const dbins &b) {
   __shared__ int off_l[SCAN_CHUNK], cnt_l[SCAN_CHUNK], fill_l[SCAN_CHUNK];
   __shared__ int bk_l[SCAN_THREADS * SORT_BATCH];
   const int c0 = blk * SCAN_CHUNK;
-  const int cell = c0 + threadIdx.x;
   const int2 *__restrict__ list = b.clist + (size_t)blk * b.ccap;
   const int fast_cap = b.ccap < SCAN_THREADS * SORT_BATCH ? b.ccap : SCAN_THREADS * SORT_BATCH;
   // the list's first entries, loaded before the chunk sum says how many are valid
@@ -145,20 +148,29 @@ __device__ inline void sort_chunk(int blk, int N, int n, const dbins &b) {
     e[u] = p < fast_cap ? list[p] : make_int2(-1, -1);
   }
   const int members = b.bsum[blk];
-  // the chunk sums before this chunk, CARRY_BATCH per thread in flight at once
+  // the chunk sums before this chunk, four per load (the sums are allocated in whole
+  // 16-byte words), CARRY_BATCH loads per thread in flight at once
+  const int4 *__restrict__ bsum4 = reinterpret_cast<const int4 *>(b.bsum);
   int carry_part = 0;
-  for (int q0 = 0; q0 < blk; q0 += SCAN_THREADS * CARRY_BATCH) {
-    int t[CARRY_BATCH];
+  for (int q0 = 0; q0 < blk; q0 += 4 * SCAN_THREADS * CARRY_BATCH) {
+    int4 t[CARRY_BATCH];
 #pragma unroll
     for (int u = 0; u < CARRY_BATCH; u++) {
-      const int q = q0 + u * SCAN_THREADS + threadIdx.x;
-      t[u] = q < blk ? b.bsum[q] : 0;
+      const int q = q0 + 4 * (u * SCAN_THREADS + (int)threadIdx.x);
+      t[u] = q < blk ? bsum4[q >> 2] : make_int4(0, 0, 0, 0);
     }
 #pragma unroll
-    for (int u = 0; u < CARRY_BATCH; u++) carry_part += t[u];
+    for (int u = 0; u < CARRY_BATCH; u++) {
+      const int q = q0 + 4 * (u * SCAN_THREADS + (int)threadIdx.x);
+      carry_part += (q < blk ? t[u].x : 0) + (q + 1 < blk ? t[u].y : 0) + (q + 2 < blk ? t[u].z : 0) +
+                    (q + 3 < blk ? t[u].w : 0);
+    }
   }
-  cnt_l[threadIdx.x] = 0;
-  fill_l[threadIdx.x] = 0;
+#pragma unroll
+  for (int q = 0; q < SCAN_ITEMS; q++) {
+    cnt_l[q * SCAN_THREADS + threadIdx.x] = 0;
+    fill_l[q * SCAN_THREADS + threadIdx.x] = 0;
+  }
   const bool fast = members <= fast_cap;  // the same in every thread
   const bool listed = members <= b.ccap;
   const int lim = listed ? members : N;
@@ -179,13 +191,25 @@ __device__ inline void sort_chunk(int blk, int N, int n, const dbins &b) {
   }
   int carry;
   block_exclusive_scan(carry_part, carry);  // its barriers also complete the counts
-  const int v = cnt_l[threadIdx.x];
+  // the thread's SCAN_ITEMS consecutive cells
+  const int l0 = threadIdx.x * SCAN_ITEMS;
+  int v[SCAN_ITEMS], sv = 0;
+#pragma unroll
+  for (int q = 0; q < SCAN_ITEMS; q++) {
+    v[q] = cnt_l[l0 + q];
+    sv += v[q];
+  }
   int total;
-  const int o = block_exclusive_scan(v, total) + carry;
-  off_l[threadIdx.x] = o;
-  if (cell < n) {
-    b.offsets[cell] = o;
-    if (cell == n - 1) b.offsets[n] = o + v;
+  int o = block_exclusive_scan(sv, total) + carry;
+#pragma unroll
+  for (int q = 0; q < SCAN_ITEMS; q++) {
+    const int cell = c0 + l0 + q;
+    off_l[l0 + q] = o;
+    if (cell < n) {
+      b.offsets[cell] = o;
+      if (cell == n - 1) b.offsets[n] = o + v[q];
+    }
+    o += v[q];
   }
   __syncthreads();  // off_l
   const int base = off_l[0];
@@ -1252,7 +1276,7 @@ void launch_device_math(int fn, const double *x, const double *y, double *out, i
 }
 
 int scan_chunk_cells() { return SCAN_CHUNK; }
-int chunk_list_cap(int N) { return N < 8 * SCAN_CHUNK ? N : 8 * SCAN_CHUNK; }
+int chunk_list_cap(int N) { return N < 4 * SCAN_CHUNK ? N : 4 * SCAN_CHUNK; }
 
 void launch_batch_hash(const batch_work &w, hipStream_t s) {
   hipLaunchKernelGGL(kb_hash_count, dim3(padded_tiles(w.ntiles)), dim3(256), 0, s, w.args, w.tiles, w.ntiles);
