@@ -53,10 +53,10 @@ assert sum(KERNEL_BYTES.values()) == STEP_BYTES
 # --pmc FETCH_SIZE / WRITE_SIZE passes and scaled by a 512 MiB copy (the commands and tables are
 # in profiles/traffic_thermal_contact_150k.md); a table of measurements, not measured in this
 # run, and reported as `traffic` only for the tuning it was measured with
-MEASURED_TRAFFIC = {("kb_interact", "fp32c", 0, 8): 662, ("kb_interact", "fp32c", 0, 32): 612,
-                    ("kb_interact", "fp32c", 1, 8): 648, ("kb_interact", "fp32c", 1, 32): 600,
-                    ("kb_interact", "fp64", 0, 8): 831, ("kb_interact", "fp64", 0, 32): 741,
-                    ("kb_interact", "fp64", 1, 8): 812, ("kb_interact", "fp64", 1, 32): 694}
+MEASURED_TRAFFIC = {("kb_interact", "fp32c", 0, 8): 664, ("kb_interact", "fp32c", 0, 32): 612,
+                    ("kb_interact", "fp32c", 1, 8): 647, ("kb_interact", "fp32c", 1, 32): 599,
+                    ("kb_interact", "fp64", 0, 8): 826, ("kb_interact", "fp64", 0, 32): 743,
+                    ("kb_interact", "fp64", 1, 8): 794, ("kb_interact", "fp64", 1, 32): 691}
 MEASURED_TRAFFIC_SOURCE = ("table: rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, thermal_contact_150k, fused interaction, "
                            "profiles/traffic_thermal_contact_150k.md")
 
