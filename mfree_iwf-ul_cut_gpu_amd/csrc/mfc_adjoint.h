@@ -435,6 +435,8 @@ struct tail_in {
   int flags;
   bool pse_flag_zero;  // PSE tool-flag quirk applies to this index
   double acc[L_N];   // neighbour sums of this step
+  double dl_rec = 0.;  // the forward's secant result for this step (-dl: its last update
+                       // folded); 0: run the secant here
 };
 
 struct tail_grad {
@@ -534,17 +536,22 @@ MFC_HD void tail_vjp(const dev_consts &k, const mfc_tool_state &tool, const tail
       plastic = true;
       double x = fmax(s[TS_EPSD] * dt * s23, 1e-8), xo;
       int it = 0;
-      while (true) {
-        const double g0 = norm - 2 * G * x - s23 * sigma_yield_value(k, e0 + s23 * fmax(x, 0.), s23 * fmax(x, 0.) / dt, TCf);
-        const double x1 = x * 1.01;
-        const double g1 = norm - 2 * G * x1 - s23 * sigma_yield_value(k, e0 + s23 * fmax(x1, 0.), s23 * fmax(x1, 0.) / dt, TCf);
-        const double slope = (g0 - g1) / (x - x1);
-        xo = x;
-        x = x - g0 / slope;
-        folded = false;
-        if (x < 0.) { x = -0.1 * x; folded = true; }
-        if (fabs(x - xo) < k.secant_tol || it > 100) break;
-        it++;
+      if (in.dl_rec != 0.) {  // the iterates of the forward's own secant, ended as recorded
+        x = fabs(in.dl_rec);
+        folded = in.dl_rec < 0.;
+      } else {
+        while (true) {
+          const double g0 = norm - 2 * G * x - s23 * sigma_yield_value(k, e0 + s23 * fmax(x, 0.), s23 * fmax(x, 0.) / dt, TCf);
+          const double x1 = x * 1.01;
+          const double g1 = norm - 2 * G * x1 - s23 * sigma_yield_value(k, e0 + s23 * fmax(x1, 0.), s23 * fmax(x1, 0.) / dt, TCf);
+          const double slope = (g0 - g1) / (x - x1);
+          xo = x;
+          x = x - g0 / slope;
+          folded = false;
+          if (x < 0.) { x = -0.1 * x; folded = true; }
+          if (fabs(x - xo) < k.secant_tol || it > 100) break;
+          it++;
+        }
       }
       dl = x;
       const double inc = s23 * fmax(dl, 0.);

@@ -59,6 +59,8 @@ namespace {
 
 int padded(int ntiles) { return ((ntiles + 7) / 8) * 8; }
 
+static_assert(L_N == ACC_N, "a replay slot holds the interaction's neighbour sums in the tail's order");
+
 // one instance of the backward pass as its kernels see it (device memory, one per instance)
 struct adj_args {
   dev_consts k;
@@ -69,7 +71,8 @@ struct adj_args {
   const double *rec;                // FP64 neighbour records
   const int *key_sorted, *offsets, *perm;
   const mfc_tool_state *tool_now;   // the tool after the reversed step's move
-  double *acc_seg;                  // the replay's neighbour sums, one slot of ACC_N x N per step
+  double *acc_seg;                  // the replay's neighbour sums and secant results, one slot of
+                                    // (ACC_N + 1) x N per step
   size_t acc_stride;                // doubles per slot
   double *lam_acc, *lam_s[2], *lam_n, *g_s, *g_rec, *theta_part, *theta_acc;
   double Fb[2];                     // cotangent over the window length
@@ -154,6 +157,7 @@ __global__ __launch_bounds__(TILE) void kb_adj_tail(const adj_args *__restrict__
     in.pse_flag_zero = k.pse_quirk && i >= k.N / 2;
 #pragma unroll
     for (int q = 0; q < L_N; q++) in.acc[q] = acc[q * N + i];
+    in.dl_rec = acc[L_N * N + i];
     const double *lam_s = d.lam_s[cur];
     double ob[TS_N];
 #pragma unroll
@@ -406,7 +410,7 @@ int setup_work(mfc_sim *s, int nsteps, int K, int w0, int w1) {
         (rc = w->alloc(&w->partials, 16 * (size_t)s->ntiles)) || (rc = w->alloc(&w->force_last, 2)) ||
         (rc = w->alloc(&w->hist, 2)) || (rc = w->alloc(&w->ctr, 1)) || (rc = w->alloc(&w->pending, 1)) ||
         (rc = w->alloc(&w->keys_keep, n)) ||
-        (rc = w->alloc(&w->acc_seg, (size_t)K * L_N * n)) || (rc = w->alloc(&w->lam_acc, L_N * n)) ||
+        (rc = w->alloc(&w->acc_seg, (size_t)K * (L_N + 1) * n)) || (rc = w->alloc(&w->lam_acc, L_N * n)) ||
         (rc = w->alloc(&w->lam_s[0], TS_N * n)) || (rc = w->alloc(&w->lam_s[1], TS_N * n)) ||
         (rc = w->alloc(&w->lam_n, 2 * n)) || (rc = w->alloc(&w->g_s, TS_N * n)) ||
         (rc = w->alloc(&w->g_rec, REC_DOUBLES * n)) || (rc = w->alloc(&w->theta_part, 6 * (size_t)s->ntiles)) ||
@@ -528,7 +532,7 @@ int adj_backward(mfc_sim *const *sims, int n, const batch_work &fw, hipStream_t 
     d.perm = s->bins.perm;
     d.tool_now = w->toolw + 1;
     d.acc_seg = w->acc_seg;
-    d.acc_stride = (size_t)L_N * N;
+    d.acc_stride = (size_t)(L_N + 1) * N;
     d.lam_acc = w->lam_acc;
     d.lam_s[0] = w->lam_s[0];
     d.lam_s[1] = w->lam_s[1];

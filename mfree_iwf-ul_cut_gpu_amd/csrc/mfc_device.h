@@ -248,11 +248,14 @@ __device__ inline int nth_set_bit(unsigned long long m, int n) {
 // negative iterates folded to -0.1 dl, stop when the step is below the tolerance or after 101
 // iterations), so each particle gets the iterates of a one-lane secant, bit for bit, in half the
 // dependent chain. Called by all 64 lanes of the wave together; need marks the lanes whose
-// particle returns, with their inputs. Returns dl on those lanes.
-__device__ inline double rr_secant_wave(const dev_consts &k, bool need, const rr_input &in, double eps0, double t) {
+// particle returns, with their inputs. Returns dl on those lanes, and in folded whether the last
+// update took the fold branch (the reverse mode differentiates the fold's fixed point there).
+__device__ inline double rr_secant_wave(const dev_consts &k, bool need, const rr_input &in, double eps0, double t,
+                                        bool &folded) {
   const int lane = (int)(threadIdx.x & 63);
   unsigned long long pending = __ballot(need);
   double out = 0.;
+  folded = false;
   const double A = k.jc_A, B = k.jc_B, n = k.jc_n, C = k.jc_C, edr = k.jc_eps_dot_ref, dt = k.dt, G = k.G;
   const double tol = k.secant_tol;
   const bool tanh_model = k.jc_tanh != 0;
@@ -266,6 +269,7 @@ __device__ inline double rr_secant_wave(const dev_consts &k, bool need, const rr
     const double norm = __shfl(in.norm, owner, 64), TC = __shfl(in.TC, owner, 64);
     const double e0 = __shfl(eps0, owner, 64), tt = __shfl(t, owner, 64);
     double dl = __shfl(in.init, owner, 64);
+    int fold = 0;
     if (active) {
       const bool second = lane & 1;
       int iter = 0;
@@ -279,7 +283,8 @@ __device__ inline double rr_secant_wave(const dev_consts &k, bool need, const rr
         const double slope = (g - g1) / (dl - dl1);
         const double dl_old = dl;
         dl = dl - g / slope;
-        if (dl < 0.) dl = -0.1 * dl;
+        fold = dl < 0.;
+        if (fold) dl = -0.1 * dl;
         if (fabs(dl - dl_old) < tol) break;
         if (iter > 100) break;
         iter++;
@@ -289,7 +294,11 @@ __device__ inline double rr_secant_wave(const dev_consts &k, bool need, const rr
     const bool mine = (pass >> lane) & 1ull;
     const int src = mine ? 2 * __popcll(pass & ((1ull << lane) - 1)) : lane;
     const double got = __shfl(dl, src, 64);
-    if (mine) out = got;
+    const int got_fold = __shfl(fold, src, 64);
+    if (mine) {
+      out = got;
+      folded = got_fold != 0;
+    }
     pending &= ~pass;
   }
   return out;

@@ -97,8 +97,9 @@ struct inst_args {
   mfc_tool_state *tool_out;
   dforce f;
   double *acc;  // split interaction only
-  double *acc_out;  // when set, the fused interaction also stores its neighbour sums there
-                    // (ACC_N arrays of N doubles; the reverse mode's replay keeps them)
+  double *acc_out;  // when set, the interaction also stores its neighbour sums there (ACC_N
+                    // arrays of N doubles), then each particle's secant result (N doubles,
+                    // wave_returns); the reverse mode's replay keeps them
   grid_geo *geo;  // adaptive grid: the geometry of the current step (written by the device)
   int ntiles;   // particle tiles of TILE
   int nscan;    // scan chunks of the cell table

@@ -715,10 +715,15 @@ __device__ inline void finish_return(const dev_consts &k, int i, double dl, defe
   dst.eps[i] = dr.eps; dst.epsd[i] = dr.epsd; dst.T[i] = dr.T;
 }
 
-// the deferred returns of the wave (all 64 lanes together)
-__device__ inline void wave_returns(const dev_consts &k, int i, deferred_return &dr, const dstate &dst) {
-  const double dl = rr_secant_wave(k, dr.need, dr.in, dr.eps, dr.T);
+// the deferred returns of the wave (all 64 lanes together); dl_out (the reverse mode's replay,
+// the same in every thread): each particle's secant result, -dl where its last update folded,
+// 0 for no return
+__device__ inline void wave_returns(const dev_consts &k, int i, deferred_return &dr, const dstate &dst,
+                                    double *__restrict__ dl_out) {
+  bool folded;
+  const double dl = rr_secant_wave(k, dr.need, dr.in, dr.eps, dr.T, folded);
   if (dr.need) finish_return(k, i, dl, dr, dst);
+  if (dl_out && i < k.N) dl_out[i] = dr.need ? (folded ? -dl : dl) : 0.;
 }
 
 // the neighbour sums of particle i as pair_acc + PSE sum
@@ -845,7 +850,7 @@ __device__ inline void interact(const dev_consts &k, int t, const drecords &r, c
     if (acc_out) store_acc(acc_out, k.N, i, acc, tpse);  // the same in every thread
     update_particle(k, i, s, acc, tpse, src, dst, pos, *tool_now, fsum, xn, yn, fln, dr);
   }
-  wave_returns(k, i, dr, dst);
+  wave_returns(k, i, dr, dst, acc_out ? acc_out + (size_t)ACC_N * k.N : nullptr);
   next_cell(k, i, i < k.N, xn, yn, fln, next, bin_next);
   force_partials(i >> 6, fsum, fo.partials);
   if (t == 0 && threadIdx.x == 0) *fo.pending = 1;
@@ -872,11 +877,11 @@ __device__ inline void sums(const dev_consts &k, int t, const drecords &r, const
   store_acc(acc, k.N, i, a, tpse);
 }
 
-// split, second half: the per-particle update from the stored sums
+// split, second half: the per-particle update from the stored sums (dl_out: as wave_returns)
 __device__ inline void update(const dev_consts &k, int t, const double *__restrict__ acc,
                               const int *__restrict__ perm, const dstate &src, const dstate &dst,
                               const dpositional &pos, const mfc_tool_state *__restrict__ tool_now, const dforce &fo,
-                              const dbins next, bool bin_next) {
+                              const dbins next, bool bin_next, double *__restrict__ dl_out) {
   const int i = t * TILE + threadIdx.x;
   double fsum[4] = {0., 0., 0., 0.};
   double xn = 0., yn = 0.;
@@ -889,7 +894,7 @@ __device__ inline void update(const dev_consts &k, int t, const double *__restri
     load_acc(acc, k.N, i, a, tpse);
     update_particle(k, i, perm[i], a, tpse, src, dst, pos, *tool_now, fsum, xn, yn, fln, dr);
   }
-  wave_returns(k, i, dr, dst);
+  wave_returns(k, i, dr, dst, dl_out);
   next_cell(k, i, i < k.N, xn, yn, fln, next, bin_next);
   force_partials(i >> 6, fsum, fo.partials);
   if (t == 0 && threadIdx.x == 0) *fo.pending = 1;
@@ -1220,7 +1225,8 @@ __global__ __launch_bounds__(TILE) void kb_update(const inst_args *__restrict__ 
   const inst_args &a = A[it.x];
   MFC_CONST(inst_args) *c = const_args(A, it.x);
   const dbins b = load_bins(&c->b);
-  update(a.k, it.y, c->acc, b.perm, a.cur, a.nxt, a.pos, a.tool_out, a.f, b, a.bin_next && !a.k.adaptive);
+  update(a.k, it.y, c->acc, b.perm, a.cur, a.nxt, a.pos, a.tool_out, a.f, b, a.bin_next && !a.k.adaptive,
+         a.acc_out ? a.acc_out + (size_t)ACC_N * a.k.N : nullptr);
 }
 
 __global__ __launch_bounds__(TILE) void kb_force_final(const inst_args *__restrict__ A) {
