@@ -49,14 +49,18 @@ assert sum(KERNEL_BYTES.values()) == STEP_BYTES
 # the split variant: the sums kernel reads nothing compulsory beyond what the interaction kernel
 # is credited with; its 288 B are credited to the pair (sums + update)
 # HBM bytes per particle of one launch of the fused interaction kernel on thermal_contact_150k,
-# by precision, pair loop and XCD chunk length (full LDS stage), measured with rocprofv3
-# --pmc FETCH_SIZE / WRITE_SIZE passes and scaled by a 512 MiB copy (the commands and tables are
-# in profiles/traffic_thermal_contact_150k.md); a table of measurements, not measured in this
-# run, and reported as `traffic` only for the tuning it was measured with
-MEASURED_TRAFFIC = {("kb_interact", "fp32c", 0, 8): 664, ("kb_interact", "fp32c", 0, 32): 612,
-                    ("kb_interact", "fp32c", 1, 8): 647, ("kb_interact", "fp32c", 1, 32): 599,
-                    ("kb_interact", "fp64", 0, 8): 826, ("kb_interact", "fp64", 0, 32): 743,
-                    ("kb_interact", "fp64", 1, 8): 794, ("kb_interact", "fp64", 1, 32): 691}
+# by precision, pair loop, XCD chunk length and LDS stage limit (0: the full 2048 candidates),
+# measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE passes and scaled by a 512 MiB copy (the
+# commands and tables are in profiles/traffic_thermal_contact_150k.md); a table of measurements,
+# not measured in this run, and reported as `traffic` only for the tuning it was measured with
+MEASURED_TRAFFIC = {("kb_interact", "fp32c", 0, 8, 0): 664, ("kb_interact", "fp32c", 0, 32, 0): 612,
+                    ("kb_interact", "fp32c", 1, 8, 0): 647, ("kb_interact", "fp32c", 1, 32, 0): 599,
+                    ("kb_interact", "fp64", 0, 8, 0): 826, ("kb_interact", "fp64", 0, 32, 0): 743,
+                    ("kb_interact", "fp64", 1, 8, 0): 794, ("kb_interact", "fp64", 1, 32, 0): 691,
+                    ("kb_interact", "fp32c", 0, 8, 1536): 669, ("kb_interact", "fp32c", 0, 32, 1536): 615,
+                    ("kb_interact", "fp32c", 1, 8, 1536): 656, ("kb_interact", "fp32c", 1, 32, 1536): 600,
+                    ("kb_interact", "fp64", 0, 8, 1536): 846, ("kb_interact", "fp64", 0, 32, 1536): 748,
+                    ("kb_interact", "fp64", 1, 8, 1536): 826, ("kb_interact", "fp64", 1, 32, 1536): 695}
 MEASURED_TRAFFIC_SOURCE = ("table: rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, thermal_contact_150k, fused interaction, "
                            "profiles/traffic_thermal_contact_150k.md")
 
@@ -107,14 +111,16 @@ def tune(pymfc, handle, steps: int) -> dict:
 def run_path(pymfc, workload: str, precision: int, steps: int, warmup: int, profile_steps: int,
              tune_steps: int) -> dict:
     """The timed steps are steps warmup + 16 .. warmup + 16 + steps of the run whatever the tuning:
-    the autotuner runs on a handle of its own (after the same warm-up), and the measured handle
-    takes its choice, runs warmup steps and 2 * GRAPH_STEPS more (its hipGraphs are captured
-    there), then the timed steps. The cost of a step changes along the cut as the chip forms."""
+    the autotuner runs on a handle of its own, advanced to the middle of the timed window (the
+    cost of a step, and which launch settings are fastest, change along the cut as the chip forms
+    and more particles yield: the tuner's candidates then run through the window's second half),
+    and the measured handle takes its choice, runs warmup steps and 2 * GRAPH_STEPS more (its
+    hipGraphs are captured there), then the timed steps."""
     cfg = pymfc.workload(workload, precision)
     tuning = {"ms_per_step": {}}
     if tune_steps > 0:
         t = pymfc.Sim(cfg)
-        t.step(warmup)
+        t.step(warmup + 2 * pymfc.GRAPH_STEPS + steps // 2)
         tuning = tune(pymfc, t, tune_steps)
         t.close()
     else:
@@ -187,14 +193,15 @@ def adjoint_ratio(pymfc, workload: str, nsteps: int, K: int) -> dict:
 def batch_bench(pymfc, n: int, steps: int, tune_steps: int) -> dict:
     """C5: n independent uneven cuts in one batch (one launch per phase for all instances),
     against the same instances run one after another as single handles, over the same steps of
-    the run (10 + 2 * GRAPH_STEPS warm-up steps, the autotuner on a batch of its own)."""
+    the run (10 + 2 * GRAPH_STEPS warm-up steps, the autotuner on a batch of its own advanced to
+    the middle of the timed window)."""
     out = {"instances": n}
     warm = 10 + 2 * pymfc.GRAPH_STEPS
     for name, prec in (("fp64", pymfc.PREC_FP64), ("fp32c", pymfc.PREC_FP32C)):
         cfgs = pymfc.batch_configs(n, precision=prec)
         if tune_steps > 0:
             tb = pymfc.Batch(cfgs)
-            tb.step(10)
+            tb.step(warm + steps // 2)  # the middle of the timed window, as run_path
             tuning = tune(pymfc, tb, max(tune_steps, 2))
             tb.close()
         else:
@@ -356,10 +363,10 @@ def main() -> int:
     # kernel's roofline
     t_int = (r["phases"]["interact"] + r["phases"]["update"]) * 1e-3 if r["phases"] else None
     achieved = N * kbytes / t_int / 1e9 if t_int else None
-    tkey = ("kb_interact", head, r["tuning"]["compact_pairs"], r["tuning"].get("xcd_chunk", 8))
+    tkey = ("kb_interact", head, r["tuning"]["compact_pairs"], r["tuning"].get("xcd_chunk", 8),
+            r["tuning"]["stage_limit"])
     traffic = None
-    if args.workload == "thermal_contact_150k" and not split and r["tuning"]["stage_limit"] == 0 \
-            and tkey in MEASURED_TRAFFIC:
+    if args.workload == "thermal_contact_150k" and not split and tkey in MEASURED_TRAFFIC:
         traffic = MEASURED_TRAFFIC[tkey] * N
     step_s = r["wall"] / args.steps
     result = {

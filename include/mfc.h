@@ -329,15 +329,23 @@ int mfc_vjp(mfc_sim *sim, mfc_vjp_req *req, double grad[6]);
  * records in memory. xcd_chunk: the interaction kernels deal their tiles to the 8 XCDs in chunks
  * of this many consecutive tiles (1..MFC_XCD_CHUNK_MAX; 0 = 8); longer chunks fetch fewer
  * neighbour columns into two L2s, shorter ones balance the XCDs better. The Python package's
- * autotuner (pymfc.autotune) times the combinations on a live handle and keeps the fastest. */
+ * autotuner (pymfc.autotune) times the combinations on a live handle and keeps the fastest.
+ * sort_fast and sort_list bound the counting sort's paths (0 = the defaults; no autotuned
+ * setting: they exist so that every path of the sort can be run on any cut): a chunk of cells
+ * with at most sort_fast particles (1..MFC_SORT_FAST_MAX) sorts them in registers and LDS, a
+ * denser one walks its particle list with a bucket in memory, and a chunk with more than
+ * sort_list particles (>= 1) passes over every particle's cell id instead of its list. */
 #define MFC_STAGE_CAP 2048
 #define MFC_XCD_CHUNK_MAX 64
+#define MFC_SORT_FAST_MAX 3072
 typedef struct mfc_tuning {
   int split_interact;
   int compact_pairs;
   int stage_limit;
   int xcd_chunk;
-  int reserved[4];
+  int sort_fast;
+  int sort_list;
+  int reserved[2];
 } mfc_tuning;
 int mfc_set_tuning(mfc_sim *sim, const mfc_tuning *tuning);
 int mfc_get_tuning(const mfc_sim *sim, mfc_tuning *tuning);

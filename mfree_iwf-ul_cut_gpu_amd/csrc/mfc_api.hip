@@ -85,6 +85,9 @@ int check_tuning(const mfc_tuning *t) {
     return fail(MFC_E_INVALID, "stage_limit must be in 0.." + std::to_string(MFC_STAGE_CAP));
   if (t->xcd_chunk < 0 || t->xcd_chunk > mfc::XCD_CHUNK_MAX)
     return fail(MFC_E_INVALID, "xcd_chunk must be in 0.." + std::to_string(mfc::XCD_CHUNK_MAX));
+  if (t->sort_fast < 0 || t->sort_fast > MFC_SORT_FAST_MAX)
+    return fail(MFC_E_INVALID, "sort_fast must be in 0.." + std::to_string(MFC_SORT_FAST_MAX));
+  if (t->sort_list < 0) return fail(MFC_E_INVALID, "sort_list must be >= 0");
   return MFC_OK;
 }
 
@@ -317,6 +320,8 @@ int mfc_set_tuning(mfc_sim *s, const mfc_tuning *t) {
   HIP_TRY(hipStreamSynchronize(s->stream));
   s->tuning = *t;
   s->k.stage_limit = t->stage_limit;
+  s->k.sort_fast = t->sort_fast;
+  s->k.sort_list = t->sort_list;
   mfc::inst_args a2[2] = {instance_args(s, 0), instance_args(s, 1)};
   HIP_TRY(hipMemcpy(s->dargs, a2, sizeof(a2), hipMemcpyHostToDevice));
   for (int p = 0; p < 2; p++)

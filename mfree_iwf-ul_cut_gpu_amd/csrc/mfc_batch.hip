@@ -17,7 +17,7 @@ struct mfc_batch {
   std::vector<mfc_sim *> sims;
   hipStream_t stream = nullptr;
   bool f32 = false;
-  mfc_tuning tuning = {0, 0, 0, 0, {0, 0, 0, 0}};
+  mfc_tuning tuning = {0, 0, 0, 0, 0, 0, {0, 0}};
   mfc::inst_args *args[2] = {nullptr, nullptr};  // descriptors per start parity (device)
   int2 *tiles = nullptr, *chunks = nullptr;
   int ntiles = 0, nchunks = 0;
@@ -184,6 +184,8 @@ int mfc_batch_set_tuning(mfc_batch *b, const mfc_tuning *t) {
     mfc_sim *s = b->sims[i];
     s->tuning = *t;
     s->k.stage_limit = t->stage_limit;
+    s->k.sort_fast = t->sort_fast;
+    s->k.sort_list = t->sort_list;
     a0[i] = instance_args(s, 0);
     a1[i] = instance_args(s, 1);
   }

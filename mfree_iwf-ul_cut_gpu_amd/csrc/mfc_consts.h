@@ -25,6 +25,8 @@ struct dev_consts {
   double tanh_a, tanh_b, tanh_c, tanh_d;
   int plastic;        // Johnson-Cook return on
   int stage_limit;    // most window candidates a tile stages in LDS (0: all it has room for)
+  int sort_fast;      // most particles of a chunk the sort holds in registers (0: all it can)
+  int sort_list;      // most list entries a chunk of the sort reads (0: all the list holds)
   // adaptive grid: the geometry fields above (bbmin, cell, gnx, gny, num_cell) are rewritten on
   // the device at the start of every step; ncell_table is the size of the binning table (the
   // counting sort runs over it), num_cell for a locked grid

@@ -44,6 +44,7 @@ constexpr int SCAN_THREADS = 256;
 constexpr int SCAN_ITEMS = 4;                         // cells per thread of the sort
 constexpr int SCAN_CHUNK = SCAN_THREADS * SCAN_ITEMS;  // cells per chunk of the sort
 constexpr int SORT_BATCH = 12;            // particles per thread the sort holds in registers
+static_assert(SCAN_THREADS * SORT_BATCH == MFC_SORT_FAST_MAX, "the sort's register path bound of the header");
 constexpr int CARRY_BATCH = 4;            // 16-byte loads of chunk sums per thread the sort
                                           // has in flight at once
 
@@ -133,13 +134,16 @@ __device__ inline void hash_count(const dev_consts &k, int i, const double *__re
 // only global loads of the work-group. A larger chunk walks its list (or, when the list
 // overflowed, every particle's cell id) three times and uses the global bucket. Several cells
 // per thread keep the work-groups of a large table resident at once (a 500k-particle cut has
-// 1.18 million cells).
-__device__ inline void sort_chunk(int blk, int N, int n, const dbins &b) {
+// 1.18 million cells). sort_fast and sort_list (launch settings, 0: none) lower the two bounds,
+// so that the other paths can be run on any cut.
+__device__ inline void sort_chunk(int blk, int N, int n, const dbins &b, int sort_fast, int sort_list) {
   __shared__ int off_l[SCAN_CHUNK], cnt_l[SCAN_CHUNK], fill_l[SCAN_CHUNK];
   __shared__ int bk_l[SCAN_THREADS * SORT_BATCH];
   const int c0 = blk * SCAN_CHUNK;
   const int2 *__restrict__ list = b.clist + (size_t)blk * b.ccap;
-  const int fast_cap = b.ccap < SCAN_THREADS * SORT_BATCH ? b.ccap : SCAN_THREADS * SORT_BATCH;
+  const int list_cap = sort_list > 0 && sort_list < b.ccap ? sort_list : b.ccap;
+  int fast_cap = list_cap < SCAN_THREADS * SORT_BATCH ? list_cap : SCAN_THREADS * SORT_BATCH;
+  if (sort_fast > 0 && sort_fast < fast_cap) fast_cap = sort_fast;
   // the list's first entries, loaded before the chunk sum says how many are valid
   int2 e[SORT_BATCH];
 #pragma unroll
@@ -172,7 +176,7 @@ __device__ inline void sort_chunk(int blk, int N, int n, const dbins &b) {
     fill_l[q * SCAN_THREADS + threadIdx.x] = 0;
   }
   const bool fast = members <= fast_cap;  // the same in every thread
-  const bool listed = members <= b.ccap;
+  const bool listed = members <= list_cap;
   const int lim = listed ? members : N;
   __syncthreads();
   int ls[SORT_BATCH];
@@ -1141,7 +1145,8 @@ __global__ __launch_bounds__(SCAN_THREADS) void kb_sort(const inst_args *__restr
   if (w < 0) return;
   const int2 it = work[w];
   MFC_CONST(inst_args) *c = const_args(A, it.x);
-  sort_chunk(it.y, A[it.x].k.N, A[it.x].k.ncell_table, load_bins(&c->b));
+  const dev_consts &k = A[it.x].k;
+  sort_chunk(it.y, k.N, k.ncell_table, load_bins(&c->b), k.sort_fast, k.sort_list);
 }
 
 template <bool F32, bool GATHER>

@@ -72,7 +72,7 @@ struct mfc_sim {
   double *acc = nullptr;           // neighbour sums between the split interaction kernels
   double *wear_part = nullptr;     // tool wear: 4 doubles per tile, then the 4 results
   double wear_accum = 0.;          // accumulated wear, sum of dt * mean rate (tool_wear.cu:59)
-  mfc_tuning tuning = {0, 0, 0, 0, {0, 0, 0, 0}};
+  mfc_tuning tuning = {0, 0, 0, 0, 0, 0, {0, 0}};
   mfc::variant variant() const {
     return {tuning.split_interact != 0, tuning.compact_pairs != 0, tuning.xcd_chunk ? tuning.xcd_chunk : mfc::XCD_CHUNK};
   }

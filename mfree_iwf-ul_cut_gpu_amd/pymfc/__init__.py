@@ -117,6 +117,7 @@ class VjpReq(C.Structure):
 
 STAGE_CAP = 2048  # MFC_STAGE_CAP: window candidates a tile can stage in LDS
 XCD_CHUNK_MAX = 64  # MFC_XCD_CHUNK_MAX: longest XCD chunk of the interaction kernels' tile map
+SORT_FAST_MAX = 3072  # MFC_SORT_FAST_MAX: most particles of a chunk the sort holds in registers
 GRAPH_STEPS = 8   # steps per captured hipGraph of a handle (one graph per buffer parity)
 
 
@@ -166,7 +167,8 @@ def autotune(handle, steps: int = 100, space: dict | None = None, rounds: int = 
 class Tuning(C.Structure):
     """mfc_tuning (include/mfc.h)."""
     _fields_ = [("split_interact", C.c_int32), ("compact_pairs", C.c_int32), ("stage_limit", C.c_int32),
-                ("xcd_chunk", C.c_int32), ("reserved", C.c_int32 * 4)]
+                ("xcd_chunk", C.c_int32), ("sort_fast", C.c_int32), ("sort_list", C.c_int32),
+                ("reserved", C.c_int32 * 2)]
 
 
 THETA = ("jc_A", "jc_B", "jc_n", "jc_C", "jc_m", "mu")  # order of the adjoint's parameters
@@ -431,7 +433,8 @@ class Sim:
 
     def set_tuning(self, **kw) -> None:
         """Launch configuration (results are independent of it): split_interact=0|1,
-        compact_pairs=0|1, stage_limit=0..STAGE_CAP (0: the full LDS stage)."""
+        compact_pairs=0|1, stage_limit=0..STAGE_CAP (0: the full LDS stage),
+        xcd_chunk=0..XCD_CHUNK_MAX, sort_fast=0..SORT_FAST_MAX, sort_list>=0 (0: no bound)."""
         t = Tuning()
         _check(lib().mfc_get_tuning(self._h, C.byref(t)), "mfc_get_tuning")
         for k, v in kw.items():
@@ -442,7 +445,7 @@ class Sim:
         t = Tuning()
         _check(lib().mfc_get_tuning(self._h, C.byref(t)), "mfc_get_tuning")
         return {"split_interact": t.split_interact, "compact_pairs": t.compact_pairs, "stage_limit": t.stage_limit,
-                "xcd_chunk": t.xcd_chunk}
+                "xcd_chunk": t.xcd_chunk, "sort_fast": t.sort_fast, "sort_list": t.sort_list}
 
     def vjp_forward(self, nsteps: int, checkpoint_every: int, w0: int, w1: int) -> np.ndarray:
         """Run nsteps steps keeping a checkpoint every checkpoint_every steps; returns the mean

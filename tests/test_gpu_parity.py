@@ -206,6 +206,38 @@ def test_xcd_chunk_lengths_match(prec, split):
         assert equal(runs[0].force_history(), g.force_history())
 
 
+@pytest.mark.parametrize("prec", [pymfc.PREC_FP64, pymfc.PREC_FP32C])
+def test_sort_paths_match(prec):
+    """The counting sort takes a chunk's particles from its list into registers and LDS when the
+    chunk holds at most sort_fast of them, walks the list with a bucket in memory when it holds
+    more, and passes over every particle's cell id when it holds more than sort_list. Each cell's
+    order is the rank by current position whatever the path, so every bound gives the same bits:
+    sort_fast=1 sends every chunk of two or more particles down the list walk, sort_list=1 down
+    the pass over all particles, and the bounds of 64 and 200 mix the three paths in one step."""
+    cfg = pymfc.cut_config(200, 60, 10.0, thermal="contact", speed_m_min=1000.0, precision=prec)
+    runs = []
+    for fast, lst in ((0, 0), (1, 0), (0, 1), (64, 0), (64, 200)):
+        g = pymfc.Sim(cfg)
+        g.set_tuning(sort_fast=fast, sort_list=lst)
+        t = g.tuning()
+        assert (t["sort_fast"], t["sort_list"]) == (fast, lst)
+        g.step(30)
+        runs.append(g)
+    for g in runs[1:]:
+        for f in ["pos", "vel", "S", "T", "eps_pl", "fc", "ft", "flags", "id"]:
+            assert equal(runs[0].field(f), g.field(f)), f
+        assert equal(runs[0].cells(), g.cells())
+        assert equal(runs[0].force_history(), g.force_history())
+
+
+def test_sort_bounds_are_checked():
+    g = pymfc.Sim(small_cut())
+    for kw in ({"sort_fast": -1}, {"sort_fast": pymfc.SORT_FAST_MAX + 1}, {"sort_list": -1}):
+        with pytest.raises(pymfc.MfcError):
+            g.set_tuning(**kw)
+    g.set_tuning(sort_fast=pymfc.SORT_FAST_MAX, sort_list=1)
+
+
 def test_xcd_chunk_is_checked():
     g = pymfc.Sim(small_cut())
     for bad in (-1, pymfc.XCD_CHUNK_MAX + 1):
