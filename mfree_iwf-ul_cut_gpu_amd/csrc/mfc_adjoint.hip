@@ -221,14 +221,15 @@ __global__ __launch_bounds__(TILE) void kb_adj_theta(const adj_args *__restrict_
 // PULL_LANES consecutive lanes share a particle: lane r takes the window candidates r,
 // r + PULL_LANES, ... (numbered column by column, as the forward walk numbers them), and the
 // lanes' partial sums are added by a butterfly of shuffles, which gives every lane the same bits
-// ((g0 + g1) + (g2 + g3)). The pass is latency-bound (a long dependent chain of record loads and
-// pair adjoints per particle, 2 waves per SIMD at its register count), so more lanes per
-// particle shorten the chain. The count is fixed, so that an instance's gradient has the same
-// bits alone and in a batch. Measured backward pass (two pull passes then), ms per step with
-// 1 / 2 / 4 lanes: 6k particles 0.87 / 0.76 / 0.71, 37.5k 0.60 / 0.55 / 0.52, the 32-instance
-// batch of 577k 2.22 / 2.20 / 2.33. A block takes the PULL_LANES-th part of a tile: grid
-// padded(ntiles * PULL_LANES).
-constexpr int PULL_LANES = 4;
+// (((g0 + g1) + (g2 + g3)) + ...). The pass is latency-bound (a long dependent chain of record
+// loads and pair adjoints per particle, 2 waves per SIMD at its 256 VGPRs; forcing 3 waves
+// spills 336 bytes per lane and takes 40 % longer), so more lanes per particle shorten the
+// chain. The count is fixed, so that an instance's gradient has the same bits alone and in a
+// batch. Measured backward pass, ms per step with 2 / 4 / 8 lanes: 6k particles 0.357 / 0.317 /
+// 0.286, 37.5k 0.340 / 0.313 / 0.295, the 32-instance batch of 577k 1.80 / 1.88 / 1.92 (the
+// calibration cuts are the ones the reverse mode is for). A block takes the PULL_LANES-th part
+// of a tile: grid padded(ntiles * PULL_LANES).
+constexpr int PULL_LANES = 8;
 
 __global__ __launch_bounds__(TILE) void kb_adj_pull(const adj_args *__restrict__ D, const int2 *__restrict__ work,
                                                     int nwork) {
