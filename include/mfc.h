@@ -337,7 +337,7 @@ int mfc_vjp(mfc_sim *sim, mfc_vjp_req *req, double grad[6]);
  * sort_list particles (>= 1) passes over every particle's cell id instead of its list. */
 #define MFC_STAGE_CAP 2048
 #define MFC_XCD_CHUNK_MAX 64
-#define MFC_SORT_FAST_MAX 3072
+#define MFC_SORT_FAST_MAX 1536
 typedef struct mfc_tuning {
   int split_interact;
   int compact_pairs;

@@ -41,9 +41,9 @@ namespace mfc {
 namespace {
 
 constexpr int SCAN_THREADS = 256;
-constexpr int SCAN_ITEMS = 4;                         // cells per thread of the sort
+constexpr int SCAN_ITEMS = 2;                         // cells per thread of the sort
 constexpr int SCAN_CHUNK = SCAN_THREADS * SCAN_ITEMS;  // cells per chunk of the sort
-constexpr int SORT_BATCH = 12;            // particles per thread the sort holds in registers
+constexpr int SORT_BATCH = 6;            // particles per thread the sort holds in registers
 static_assert(SCAN_THREADS * SORT_BATCH == MFC_SORT_FAST_MAX, "the sort's register path bound of the header");
 constexpr int CARRY_BATCH = 4;            // 16-byte loads of chunk sums per thread the sort
                                           // has in flight at once
@@ -132,9 +132,9 @@ __device__ inline void hash_count(const dev_consts &k, int i, const double *__re
 // chunk holds about 2.25 per cell at the lattice spacing) keeps them in registers and their
 // buckets in LDS, and its list, chunk sum and preceding chunk sums are loaded at once: the
 // only global loads of the work-group. A larger chunk walks its list (or, when the list
-// overflowed, every particle's cell id) three times and uses the global bucket. Several cells
-// per thread keep the work-groups of a large table resident at once (a 500k-particle cut has
-// 1.18 million cells). sort_fast and sort_list (launch settings, 0: none) lower the two bounds,
+// overflowed, every particle's cell id) three times and uses the global bucket. Two cells per
+// thread keep most work-groups of a large table resident at once (a 500k-particle cut has 1.18
+// million cells: 2,310 work-groups). sort_fast and sort_list (launch settings, 0: none) lower the two bounds,
 // so that the other paths can be run on any cut.
 __device__ inline void sort_chunk(int blk, int N, int n, const dbins &b, int sort_fast, int sort_list) {
   __shared__ int off_l[SCAN_CHUNK], cnt_l[SCAN_CHUNK], fill_l[SCAN_CHUNK];

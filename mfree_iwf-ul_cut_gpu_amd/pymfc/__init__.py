@@ -117,7 +117,7 @@ class VjpReq(C.Structure):
 
 STAGE_CAP = 2048  # MFC_STAGE_CAP: window candidates a tile can stage in LDS
 XCD_CHUNK_MAX = 64  # MFC_XCD_CHUNK_MAX: longest XCD chunk of the interaction kernels' tile map
-SORT_FAST_MAX = 3072  # MFC_SORT_FAST_MAX: most particles of a chunk the sort holds in registers
+SORT_FAST_MAX = 1536  # MFC_SORT_FAST_MAX: most particles of a chunk the sort holds in registers
 GRAPH_STEPS = 8   # steps per captured hipGraph of a handle (one graph per buffer parity)
 
 
