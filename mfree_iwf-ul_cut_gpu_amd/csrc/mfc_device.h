@@ -176,6 +176,25 @@ __device__ __attribute__((noinline)) double rr_residual_jc(double dl, double nor
   return norm - 2 * G * dl - s23 * sY;
 }
 
+// The same residual for the FP32C path, with the device library's pow and log: FP64 throughout
+// (the return stays in double precision), but not the oracle's own pow and log, whose bits the
+// FP64 path must reproduce; the library's are about half the instructions and branch-free, and
+// the secant's chain of residuals sets the end of the interaction launch late in a cut.
+__device__ __attribute__((noinline)) double rr_residual_jc_lib(double dl, double norm, double eps0, double TC,
+                                                              double A, double B, double n, double C, double edr,
+                                                              double dt, double G) {
+  const double s23 = sqrt(2.0 / 3.0);
+  double e = eps0 + s23 * fmax(dl, 0.);
+  double ed = s23 * fmax(dl, 0.) / dt;
+  double TA = A + B * pow(e, n);
+  double TB = 1.0;
+  double es = ed / edr;
+  if (es > 1.0) TB = 1.0 + C * log(es);
+  else TB = pow((1.0 + es), C);
+  double sY = TA * TB * TC;
+  return norm - 2 * G * dl - s23 * sY;
+}
+
 __device__ __attribute__((noinline)) double rr_residual_tanh(const dev_consts &k, double dl, double norm, double eps0,
                                                             double t, double TC) {
   const double s23 = sqrt(2.0 / 3.0);
@@ -250,6 +269,8 @@ __device__ inline int nth_set_bit(unsigned long long m, int n) {
 // dependent chain. Called by all 64 lanes of the wave together; need marks the lanes whose
 // particle returns, with their inputs. Returns dl on those lanes, and in folded whether the last
 // update took the fold branch (the reverse mode differentiates the fold's fixed point there).
+// LIB (the FP32C path): plain Johnson-Cook residuals with the device library's pow and log.
+template <bool LIB>
 __device__ inline double rr_secant_wave(const dev_consts &k, bool need, const rr_input &in, double eps0, double t,
                                         bool &folded) {
   const int lane = (int)(threadIdx.x & 63);
@@ -277,6 +298,7 @@ __device__ inline double rr_secant_wave(const dev_consts &k, bool need, const rr
         const double dl1 = dl * 1.01;
         const double x = second ? dl1 : dl;
         const double gx = tanh_model ? rr_residual_tanh(k, x, norm, e0, tt, TC)
+                          : LIB      ? rr_residual_jc_lib(x, norm, e0, TC, A, B, n, C, edr, dt, G)
                                      : rr_residual_jc(x, norm, e0, TC, A, B, n, C, edr, dt, G);
         const double gy = __shfl_xor(gx, 1, 64);
         const double g = second ? gy : gx, g1 = second ? gx : gy;

@@ -721,11 +721,12 @@ __device__ inline void finish_return(const dev_consts &k, int i, double dl, defe
 
 // the deferred returns of the wave (all 64 lanes together); dl_out (the reverse mode's replay,
 // the same in every thread): each particle's secant result, -dl where its last update folded,
-// 0 for no return
+// 0 for no return. F32: the FP32C path's secant (rr_secant_wave<true>)
+template <bool F32>
 __device__ inline void wave_returns(const dev_consts &k, int i, deferred_return &dr, const dstate &dst,
                                     double *__restrict__ dl_out) {
   bool folded;
-  const double dl = rr_secant_wave(k, dr.need, dr.in, dr.eps, dr.T, folded);
+  const double dl = rr_secant_wave<F32>(k, dr.need, dr.in, dr.eps, dr.T, folded);
   if (dr.need) finish_return(k, i, dl, dr, dst);
   if (dl_out && i < k.N) dl_out[i] = dr.need ? (folded ? -dl : dl) : 0.;
 }
@@ -854,7 +855,7 @@ __device__ inline void interact(const dev_consts &k, int t, const drecords &r, c
     if (acc_out) store_acc(acc_out, k.N, i, acc, tpse);  // the same in every thread
     update_particle(k, i, s, acc, tpse, src, dst, pos, *tool_now, fsum, xn, yn, fln, dr);
   }
-  wave_returns(k, i, dr, dst, acc_out ? acc_out + (size_t)ACC_N * k.N : nullptr);
+  wave_returns<F32>(k, i, dr, dst, acc_out ? acc_out + (size_t)ACC_N * k.N : nullptr);
   next_cell(k, i, i < k.N, xn, yn, fln, next, bin_next);
   force_partials(i >> 6, fsum, fo.partials);
   if (t == 0 && threadIdx.x == 0) *fo.pending = 1;
@@ -882,6 +883,7 @@ __device__ inline void sums(const dev_consts &k, int t, const drecords &r, const
 }
 
 // split, second half: the per-particle update from the stored sums (dl_out: as wave_returns)
+template <bool F32>
 __device__ inline void update(const dev_consts &k, int t, const double *__restrict__ acc,
                               const int *__restrict__ perm, const dstate &src, const dstate &dst,
                               const dpositional &pos, const mfc_tool_state *__restrict__ tool_now, const dforce &fo,
@@ -898,7 +900,7 @@ __device__ inline void update(const dev_consts &k, int t, const double *__restri
     load_acc(acc, k.N, i, a, tpse);
     update_particle(k, i, perm[i], a, tpse, src, dst, pos, *tool_now, fsum, xn, yn, fln, dr);
   }
-  wave_returns(k, i, dr, dst, dl_out);
+  wave_returns<F32>(k, i, dr, dst, dl_out);
   next_cell(k, i, i < k.N, xn, yn, fln, next, bin_next);
   force_partials(i >> 6, fsum, fo.partials);
   if (t == 0 && threadIdx.x == 0) *fo.pending = 1;
@@ -1222,6 +1224,7 @@ __global__ __launch_bounds__(TILE) SUMS_OCCUPANCY void kb_sums(const inst_args *
                      a.cur.flags, c->acc);
 }
 
+template <bool F32>
 __global__ __launch_bounds__(TILE) void kb_update(const inst_args *__restrict__ A, const int2 *__restrict__ work,
                                                   int nwork) {
   const int w = xcd_tile(blockIdx.x, nwork);
@@ -1230,8 +1233,8 @@ __global__ __launch_bounds__(TILE) void kb_update(const inst_args *__restrict__ 
   const inst_args &a = A[it.x];
   MFC_CONST(inst_args) *c = const_args(A, it.x);
   const dbins b = load_bins(&c->b);
-  update(a.k, it.y, c->acc, b.perm, a.cur, a.nxt, a.pos, a.tool_out, a.f, b, a.bin_next && !a.k.adaptive,
-         a.acc_out ? a.acc_out + (size_t)ACC_N * a.k.N : nullptr);
+  update<F32>(a.k, it.y, c->acc, b.perm, a.cur, a.nxt, a.pos, a.tool_out, a.f, b, a.bin_next && !a.k.adaptive,
+              a.acc_out ? a.acc_out + (size_t)ACC_N * a.k.N : nullptr);
 }
 
 __global__ __launch_bounds__(TILE) void kb_force_final(const inst_args *__restrict__ A) {
@@ -1337,7 +1340,8 @@ void launch_batch_sums(const batch_work &w, hipStream_t s, hipEvent_t t0, hipEve
 }
 
 void launch_batch_update(const batch_work &w, hipStream_t s, hipEvent_t t0, hipEvent_t t1) {
-  launch_timed(kb_update, dim3(padded_tiles(w.ntiles)), w, s, t0, t1);
+  if (w.f32) launch_timed(kb_update<true>, dim3(padded_tiles(w.ntiles)), w, s, t0, t1);
+  else launch_timed(kb_update<false>, dim3(padded_tiles(w.ntiles)), w, s, t0, t1);
 }
 
 void launch_batch_interact_fused(const batch_work &w, hipStream_t s, hipEvent_t t0, hipEvent_t t1) {
