@@ -134,8 +134,8 @@ __device__ inline void hash_count(const dev_consts &k, int i, const double *__re
 // only global loads of the work-group. A larger chunk walks its list (or, when the list
 // overflowed, every particle's cell id) three times and uses the global bucket. Two cells per
 // thread keep most work-groups of a large table resident at once (a 500k-particle cut has 1.18
-// million cells: 2,310 work-groups). sort_fast and sort_list (launch settings, 0: none) lower the two bounds,
-// so that the other paths can be run on any cut.
+// million cells: 2,310 work-groups). sort_fast and sort_list (launch settings, 0: none) lower
+// the two bounds, so that the other paths can be run on any cut.
 __device__ inline void sort_chunk(int blk, int N, int n, const dbins &b, int sort_fast, int sort_list) {
   __shared__ int off_l[SCAN_CHUNK], cnt_l[SCAN_CHUNK], fill_l[SCAN_CHUNK];
   __shared__ int bk_l[SCAN_THREADS * SORT_BATCH];
