@@ -906,6 +906,8 @@ __device__ inline void update(const dev_consts &k, int t, const double *__restri
   if (t == 0 && threadIdx.x == 0) *fo.pending = 1;
 }
 
+constexpr int FF_BATCH = 8;  // wave partials a thread of force_final loads at a time
+
 // tool force of the step from the wave partials, when the interaction of a step has left them
 // (pending): lane l adds the results of waves l, l + TILE, ... in ascending order, then the lane
 // sums are reduced by halving strides; F = sum(fc) + sum(ft). All threads of the work-group
@@ -914,9 +916,20 @@ __device__ inline void force_final(int ntiles, const dforce &fo) {
   if (*fo.pending == 0) return;  // the same value in every thread
   __shared__ double red[4][TILE];
   double a0 = 0., a1 = 0., a2 = 0., a3 = 0.;
-  for (int q = threadIdx.x; q < 4 * ntiles; q += TILE) {  // the wave partials, 4 per tile
-    const double4 v = reinterpret_cast<const double4 *>(fo.partials)[q];
-    a0 = a0 + v.x; a1 = a1 + v.y; a2 = a2 + v.z; a3 = a3 + v.w;
+  // each thread adds the wave partials q = threadIdx.x, + TILE, + 2 TILE, ... in that order (4
+  // per tile), eight loads in flight at a time: this work-group is the first of the step's
+  // predict launch, and one load at a time made it that launch's last
+  const double4 *__restrict__ p4 = reinterpret_cast<const double4 *>(fo.partials);
+  const int nq = 4 * ntiles;
+  for (int q0 = threadIdx.x; q0 < nq; q0 += TILE * FF_BATCH) {
+    double4 v[FF_BATCH];
+#pragma unroll
+    for (int u = 0; u < FF_BATCH; u++) v[u] = q0 + u * TILE < nq ? p4[q0 + u * TILE] : make_double4(0., 0., 0., 0.);
+#pragma unroll
+    for (int u = 0; u < FF_BATCH; u++)
+      if (q0 + u * TILE < nq) {
+        a0 = a0 + v[u].x; a1 = a1 + v[u].y; a2 = a2 + v[u].z; a3 = a3 + v[u].w;
+      }
   }
   red[0][threadIdx.x] = a0; red[1][threadIdx.x] = a1;
   red[2][threadIdx.x] = a2; red[3][threadIdx.x] = a3;
