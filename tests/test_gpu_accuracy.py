@@ -66,16 +66,16 @@ def test_fp32c_full_thermal_contact_cut():
 
 
 def test_fp32c_pse_500k_cut():
-    """The 1250 x 400 jittered PSE cut at 1.2 um (500,000 particles), 10,000 steps. The 1 K
-    peak-temperature target is the thermal-contact cut's; here the hottest particle at the last
-    step differs by about 1 K (after 10,000 steps of PSE sums over all 25 window cells), so the
-    bound on it is 2 K, and the peak through the second half is held to 1 K."""
+    """The 1250 x 400 jittered PSE cut at 1.2 um (500,000 particles), 10,000 steps, held to the
+    thermal-contact cut's targets: the peak temperature at the last step and through the second
+    half within 1 K of the FP64 path's (measured: 0.73 K and 0.08 K; the kernels sum in a fixed
+    order, so the figures repeat from run to run)."""
     steps = 10000
     a, b = run_both("pse_500k", steps)
     err = errors(a, b, 1.2e-4, steps)
     print("fp32c vs fp64, pse_500k, 10000 steps:", err)
     assert err["force_rel"][0] < 5e-3
     assert err["Tpeak_mean_K"] < 1.0
-    assert err["Tpeak_K"] < 2.0            # one particle's temperature, see below
+    assert err["Tpeak_K"] < 1.0
     assert err["blanked"][1] == err["blanked"][0]
 
